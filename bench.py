@@ -1,0 +1,156 @@
+"""Benchmark of the Shan-Chen lattice-Boltzmann step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+
+A step is one time step of the whole lattice (force + collision + streaming)
+with the populations already resident in HBM.  Default workload: D3Q19
+256^3 periodic, FP64, tau = 1, G = -5.5, droplet R = 48 (the drop-in FP64
+configuration).  Prints ONE JSON line with MLUPS, the HBM roofline of the
+step kernel (algorithmic bytes 2*Q*b per lattice update over the kernel's
+measured time) and a CPU baseline: the oracle (a restatement of the
+reference's CPU code) timed on this host's cores on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import mesobasic_amd as mb  # noqa: E402
+
+METRIC = "D3Q19 Shan-Chen MLUPS & % HBM roofline; batched-serve p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # name: q, shape, precision, tau, G, init
+    "d3q19_fp64_256_droplet": (19, (256, 256, 256), mb.FP64, 1.0, -5.5, ("droplet", 48.0)),
+    "d3q19_fp64_512_noise": (19, (512, 512, 512), mb.FP64, 1.0, -5.0, ("noise", 1)),
+    "d3q19_fp32_512_noise": (19, (512, 512, 512), mb.FP32, 1.0, -5.0, ("noise", 1)),
+    "d3q19_fp16_512_noise": (19, (512, 512, 512), mb.FP16S, 1.0, -5.0, ("noise", 1)),
+    "d3q19_bf16_512_noise": (19, (512, 512, 512), mb.BF16S, 1.0, -5.0, ("noise", 1)),
+    "d2q9_fp64_256_noise": (9, (256, 256), mb.FP64, 1.0, -5.0, ("noise", 1)),
+}
+BYTES_PER_POP = {mb.FP64: 8, mb.FP32: 4, mb.FP16S: 2, mb.BF16S: 2}
+DTYPE = {mb.FP64: "f64", mb.FP32: "f32", mb.FP16S: "f32 compute / f16 storage",
+         mb.BF16S: "f32 compute / bf16 storage"}
+KERNEL_NAMES = {mb.KERNEL_TWO_PASS: "two_pass", mb.KERNEL_FUSED: "fused"}
+
+
+def initial_rho(shape, init):
+    if init[0] == "droplet":
+        return mb.droplet_rho(shape, init[1])
+    return mb.noise_rho(shape, init[1])
+
+
+def cpu_baseline(q, shape, tau, G, init, budget_s=15.0):
+    """Times the oracle (CPU restatement of the reference step, x-slab
+    threads) on a bounded sample: the same lattice, as many steps as fit
+    in about budget_s seconds after one warm-up step."""
+    import oracle
+    cores = min(os.cpu_count() or 1, 16)
+    o = oracle.Oracle(q, shape, tau, G, nthreads=cores)
+    o.set_rho(initial_rho(shape, init))
+    o.init()
+    t0 = time.perf_counter()
+    o.step(1)
+    t1 = time.perf_counter()
+    per = max(t1 - t0, 1e-6)
+    k = max(1, min(200, int(budget_s / per)))
+    t0 = time.perf_counter()
+    o.step(k)
+    dt = time.perf_counter() - t0
+    n = int(np.prod(shape))
+    o.close()
+    return {"value": round(n * k / dt / 1e6, 3), "unit": "MLUPS", "cores": cores, "kind": "port",
+            "sample": f"oracle (CPU restatement of the reference step, AoS FP64, {cores} x-slab "
+                      f"threads), {'x'.join(map(str, shape))} lattice, {k} timed steps after 1 warm-up"}
+
+
+def load_traffic(kernel_name, workload):
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel_name}_{workload}.json")
+    if os.path.exists(path):
+        with open(path) as fh:
+            return json.load(fh).get("bytes_per_launch")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="d3q19_fp64_256_droplet", choices=sorted(WORKLOADS))
+    ap.add_argument("--kernel", default="auto", choices=["auto", "two_pass", "fused"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the lattice would shard along x; see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    q, shape, prec, tau, G, init = WORKLOADS[args.workload]
+    kernel = {"auto": mb.KERNEL_AUTO, "two_pass": mb.KERNEL_TWO_PASS,
+              "fused": mb.KERNEL_FUSED}[args.kernel]
+    n = int(np.prod(shape))
+    lat = mb.Lattice(q, shape, tau, G, precision=prec, kernel=kernel)
+    lat.set_rho(initial_rho(shape, init))
+    lat.init()
+    print(f"# workload {args.workload} kernel {KERNEL_NAMES.get(lat.kernel)} "
+          f"device bytes {lat.device_bytes / 2**30:.2f} GiB", file=sys.stderr, flush=True)
+    lat.step_timed(args.warmup)
+    lat.synchronize()
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+    t0 = time.perf_counter()
+    ms = lat.step_timed(args.steps)     # HIP events on the kernels' stream
+    wall = time.perf_counter() - t0
+    lat.synchronize()
+    ms_per_step = ms / args.steps
+    mlups = n / (ms_per_step * 1e-3) / 1e6
+    bpl = 2 * q * BYTES_PER_POP[prec]
+    cons = lat.conservation()
+    kname = KERNEL_NAMES.get(lat.kernel, "unknown")
+    achieved = bpl * n / (ms_per_step * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(mlups, 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": DTYPE[prec],
+        "data": "synthetic (seeded initial density; no dataset)",
+        "config": {"workload": args.workload, "stencil": "D3Q19" if q == 19 else "D2Q9",
+                   "lattice": list(shape), "tau": tau, "G": G, "init": list(init),
+                   "boundary": "periodic", "kernel": kname,
+                   "wall_s_timed_region": round(wall, 4)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": load_traffic(kname, args.workload),
+                     "algorithmic_bytes_per_lu": bpl,
+                     "kernel": f"step ({kname}); HIP-event time per step over the timed region"},
+        "mass_after": cons[0],
+    }
+    lat.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(q, shape, tau, G, init)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,156 @@
+/*
+ * mesobasic_lbm.h -- C ABI of the MI355X Shan-Chen lattice-Boltzmann step.
+ *
+ * One handle = one lattice resident in HBM (structure-of-arrays populations,
+ * A/B buffers).  Plain C types only; host arrays are interior nodes in the
+ * reference's order (x slowest, last axis fastest: index (x*NY + y)*NZ + z,
+ * or x*NY + y in 2-D), caller-owned.  Every call returns 0 on success or a
+ * negative LBM_E* code; lbm_last_error() gives the message.  A handle is not
+ * thread-safe; distinct handles are independent.
+ *
+ * Each entry point replaces a piece of the reference's LBfluid3D/LBfluid2D
+ * and scmp3D interface (flowzario/mesoBasic, src/lattice_boltzmann); the
+ * reference's C++ classes are what a maintainer binds these to (the drop-in
+ * classes in mesobasic_amd/host do exactly that).
+ */
+#ifndef MESOBASIC_LBM_H
+#define MESOBASIC_LBM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_ABI_VERSION 1
+
+enum lbm_status {
+    LBM_OK = 0,
+    LBM_EINVAL = -1,   /* bad argument or description */
+    LBM_ENOMEM = -2,   /* device allocation failed or over the HBM budget */
+    LBM_EHIP = -3,     /* HIP runtime error */
+    LBM_ESTATE = -4,   /* call out of order (e.g. step before init) */
+    LBM_ENODEV = -5    /* no usable gfx950 device */
+};
+
+enum lbm_precision {
+    LBM_FP64 = 0,      /* FP64 compute and storage: the drop-in parity path */
+    LBM_FP32 = 1,      /* FP32 compute and storage */
+    LBM_FP16S = 2,     /* FP32 compute, FP16 storage of f_i - w_i*rho_shift */
+    LBM_BF16S = 3      /* FP32 compute, BF16 storage of f_i - w_i*rho_shift */
+};
+
+enum lbm_kernel {
+    LBM_KERNEL_AUTO = 0,
+    LBM_KERNEL_TWO_PASS = 1, /* density pass, then force+collide+stream pass */
+    LBM_KERNEL_FUSED = 2     /* one pass, psi halo recomputed in LDS */
+};
+
+/* Lattice description.  Mirrors the keys the reference reads:
+ *   q          "LBApp/stencil" D2Q9 -> 9, D3Q19 -> 19 (scmp3D.cpp:30-32)
+ *   nx,ny,nz   "Domain/nx,ny,nz" (LBApp.cpp:18-20); nz = 1 for q = 9
+ *   tau        "LBApp/tauA" (scmp3D.cpp:24-25)
+ *   G          "LBApp/G" (scmp3D.cpp:21)
+ *   order      3: scmp3D step order (scmp3D.cpp:90-119)
+ *              2: LBfluid2D step order (LBfluid2D.cpp:227-312)
+ *   walls      1: halfway bounce-back walls on the last axis
+ *              (LBfluid3D.cpp:316-347 in 3-D, LBfluid2D.cpp:363-388 in 2-D)
+ *   precision  enum lbm_precision
+ *   rho_shift  density whose equilibrium is subtracted before 16-bit storage
+ *   kernel     enum lbm_kernel
+ */
+typedef struct lbm_desc {
+    int q;
+    int nx, ny, nz;
+    double tau;
+    double G;
+    int order;
+    int walls;
+    int precision;
+    double rho_shift;
+    int kernel;
+} lbm_desc;
+
+typedef struct lbm_handle lbm_handle;
+
+/* Allocates the lattice on the current device.  Replaces the LBfluid3D /
+ * LBfluid2D constructors and allocateFs (LBfluid3D.cpp:26-95,
+ * LBfluid2D.cpp:26-89). */
+int lbm_create(const lbm_desc* desc, lbm_handle** out);
+void lbm_destroy(lbm_handle* h);
+
+/* Initial density (host, nx*ny*nz doubles); velocities zero.  Replaces the
+ * setRho/setU/setV/setW fill of scmp3D::initLatticeBoltzmann
+ * (scmp3D.cpp:63-76). */
+int lbm_set_rho(lbm_handle* h, const double* rho0);
+
+/* ghost rho -> Shan-Chen force -> f = feq(rho, u + tau*F/rho).  Replaces
+ * scmp3D.cpp:78-80 (ghostNodesRho, calculateShanChenForces, setFtoFeq). */
+int lbm_init_equilibrium(lbm_handle* h);
+
+/* Advances nsteps time steps on the handle's stream (asynchronous).
+ * Replaces scmp3D::updateLatticeBoltzmann (scmp3D.cpp:90-119) or, in order
+ * 2, force + LBfluid2D::updateFluid_SC (mcmp2D.cpp:111-121), including the
+ * bounce-back walls when enabled. */
+int lbm_step(lbm_handle* h, int nsteps);
+
+/* As lbm_step, bracketed by HIP events on the handle's stream; waits and
+ * returns the elapsed milliseconds. */
+int lbm_step_timed(lbm_handle* h, int nsteps, double* ms);
+
+enum lbm_fields_kind {
+    LBM_FIELDS_REFERENCE = 0, /* the reference's r,u,v,w member arrays:
+                                 order 3 -> moments of f(N-1),
+                                 order 2 -> moments of f(N); rho0, 0 at N=0 */
+    LBM_FIELDS_CURRENT = 1    /* moments of the current f(N) */
+};
+
+/* Density and bare velocity (LBfluid3D::macros, LBfluid3D.cpp:208-244),
+ * host arrays of nx*ny*nz doubles; any pointer may be NULL; w ignored in
+ * 2-D.  Synchronous. */
+int lbm_get_fields(lbm_handle* h, int kind, double* rho, double* u, double* v,
+                   double* w);
+
+/* Post-streaming populations f (after walls), host [node][q] doubles as the
+ * reference's f array (LBfluid3D.hpp:29).  Synchronous. */
+int lbm_get_f(lbm_handle* h, double* f);
+
+/* Replaces the populations (host [node][q]); the next step starts from them
+ * with the reference r,u arrays taken as their moments. */
+int lbm_set_f(lbm_handle* h, const double* f);
+
+/* FP64 totals over the lattice of the current f: out[0] = mass,
+ * out[1..3] = momentum (x, y, z; z = 0 in 2-D).  Synchronous. */
+int lbm_conservation(lbm_handle* h, double out[4]);
+
+/* Number of completed steps since lbm_init_equilibrium / lbm_set_f. */
+int64_t lbm_steps_done(const lbm_handle* h);
+
+/* Stream the handle launches on (a hipStream_t); NULL restores the handle's
+ * own stream.  The caller keeps ownership of a stream it passes. */
+int lbm_set_stream(lbm_handle* h, void* stream);
+void* lbm_get_stream(const lbm_handle* h);
+int lbm_synchronize(lbm_handle* h);
+
+/* Device bytes held by the handle, and the kernel it resolved to. */
+int64_t lbm_device_bytes(const lbm_handle* h);
+int lbm_kernel_in_use(const lbm_handle* h);
+
+/* Message for the last failing call on this thread. */
+const char* lbm_last_error(void);
+
+/* Seeded inputs (host only, no device work).  noise: rho = mean +
+ * amp*(x/4294967295.0 - 0.5) for successive std::mt19937(seed) outputs x in
+ * reference loop order -- the scmp3D.cpp:67-69 fill with a reproducible
+ * generator.  droplet: (rl+rv)/2 - (rl-rv)/2*tanh(2(r-R)/W), r from the
+ * geometric centre ((N-1)/2 per axis).  nz = 1 for 2-D. */
+int lbm_fill_noise(uint32_t seed, int64_t n, double mean, double amp, double* out);
+int lbm_fill_droplet(int nx, int ny, int nz, double R, double W, double rl,
+                     double rv, double* out);
+
+int lbm_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

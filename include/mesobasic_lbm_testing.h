@@ -1,0 +1,21 @@
+/*
+ * mesobasic_lbm_testing.h -- test hooks of libmesobasic_lbm.so.  These have
+ * no counterpart in the reference; they expose device helpers to the parity
+ * tests.
+ */
+#ifndef MESOBASIC_LBM_TESTING_H
+#define MESOBASIC_LBM_TESTING_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* y[i] = the device exp() used by the FP64 kernels, for n host values x. */
+int lbm_probe_exp(const double* x, double* y, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,451 @@
+// lbm_capi.cpp -- the C ABI declared in include/mesobasic_lbm.h: handle
+// lifetime, device memory, step sequencing and host<->device copies.  All
+// arithmetic runs in the HIP kernels; there is no host fallback.
+#include "../../include/mesobasic_lbm.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "lbm_internal.h"
+#include "lbm_lowp.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_OK(expr)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return fail(LBM_EHIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_),      \
+                        __FILE__, __LINE__);                                          \
+    } while (0)
+
+// Plane stride: rounded up to 256 B and skewed by 512 B per population so
+// that the Q planes of a large power-of-two lattice do not start on the same
+// HBM channel.
+int64_t plane_stride(int64_t n, int elem_bytes)
+{
+    const int64_t per256 = 256 / elem_bytes;
+    int64_t ps = (n + per256 - 1) / per256 * per256;
+    if ((ps * elem_bytes) % 4096 == 0) ps += 512 / elem_bytes;
+    return ps;
+}
+
+}  // namespace
+
+struct lbm_handle {
+    lbm_desc d{};
+    lbm::Grid g{};
+    lbm::StepArgs a{};
+    int elem = 8;             // bytes per stored population
+    void* buf[2] = {nullptr, nullptr};
+    int kind[2] = {lbm::kHoldsF, lbm::kHoldsF};
+    int cur = 0;
+    double* psi = nullptr;    // FP64 two-pass density field
+    double* rho0 = nullptr;   // initial density (reference r array at N=0)
+    double* tmp = nullptr;    // staging for fields / f / partials
+    int64_t tmp_bytes = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int64_t steps = 0;
+    bool initialised = false;
+    bool first_step_stored = false;  // LBfluid2D order: step 1 uses rho0, u=0
+    bool prev_valid = false;         // buf[1-cur] holds f(N-1)
+    bool ref_is_rho0 = false;        // reference arrays still hold rho0, u = 0
+    int kernel = LBM_KERNEL_TWO_PASS;
+    int64_t bytes = 0;
+};
+
+namespace {
+
+int ensure_tmp(lbm_handle* h, int64_t bytes)
+{
+    if (h->tmp_bytes >= bytes) return LBM_OK;
+    if (h->tmp) hipFree(h->tmp);
+    h->tmp = nullptr;
+    h->tmp_bytes = 0;
+    HIP_OK(hipMalloc(&h->tmp, bytes));
+    h->tmp_bytes = bytes;
+    return LBM_OK;
+}
+
+bool is_lowp(const lbm_handle* h) { return h->d.precision != LBM_FP64; }
+
+int run_step(lbm_handle* h)
+{
+    const int src = h->cur, dst = 1 - h->cur;
+    if (!is_lowp(h)) {
+        auto* Pin = static_cast<const double*>(h->buf[src]);
+        auto* Pout = static_cast<double*>(h->buf[dst]);
+        const int mode = h->first_step_stored ? 1 : 0;
+        if (h->kernel == LBM_KERNEL_FUSED && mode == 0) {
+            HIP_OK(lbm::fp64_fused_step(h->a, h->g, Pin, h->kind[src], Pout, h->stream));
+        } else {
+            if (mode == 1) HIP_OK(lbm::fp64_psi_from_rho(h->rho0, h->psi, h->g.n, h->stream));
+            else HIP_OK(lbm::fp64_psi_from_pops(h->a, h->g, Pin, h->kind[src], h->psi, h->stream));
+            HIP_OK(lbm::fp64_collide(h->a, h->g, Pin, h->kind[src], Pout, h->psi, h->rho0, mode,
+                                     h->stream));
+        }
+    } else {
+        HIP_OK(lbm::lowp_step(h->d.precision, h->a, h->g, h->d.rho_shift, h->buf[src],
+                              h->kind[src], h->buf[dst], h->first_step_stored ? h->rho0 : nullptr,
+                              reinterpret_cast<float*>(h->psi), h->kernel, h->stream));
+    }
+    h->kind[dst] = lbm::kHoldsPost;
+    h->cur = dst;
+    h->first_step_stored = false;
+    h->prev_valid = true;
+    h->steps++;
+    return LBM_OK;
+}
+
+// Moments of buffer b into the four host arrays.
+int moments_to_host(lbm_handle* h, int b, double* rho, double* u, double* v, double* w)
+{
+    const int64_t n = h->g.n;
+    int rc = ensure_tmp(h, 4 * n * 8);
+    if (rc) return rc;
+    double* d = h->tmp;
+    if (!is_lowp(h))
+        HIP_OK(lbm::fp64_moments(h->a, h->g, static_cast<const double*>(h->buf[b]), h->kind[b],
+                                 d, d + n, d + 2 * n, d + 3 * n, h->stream));
+    else
+        HIP_OK(lbm::lowp_moments(h->d.precision, h->a, h->g, h->d.rho_shift, h->buf[b],
+                                 h->kind[b], d, d + n, d + 2 * n, d + 3 * n, h->stream));
+    double* outs[4] = {rho, u, v, w};
+    for (int k = 0; k < 4; k++)
+        if (outs[k])
+            HIP_OK(hipMemcpyAsync(outs[k], d + k * n, n * 8, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    return LBM_OK;
+}
+
+int check_desc(const lbm_desc* d)
+{
+    if (!d) return fail(LBM_EINVAL, "lbm_create: desc is NULL");
+    if (d->q != 9 && d->q != 19) return fail(LBM_EINVAL, "q must be 9 or 19 (got %d)", d->q);
+    if (d->nx < 1 || d->ny < 1 || d->nz < 1)
+        return fail(LBM_EINVAL, "lattice sizes must be >= 1 (got %d x %d x %d)", d->nx, d->ny, d->nz);
+    if (d->q == 9 && d->nz != 1) return fail(LBM_EINVAL, "D2Q9 needs nz == 1 (got %d)", d->nz);
+    if (d->order != 2 && d->order != 3)
+        return fail(LBM_EINVAL, "order must be 2 (LBfluid2D) or 3 (scmp3D) (got %d)", d->order);
+    if (!(d->tau > 0.0) || !std::isfinite(d->tau)) return fail(LBM_EINVAL, "tau must be > 0");
+    if (!std::isfinite(d->G)) return fail(LBM_EINVAL, "G must be finite");
+    if (d->walls) {
+        const int last = d->q == 19 ? d->nz : d->ny;
+        if (last < 3) return fail(LBM_EINVAL, "walls need >= 3 nodes on the wall axis (got %d)", last);
+    }
+    if (d->precision < LBM_FP64 || d->precision > LBM_BF16S)
+        return fail(LBM_EINVAL, "unknown precision %d", d->precision);
+    if (d->kernel < LBM_KERNEL_AUTO || d->kernel > LBM_KERNEL_FUSED)
+        return fail(LBM_EINVAL, "unknown kernel %d", d->kernel);
+    const int64_t n = static_cast<int64_t>(d->nx) * d->ny * d->nz;
+    if (n >= (int64_t(1) << 32)) return fail(LBM_EINVAL, "lattice too large (%lld nodes)", (long long)n);
+    return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_abi_version(void) { return LBM_ABI_VERSION; }
+
+const char* lbm_last_error(void) { return g_err.c_str(); }
+
+int lbm_create(const lbm_desc* desc, lbm_handle** out)
+{
+    if (!out) return fail(LBM_EINVAL, "lbm_create: out is NULL");
+    *out = nullptr;
+    int rc = check_desc(desc);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(LBM_ENODEV, "lbm_create: no HIP device");
+    auto* h = new lbm_handle();
+    h->d = *desc;
+    h->elem = (desc->precision == LBM_FP64) ? 8 : (desc->precision == LBM_FP32 ? 4 : 2);
+    h->g.nx = desc->nx; h->g.ny = desc->ny; h->g.nz = desc->nz;
+    h->g.n = static_cast<int64_t>(desc->nx) * desc->ny * desc->nz;
+    h->g.ps = plane_stride(h->g.n, h->elem);
+    h->a.q = desc->q; h->a.walls = desc->walls ? 1 : 0;
+    h->a.tau = desc->tau; h->a.G = desc->G;
+    if (desc->precision == LBM_FP64) {
+        const bool fused_ok = lbm::fp64_fused_supported(h->a, h->g);
+        if (desc->kernel == LBM_KERNEL_FUSED && !fused_ok) {
+            delete h;
+            return fail(LBM_EINVAL, "fused FP64 kernel does not support this lattice");
+        }
+        h->kernel = desc->kernel == LBM_KERNEL_AUTO ? (fused_ok ? LBM_KERNEL_FUSED : LBM_KERNEL_TWO_PASS)
+                                                    : desc->kernel;
+    } else {
+        h->kernel = lbm::lowp_resolve_kernel(desc->precision, h->a, h->g, desc->kernel);
+        if (h->kernel < 0) {
+            delete h;
+            return fail(LBM_EINVAL, "requested kernel does not support this lattice/precision");
+        }
+    }
+    const int64_t pop_bytes = h->g.ps * desc->q * h->elem;
+    auto cleanup = [&](int code) {
+        lbm_destroy(h);
+        return code;
+    };
+    for (int b = 0; b < 2; b++) {
+        hipError_t e = hipMalloc(&h->buf[b], pop_bytes);
+        if (e != hipSuccess)
+            return cleanup(fail(LBM_ENOMEM, "hipMalloc(%lld B) for populations: %s",
+                                (long long)pop_bytes, hipGetErrorString(e)));
+    }
+    h->bytes = 2 * pop_bytes;
+    if (h->kernel == LBM_KERNEL_TWO_PASS || desc->precision != LBM_FP64) {
+        if (hipMalloc(&h->psi, h->g.n * 8) != hipSuccess)
+            return cleanup(fail(LBM_ENOMEM, "hipMalloc for psi failed"));
+        h->bytes += h->g.n * 8;
+    }
+    if (hipMalloc(&h->rho0, h->g.n * 8) != hipSuccess)
+        return cleanup(fail(LBM_ENOMEM, "hipMalloc for rho0 failed"));
+    h->bytes += h->g.n * 8;
+    if (hipStreamCreateWithFlags(&h->own, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(LBM_EHIP, "hipStreamCreate failed"));
+    h->stream = h->own;
+    if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
+        return cleanup(fail(LBM_EHIP, "hipEventCreate failed"));
+    *out = h;
+    return LBM_OK;
+}
+
+void lbm_destroy(lbm_handle* h)
+{
+    if (!h) return;
+    if (h->stream) hipStreamSynchronize(h->stream);
+    for (void* b : h->buf)
+        if (b) hipFree(b);
+    if (h->psi) hipFree(h->psi);
+    if (h->rho0) hipFree(h->rho0);
+    if (h->tmp) hipFree(h->tmp);
+    if (h->ev0) hipEventDestroy(h->ev0);
+    if (h->ev1) hipEventDestroy(h->ev1);
+    if (h->own) hipStreamDestroy(h->own);
+    delete h;
+}
+
+int lbm_set_rho(lbm_handle* h, const double* rho0)
+{
+    if (!h || !rho0) return fail(LBM_EINVAL, "lbm_set_rho: NULL argument");
+    HIP_OK(hipMemcpyAsync(h->rho0, rho0, h->g.n * 8, hipMemcpyHostToDevice, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    h->initialised = false;
+    return LBM_OK;
+}
+
+int lbm_init_equilibrium(lbm_handle* h)
+{
+    if (!h) return fail(LBM_EINVAL, "lbm_init_equilibrium: NULL handle");
+    if (!is_lowp(h)) {
+        double* psi = h->psi;
+        int rc = LBM_OK;
+        if (!psi) {
+            rc = ensure_tmp(h, h->g.n * 8);
+            if (rc) return rc;
+            psi = h->tmp;
+        }
+        HIP_OK(lbm::fp64_psi_from_rho(h->rho0, psi, h->g.n, h->stream));
+        HIP_OK(lbm::fp64_collide(h->a, h->g, nullptr, lbm::kHoldsF, static_cast<double*>(h->buf[0]),
+                                 psi, h->rho0, 2, h->stream));
+    } else {
+        HIP_OK(lbm::lowp_init(h->d.precision, h->a, h->g, h->d.rho_shift, h->rho0, h->buf[0],
+                              reinterpret_cast<float*>(h->psi), h->stream));
+    }
+    HIP_OK(hipStreamSynchronize(h->stream));
+    h->cur = 0;
+    h->kind[0] = lbm::kHoldsF;
+    h->steps = 0;
+    h->initialised = true;
+    h->first_step_stored = (h->d.order == 2);
+    h->prev_valid = false;
+    h->ref_is_rho0 = true;
+    return LBM_OK;
+}
+
+int lbm_step(lbm_handle* h, int nsteps)
+{
+    if (!h) return fail(LBM_EINVAL, "lbm_step: NULL handle");
+    if (!h->initialised) return fail(LBM_ESTATE, "lbm_step before lbm_init_equilibrium / lbm_set_f");
+    if (nsteps < 0) return fail(LBM_EINVAL, "lbm_step: nsteps < 0");
+    for (int s = 0; s < nsteps; s++) {
+        int rc = run_step(h);
+        if (rc) return rc;
+    }
+    return LBM_OK;
+}
+
+int lbm_step_timed(lbm_handle* h, int nsteps, double* ms)
+{
+    if (!h || !ms) return fail(LBM_EINVAL, "lbm_step_timed: NULL argument");
+    HIP_OK(hipEventRecord(h->ev0, h->stream));
+    int rc = lbm_step(h, nsteps);
+    if (rc) return rc;
+    HIP_OK(hipEventRecord(h->ev1, h->stream));
+    HIP_OK(hipEventSynchronize(h->ev1));
+    float t = 0.f;
+    HIP_OK(hipEventElapsedTime(&t, h->ev0, h->ev1));
+    *ms = t;
+    return LBM_OK;
+}
+
+int lbm_get_fields(lbm_handle* h, int kind, double* rho, double* u, double* v, double* w)
+{
+    if (!h) return fail(LBM_EINVAL, "lbm_get_fields: NULL handle");
+    if (!h->initialised) return fail(LBM_ESTATE, "lbm_get_fields before initialisation");
+    const int64_t n = h->g.n;
+    if (kind == LBM_FIELDS_CURRENT) return moments_to_host(h, h->cur, rho, u, v, w);
+    if (kind != LBM_FIELDS_REFERENCE) return fail(LBM_EINVAL, "unknown fields kind %d", kind);
+    if (h->steps == 0 && h->ref_is_rho0) {
+        // r = rho0, u = v = w = 0 (scmp3D.cpp:63-76 before any step)
+        if (rho) HIP_OK(hipMemcpyAsync(rho, h->rho0, n * 8, hipMemcpyDeviceToHost, h->stream));
+        HIP_OK(hipStreamSynchronize(h->stream));
+        for (double* p : {u, v, w})
+            if (p) std::memset(p, 0, n * 8);
+        return LBM_OK;
+    }
+    if (h->d.order == 2 || h->steps == 0) return moments_to_host(h, h->cur, rho, u, v, w);
+    if (h->steps == 1 && h->d.order == 3 && !h->prev_valid)
+        return fail(LBM_ESTATE, "previous populations unavailable");
+    // order 3: the arrays hold the moments computed at the start of step N
+    return moments_to_host(h, 1 - h->cur, rho, u, v, w);
+}
+
+int lbm_get_f(lbm_handle* h, double* f)
+{
+    if (!h || !f) return fail(LBM_EINVAL, "lbm_get_f: NULL argument");
+    if (!h->initialised) return fail(LBM_ESTATE, "lbm_get_f before initialisation");
+    const int64_t n = h->g.n * h->d.q;
+    int rc = ensure_tmp(h, n * 8);
+    if (rc) return rc;
+    if (!is_lowp(h))
+        HIP_OK(lbm::fp64_get_f(h->a, h->g, static_cast<const double*>(h->buf[h->cur]),
+                               h->kind[h->cur], h->tmp, h->stream));
+    else
+        HIP_OK(lbm::lowp_get_f(h->d.precision, h->a, h->g, h->d.rho_shift, h->buf[h->cur],
+                               h->kind[h->cur], h->tmp, h->stream));
+    HIP_OK(hipMemcpyAsync(f, h->tmp, n * 8, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    return LBM_OK;
+}
+
+int lbm_set_f(lbm_handle* h, const double* f)
+{
+    if (!h || !f) return fail(LBM_EINVAL, "lbm_set_f: NULL argument");
+    const int64_t n = h->g.n * h->d.q;
+    int rc = ensure_tmp(h, n * 8);
+    if (rc) return rc;
+    HIP_OK(hipMemcpyAsync(h->tmp, f, n * 8, hipMemcpyHostToDevice, h->stream));
+    if (!is_lowp(h))
+        HIP_OK(lbm::fp64_set_f(h->a, h->g, h->tmp, static_cast<double*>(h->buf[0]), h->stream));
+    else
+        HIP_OK(lbm::lowp_set_f(h->d.precision, h->a, h->g, h->d.rho_shift, h->tmp, h->buf[0],
+                               h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    h->cur = 0;
+    h->kind[0] = lbm::kHoldsF;
+    h->steps = 0;
+    h->initialised = true;
+    h->first_step_stored = false;
+    h->prev_valid = false;
+    h->ref_is_rho0 = false;
+    return LBM_OK;
+}
+
+int lbm_conservation(lbm_handle* h, double out[4])
+{
+    if (!h || !out) return fail(LBM_EINVAL, "lbm_conservation: NULL argument");
+    if (!h->initialised) return fail(LBM_ESTATE, "lbm_conservation before initialisation");
+    int nb = 0;
+    const int maxb = lbm::fp64_conservation_blocks(h->g);
+    int rc = ensure_tmp(h, int64_t(maxb) * 4 * 8);
+    if (rc) return rc;
+    if (!is_lowp(h))
+        HIP_OK(lbm::fp64_conservation(h->a, h->g, static_cast<const double*>(h->buf[h->cur]),
+                                      h->kind[h->cur], h->tmp, &nb, h->stream));
+    else
+        HIP_OK(lbm::lowp_conservation(h->d.precision, h->a, h->g, h->d.rho_shift,
+                                      h->buf[h->cur], h->kind[h->cur], h->tmp, &nb, h->stream));
+    std::vector<double> part(static_cast<size_t>(nb) * 4);
+    HIP_OK(hipMemcpyAsync(part.data(), h->tmp, part.size() * 8, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    for (int k = 0; k < 4; k++) out[k] = 0.0;
+    for (int b = 0; b < nb; b++)
+        for (int k = 0; k < 4; k++) out[k] += part[static_cast<size_t>(b) * 4 + k];
+    return LBM_OK;
+}
+
+int64_t lbm_steps_done(const lbm_handle* h) { return h ? h->steps : -1; }
+
+int lbm_set_stream(lbm_handle* h, void* stream)
+{
+    if (!h) return fail(LBM_EINVAL, "lbm_set_stream: NULL handle");
+    HIP_OK(hipStreamSynchronize(h->stream));
+    h->stream = stream ? static_cast<hipStream_t>(stream) : h->own;
+    return LBM_OK;
+}
+
+void* lbm_get_stream(const lbm_handle* h) { return h ? static_cast<void*>(h->stream) : nullptr; }
+
+int lbm_synchronize(lbm_handle* h)
+{
+    if (!h) return fail(LBM_EINVAL, "lbm_synchronize: NULL handle");
+    HIP_OK(hipStreamSynchronize(h->stream));
+    return LBM_OK;
+}
+
+int64_t lbm_device_bytes(const lbm_handle* h) { return h ? h->bytes : 0; }
+
+int lbm_kernel_in_use(const lbm_handle* h) { return h ? h->kernel : -1; }
+
+int lbm_fill_noise(uint32_t seed, int64_t n, double mean, double amp, double* out)
+{
+    if (!out || n < 0) return fail(LBM_EINVAL, "lbm_fill_noise: bad argument");
+    std::mt19937 gen(seed);
+    for (int64_t c = 0; c < n; c++) {
+        const double x = static_cast<double>(gen()) / 4294967295.0;
+        out[c] = mean + amp * (x - 0.5);
+    }
+    return LBM_OK;
+}
+
+int lbm_fill_droplet(int nx, int ny, int nz, double R, double W, double rl, double rv, double* out)
+{
+    if (!out || nx < 1 || ny < 1 || nz < 1 || !(W > 0.0))
+        return fail(LBM_EINVAL, "lbm_fill_droplet: bad argument");
+    const double cx = 0.5 * (nx - 1), cy = 0.5 * (ny - 1), cz = 0.5 * (nz - 1);
+    int64_t c = 0;
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++)
+            for (int k = 0; k < nz; k++) {
+                const double dx = i - cx, dy = j - cy, dz = k - cz;
+                const double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+                out[c++] = 0.5 * (rl + rv) - 0.5 * (rl - rv) * std::tanh(2.0 * (r - R) / W);
+            }
+    return LBM_OK;
+}
+
+}  // extern "C"

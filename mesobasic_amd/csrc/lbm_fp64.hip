@@ -1,0 +1,411 @@
+// lbm_fp64.hip -- FP64 Shan-Chen step for gfx950, bit-faithful to the
+// reference CPU code (flowzario/mesoBasic, src/lattice_boltzmann).
+//
+// Built with -ffp-contract=off: the reference is compiled for baseline
+// x86-64 (no FMA), so every product and sum rounds on its own.  The
+// operation order of each formula is the reference's left-to-right order:
+//   moments   LBfluid3D.cpp:223-233   rho = sum f; u = (sum f*ex)/rho
+//   psi       scmp3D.cpp:189          1.0 - exp(-rho)   (exp: lbm_exp.h)
+//   force     scmp3D.cpp:158-173      F = ((-G)*psi(x)) * sum (w*e)*psi(x+e)
+//   feq       LBfluid3D.cpp:263-271   ueq = u + (tau*F)/rho,
+//                                     feq = (((1+3ev)+(4.5ev)ev)-1.5usq)*(rho*w)
+//   collide   LBfluid3D.cpp:272       f - (f - feq)/tau  (IEEE divide)
+//   stream    LBfluid3D.cpp:288-302   f(x) = f*(x - e)   (pull)
+//   walls     LBfluid3D.cpp:316-347, LBfluid2D.cpp:363-388
+//
+// Storage: populations are structure-of-arrays, and the buffer written by a
+// step holds post-collision values f*(x).  The next step pulls
+// f(x) = f*(x - e) with periodic wrap by index arithmetic, which replaces the
+// reference's ghost shell and its MPI halo exchange (LBfluid3D.cpp:430-543).
+// At a wall row the reference overwrites the streamed value with the same
+// node's post-collision value of the opposite direction; the pull does the
+// same substitution when it reads.
+#pragma clang fp contract(off)
+
+#include "lbm_exp.h"
+#include "lbm_internal.h"
+#include "lbm_stencil.h"
+
+namespace lbm {
+namespace {
+
+constexpr int kBlock = 256;
+
+struct Coord {
+    int x, y, z;
+};
+
+__device__ __forceinline__ Coord decompose(uint32_t c, const Grid& g)
+{
+    Coord r;
+    const uint32_t nz = static_cast<uint32_t>(g.nz), ny = static_cast<uint32_t>(g.ny);
+    const uint32_t t = c / nz;
+    r.z = static_cast<int>(c - t * nz);
+    r.x = static_cast<int>(t / ny);
+    r.y = static_cast<int>(t - static_cast<uint32_t>(r.x) * ny);
+    return r;
+}
+
+// Periodic neighbour coordinates along each axis: [0] = -1, [1] = 0, [2] = +1.
+struct Nbr {
+    int64_t xy[3][3];  // (x + dx)*ny + (y + dy), premultiplied by nz
+    int z[3];
+};
+
+__device__ __forceinline__ Nbr neighbours(const Coord& p, const Grid& g)
+{
+    Nbr nb;
+    const int xs[3] = {p.x == 0 ? g.nx - 1 : p.x - 1, p.x, p.x == g.nx - 1 ? 0 : p.x + 1};
+    const int ys[3] = {p.y == 0 ? g.ny - 1 : p.y - 1, p.y, p.y == g.ny - 1 ? 0 : p.y + 1};
+    nb.z[0] = p.z == 0 ? g.nz - 1 : p.z - 1;
+    nb.z[1] = p.z;
+    nb.z[2] = p.z == g.nz - 1 ? 0 : p.z + 1;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+            nb.xy[a][b] = (static_cast<int64_t>(xs[a]) * g.ny + ys[b]) * g.nz;
+    return nb;
+}
+
+// Node at offset (dx, dy, dz) in {-1,0,1}^3.
+__device__ __forceinline__ int64_t at(const Nbr& nb, int dx, int dy, int dz)
+{
+    return nb.xy[dx + 1][dy + 1] + nb.z[dz + 1];
+}
+
+// f at node c: read in place, or pulled from the upstream neighbours with
+// the wall substitution.
+template <int Q, bool WALLS>
+__device__ __forceinline__ void load_f(const double* __restrict__ P, const Grid& g, int kind,
+                                       const Coord& p, const Nbr& nb, int64_t c, double (&f)[Q])
+{
+    using S = St<Q>;
+    if (kind == kHoldsF) {
+#pragma unroll
+        for (int n = 0; n < Q; n++) f[n] = P[n * g.ps + c];
+        return;
+    }
+#pragma unroll
+    for (int n = 0; n < Q; n++) f[n] = P[n * g.ps + at(nb, -S::ex[n], -S::ey[n], -S::ez[n])];
+    if constexpr (WALLS) {
+        const int last = (S::D == 3) ? p.z : p.y;
+        const int nl = (S::D == 3) ? g.nz : g.ny;
+        if (last == 1) {
+#pragma unroll
+            for (int m = 0; m < S::nwall; m++) f[S::wall_up[m]] = P[S::wall_dn[m] * g.ps + c];
+        }
+        if (last == nl - 2) {
+#pragma unroll
+            for (int m = 0; m < S::nwall; m++) f[S::wall_dn[m]] = P[S::wall_up[m] * g.ps + c];
+        }
+    }
+}
+
+template <int Q>
+__device__ __forceinline__ void moments(const double (&f)[Q], double& rho, double& sx, double& sy,
+                                        double& sz)
+{
+    using S = St<Q>;
+    rho = 0.0; sx = 0.0; sy = 0.0; sz = 0.0;
+#pragma unroll
+    for (int n = 0; n < Q; n++) {
+        rho = rho + f[n];
+        if (S::ex[n] == 1) sx = sx + f[n];
+        if (S::ex[n] == -1) sx = sx - f[n];
+        if (S::ey[n] == 1) sy = sy + f[n];
+        if (S::ey[n] == -1) sy = sy - f[n];
+        if (S::ez[n] == 1) sz = sz + f[n];
+        if (S::ez[n] == -1) sz = sz - f[n];
+    }
+}
+
+__device__ __forceinline__ double psi_of(double rho) { return 1.0 - exp_libm(-rho); }
+
+__global__ void __launch_bounds__(kBlock) k_psi_rho(const double* __restrict__ rho,
+                                                    double* __restrict__ psi, int64_t n)
+{
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (c < n) psi[c] = psi_of(rho[c]);
+}
+
+template <int Q, bool WALLS>
+__global__ void __launch_bounds__(kBlock) k_psi_pops(const double* __restrict__ P, int kind,
+                                                     double* __restrict__ psi, Grid g)
+{
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (c >= g.n) return;
+    const Coord p = decompose(static_cast<uint32_t>(c), g);
+    const Nbr nb = neighbours(p, g);
+    double f[Q];
+    load_f<Q, WALLS>(P, g, kind, p, nb, c, f);
+    double rho = 0.0;
+#pragma unroll
+    for (int n = 0; n < Q; n++) rho = rho + f[n];
+    psi[c] = psi_of(rho);
+}
+
+// Force + collision (+ the pull stream that produced f).  MODE 0: moments of
+// f; MODE 1: rho = rho0, u = 0 (LBfluid2D's first step uses the stored
+// initial arrays, LBfluid2D.cpp:238-240 after setFtoFeq); MODE 2: initial
+// equilibrium (LBfluid3D::setFtoFeq, LBfluid3D.cpp:180-200).
+template <int Q, bool WALLS, int MODE>
+__global__ void __launch_bounds__(kBlock)
+    k_collide(const double* __restrict__ Pin, int kind, double* __restrict__ Pout,
+              const double* __restrict__ psi, const double* __restrict__ rho0, Grid g,
+              double tau, double G)
+{
+    using S = St<Q>;
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (c >= g.n) return;
+    const Coord p = decompose(static_cast<uint32_t>(c), g);
+    const Nbr nb = neighbours(p, g);
+
+    double f[Q];
+    double rho, u, v, w;
+    if constexpr (MODE == 0) {
+        load_f<Q, WALLS>(Pin, g, kind, p, nb, c, f);
+        double sx, sy, sz;
+        moments<Q>(f, rho, sx, sy, sz);
+        u = sx / rho;
+        v = sy / rho;
+        w = sz / rho;
+    } else {
+        if constexpr (MODE == 1) load_f<Q, WALLS>(Pin, g, kind, p, nb, c, f);
+        rho = rho0[c];
+        u = 0.0; v = 0.0; w = 0.0;
+    }
+
+    // Shan-Chen force, scmp3D.cpp:141-179
+    const double pc = psi[c];
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+#pragma unroll
+    for (int n = 1; n < Q; n++) {
+        const double t = S::w[n] * psi[at(nb, S::ex[n], S::ey[n], S::ez[n])];
+        if (S::ex[n] == 1) gx = gx + t;
+        if (S::ex[n] == -1) gx = gx - t;
+        if (S::ey[n] == 1) gy = gy + t;
+        if (S::ey[n] == -1) gy = gy - t;
+        if (S::ez[n] == 1) gz = gz + t;
+        if (S::ez[n] == -1) gz = gz - t;
+    }
+    const double mg = -G;
+    const double Fx = (mg * pc) * gx;
+    const double Fy = (mg * pc) * gy;
+    const double Fz = (mg * pc) * gz;
+
+    const double ueq = u + (tau * Fx) / rho;
+    const double veq = v + (tau * Fy) / rho;
+    double weq, usq;
+    if constexpr (S::D == 3) {
+        weq = w + (tau * Fz) / rho;
+        usq = (ueq * ueq + veq * veq) + weq * weq;
+    } else {
+        weq = 0.0;
+        usq = ueq * ueq + veq * veq;
+    }
+    const double usq15 = 1.5 * usq;
+    static_for<Q>([&](auto I) {
+        constexpr int n = decltype(I)::value;
+        const double ev = edot<S::ex[n], S::ey[n], S::ez[n]>(ueq, veq, weq);
+        const double feq = (((1.0 + 3.0 * ev) + (4.5 * ev) * ev) - usq15) * (rho * S::w[n]);
+        double out;
+        if constexpr (MODE == 2) out = feq;
+        else out = f[n] - (f[n] - feq) / tau;
+        Pout[n * g.ps + c] = out;
+    });
+}
+
+template <int Q, bool WALLS>
+__global__ void __launch_bounds__(kBlock)
+    k_moments(const double* __restrict__ P, int kind, Grid g, double* __restrict__ rho,
+              double* __restrict__ u, double* __restrict__ v, double* __restrict__ w)
+{
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (c >= g.n) return;
+    const Coord p = decompose(static_cast<uint32_t>(c), g);
+    const Nbr nb = neighbours(p, g);
+    double f[Q];
+    load_f<Q, WALLS>(P, g, kind, p, nb, c, f);
+    double r, sx, sy, sz;
+    moments<Q>(f, r, sx, sy, sz);
+    rho[c] = r;
+    u[c] = sx / r;
+    v[c] = sy / r;
+    w[c] = (St<Q>::D == 3) ? sz / r : 0.0;
+}
+
+template <int Q, bool WALLS>
+__global__ void __launch_bounds__(kBlock)
+    k_get_f(const double* __restrict__ P, int kind, Grid g, double* __restrict__ aos)
+{
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (c >= g.n) return;
+    const Coord p = decompose(static_cast<uint32_t>(c), g);
+    const Nbr nb = neighbours(p, g);
+    double f[Q];
+    load_f<Q, WALLS>(P, g, kind, p, nb, c, f);
+#pragma unroll
+    for (int n = 0; n < Q; n++) aos[c * Q + n] = f[n];
+}
+
+template <int Q>
+__global__ void __launch_bounds__(kBlock)
+    k_set_f(const double* __restrict__ aos, Grid g, double* __restrict__ P)
+{
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (c >= g.n) return;
+#pragma unroll
+    for (int n = 0; n < Q; n++) P[n * g.ps + c] = aos[c * Q + n];
+}
+
+// Fixed-shape block reduction: deterministic for a given lattice.
+constexpr int kConsBlock = 256;
+constexpr int kConsPerThread = 16;
+
+template <int Q, bool WALLS>
+__global__ void __launch_bounds__(kConsBlock)
+    k_conservation(const double* __restrict__ P, int kind, Grid g, double* __restrict__ part)
+{
+    using S = St<Q>;
+    __shared__ double red[4][kConsBlock];
+    double m = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kConsBlock * kConsPerThread;
+    for (int i = 0; i < kConsPerThread; i++) {
+        const int64_t c = base + static_cast<int64_t>(i) * kConsBlock + threadIdx.x;
+        if (c >= g.n) break;
+        const Coord p = decompose(static_cast<uint32_t>(c), g);
+        const Nbr nb = neighbours(p, g);
+        double f[Q];
+        load_f<Q, WALLS>(P, g, kind, p, nb, c, f);
+        double r, sx, sy, sz;
+        moments<Q>(f, r, sx, sy, sz);
+        m += r; jx += sx; jy += sy; jz += sz;
+        (void)S::D;
+    }
+    red[0][threadIdx.x] = m; red[1][threadIdx.x] = jx;
+    red[2][threadIdx.x] = jy; red[3][threadIdx.x] = jz;
+    __syncthreads();
+    for (int s = kConsBlock / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int k = 0; k < 4; k++) red[k][threadIdx.x] += red[k][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) part[blockIdx.x * 4 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+inline unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+// ---- launchers -------------------------------------------------------------
+
+hipError_t fp64_psi_from_rho(const double* rho, double* psi, int64_t n, hipStream_t s)
+{
+    k_psi_rho<<<blocks_for(n), kBlock, 0, s>>>(rho, psi, n);
+    return hipGetLastError();
+}
+
+#define LBM_DISPATCH_QW(a, CALL)                                   \
+    do {                                                          \
+        if ((a).q == 9) {                                         \
+            if ((a).walls) { constexpr int Q = 9; constexpr bool W = true; CALL; } \
+            else { constexpr int Q = 9; constexpr bool W = false; CALL; }          \
+        } else {                                                  \
+            if ((a).walls) { constexpr int Q = 19; constexpr bool W = true; CALL; } \
+            else { constexpr int Q = 19; constexpr bool W = false; CALL; }          \
+        }                                                         \
+    } while (0)
+
+hipError_t fp64_psi_from_pops(const StepArgs& a, const Grid& g, const double* P, int kind,
+                              double* psi, hipStream_t s)
+{
+    LBM_DISPATCH_QW(a, (k_psi_pops<Q, W><<<blocks_for(g.n), kBlock, 0, s>>>(P, kind, psi, g)));
+    return hipGetLastError();
+}
+
+hipError_t fp64_collide(const StepArgs& a, const Grid& g, const double* Pin, int kind,
+                        double* Pout, const double* psi, const double* rho0, int mode,
+                        hipStream_t s)
+{
+    const unsigned nb = blocks_for(g.n);
+    if (mode == 0)
+        LBM_DISPATCH_QW(a, (k_collide<Q, W, 0><<<nb, kBlock, 0, s>>>(Pin, kind, Pout, psi, rho0, g, a.tau, a.G)));
+    else if (mode == 1)
+        LBM_DISPATCH_QW(a, (k_collide<Q, W, 1><<<nb, kBlock, 0, s>>>(Pin, kind, Pout, psi, rho0, g, a.tau, a.G)));
+    else
+        LBM_DISPATCH_QW(a, (k_collide<Q, W, 2><<<nb, kBlock, 0, s>>>(Pin, kind, Pout, psi, rho0, g, a.tau, a.G)));
+    return hipGetLastError();
+}
+
+hipError_t fp64_moments(const StepArgs& a, const Grid& g, const double* P, int kind, double* rho,
+                        double* u, double* v, double* w, hipStream_t s)
+{
+    LBM_DISPATCH_QW(a, (k_moments<Q, W><<<blocks_for(g.n), kBlock, 0, s>>>(P, kind, g, rho, u, v, w)));
+    return hipGetLastError();
+}
+
+hipError_t fp64_get_f(const StepArgs& a, const Grid& g, const double* P, int kind, double* aos,
+                      hipStream_t s)
+{
+    LBM_DISPATCH_QW(a, (k_get_f<Q, W><<<blocks_for(g.n), kBlock, 0, s>>>(P, kind, g, aos)));
+    return hipGetLastError();
+}
+
+hipError_t fp64_set_f(const StepArgs& a, const Grid& g, const double* aos, double* P,
+                      hipStream_t s)
+{
+    if (a.q == 9) k_set_f<9><<<blocks_for(g.n), kBlock, 0, s>>>(aos, g, P);
+    else k_set_f<19><<<blocks_for(g.n), kBlock, 0, s>>>(aos, g, P);
+    return hipGetLastError();
+}
+
+int fp64_conservation_blocks(const Grid& g)
+{
+    const int64_t per = static_cast<int64_t>(kConsBlock) * kConsPerThread;
+    return static_cast<int>((g.n + per - 1) / per);
+}
+
+hipError_t fp64_conservation(const StepArgs& a, const Grid& g, const double* P, int kind,
+                             double* partials, int* nblocks, hipStream_t s)
+{
+    const int nb = fp64_conservation_blocks(g);
+    *nblocks = nb;
+    LBM_DISPATCH_QW(a, (k_conservation<Q, W><<<nb, kConsBlock, 0, s>>>(P, kind, g, partials)));
+    return hipGetLastError();
+}
+
+bool fp64_fused_supported(const StepArgs&, const Grid&) { return false; }
+
+hipError_t fp64_fused_step(const StepArgs&, const Grid&, const double*, int, double*, hipStream_t)
+{
+    return hipErrorNotSupported;
+}
+
+}  // namespace lbm
+
+// ---- test hook (include/mesobasic_lbm_testing.h) ---------------------------
+
+namespace {
+__global__ void k_probe_exp(const double* __restrict__ x, double* __restrict__ y, int64_t n)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (i < n) y[i] = lbm::exp_libm(x[i]);
+}
+}  // namespace
+
+extern "C" int lbm_probe_exp(const double* x, double* y, int64_t n)
+{
+    if (n <= 0) return 0;
+    double* d = nullptr;
+    if (hipMalloc(&d, 2 * n * sizeof(double)) != hipSuccess) return -2;
+    int rc = 0;
+    if (hipMemcpy(d, x, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) rc = -3;
+    if (!rc) {
+        k_probe_exp<<<static_cast<unsigned>((n + 255) / 256), 256>>>(d, d + n, n);
+        if (hipGetLastError() != hipSuccess) rc = -3;
+    }
+    if (!rc && hipMemcpy(y, d + n, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) rc = -3;
+    (void)hipFree(d);
+    return rc;
+}

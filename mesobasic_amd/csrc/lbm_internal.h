@@ -1,0 +1,59 @@
+// lbm_internal.h -- types shared by the C-ABI layer and the kernel
+// translation units.  Not part of the public interface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lbm {
+
+// Lattice geometry on the device.  Node index c = (x*ny + y)*nz + z
+// (nz = 1 in 2-D), the interior order of the reference arrays without the
+// ghost shell; population n of node c lives at n*ps + c (structure of
+// arrays, ps >= n rounded up and offset so that the 19 planes do not share
+// HBM channel alignment).
+struct Grid {
+    int nx, ny, nz;
+    int64_t n;
+    int64_t ps;
+};
+
+// What a population buffer holds.
+enum BufKind : int {
+    kHoldsF = 0,   // post-streaming f (read in place)
+    kHoldsPost = 1 // post-collision values (f is gathered from neighbours)
+};
+
+struct StepArgs {
+    int q;
+    int walls;
+    double tau, G;
+};
+
+// FP64 path (lbm_fp64.hip).  All launches go to `s`.
+hipError_t fp64_psi_from_rho(const double* rho, double* psi, int64_t n, hipStream_t s);
+hipError_t fp64_psi_from_pops(const StepArgs& a, const Grid& g, const double* P, int kind,
+                              double* psi, hipStream_t s);
+// mode 0: moments of the gathered f; 1: rho = rho0, u = 0 (the LBfluid2D
+// first step); 2: initial equilibrium (no f read; writes feq).
+hipError_t fp64_collide(const StepArgs& a, const Grid& g, const double* Pin, int kind,
+                        double* Pout, const double* psi, const double* rho0, int mode,
+                        hipStream_t s);
+// One fused pass (psi halo recomputed on chip).  Returns hipErrorNotSupported
+// when the geometry does not suit it.
+hipError_t fp64_fused_step(const StepArgs& a, const Grid& g, const double* Pin, int kind,
+                           double* Pout, hipStream_t s);
+bool fp64_fused_supported(const StepArgs& a, const Grid& g);
+hipError_t fp64_moments(const StepArgs& a, const Grid& g, const double* P, int kind, double* rho,
+                        double* u, double* v, double* w, hipStream_t s);
+hipError_t fp64_get_f(const StepArgs& a, const Grid& g, const double* P, int kind, double* aos,
+                      hipStream_t s);
+hipError_t fp64_set_f(const StepArgs& a, const Grid& g, const double* aos, double* P,
+                      hipStream_t s);
+// Per-block partial sums (mass, jx, jy, jz) of the current f; returns the
+// number of blocks written (4 doubles each) through nblocks.
+hipError_t fp64_conservation(const StepArgs& a, const Grid& g, const double* P, int kind,
+                             double* partials, int* nblocks, hipStream_t s);
+int fp64_conservation_blocks(const Grid& g);
+
+}  // namespace lbm

@@ -1,0 +1,213 @@
+"""ctypes binding of libmesobasic_lbm.so (include/mesobasic_lbm.h).
+
+Python is plumbing here: tests and the bench drive the C ABI through this
+module; all lattice arithmetic runs in the HIP kernels.  Loading fails loudly
+when the library has not been built -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmesobasic_lbm.so")
+
+FP64, FP32, FP16S, BF16S = 0, 1, 2, 3
+KERNEL_AUTO, KERNEL_TWO_PASS, KERNEL_FUSED = 0, 1, 2
+FIELDS_REFERENCE, FIELDS_CURRENT = 0, 1
+PRECISIONS = {"fp64": FP64, "fp32": FP32, "fp16": FP16S, "bf16": BF16S}
+
+_D = ctypes.POINTER(ctypes.c_double)
+
+
+class LbmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"lbm error {code}: {msg}")
+        self.code = code
+
+
+class Desc(ctypes.Structure):
+    _fields_ = [("q", ctypes.c_int), ("nx", ctypes.c_int), ("ny", ctypes.c_int),
+                ("nz", ctypes.c_int), ("tau", ctypes.c_double), ("G", ctypes.c_double),
+                ("order", ctypes.c_int), ("walls", ctypes.c_int), ("precision", ctypes.c_int),
+                ("rho_shift", ctypes.c_double), ("kernel", ctypes.c_int)]
+
+
+_lib = None
+
+
+def load():
+    """Loads the built library (raises if it is missing)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built: run `python -m mesobasic_amd.build`")
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    sig = {
+        "lbm_create": (ctypes.c_int, [ctypes.POINTER(Desc), ctypes.POINTER(P)]),
+        "lbm_destroy": (None, [P]),
+        "lbm_set_rho": (ctypes.c_int, [P, _D]),
+        "lbm_init_equilibrium": (ctypes.c_int, [P]),
+        "lbm_step": (ctypes.c_int, [P, ctypes.c_int]),
+        "lbm_step_timed": (ctypes.c_int, [P, ctypes.c_int, _D]),
+        "lbm_get_fields": (ctypes.c_int, [P, ctypes.c_int, _D, _D, _D, _D]),
+        "lbm_get_f": (ctypes.c_int, [P, _D]),
+        "lbm_set_f": (ctypes.c_int, [P, _D]),
+        "lbm_conservation": (ctypes.c_int, [P, _D]),
+        "lbm_steps_done": (ctypes.c_int64, [P]),
+        "lbm_set_stream": (ctypes.c_int, [P, P]),
+        "lbm_get_stream": (P, [P]),
+        "lbm_synchronize": (ctypes.c_int, [P]),
+        "lbm_device_bytes": (ctypes.c_int64, [P]),
+        "lbm_kernel_in_use": (ctypes.c_int, [P]),
+        "lbm_last_error": (ctypes.c_char_p, []),
+        "lbm_fill_noise": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_int64, ctypes.c_double,
+                                          ctypes.c_double, _D]),
+        "lbm_fill_droplet": (ctypes.c_int, [ctypes.c_int] * 3 + [ctypes.c_double] * 4 + [_D]),
+        "lbm_abi_version": (ctypes.c_int, []),
+        "lbm_probe_exp": (ctypes.c_int, [_D, _D, ctypes.c_int64]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise LbmError(rc, _lib.lbm_last_error().decode())
+
+
+def _dptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_D)
+
+
+def noise_rho(shape, seed, mean=0.7, amp=0.02):
+    """Seeded noisy density (std::mt19937), reference loop order."""
+    lib = load()
+    n = int(np.prod(shape))
+    out = np.empty(n)
+    _check(lib.lbm_fill_noise(seed & 0xFFFFFFFF, n, float(mean), float(amp), _dptr(out)))
+    return out.reshape(tuple(shape))
+
+
+def droplet_rho(shape, R, W=2.0, rl=2.0, rv=0.15):
+    lib = load()
+    sh = tuple(int(s) for s in shape)
+    nz = sh[2] if len(sh) == 3 else 1
+    out = np.empty(int(np.prod(sh)))
+    _check(lib.lbm_fill_droplet(sh[0], sh[1], nz, float(R), float(W), float(rl), float(rv), _dptr(out)))
+    return out.reshape(sh)
+
+
+def probe_exp(x):
+    """The FP64 kernels' device exp() applied to x (test hook)."""
+    lib = load()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(x)
+    _check(lib.lbm_probe_exp(_dptr(x), _dptr(y), x.size))
+    return y
+
+
+class Lattice:
+    """One Shan-Chen lattice resident on the GPU (a C-ABI handle)."""
+
+    def __init__(self, q, shape, tau, G, order=None, walls=False, precision=FP64,
+                 rho_shift=0.7, kernel=KERNEL_AUTO):
+        lib = load()
+        shape = tuple(int(s) for s in shape)
+        if q == 9:
+            nx, ny, nz = shape[0], shape[1], 1
+            order = 2 if order is None else order
+        else:
+            nx, ny, nz = shape
+            order = 3 if order is None else order
+        if isinstance(precision, str):
+            precision = PRECISIONS[precision]
+        self.q, self.shape, self.order, self.precision = q, shape, order, precision
+        self.n = nx * ny * nz
+        d = Desc(q, nx, ny, nz, float(tau), float(G), int(order), int(bool(walls)),
+                 int(precision), float(rho_shift), int(kernel))
+        h = ctypes.c_void_p()
+        _check(lib.lbm_create(ctypes.byref(d), ctypes.byref(h)))
+        self._h = h
+        self._lib = lib
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.lbm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def kernel(self):
+        return self._lib.lbm_kernel_in_use(self._h)
+
+    @property
+    def device_bytes(self):
+        return self._lib.lbm_device_bytes(self._h)
+
+    @property
+    def steps_done(self):
+        return self._lib.lbm_steps_done(self._h)
+
+    @property
+    def stream(self):
+        return self._lib.lbm_get_stream(self._h)
+
+    def set_rho(self, rho0):
+        r = np.ascontiguousarray(rho0, dtype=np.float64).reshape(-1)
+        assert r.size == self.n
+        _check(self._lib.lbm_set_rho(self._h, _dptr(r)))
+
+    def init(self):
+        _check(self._lib.lbm_init_equilibrium(self._h))
+
+    def step(self, nsteps=1):
+        _check(self._lib.lbm_step(self._h, int(nsteps)))
+
+    def step_timed(self, nsteps):
+        ms = ctypes.c_double()
+        _check(self._lib.lbm_step_timed(self._h, int(nsteps), ctypes.byref(ms)))
+        return ms.value
+
+    def synchronize(self):
+        _check(self._lib.lbm_synchronize(self._h))
+
+    def fields(self, kind=FIELDS_REFERENCE):
+        rho, u, v, w = (np.empty(self.n) for _ in range(4))
+        _check(self._lib.lbm_get_fields(self._h, kind, _dptr(rho), _dptr(u), _dptr(v), _dptr(w)))
+        sh = self.shape
+        return dict(rho=rho.reshape(sh), u=u.reshape(sh), v=v.reshape(sh), w=w.reshape(sh))
+
+    def get_f(self):
+        f = np.empty(self.n * self.q)
+        _check(self._lib.lbm_get_f(self._h, _dptr(f)))
+        return f.reshape(self.shape + (self.q,))
+
+    def set_f(self, f):
+        a = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
+        assert a.size == self.n * self.q
+        _check(self._lib.lbm_set_f(self._h, _dptr(a)))
+
+    def conservation(self):
+        out = np.empty(4)
+        _check(self._lib.lbm_conservation(self._h, _dptr(out)))
+        return out

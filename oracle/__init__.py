@@ -1,0 +1,163 @@
+"""ctypes binding of the CPU oracle (oracle/scmp_oracle.c).
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, the smoke check and the CPU
+baseline leg of bench.py, never by the product package.  The oracle restates
+the reference SCMP step (flowzario/mesoBasic, src/lattice_boltzmann) in the
+reference's own AoS layout and floating-point order; see scmp_oracle.h for
+the file:line map.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libscmp_oracle.so")
+_lib = None
+
+_D = ctypes.POINTER(ctypes.c_double)
+
+
+def build() -> None:
+    """Compiles libscmp_oracle.so (gcc, no FMA contraction)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        build()
+    lib = ctypes.CDLL(_LIB_PATH)
+    lib.scmp_oracle_create.restype = ctypes.c_void_p
+    lib.scmp_oracle_create.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double] * 2 + [ctypes.c_int] * 3
+    lib.scmp_oracle_destroy.argtypes = [ctypes.c_void_p]
+    lib.scmp_oracle_set_rho.argtypes = [ctypes.c_void_p, _D]
+    lib.scmp_oracle_init.argtypes = [ctypes.c_void_p]
+    lib.scmp_oracle_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.scmp_oracle_get.argtypes = [ctypes.c_void_p] + [_D] * 5
+    lib.scmp_oracle_get_force.argtypes = [ctypes.c_void_p] + [_D] * 3
+    lib.scmp_oracle_moments.argtypes = [ctypes.c_void_p] + [_D] * 4
+    lib.scmp_oracle_set_f.argtypes = [ctypes.c_void_p, _D]
+    lib.scmp_oracle_set_exp.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    lib.scmp_oracle_nodes.restype = ctypes.c_int64
+    lib.scmp_oracle_nodes.argtypes = [ctypes.c_void_p]
+    lib.scmp_oracle_noise_rho.argtypes = [ctypes.c_uint32, ctypes.c_int64, ctypes.c_double,
+                                          ctypes.c_double, _D]
+    lib.scmp_oracle_mt19937.argtypes = [ctypes.c_uint32, ctypes.c_int64,
+                                        ctypes.POINTER(ctypes.c_uint32)]
+    lib.scmp_oracle_droplet_rho.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 4 + [_D]
+    _lib = lib
+    return lib
+
+
+def _ptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_D)
+
+
+class Oracle:
+    """One SCMP lattice on the CPU, reference semantics.
+
+    q: 9 or 19; shape (nx, ny) for q=9 or (nx, ny, nz) for q=19.
+    order: 3 = scmp3D step order, 2 = LBfluid2D step order.
+    walls: bounce-back walls on the last axis.
+    """
+
+    def __init__(self, q, shape, tau, G, order=None, walls=False, nthreads=1):
+        lib = _load()
+        shape = tuple(int(s) for s in shape)
+        if q == 9:
+            assert len(shape) == 2
+            nx, ny, nz = shape[0], shape[1], 1
+            order = 2 if order is None else order
+        else:
+            assert len(shape) == 3
+            nx, ny, nz = shape
+            order = 3 if order is None else order
+        self.q, self.shape, self.order = q, shape, order
+        self.n = nx * ny * nz
+        self._h = lib.scmp_oracle_create(q, nx, ny, nz, float(tau), float(G),
+                                         int(order), int(bool(walls)), int(nthreads))
+        if not self._h:
+            raise ValueError("oracle: bad lattice description")
+        self._lib = lib
+
+    def close(self):
+        if self._h:
+            self._lib.scmp_oracle_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def set_rho(self, rho0: np.ndarray):
+        r = np.ascontiguousarray(rho0, dtype=np.float64).reshape(-1)
+        assert r.size == self.n
+        self._lib.scmp_oracle_set_rho(self._h, _ptr(r))
+
+    def init(self):
+        self._lib.scmp_oracle_init(self._h)
+
+    def step(self, nsteps: int = 1):
+        self._lib.scmp_oracle_step(self._h, int(nsteps))
+
+    def fields(self):
+        """Reference r,u,v,w arrays and f ([node][q]), shaped to the lattice."""
+        rho, u, v, w = (np.empty(self.n) for _ in range(4))
+        f = np.empty(self.n * self.q)
+        self._lib.scmp_oracle_get(self._h, _ptr(rho), _ptr(u), _ptr(v), _ptr(w), _ptr(f))
+        sh = self.shape
+        return dict(rho=rho.reshape(sh), u=u.reshape(sh), v=v.reshape(sh),
+                    w=w.reshape(sh), f=f.reshape(sh + (self.q,)))
+
+    def force(self):
+        fx, fy, fz = (np.empty(self.n) for _ in range(3))
+        self._lib.scmp_oracle_get_force(self._h, _ptr(fx), _ptr(fy), _ptr(fz))
+        return fx.reshape(self.shape), fy.reshape(self.shape), fz.reshape(self.shape)
+
+    def moments(self):
+        rho, u, v, w = (np.empty(self.n) for _ in range(4))
+        self._lib.scmp_oracle_moments(self._h, _ptr(rho), _ptr(u), _ptr(v), _ptr(w))
+        sh = self.shape
+        return dict(rho=rho.reshape(sh), u=u.reshape(sh), v=v.reshape(sh), w=w.reshape(sh))
+
+    def set_f(self, f: np.ndarray):
+        a = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
+        assert a.size == self.n * self.q
+        self._lib.scmp_oracle_set_f(self._h, _ptr(a))
+
+    def set_exp(self, fnptr):
+        """fnptr: ctypes function pointer double(double), or None for libm."""
+        self._lib.scmp_oracle_set_exp(self._h, fnptr)
+
+
+def mt19937(seed: int, n: int) -> np.ndarray:
+    """First n outputs of std::mt19937(seed) (uint32)."""
+    out = np.empty(int(n), dtype=np.uint32)
+    _load().scmp_oracle_mt19937(seed & 0xFFFFFFFF, out.size,
+                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+    return out
+
+
+def noise_rho(shape, seed: int, mean: float = 0.7, amp: float = 0.02) -> np.ndarray:
+    """Seeded noisy density, reference loop order (x slowest)."""
+    n = int(np.prod(shape))
+    out = np.empty(n)
+    _load().scmp_oracle_noise_rho(seed & 0xFFFFFFFF, n, float(mean), float(amp), _ptr(out))
+    return out.reshape(tuple(shape))
+
+
+def droplet_rho(shape, R: float, W: float = 2.0, rl: float = 2.0, rv: float = 0.15) -> np.ndarray:
+    """tanh droplet centred at (N-1)/2 per axis."""
+    sh = tuple(int(s) for s in shape)
+    nx, ny = sh[0], sh[1]
+    nz = sh[2] if len(sh) == 3 else 1
+    out = np.empty(nx * ny * nz)
+    _load().scmp_oracle_droplet_rho(nx, ny, nz, float(R), float(W), float(rl), float(rv), _ptr(out))
+    return out.reshape(sh)

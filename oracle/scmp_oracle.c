@@ -1,0 +1,510 @@
+/*
+ * scmp_oracle.c -- CPU restatement of the reference SCMP lattice-Boltzmann
+ * step.  TEST INFRASTRUCTURE ONLY: the HIP product path never links or calls
+ * this file; tests, the smoke check and the bench's CPU-baseline leg do.
+ *
+ * Every loop below keeps the reference's array layout (AoS, one-node ghost
+ * shell, x slowest / last axis fastest), loop order and left-to-right
+ * floating-point evaluation order, so that results are bit-identical to the
+ * reference compiled with g++ -O3 on x86-64 (no FMA contraction; build this
+ * file with -ffp-contract=off and no -march flags).  File:line citations are
+ * to the reference tree (flowzario/mesoBasic).
+ */
+#include "scmp_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+struct scmp_oracle {
+    int q, dim, order, walls, nthreads;
+    int nx, ny, nz;                /* interior sizes (nz = 1 in 2-D)      */
+    int64_t gx, gy, gz;            /* sizes with the ghost shell          */
+    int64_t deli, delj, delk;      /* LBfluid3D.cpp:27-29 / LBfluid2D.cpp:52-53 */
+    int64_t gn;                    /* number of nodes with ghosts          */
+    double tau, G;
+    int exi[19], eyi[19], ezi[19];
+    double ex[19], ey[19], ez[19], wa[19];
+    double *r, *u, *v, *w, *fx, *fy, *fz, *f, *fs;
+    double (*expfn)(double);
+};
+
+/* Stencil::setStencil, Stencil.cpp:5-120.  The direction order fixes every
+ * summation order below and the bounce-back pairs. */
+static void set_stencil(scmp_oracle* h)
+{
+    static const int d2[9][2] = {{0, 0}, {1, 0}, {-1, 0}, {0, 1}, {0, -1},
+                                 {1, 1}, {-1, -1}, {-1, 1}, {1, -1}};
+    static const int d3[19][3] = {
+        {0, 0, 0},  {1, 0, 0},  {-1, 0, 0}, {0, 1, 0},  {0, -1, 0},
+        {0, 0, 1},  {0, 0, -1}, {1, 1, 0},  {-1, -1, 0}, {1, 0, 1},
+        {-1, 0, -1}, {0, 1, 1}, {0, -1, -1}, {1, -1, 0}, {-1, 1, 0},
+        {1, 0, -1}, {-1, 0, 1}, {0, 1, -1}, {0, -1, 1}};
+    if (h->q == 9) {
+        for (int n = 0; n < 9; n++) {
+            h->exi[n] = d2[n][0]; h->eyi[n] = d2[n][1]; h->ezi[n] = 0;
+        }
+        h->wa[0] = 4.0 / 9.0;                                   /* :32 */
+        for (int n = 1; n < 5; n++) h->wa[n] = 1.0 / 9.0;       /* :33-36 */
+        for (int n = 5; n < 9; n++) h->wa[n] = 1.0 / 36.0;      /* :37-40 */
+    } else {
+        for (int n = 0; n < 19; n++) {
+            h->exi[n] = d3[n][0]; h->eyi[n] = d3[n][1]; h->ezi[n] = d3[n][2];
+        }
+        h->wa[0] = 1.0 / 3.0;                                   /* :79 */
+        for (int n = 1; n < 7; n++) h->wa[n] = 1.0 / 18.0;      /* :80-85 */
+        for (int n = 7; n < 19; n++) h->wa[n] = 1.0 / 36.0;     /* :86-97 */
+    }
+    for (int n = 0; n < h->q; n++) {
+        h->ex[n] = (double)h->exi[n];
+        h->ey[n] = (double)h->eyi[n];
+        h->ez[n] = (double)h->ezi[n];
+    }
+}
+
+scmp_oracle* scmp_oracle_create(int q, int nx, int ny, int nz, double tau,
+                                double G, int order, int walls, int nthreads)
+{
+    if (!(q == 9 || q == 19) || nx < 1 || ny < 1 || nz < 1) return NULL;
+    if (q == 9 && nz != 1) return NULL;
+    scmp_oracle* h = (scmp_oracle*)calloc(1, sizeof(*h));
+    if (!h) return NULL;
+    h->q = q;
+    h->dim = (q == 9) ? 2 : 3;
+    h->order = order;
+    h->walls = walls;
+    h->nthreads = nthreads < 1 ? 1 : nthreads;
+    h->nx = nx; h->ny = ny; h->nz = nz;
+    h->tau = tau; h->G = G;
+    h->gx = nx + 2;
+    h->gy = ny + 2;
+    if (h->dim == 3) {
+        h->gz = nz + 2;
+        h->deli = h->gy * h->gz; h->delj = h->gz; h->delk = 1;
+    } else {
+        h->gz = 1;
+        h->deli = h->gy; h->delj = 1; h->delk = 0;
+    }
+    h->gn = h->gx * h->gy * h->gz;
+    set_stencil(h);
+    h->expfn = exp;
+    size_t sn = (size_t)h->gn * sizeof(double);
+    size_t sf = (size_t)h->gn * (size_t)q * sizeof(double);
+    h->r = (double*)calloc(1, sn);  h->u = (double*)calloc(1, sn);
+    h->v = (double*)calloc(1, sn);  h->w = (double*)calloc(1, sn);
+    h->fx = (double*)calloc(1, sn); h->fy = (double*)calloc(1, sn);
+    h->fz = (double*)calloc(1, sn);
+    h->f = (double*)calloc(1, sf);  h->fs = (double*)calloc(1, sf);
+    if (!h->r || !h->u || !h->v || !h->w || !h->fx || !h->fy || !h->fz ||
+        !h->f || !h->fs) {
+        scmp_oracle_destroy(h);
+        return NULL;
+    }
+    return h;
+}
+
+void scmp_oracle_destroy(scmp_oracle* h)
+{
+    if (!h) return;
+    free(h->r); free(h->u); free(h->v); free(h->w);
+    free(h->fx); free(h->fy); free(h->fz); free(h->f); free(h->fs);
+    free(h);
+}
+
+int64_t scmp_oracle_nodes(const scmp_oracle* h)
+{
+    return (int64_t)h->nx * h->ny * h->nz;
+}
+
+void scmp_oracle_set_exp(scmp_oracle* h, double (*fn)(double))
+{
+    h->expfn = fn ? fn : exp;
+}
+
+static inline int64_t nd(const scmp_oracle* h, int64_t i, int64_t j, int64_t k)
+{
+    return i * h->deli + j * h->delj + k * h->delk;
+}
+
+/* scmp3D::psi, scmp3D.cpp:187-190 (rho0 = 1 is hard-wired there). */
+static inline double psi(const scmp_oracle* h, double rho)
+{
+    return (1.0 - h->expfn(-rho));
+}
+
+/* Interior iteration: x slabs over threads, the rest serial. */
+#define FOR_INTERIOR_BEGIN(h)                                                \
+    _Pragma("omp parallel for num_threads(h->nthreads) schedule(static)")    \
+    for (int i = 1; i < (h)->nx + 1; i++) {                                  \
+        for (int j = 1; j < (h)->ny + 1; j++) {                              \
+            for (int k = ((h)->dim == 3 ? 1 : 0);                            \
+                 k < ((h)->dim == 3 ? (h)->nz + 1 : 1); k++) {               \
+                int64_t ndx = nd((h), i, j, k);
+#define FOR_INTERIOR_END }}}
+
+/* Periodic fill of the ghost shell of a field with `per` values per node.
+ * Order as LBfluid3D::ghostNodesRho / ghostNodesStreaming
+ * (LBfluid3D.cpp:430-506): x planes first (the single-rank mpiExchange,
+ * :515-543, with both neighbours equal to self), then y over all i, then z
+ * over all i and j, so edges and corners come out periodic.
+ * LBfluid2D.cpp:495-543 is the same without z. */
+static void ghost_fill(const scmp_oracle* h, double* a, int per)
+{
+    const int64_t plane = h->deli * per;
+    /* send-left/recv-right then send-right/recv-left (mpiExchange :524-541) */
+    memcpy(a + (int64_t)(h->nx + 1) * plane, a + 1 * plane,
+           (size_t)plane * sizeof(double));
+    memcpy(a + 0, a + (int64_t)h->nx * plane, (size_t)plane * sizeof(double));
+    for (int64_t i = 0; i < h->nx + 2; i++) {
+        for (int64_t k = 0; k < h->gz; k++) {
+            for (int n = 0; n < per; n++) {
+                a[nd(h, i, 0, k) * per + n] = a[nd(h, i, h->ny, k) * per + n];
+                a[nd(h, i, h->ny + 1, k) * per + n] = a[nd(h, i, 1, k) * per + n];
+            }
+        }
+    }
+    if (h->dim == 3) {
+        for (int64_t i = 0; i < h->nx + 2; i++) {
+            for (int64_t j = 0; j < h->ny + 2; j++) {
+                for (int n = 0; n < per; n++) {
+                    a[nd(h, i, j, 0) * per + n] = a[nd(h, i, j, h->nz) * per + n];
+                    a[nd(h, i, j, h->nz + 1) * per + n] = a[nd(h, i, j, 1) * per + n];
+                }
+            }
+        }
+    }
+}
+
+/* scmp3D::calculateShanChenForces, scmp3D.cpp:141-179; in 2-D the same
+ * formula on the D2Q9 stencil (the loop shape of mcmp2D.cpp:145-183 for a
+ * single component). */
+static void shan_chen_forces(scmp_oracle* h)
+{
+    const int q = h->q;
+    FOR_INTERIOR_BEGIN(h)
+        double rhoA = h->r[ndx];
+        double psi_rhoA = psi(h, rhoA);
+        double GsumxA = 0.0, GsumyA = 0.0, GsumzA = 0.0;
+        for (int n = 0; n < q; n++) {
+            int64_t nbr = nd(h, i + h->exi[n], j + h->eyi[n],
+                             h->dim == 3 ? k + h->ezi[n] : 0);
+            double psi_nbr = psi(h, h->r[nbr]);
+            GsumxA += h->wa[n] * h->ex[n] * psi_nbr;
+            GsumyA += h->wa[n] * h->ey[n] * psi_nbr;
+            GsumzA += h->wa[n] * h->ez[n] * psi_nbr;
+        }
+        h->fx[ndx] = -h->G * psi_rhoA * GsumxA;
+        h->fy[ndx] = -h->G * psi_rhoA * GsumyA;
+        if (h->dim == 3) h->fz[ndx] = -h->G * psi_rhoA * GsumzA;
+    FOR_INTERIOR_END
+}
+
+/* Equilibrium at a node with the velocity shift ueq = u + tau*F/rho.
+ * LBfluid3D.cpp:180-200 (setFtoFeq) and :259-276 (collision);
+ * LBfluid2D.cpp:201-218 and :235-249 for D2Q9 (no z terms). */
+static inline void shifted_velocity(const scmp_oracle* h, int64_t ndx,
+                                    double* ueq, double* veq, double* weq,
+                                    double* usq)
+{
+    const double tau = h->tau;
+    *ueq = h->u[ndx] + tau * h->fx[ndx] / h->r[ndx];
+    *veq = h->v[ndx] + tau * h->fy[ndx] / h->r[ndx];
+    if (h->dim == 3) {
+        *weq = h->w[ndx] + tau * h->fz[ndx] / h->r[ndx];
+        *usq = *ueq * *ueq + *veq * *veq + *weq * *weq;
+    } else {
+        *weq = 0.0;
+        *usq = *ueq * *ueq + *veq * *veq;
+    }
+}
+
+static inline double feq_n(const scmp_oracle* h, int n, double rho,
+                           double ueq, double veq, double weq, double usq)
+{
+    double evel = (h->dim == 3)
+                      ? h->ex[n] * ueq + h->ey[n] * veq + h->ez[n] * weq
+                      : h->ex[n] * ueq + h->ey[n] * veq;
+    double feq = 1.0 + 3.0 * evel + 4.5 * evel * evel - 1.5 * usq;
+    feq *= rho * h->wa[n];
+    return feq;
+}
+
+static void set_f_to_feq(scmp_oracle* h)
+{
+    const int q = h->q;
+    FOR_INTERIOR_BEGIN(h)
+        double ueq, veq, weq, usq;
+        shifted_velocity(h, ndx, &ueq, &veq, &weq, &usq);
+        for (int n = 0; n < q; n++)
+            h->f[ndx * q + n] = feq_n(h, n, h->r[ndx], ueq, veq, weq, usq);
+    FOR_INTERIOR_END
+}
+
+/* LBfluid3D::macros, LBfluid3D.cpp:208-244 (bare velocity, no F/2 shift);
+ * LBfluid2D.cpp:285-301 for D2Q9. */
+static void macros(scmp_oracle* h)
+{
+    const int q = h->q;
+    FOR_INTERIOR_BEGIN(h)
+        double sum = 0.0, sumx = 0.0, sumy = 0.0, sumz = 0.0;
+        for (int n = 0; n < q; n++) {
+            int64_t ndxn = ndx * q + n;
+            sum += h->f[ndxn];
+            sumx += h->f[ndxn] * h->ex[n];
+            sumy += h->f[ndxn] * h->ey[n];
+            if (h->dim == 3) sumz += h->f[ndxn] * h->ez[n];
+        }
+        h->r[ndx] = sum;
+        h->u[ndx] = sumx / h->r[ndx];
+        h->v[ndx] = sumy / h->r[ndx];
+        if (h->dim == 3) h->w[ndx] = sumz / h->r[ndx];
+    FOR_INTERIOR_END
+}
+
+/* Collision into fs, then periodic ghosts of fs, then pull streaming:
+ * LBfluid3D::collideStreamUpdate, LBfluid3D.cpp:252-304;
+ * LBfluid2D::updateFluid_SC, LBfluid2D.cpp:235-272. */
+static void collide_stream(scmp_oracle* h)
+{
+    const int q = h->q;
+    const double tau = h->tau;
+    FOR_INTERIOR_BEGIN(h)
+        double ueq, veq, weq, usq;
+        shifted_velocity(h, ndx, &ueq, &veq, &weq, &usq);
+        for (int n = 0; n < q; n++) {
+            int64_t ndxn = ndx * q + n;
+            double feq = feq_n(h, n, h->r[ndx], ueq, veq, weq, usq);
+            h->fs[ndxn] = h->f[ndxn] - (h->f[ndxn] - feq) / tau;
+        }
+    FOR_INTERIOR_END
+    ghost_fill(h, h->fs, q);
+    FOR_INTERIOR_BEGIN(h)
+        for (int n = 0; n < q; n++) {
+            int64_t nbr = nd(h, i - h->exi[n], j - h->eyi[n],
+                             h->dim == 3 ? k - h->ezi[n] : 0);
+            h->f[ndx * q + n] = h->fs[nbr * q + n];
+        }
+    FOR_INTERIOR_END
+}
+
+/* Halfway bounce-back, applied after streaming from the same node's
+ * post-collision values.  3-D: LBfluid3D::bounceBackWallsZdir,
+ * LBfluid3D.cpp:316-347 (rows k=2 and k=nz-1).  2-D:
+ * LBfluid2D::bounceBackWallsYdir, LBfluid2D.cpp:363-388 (rows j=2, ny-1). */
+static void bounce_back(scmp_oracle* h)
+{
+    const int q = h->q;
+    if (h->dim == 3) {
+        static const int lo_dst[5] = {5, 9, 11, 16, 18};
+        static const int lo_src[5] = {6, 10, 12, 15, 17};
+        for (int64_t i = 1; i < h->nx + 1; i++) {
+            for (int64_t j = 1; j < h->ny + 1; j++) {
+                int64_t a = nd(h, i, j, 2), b = nd(h, i, j, h->nz - 1);
+                for (int m = 0; m < 5; m++)
+                    h->f[a * q + lo_dst[m]] = h->fs[a * q + lo_src[m]];
+                for (int m = 0; m < 5; m++)
+                    h->f[b * q + lo_src[m]] = h->fs[b * q + lo_dst[m]];
+            }
+        }
+    } else {
+        static const int lo_dst[3] = {3, 5, 7};
+        static const int lo_src[3] = {4, 6, 8};
+        for (int64_t i = 1; i < h->nx + 1; i++) {
+            int64_t a = nd(h, i, 2, 0), b = nd(h, i, h->ny - 1, 0);
+            for (int m = 0; m < 3; m++)
+                h->f[a * q + lo_dst[m]] = h->fs[a * q + lo_src[m]];
+            for (int m = 0; m < 3; m++)
+                h->f[b * q + lo_src[m]] = h->fs[b * q + lo_dst[m]];
+        }
+    }
+}
+
+void scmp_oracle_set_rho(scmp_oracle* h, const double* rho0)
+{
+    int64_t c = 0;
+    for (int i = 1; i < h->nx + 1; i++)
+        for (int j = 1; j < h->ny + 1; j++)
+            for (int k = (h->dim == 3 ? 1 : 0); k < (h->dim == 3 ? h->nz + 1 : 1); k++) {
+                int64_t ndx = nd(h, i, j, k);
+                h->r[ndx] = rho0[c++];
+                h->u[ndx] = 0.0;
+                h->v[ndx] = 0.0;
+                h->w[ndx] = 0.0;
+            }
+}
+
+void scmp_oracle_init(scmp_oracle* h)
+{
+    ghost_fill(h, h->r, 1);          /* scmp3D.cpp:78 */
+    shan_chen_forces(h);             /* scmp3D.cpp:79 */
+    set_f_to_feq(h);                 /* scmp3D.cpp:80 */
+}
+
+void scmp_oracle_step(scmp_oracle* h, int nsteps)
+{
+    for (int s = 0; s < nsteps; s++) {
+        if (h->order == 3) {
+            /* scmp3D.cpp:90-119; walls as mcmp3DFilm.cpp:132-140 */
+            macros(h);
+            ghost_fill(h, h->r, 1);
+            shan_chen_forces(h);
+            collide_stream(h);
+            if (h->walls) bounce_back(h);
+        } else {
+            /* mcmp2D.cpp:111-121 -> LBfluid2D.cpp:227-312 */
+            shan_chen_forces(h);
+            collide_stream(h);
+            if (h->walls) bounce_back(h);
+            macros(h);
+            ghost_fill(h, h->r, 1);
+        }
+    }
+}
+
+void scmp_oracle_get(const scmp_oracle* h, double* rho, double* u, double* v,
+                     double* w, double* f)
+{
+    int64_t c = 0;
+    for (int i = 1; i < h->nx + 1; i++)
+        for (int j = 1; j < h->ny + 1; j++)
+            for (int k = (h->dim == 3 ? 1 : 0); k < (h->dim == 3 ? h->nz + 1 : 1); k++) {
+                int64_t ndx = nd(h, i, j, k);
+                if (rho) rho[c] = h->r[ndx];
+                if (u) u[c] = h->u[ndx];
+                if (v) v[c] = h->v[ndx];
+                if (w) w[c] = (h->dim == 3) ? h->w[ndx] : 0.0;
+                if (f) memcpy(f + c * h->q, h->f + ndx * h->q,
+                              (size_t)h->q * sizeof(double));
+                c++;
+            }
+}
+
+void scmp_oracle_get_force(const scmp_oracle* h, double* fx, double* fy,
+                           double* fz)
+{
+    int64_t c = 0;
+    for (int i = 1; i < h->nx + 1; i++)
+        for (int j = 1; j < h->ny + 1; j++)
+            for (int k = (h->dim == 3 ? 1 : 0); k < (h->dim == 3 ? h->nz + 1 : 1); k++) {
+                int64_t ndx = nd(h, i, j, k);
+                if (fx) fx[c] = h->fx[ndx];
+                if (fy) fy[c] = h->fy[ndx];
+                if (fz) fz[c] = (h->dim == 3) ? h->fz[ndx] : 0.0;
+                c++;
+            }
+}
+
+void scmp_oracle_moments(const scmp_oracle* h, double* rho, double* u,
+                         double* v, double* w)
+{
+    const int q = h->q;
+    int64_t c = 0;
+    for (int i = 1; i < h->nx + 1; i++)
+        for (int j = 1; j < h->ny + 1; j++)
+            for (int k = (h->dim == 3 ? 1 : 0); k < (h->dim == 3 ? h->nz + 1 : 1); k++) {
+                int64_t ndx = nd(h, i, j, k);
+                double sum = 0.0, sumx = 0.0, sumy = 0.0, sumz = 0.0;
+                for (int n = 0; n < q; n++) {
+                    double fv = h->f[ndx * q + n];
+                    sum += fv;
+                    sumx += fv * h->ex[n];
+                    sumy += fv * h->ey[n];
+                    sumz += fv * h->ez[n];
+                }
+                if (rho) rho[c] = sum;
+                if (u) u[c] = sumx / sum;
+                if (v) v[c] = sumy / sum;
+                if (w) w[c] = (h->dim == 3) ? sumz / sum : 0.0;
+                c++;
+            }
+}
+
+void scmp_oracle_set_f(scmp_oracle* h, const double* f)
+{
+    int64_t c = 0;
+    for (int i = 1; i < h->nx + 1; i++)
+        for (int j = 1; j < h->ny + 1; j++)
+            for (int k = (h->dim == 3 ? 1 : 0); k < (h->dim == 3 ? h->nz + 1 : 1); k++) {
+                int64_t ndx = nd(h, i, j, k);
+                memcpy(h->f + ndx * h->q, f + c * h->q,
+                       (size_t)h->q * sizeof(double));
+                c++;
+            }
+    /* The r,u arrays become the moments of the new f, as after a step of
+     * the LBfluid2D order (LBfluid2D.cpp:285-310). */
+    macros(h);
+    ghost_fill(h, h->r, 1);
+}
+
+/* Seeded initial densities (the reference seeds rand() from time(), which is
+ * not reproducible; these generators define the seeded inputs instead). */
+
+/* std::mt19937(seed): the Mersenne Twister of Matsumoto & Nishimura (1998),
+ * the same generator as src/utils/rand.cpp:24-78. */
+typedef struct { uint32_t mt[624]; int idx; } mt19937_state;
+
+static void mt_seed(mt19937_state* s, uint32_t seed)
+{
+    s->mt[0] = seed;
+    for (int i = 1; i < 624; i++)
+        s->mt[i] = 1812433253u * (s->mt[i - 1] ^ (s->mt[i - 1] >> 30)) + (uint32_t)i;
+    s->idx = 624;
+}
+
+static uint32_t mt_next(mt19937_state* s)
+{
+    if (s->idx >= 624) {
+        for (int i = 0; i < 624; i++) {
+            uint32_t y = (s->mt[i] & 0x80000000u) | (s->mt[(i + 1) % 624] & 0x7fffffffu);
+            uint32_t v = s->mt[(i + 397) % 624] ^ (y >> 1);
+            if (y & 1u) v ^= 0x9908b0dfu;
+            s->mt[i] = v;
+        }
+        s->idx = 0;
+    }
+    uint32_t y = s->mt[s->idx++];
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= y >> 18;
+    return y;
+}
+
+/* rho = mean + amp*(x/4294967295.0 - 0.5), x = successive mt19937 outputs in
+ * reference loop order; the mapping of scmp3D.cpp:67-69 with rand()/RAND_MAX
+ * replaced by Rand::uniform (rand.cpp:81-84). */
+void scmp_oracle_noise_rho(uint32_t seed, int64_t n, double mean, double amp,
+                           double* out)
+{
+    mt19937_state s;
+    mt_seed(&s, seed);
+    for (int64_t c = 0; c < n; c++) {
+        double x = (double)mt_next(&s) / 4294967295.0;
+        out[c] = mean + amp * (x - 0.5);
+    }
+}
+
+/* Raw generator output, for the known-answer test of mt19937. */
+void scmp_oracle_mt19937(uint32_t seed, int64_t n, uint32_t* out)
+{
+    mt19937_state s;
+    mt_seed(&s, seed);
+    for (int64_t c = 0; c < n; c++) out[c] = mt_next(&s);
+}
+
+/* Droplet: rho = (rl+rv)/2 - (rl-rv)/2 * tanh(2(r-R)/W), r measured from the
+ * geometric centre ((N-1)/2 per axis) without periodic images.  dims has 2
+ * or 3 entries; nz = 1 for 2-D. */
+void scmp_oracle_droplet_rho(int nx, int ny, int nz, double R, double W,
+                             double rl, double rv, double* out)
+{
+    double cx = 0.5 * (nx - 1), cy = 0.5 * (ny - 1), cz = 0.5 * (nz - 1);
+    int64_t c = 0;
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++)
+            for (int k = 0; k < nz; k++) {
+                double dx = i - cx, dy = j - cy, dz = k - cz;
+                double r = sqrt(dx * dx + dy * dy + dz * dz);
+                out[c++] = 0.5 * (rl + rv) - 0.5 * (rl - rv) * tanh(2.0 * (r - R) / W);
+            }
+}

@@ -1,0 +1,98 @@
+/*
+ * scmp_oracle.h -- CPU restatement of mesoBasic's Shan-Chen single-component
+ * multiphase (SCMP) lattice-Boltzmann step, used ONLY as a test checker and
+ * as the timed CPU baseline.  Nothing in the product path links this file.
+ *
+ * It follows the reference arithmetic statement by statement, in the
+ * reference's own array-of-structures layout with a one-node ghost shell:
+ *   src/lattice_boltzmann/LBUtils/Stencil.cpp:5-120      (tables, order)
+ *   src/lattice_boltzmann/LBUtils/LBfluid3D.cpp:180-347  (feq, macros,
+ *                                                          collide, stream,
+ *                                                          z bounce-back)
+ *   src/lattice_boltzmann/LBUtils/LBfluid3D.cpp:430-543  (ghost shells)
+ *   src/lattice_boltzmann/LBUtils/LBfluid2D.cpp:201-388  (D2Q9 engine)
+ *   src/lattice_boltzmann/LBUtils/LBfluid2D.cpp:495-580  (D2Q9 ghosts)
+ *   src/lattice_boltzmann/LBTypes/scmp3D.cpp:54-190      (model, psi, force)
+ *   src/lattice_boltzmann/LBTypes/mcmp2D.cpp:97-201      (2-D step order)
+ * Indices are 64-bit (the reference uses int, which overflows at 512^3 on
+ * one rank); the x-slab split over threads replaces the MPI slab split of
+ * src/lattice_boltzmann/LBApp.cpp:46-60 and leaves results unchanged because
+ * every per-node operation reads only the previous phase's arrays.
+ *
+ * Parity status: pinned bit-for-bit against the compiled reference sources
+ * (oracle/_ref harness, fixtures in tests/golden) and by closed-form checks
+ * (stencil moments, uniform fixed point, mass/momentum conservation,
+ * periodic-shift invariance, D3Q19 NZ=1 vs D2Q9).  psi() calls the host
+ * libm exp(), exactly as scmp3D.cpp:189 does.
+ */
+#ifndef SCMP_ORACLE_H
+#define SCMP_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct scmp_oracle scmp_oracle;
+
+/* q = 9 (D2Q9, nz must be 1) or 19 (D3Q19).
+ * order = 3: step = macros -> ghost rho -> force -> collide -> stream
+ *            (scmp3D::updateLatticeBoltzmann, scmp3D.cpp:90-119)
+ * order = 2: step = force -> collide -> stream -> walls -> macros -> ghost rho
+ *            (mcmp2D::updateLatticeBoltzmann + LBfluid2D::updateFluid_SC,
+ *             mcmp2D.cpp:97-123, LBfluid2D.cpp:227-312)
+ * walls = 1: halfway bounce-back walls on the last axis (z in 3-D,
+ *            LBfluid3D.cpp:316-347; y in 2-D, LBfluid2D.cpp:363-388).
+ * nthreads: x-slab worker threads (1 = serial, as one MPI rank). */
+scmp_oracle* scmp_oracle_create(int q, int nx, int ny, int nz, double tau,
+                                double G, int order, int walls, int nthreads);
+void scmp_oracle_destroy(scmp_oracle* h);
+
+/* Initial density, interior nodes, reference order (x slowest, last axis
+ * fastest).  Velocities are set to zero, as scmp3D.cpp:67-75 does. */
+void scmp_oracle_set_rho(scmp_oracle* h, const double* rho0);
+
+/* scmp3D::initLatticeBoltzmann minus the rand() fill (scmp3D.cpp:76-81):
+ * ghost rho, Shan-Chen force, f = feq(rho, u + tau*F/rho). */
+void scmp_oracle_init(scmp_oracle* h);
+
+void scmp_oracle_step(scmp_oracle* h, int nsteps);
+
+/* Copies the reference arrays, interior only, reference order.  rho/u/v/w
+ * are the r,u,v,w member arrays (their time level depends on the step
+ * order); f is [node][q].  Any pointer may be NULL.  w is ignored in 2-D. */
+void scmp_oracle_get(const scmp_oracle* h, double* rho, double* u, double* v,
+                     double* w, double* f);
+/* Shan-Chen force arrays fx, fy, fz (interior, reference order). */
+void scmp_oracle_get_force(const scmp_oracle* h, double* fx, double* fy,
+                           double* fz);
+/* Moments of the current f (rho = sum f, u = sum f e / rho), computed with
+ * the reference's summation order. */
+void scmp_oracle_moments(const scmp_oracle* h, double* rho, double* u,
+                         double* v, double* w);
+
+/* Overwrites f (interior, [node][q]) and sets the r,u,v,w arrays to its
+ * moments; used to start from a snapshot. */
+void scmp_oracle_set_f(scmp_oracle* h, const double* f);
+
+/* Replaces the exp() used inside psi(); NULL restores libm exp.  Only for
+ * sensitivity experiments (how far a 1-ulp change in psi travels). */
+void scmp_oracle_set_exp(scmp_oracle* h, double (*fn)(double));
+
+int64_t scmp_oracle_nodes(const scmp_oracle* h);
+
+/* Seeded inputs.  noise: rho = mean + amp*(x/4294967295.0 - 0.5) with x the
+ * successive outputs of std::mt19937(seed) in reference loop order.
+ * droplet: tanh profile of radius R and width W between rl and rv, centred
+ * at ((N-1)/2) per axis.  mt19937: raw outputs (known-answer tests). */
+void scmp_oracle_noise_rho(uint32_t seed, int64_t n, double mean, double amp,
+                           double* out);
+void scmp_oracle_mt19937(uint32_t seed, int64_t n, uint32_t* out);
+void scmp_oracle_droplet_rho(int nx, int ny, int nz, double R, double W,
+                             double rl, double rv, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

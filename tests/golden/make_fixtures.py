@@ -1,0 +1,80 @@
+"""Writes the golden fixtures in this folder by running the REFERENCE's own
+lattice-Boltzmann classes (compiled in place by oracle/Makefile.ref into
+oracle/_ref/ref_harness) on seeded inputs.
+
+Only runnable where the reference tree and MPICH are present:
+    make -C oracle ref && python tests/golden/make_fixtures.py
+
+Each case is one .npz: inputs (rho0, parameters) and outputs at the listed
+steps -- the reference's r,u,v,w member arrays and, where stored, f
+([node][q]) -- all float64, interior nodes, reference order.
+"""
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+import oracle  # noqa: E402
+
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+MPIRUN = "/opt/conda/bin/mpirun"
+
+# name, q, shape, tau, G, walls, init, steps, steps_with_f, ranks
+CASES = [
+    ("d3q19_7x6x5_tau1", 19, (7, 6, 5), 1.0, -5.5, False, ("noise", 11), [0, 1, 2, 10, 100], [0, 1, 100], 3),
+    ("d3q19_7x6x5_tau07", 19, (7, 6, 5), 0.7, -5.5, False, ("noise", 12), [0, 1, 2, 10, 100], [1, 100], 2),
+    ("d3q19_walls_8x8x10", 19, (8, 8, 10), 1.0, -5.5, True, ("noise", 13), [1, 50], [50], 2),
+    ("d3q19_droplet_16", 19, (16, 16, 16), 1.0, -5.5, False, ("droplet", 4.0), [1, 200], [], 4),
+    ("d2q9_13x11", 9, (13, 11), 1.0, -5.0, False, ("noise", 14), [0, 1, 2, 10, 1000], [1, 1000], 3),
+    ("d2q9_walls_12x10_tau08", 9, (12, 10), 0.8, -5.0, True, ("noise", 15), [1, 10, 300], [300], 2),
+    ("d2q9_64x64", 9, (64, 64), 1.0, -5.0, False, ("noise", 16), [1000], [], 4),
+]
+
+
+def run_case(name, q, shape, tau, G, walls, init, steps, steps_f, ranks):
+    if init[0] == "noise":
+        rho0 = oracle.noise_rho(shape, init[1])
+    else:
+        rho0 = oracle.droplet_rho(shape, init[1])
+    nx, ny = shape[0], shape[1]
+    nz = shape[2] if len(shape) == 3 else 1
+    n = nx * ny * nz
+    with tempfile.TemporaryDirectory() as td:
+        pre = os.path.join(td, name)
+        rho0.astype(np.float64).tofile(pre + "_rho0.bin")
+        cmd = [MPIRUN, "-np", str(ranks), HARNESS, str(q), str(nx), str(ny), str(nz), repr(tau),
+               repr(G), str(int(walls)), pre + "_rho0.bin", pre, ",".join(map(str, steps))]
+        subprocess.run(cmd, check=True)
+        out = {"rho0": rho0.reshape(-1)}
+        for s in steps:
+            a = np.fromfile(f"{pre}_{s}.bin")
+            out[f"rho_{s}"] = a[:n]
+            out[f"u_{s}"] = a[n:2 * n]
+            out[f"v_{s}"] = a[2 * n:3 * n]
+            if q == 19:
+                out[f"w_{s}"] = a[3 * n:4 * n]
+            if s in steps_f:
+                out[f"f_{s}"] = a[4 * n:]
+    meta = dict(q=q, shape=list(shape), tau=tau, G=G, walls=walls, init=list(init), steps=steps,
+                steps_f=steps_f, ranks=ranks, order=3 if q == 19 else 2,
+                source="reference classes via oracle/_ref/ref_harness")
+    out["meta"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    print(name, "ok")
+
+
+def main():
+    if not os.path.exists(HARNESS):
+        sys.exit("build the harness first: make -C oracle ref")
+    for c in CASES:
+        run_case(*c)
+
+
+if __name__ == "__main__":
+    main()

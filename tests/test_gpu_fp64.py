@@ -1,0 +1,136 @@
+"""GPU parity of the FP64 drop-in path: bit-for-bit against the reference
+fixtures (written by the reference's own classes) and against the oracle on
+the same seeded inputs, for both step orders, walls, tau != 1 and every
+kernel variant."""
+import numpy as np
+import pytest
+
+import mesobasic_amd as mb
+from mesobasic_amd import lbm
+import oracle
+from conftest import FIXTURES, load_fixture, rel_l2, same_bits
+
+pytestmark = pytest.mark.gpu
+
+KERNELS = [mb.KERNEL_TWO_PASS, mb.KERNEL_FUSED]
+
+
+def _lattice(meta, kernel):
+    try:
+        return mb.Lattice(meta["q"], tuple(meta["shape"]), meta["tau"], meta["G"],
+                          order=meta["order"], walls=meta["walls"], precision=mb.FP64,
+                          kernel=kernel)
+    except mb.LbmError as e:
+        if kernel == mb.KERNEL_FUSED and e.code == -1:
+            pytest.skip("fused kernel does not take this lattice")
+        raise
+
+
+def test_device_exp_matches_libm():
+    import ctypes
+    libm = ctypes.CDLL("libm.so.6")
+    libm.exp.restype = ctypes.c_double
+    libm.exp.argtypes = [ctypes.c_double]
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.uniform(-40, 0, 200_000), rng.uniform(-511, 511, 50_000),
+                        np.array([0.0, -0.0, np.inf, -np.inf, 1e-300, 709.78, 709.79])])
+    y = lbm.probe_exp(x)
+    ref = np.array([libm.exp(v) for v in x])
+    bad = np.flatnonzero(y.view(np.uint64) != ref.view(np.uint64))
+    assert bad.size == 0, f"{bad.size} mismatches, first x={x[bad[0]]!r}"
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("name", FIXTURES)
+def test_fixture_bitwise(name, kernel):
+    meta, data = load_fixture(name)
+    print(f"case {name} kernel {kernel}", flush=True)
+    lat = _lattice(meta, kernel)
+    lat.set_rho(data["rho0"])
+    lat.init()
+    cur = 0
+    for s in sorted(meta["steps"]):
+        lat.step(s - cur)
+        cur = s
+        got = lat.fields(mb.FIELDS_REFERENCE)
+        for k in ("rho", "u", "v") + (("w",) if meta["q"] == 19 else ()):
+            assert same_bits(got[k].ravel(), data[f"{k}_{s}"]), (name, s, k,
+                                                               rel_l2(got[k], data[f"{k}_{s}"]))
+        if s in meta["steps_f"]:
+            assert same_bits(lat.get_f().ravel(), data[f"f_{s}"]), (name, s, "f")
+    lat.close()
+
+
+CASES = [
+    # q, shape, tau, G, order, walls, steps
+    (19, (9, 7, 6), 1.0, -5.5, 2, False, 25),
+    (19, (9, 7, 6), 0.65, -5.0, 3, True, 25),
+    (19, (6, 10, 7), 1.3, -5.5, 2, True, 25),
+    (9, (17, 12), 1.0, -5.0, 3, False, 40),
+    (9, (11, 16), 0.7, -4.5, 3, True, 40),
+    (9, (33, 31), 1.0, -5.0, 2, False, 200),
+]
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("case", CASES)
+def test_matches_oracle_bitwise(case, kernel):
+    q, shape, tau, G, order, walls, steps = case
+    print(f"case {case} kernel {kernel}", flush=True)
+    rho0 = mb.noise_rho(shape, 21)
+    o = oracle.Oracle(q, shape, tau, G, order=order, walls=walls)
+    o.set_rho(rho0)
+    o.init()
+    meta = dict(q=q, shape=shape, tau=tau, G=G, order=order, walls=walls)
+    lat = _lattice(meta, kernel)
+    lat.set_rho(rho0)
+    lat.init()
+    for chunk in (1, 1, steps - 2):
+        o.step(chunk)
+        lat.step(chunk)
+        ref = o.fields()
+        got = lat.fields(mb.FIELDS_REFERENCE)
+        for k in ("rho", "u", "v", "w"):
+            assert same_bits(got[k], ref[k]), (k, rel_l2(got[k], ref[k]))
+        assert same_bits(lat.get_f(), ref["f"])
+        cur = lat.fields(mb.FIELDS_CURRENT)
+        mom = o.moments()
+        for k in ("rho", "u", "v", "w"):
+            assert same_bits(cur[k], mom[k]), k
+    lat.close()
+
+
+def test_set_f_roundtrip_and_restart():
+    shape = (8, 6, 5)
+    rho0 = mb.noise_rho(shape, 4)
+    a = mb.Lattice(19, shape, 1.0, -5.5, kernel=mb.KERNEL_TWO_PASS)
+    a.set_rho(rho0); a.init(); a.step(7)
+    f7 = a.get_f()
+    b = mb.Lattice(19, shape, 1.0, -5.5, kernel=mb.KERNEL_TWO_PASS)
+    b.set_f(f7)
+    assert same_bits(b.get_f(), f7)
+    a.step(5); b.step(5)
+    assert same_bits(a.get_f(), b.get_f())
+
+
+@pytest.mark.parametrize("q,shape", [(19, (16, 12, 10)), (9, (64, 48))])
+def test_conservation_fp64(q, shape):
+    rho0 = mb.noise_rho(shape, 6)
+    lat = mb.Lattice(q, shape, 1.0, -5.5 if q == 19 else -5.0)
+    lat.set_rho(rho0); lat.init()
+    c0 = lat.conservation()
+    lat.step(100)
+    c1 = lat.conservation()
+    assert abs(c1[0] - c0[0]) / c0[0] < 1e-12
+    # total momentum: sum of forces is zero, so momentum stays ~ round-off
+    assert np.all(np.abs(c1[1:]) < 1e-9 * c0[0])
+    m = lat.fields(mb.FIELDS_CURRENT)["rho"].sum()
+    assert abs(m - c1[0]) / c1[0] < 1e-12
+
+
+def test_errors_are_reported():
+    with pytest.raises(mb.LbmError):
+        mb.Lattice(19, (4, 4, 4), -1.0, -5.5)
+    lat = mb.Lattice(19, (4, 4, 4), 1.0, -5.5)
+    with pytest.raises(mb.LbmError):
+        lat.step(1)        # before init
