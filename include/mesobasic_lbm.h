@@ -111,6 +111,13 @@ enum lbm_fields_kind {
 int lbm_get_fields(lbm_handle* h, int kind, double* rho, double* u, double* v,
                    double* w);
 
+/* Shan-Chen force held in the reference's fx,fy,fz arrays after the last
+ * completed step: F(psi(rho)) of the density the step started from
+ * (scmp3D.cpp:141-179; rho0 before the first step).  Host arrays of
+ * nx*ny*nz doubles; fz ignored in 2-D.  Synchronous.  Used for the
+ * Shan-Chen physical velocity u* = u + F/(2 rho) (LBfluid2D.cpp:164-172). */
+int lbm_get_force(lbm_handle* h, double* fx, double* fy, double* fz);
+
 /* Post-streaming populations f (after walls), host [node][q] doubles as the
  * reference's f array (LBfluid3D.hpp:29).  Synchronous. */
 int lbm_get_f(lbm_handle* h, double* f);

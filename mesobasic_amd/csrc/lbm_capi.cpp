@@ -101,12 +101,14 @@ int run_step(lbm_handle* h)
         if (h->kernel == LBM_KERNEL_FUSED && mode == 0) {
             HIP_OK(lbm::fp64_fused_step(h->a, h->g, Pin, h->kind[src], Pout, h->stream));
         } else {
+            if (!h->psi) return fail(LBM_ESTATE, "psi field not allocated");
             if (mode == 1) HIP_OK(lbm::fp64_psi_from_rho(h->rho0, h->psi, h->g.n, h->stream));
             else HIP_OK(lbm::fp64_psi_from_pops(h->a, h->g, Pin, h->kind[src], h->psi, h->stream));
             HIP_OK(lbm::fp64_collide(h->a, h->g, Pin, h->kind[src], Pout, h->psi, h->rho0, mode,
                                      h->stream));
         }
     } else {
+        if (!h->psi) return fail(LBM_ESTATE, "psi field not allocated");
         HIP_OK(lbm::lowp_step(h->d.precision, h->a, h->g, h->d.rho_shift, h->buf[src],
                               h->kind[src], h->buf[dst], h->first_step_stored ? h->rho0 : nullptr,
                               reinterpret_cast<float*>(h->psi), h->kernel, h->stream));
@@ -216,7 +218,10 @@ int lbm_create(const lbm_desc* desc, lbm_handle** out)
                                 (long long)pop_bytes, hipGetErrorString(e)));
     }
     h->bytes = 2 * pop_bytes;
-    if (h->kernel == LBM_KERNEL_TWO_PASS || desc->precision != LBM_FP64) {
+    {
+        // psi field: the two-pass kernels use it every step, and every
+        // handle runs them for initialisation and for the LBfluid2D-order
+        // first step, so it is always allocated.
         if (hipMalloc(&h->psi, h->g.n * 8) != hipSuccess)
             return cleanup(fail(LBM_ENOMEM, "hipMalloc for psi failed"));
         h->bytes += h->g.n * 8;
@@ -260,14 +265,9 @@ int lbm_set_rho(lbm_handle* h, const double* rho0)
 int lbm_init_equilibrium(lbm_handle* h)
 {
     if (!h) return fail(LBM_EINVAL, "lbm_init_equilibrium: NULL handle");
+    if (!h->psi) return fail(LBM_ESTATE, "psi field not allocated");
     if (!is_lowp(h)) {
         double* psi = h->psi;
-        int rc = LBM_OK;
-        if (!psi) {
-            rc = ensure_tmp(h, h->g.n * 8);
-            if (rc) return rc;
-            psi = h->tmp;
-        }
         HIP_OK(lbm::fp64_psi_from_rho(h->rho0, psi, h->g.n, h->stream));
         HIP_OK(lbm::fp64_collide(h->a, h->g, nullptr, lbm::kHoldsF, static_cast<double*>(h->buf[0]),
                                  psi, h->rho0, 2, h->stream));
@@ -332,6 +332,38 @@ int lbm_get_fields(lbm_handle* h, int kind, double* rho, double* u, double* v, d
         return fail(LBM_ESTATE, "previous populations unavailable");
     // order 3: the arrays hold the moments computed at the start of step N
     return moments_to_host(h, 1 - h->cur, rho, u, v, w);
+}
+
+int lbm_get_force(lbm_handle* h, double* fx, double* fy, double* fz)
+{
+    if (!h) return fail(LBM_EINVAL, "lbm_get_force: NULL handle");
+    if (!h->initialised) return fail(LBM_ESTATE, "lbm_get_force before initialisation");
+    const int64_t n = h->g.n;
+    int rc = ensure_tmp(h, 8 * n * 8);
+    if (rc) return rc;
+    double* rho = h->tmp;            // density the last step started from
+    double* psi = h->tmp + 4 * n;
+    double* F = h->tmp + 5 * n;
+    const bool from_rho0 = h->ref_is_rho0 && (h->steps == 0 || (h->steps == 1 && h->d.order == 2));
+    if (from_rho0) {
+        HIP_OK(hipMemcpyAsync(rho, h->rho0, n * 8, hipMemcpyDeviceToDevice, h->stream));
+    } else {
+        const int b = (h->steps == 0) ? h->cur : 1 - h->cur;
+        if (!is_lowp(h))
+            HIP_OK(lbm::fp64_moments(h->a, h->g, static_cast<const double*>(h->buf[b]), h->kind[b],
+                                     rho, rho + n, rho + 2 * n, rho + 3 * n, h->stream));
+        else
+            HIP_OK(lbm::lowp_moments(h->d.precision, h->a, h->g, h->d.rho_shift, h->buf[b],
+                                     h->kind[b], rho, rho + n, rho + 2 * n, rho + 3 * n, h->stream));
+    }
+    HIP_OK(lbm::fp64_psi_from_rho(rho, psi, n, h->stream));
+    HIP_OK(lbm::fp64_force(h->a, h->g, psi, F, F + n, F + 2 * n, h->stream));
+    double* outs[3] = {fx, fy, fz};
+    for (int k = 0; k < 3; k++)
+        if (outs[k])
+            HIP_OK(hipMemcpyAsync(outs[k], F + k * n, n * 8, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    return LBM_OK;
 }
 
 int lbm_get_f(lbm_handle* h, double* f)

@@ -216,6 +216,34 @@ __global__ void __launch_bounds__(kBlock)
     });
 }
 
+template <int Q>
+__global__ void __launch_bounds__(kBlock)
+    k_force(const double* __restrict__ psi, Grid g, double G, double* __restrict__ fx,
+            double* __restrict__ fy, double* __restrict__ fz)
+{
+    using S = St<Q>;
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (c >= g.n) return;
+    const Coord p = decompose(static_cast<uint32_t>(c), g);
+    const Nbr nb = neighbours(p, g);
+    const double pc = psi[c];
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+#pragma unroll
+    for (int n = 1; n < Q; n++) {
+        const double t = S::w[n] * psi[at(nb, S::ex[n], S::ey[n], S::ez[n])];
+        if (S::ex[n] == 1) gx = gx + t;
+        if (S::ex[n] == -1) gx = gx - t;
+        if (S::ey[n] == 1) gy = gy + t;
+        if (S::ey[n] == -1) gy = gy - t;
+        if (S::ez[n] == 1) gz = gz + t;
+        if (S::ez[n] == -1) gz = gz - t;
+    }
+    const double mg = -G;
+    fx[c] = (mg * pc) * gx;
+    fy[c] = (mg * pc) * gy;
+    fz[c] = (S::D == 3) ? (mg * pc) * gz : 0.0;
+}
+
 template <int Q, bool WALLS>
 __global__ void __launch_bounds__(kBlock)
     k_moments(const double* __restrict__ P, int kind, Grid g, double* __restrict__ rho,
@@ -338,6 +366,14 @@ hipError_t fp64_collide(const StepArgs& a, const Grid& g, const double* Pin, int
     return hipGetLastError();
 }
 
+hipError_t fp64_force(const StepArgs& a, const Grid& g, const double* psi, double* fx, double* fy,
+                      double* fz, hipStream_t s)
+{
+    if (a.q == 9) k_force<9><<<blocks_for(g.n), kBlock, 0, s>>>(psi, g, a.G, fx, fy, fz);
+    else k_force<19><<<blocks_for(g.n), kBlock, 0, s>>>(psi, g, a.G, fx, fy, fz);
+    return hipGetLastError();
+}
+
 hipError_t fp64_moments(const StepArgs& a, const Grid& g, const double* P, int kind, double* rho,
                         double* u, double* v, double* w, hipStream_t s)
 {
@@ -373,13 +409,6 @@ hipError_t fp64_conservation(const StepArgs& a, const Grid& g, const double* P, 
     *nblocks = nb;
     LBM_DISPATCH_QW(a, (k_conservation<Q, W><<<nb, kConsBlock, 0, s>>>(P, kind, g, partials)));
     return hipGetLastError();
-}
-
-bool fp64_fused_supported(const StepArgs&, const Grid&) { return false; }
-
-hipError_t fp64_fused_step(const StepArgs&, const Grid&, const double*, int, double*, hipStream_t)
-{
-    return hipErrorNotSupported;
 }
 
 }  // namespace lbm

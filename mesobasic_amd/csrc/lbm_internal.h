@@ -44,6 +44,9 @@ hipError_t fp64_collide(const StepArgs& a, const Grid& g, const double* Pin, int
 hipError_t fp64_fused_step(const StepArgs& a, const Grid& g, const double* Pin, int kind,
                            double* Pout, hipStream_t s);
 bool fp64_fused_supported(const StepArgs& a, const Grid& g);
+// Shan-Chen force from a psi field (FP64, reference arithmetic).
+hipError_t fp64_force(const StepArgs& a, const Grid& g, const double* psi, double* fx, double* fy,
+                      double* fz, hipStream_t s);
 hipError_t fp64_moments(const StepArgs& a, const Grid& g, const double* P, int kind, double* rho,
                         double* u, double* v, double* w, hipStream_t s);
 hipError_t fp64_get_f(const StepArgs& a, const Grid& g, const double* P, int kind, double* aos,

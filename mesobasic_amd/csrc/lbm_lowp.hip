@@ -11,29 +11,14 @@
 
 #include "../../include/mesobasic_lbm.h"
 #include "lbm_lowp.h"
+#include "lbm_fused.h"
 #include "lbm_stencil.h"
+#include "lbm_storage.h"
 
 namespace lbm {
 namespace {
 
 constexpr int kBlock = 256;
-
-// Storage policies: load returns f (FP32), store takes f.
-struct StF32 {
-    using T = float;
-    static __device__ __forceinline__ float load(T v, float) { return v; }
-    static __device__ __forceinline__ T store(float f, float) { return f; }
-};
-struct StF16 {
-    using T = __half;
-    static __device__ __forceinline__ float load(T v, float sh) { return __half2float(v) + sh; }
-    static __device__ __forceinline__ T store(float f, float sh) { return __float2half_rn(f - sh); }
-};
-struct StBF16 {
-    using T = __hip_bfloat16;
-    static __device__ __forceinline__ float load(T v, float sh) { return __bfloat162float(v) + sh; }
-    static __device__ __forceinline__ T store(float f, float sh) { return __float2bfloat16(f - sh); }
-};
 
 struct Coord {
     int x, y, z;
@@ -325,10 +310,12 @@ inline unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + kBlock
 
 }  // namespace
 
-int lowp_resolve_kernel(int, const StepArgs&, const Grid&, int requested)
+int lowp_resolve_kernel(int, const StepArgs& a, const Grid& g, int requested)
 {
-    if (requested == LBM_KERNEL_FUSED) return -1;
-    return LBM_KERNEL_TWO_PASS;
+    const bool fused_ok = fused::supported(g, a.q);
+    if (requested == LBM_KERNEL_FUSED) return fused_ok ? LBM_KERNEL_FUSED : -1;
+    if (requested == LBM_KERNEL_TWO_PASS) return LBM_KERNEL_TWO_PASS;
+    return fused_ok ? LBM_KERNEL_FUSED : LBM_KERNEL_TWO_PASS;
 }
 
 hipError_t lowp_init(int precision, const StepArgs& a, const Grid& g, double rho_shift,
@@ -345,10 +332,30 @@ hipError_t lowp_init(int precision, const StepArgs& a, const Grid& g, double rho
 }
 
 hipError_t lowp_step(int precision, const StepArgs& a, const Grid& g, double rho_shift,
-                     const void* Pin, int kind, void* Pout, const double* rho0, float* psi, int,
-                     hipStream_t s)
+                     const void* Pin, int kind, void* Pout, const double* rho0, float* psi,
+                     int kernel, hipStream_t s)
 {
     const float sh = static_cast<float>(rho_shift);
+    if (kernel == LBM_KERNEL_FUSED && rho0 == nullptr && fused::supported(g, a.q)) {
+        const int ncx = (g.nx + fused::kChunkX - 1) / fused::kChunkX;
+        const unsigned nb = static_cast<unsigned>((g.ny / fused::TY) * (g.nz / fused::TZ) * ncx);
+        auto go = [&](auto sptag) {
+            using SP = decltype(sptag);
+            using T = typename SP::T;
+            const T* in = static_cast<const T*>(Pin);
+            T* out = static_cast<T*>(Pout);
+            if (a.walls)
+                fused::k_fused<SP, float, true, false, false><<<nb, fused::NT, 0, s>>>(
+                    in, kind, out, g, (float)a.tau, (float)a.G, sh);
+            else
+                fused::k_fused<SP, float, false, false, false><<<nb, fused::NT, 0, s>>>(
+                    in, kind, out, g, (float)a.tau, (float)a.G, sh);
+        };
+        if (precision == LBM_FP32) go(StF32{});
+        else if (precision == LBM_FP16S) go(StF16{});
+        else go(StBF16{});
+        return hipGetLastError();
+    }
     LOWP_DISPATCH(precision, a, ({
         using T = typename SP::T;
         const T* in = static_cast<const T*>(Pin);

@@ -1,0 +1,147 @@
+#include "ScmpModel.hpp"
+
+#include <ctime>
+#include <stdexcept>
+#include <string>
+
+#include "VtkWriter.hpp"
+
+namespace mesobasic {
+
+namespace {
+
+int precision_of(const std::string& s)
+{
+    if (s == "fp64") return LBM_FP64;
+    if (s == "fp32") return LBM_FP32;
+    if (s == "fp16") return LBM_FP16S;
+    if (s == "bf16") return LBM_BF16S;
+    throw std::runtime_error("LBApp/precision must be fp64, fp32, fp16 or bf16 (got " + s + ")");
+}
+
+int kernel_of(const std::string& s)
+{
+    if (s == "auto") return LBM_KERNEL_AUTO;
+    if (s == "two_pass") return LBM_KERNEL_TWO_PASS;
+    if (s == "fused") return LBM_KERNEL_FUSED;
+    throw std::runtime_error("LBApp/kernel must be auto, two_pass or fused (got " + s + ")");
+}
+
+}  // namespace
+
+ScmpModel::ScmpModel(const CommonParams& pin, const InputParams& in, int q, int default_order)
+    : p(pin), q_(q)
+{
+    // scmp3D.cpp:21-31
+    G = in("LBApp/G", -6.0);
+    rhoAi = in("LBApp/rhoAi", 0.5);
+    rhoAi_noise = in("LBApp/rhoAi_noise", 0.1);
+    tauA = in("LBApp/tauA", 1.0);
+    const std::string stype = in("LBApp/stencil", q == 19 ? "D3Q19" : "D2Q9");
+    if ((q == 19 && stype != "D3Q19") || (q == 9 && stype != "D2Q9"))
+        throw std::runtime_error("LBApp/stencil " + stype + " does not match this model");
+    init_kind = in("LBApp/init", "noise");
+    seed = in.has("LBApp/seed") ? static_cast<unsigned>(in("LBApp/seed", 0.0))
+                                : static_cast<unsigned>(std::time(nullptr) * (p.rank + 1));
+    R = in("LBApp/R", 0.25 * p.NX);
+    W = in("LBApp/W", 2.0);
+    rho_l = in("LBApp/rho_l", 2.0);
+    rho_v = in("LBApp/rho_v", 0.15);
+
+    lbm_desc d{};
+    d.q = q;
+    d.nx = p.nx;
+    d.ny = p.ny;
+    d.nz = (q == 19) ? p.nz : 1;
+    d.tau = tauA;
+    d.G = G;
+    d.order = in("LBApp/order", default_order);
+    d.walls = in("LBApp/walls", 0);
+    d.precision = precision_of(in("LBApp/precision", "fp64"));
+    d.rho_shift = in("LBApp/rho_shift", rhoAi);
+    d.kernel = kernel_of(in("LBApp/kernel", "auto"));
+    if (q == 9 && p.nz != 1) throw std::runtime_error("scmp2D needs Domain/nz = 1");
+    check(lbm_create(&d, &h_), "lbm_create");
+}
+
+ScmpModel::~ScmpModel() { lbm_destroy(h_); }
+
+void ScmpModel::check(int rc, const char* what) const
+{
+    if (rc != LBM_OK) throw std::runtime_error(std::string(what) + ": " + lbm_last_error());
+}
+
+void ScmpModel::initLatticeBoltzmann()
+{
+    // scmp3D.cpp:54-82: initial density, zero velocity, ghost rho, force,
+    // f = feq.  The noise uses a seeded mt19937 in place of rand().
+    std::vector<double> rho(static_cast<size_t>(nodes()));
+    if (init_kind == "droplet") {
+        check(lbm_fill_droplet(p.nx, p.ny, q_ == 19 ? p.nz : 1, R, W, rho_l, rho_v, rho.data()),
+              "lbm_fill_droplet");
+    } else if (init_kind == "noise") {
+        check(lbm_fill_noise(seed, nodes(), rhoAi, rhoAi_noise, rho.data()), "lbm_fill_noise");
+    } else {
+        throw std::runtime_error("LBApp/init must be noise or droplet (got " + init_kind + ")");
+    }
+    check(lbm_set_rho(h_, rho.data()), "lbm_set_rho");
+    check(lbm_init_equilibrium(h_), "lbm_init_equilibrium");
+}
+
+void ScmpModel::updateLatticeBoltzmann()
+{
+    // scmp3D.cpp:90-119 (the Barrier of :97 has no counterpart on one GPU)
+    check(lbm_step(h_, 1), "lbm_step");
+}
+
+void ScmpModel::fields(std::vector<double>& rho, std::vector<double>& u, std::vector<double>& v,
+                       std::vector<double>& w)
+{
+    const size_t n = static_cast<size_t>(nodes());
+    rho.resize(n); u.resize(n); v.resize(n); w.resize(n);
+    check(lbm_get_fields(h_, LBM_FIELDS_REFERENCE, rho.data(), u.data(), v.data(), w.data()),
+          "lbm_get_fields");
+}
+
+void ScmpModel::force(std::vector<double>& fx, std::vector<double>& fy, std::vector<double>& fz)
+{
+    const size_t n = static_cast<size_t>(nodes());
+    fx.resize(n); fy.resize(n); fz.resize(n);
+    check(lbm_get_force(h_, fx.data(), fy.data(), fz.data()), "lbm_get_force");
+}
+
+void scmp3D::outputLatticeBoltzmann()
+{
+    // scmp3D.cpp:127-133
+    std::vector<double> rho, u, v, w;
+    fields(rho, u, v, w);
+    write_vtk_3d("vtkoutput", "rhoA", current_step, p.NX, p.NY, p.NZ, p.iskip, p.jskip, p.kskip, rho);
+}
+
+void scmp2D::outputLatticeBoltzmann()
+{
+    // as mcmp2D::outputLatticeBoltzmann (mcmp2D.cpp:131-137) with the
+    // Shan-Chen velocity u* = u + F/(2 rho) (LBfluid2D.cpp:164-172)
+    std::vector<double> rho, u, v, w, fx, fy, fz;
+    fields(rho, u, v, w);
+    force(fx, fy, fz);
+    std::vector<double> us(rho.size()), vs(rho.size());
+    for (size_t c = 0; c < rho.size(); c++) {
+        us[c] = u[c] + 0.5 * fx[c] / rho[c];
+        vs[c] = v[c] + 0.5 * fy[c] / rho[c];
+    }
+    write_vtk_2d("vtkoutput", "rhoA", current_step, p.NX, p.NY, p.iskip, p.jskip, rho, us, vs);
+}
+
+// LBBaseClass::LBFactory (LBBaseClass.cpp:21-45).  The reference's default
+// type is mcmp2D; the multi-component, film and capsule models are not part
+// of this build, so they (like unknown names) yield nullptr.
+LBBaseClass* LBBaseClass::LBFactory(const CommonParams& p, const InputParams& in)
+{
+    const std::string lb_type = in("LBApp/type", "mcmp2D");
+    if (lb_type == "scmp3D") return new scmp3D(p, in);
+    if (lb_type == "scmp2D") return new scmp2D(p, in);
+    return nullptr;
+}
+
+}  // namespace mesobasic

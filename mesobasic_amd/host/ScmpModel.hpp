@@ -1,0 +1,74 @@
+// ScmpModel.hpp -- single-component Shan-Chen model on the GPU, the common
+// part of scmp3D (D3Q19) and scmp2D (D2Q9).  The lattice lives in HBM behind
+// the C ABI (include/mesobasic_lbm.h); this class owns one handle.
+//
+// Keys (read as scmp3D.cpp:21-31 reads them, same defaults):
+//   LBApp/G (-6.0)  LBApp/rhoAi (0.5)  LBApp/rhoAi_noise (0.1)
+//   LBApp/tauA (1.0)  LBApp/stencil (D3Q19 for scmp3D, D2Q9 for scmp2D)
+// Keys of this build only (the reference ignores unknown keys):
+//   LBApp/seed       mt19937 seed of the noisy start (default: time(NULL),
+//                    the reference's srand(time(NULL)) at rank 0)
+//   LBApp/init       noise | droplet
+//   LBApp/R, W, rho_l, rho_v   droplet radius, interface width, densities
+//   LBApp/walls      1 = bounce-back walls on the last axis
+//   LBApp/precision  fp64 | fp32 | fp16 | bf16
+//   LBApp/rho_shift  shift of the 16-bit storage (default rhoAi)
+//   LBApp/kernel     auto | two_pass | fused
+//   LBApp/order      3 (scmp3D step order) | 2 (LBfluid2D step order)
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "../../include/mesobasic_lbm.h"
+#include "LBBaseClass.hpp"
+
+namespace mesobasic {
+
+class ScmpModel : public LBBaseClass {
+public:
+    ScmpModel(const CommonParams& p, const InputParams& in, int q, int default_order);
+    ~ScmpModel() override;
+    ScmpModel(const ScmpModel&) = delete;
+    ScmpModel& operator=(const ScmpModel&) = delete;
+
+    void setTimeStep(int step) override { current_step = step; }
+    void initLatticeBoltzmann() override;
+    void updateLatticeBoltzmann() override;
+
+    // Host copies of the reference's r,u,v,w arrays (interior, x slowest).
+    void fields(std::vector<double>& rho, std::vector<double>& u, std::vector<double>& v,
+                std::vector<double>& w);
+    void force(std::vector<double>& fx, std::vector<double>& fy, std::vector<double>& fz);
+    lbm_handle* handle() { return h_; }
+    long long nodes() const { return static_cast<long long>(p.nx) * p.ny * p.nz; }
+
+protected:
+    const CommonParams& p;
+    int q_;
+    int current_step = 0;
+    double G, rhoAi, rhoAi_noise, tauA;
+    std::string init_kind;
+    unsigned seed;
+    double R, W, rho_l, rho_v;
+    lbm_handle* h_ = nullptr;
+
+    void check(int rc, const char* what) const;
+};
+
+class scmp3D : public ScmpModel {
+public:
+    scmp3D(const CommonParams& p, const InputParams& in) : ScmpModel(p, in, 19, 3) {}
+    void outputLatticeBoltzmann() override;
+};
+
+// D2Q9 single-component Shan-Chen: the scmp3D force on the D2Q9 stencil in
+// the LBfluid2D step order (LBfluid2D.cpp:227-312).  The reference has no such
+// class; its factory type name here is "scmp2D".
+class scmp2D : public ScmpModel {
+public:
+    scmp2D(const CommonParams& p, const InputParams& in) : ScmpModel(p, in, 9, 2) {}
+    void outputLatticeBoltzmann() override;
+};
+
+}  // namespace mesobasic

@@ -1,0 +1,87 @@
+#include "VtkWriter.hpp"
+
+#include <fstream>
+#include <iomanip>
+#include <sstream>
+#include <stdexcept>
+
+namespace mesobasic {
+
+namespace {
+
+std::ofstream open_vtk(const std::string& dir, const std::string& tag, int step)
+{
+    std::stringstream name;
+    name << dir << "/" << tag << "_" << step << ".vtk";
+    std::ofstream out(name.str(), std::ios::out | std::ios::app);
+    if (!out) throw std::runtime_error("cannot open " + name.str());
+    return out;
+}
+
+}  // namespace
+
+void write_vtk_3d(const std::string& dir, const std::string& tag, int step, int NX, int NY,
+                  int NZ, int iskip, int jskip, int kskip, const std::vector<double>& rho)
+{
+    std::ofstream out = open_vtk(dir, tag, step);
+    const std::string d = "   ";
+    out << "# vtk DataFile Version 3.1" << std::endl;
+    out << "VTK file containing grid data" << std::endl;
+    out << "ASCII" << std::endl;
+    out << " " << std::endl;
+    out << "DATASET STRUCTURED_POINTS" << std::endl;
+    out << "DIMENSIONS" << d << NX / iskip << d << NY / jskip << d << NZ / kskip << std::endl;
+    out << "ORIGIN " << d << 0 << d << 0 << d << 0 << std::endl;
+    out << "SPACING" << d << 1.0 * iskip << d << 1.0 * jskip << d << 1.0 * kskip << std::endl;
+    out << " " << std::endl;
+    out << "POINT_DATA " << (NX / iskip) * (NY / jskip) * (NZ / kskip) << std::endl;
+    out << "SCALARS " << tag << " float" << std::endl;
+    out << "LOOKUP_TABLE default" << std::endl;
+    // loop order and selection of LBfluid3D.cpp:399-414 (one rank, xOffset 0)
+    for (int k = 1; k < NZ + 1; k += kskip)
+        for (int j = 1; j < NY + 1; j += jskip)
+            for (int i = 1; i < NX + 1; i++) {
+                if (i == 0 || i % iskip == 0) {
+                    const long long c = (static_cast<long long>(i - 1) * NY + (j - 1)) * NZ + (k - 1);
+                    out << std::fixed << std::setprecision(3) << rho[c] << std::endl;
+                }
+            }
+}
+
+void write_vtk_2d(const std::string& dir, const std::string& tag, int step, int NX, int NY,
+                  int iskip, int jskip, const std::vector<double>& rho,
+                  const std::vector<double>& ustar, const std::vector<double>& vstar)
+{
+    std::ofstream out = open_vtk(dir, tag, step);
+    const std::string d = "   ";
+    out << "# vtk DataFile Version 3.1" << std::endl;
+    out << "VTK file containing grid data" << std::endl;
+    out << "ASCII" << std::endl;
+    out << " " << std::endl;
+    out << "DATASET STRUCTURED_POINTS" << std::endl;
+    out << "DIMENSIONS" << d << NX / iskip << d << NY / jskip << d << 1 << std::endl;
+    out << "ORIGIN " << d << 1 << d << 1 << d << 0 << std::endl;
+    out << "SPACING" << d << 1.0 * iskip << d << 1.0 * jskip << d << 1.0 << std::endl;
+    out << " " << std::endl;
+    out << "POINT_DATA " << (NX / iskip) * (NY / jskip) << std::endl;
+    out << "SCALARS " << tag << " float" << std::endl;
+    out << "LOOKUP_TABLE default" << std::endl;
+    // LBfluid2D.cpp:440-453
+    for (int j = 1; j < NY + 1; j += jskip)
+        for (int i = 1; i < NX + 1; i++)
+            if (i == 0 || i % iskip == 0)
+                out << std::fixed << std::setprecision(3)
+                    << rho[static_cast<long long>(i - 1) * NY + (j - 1)] << std::endl;
+    // LBfluid2D.cpp:461-479
+    out << "   " << std::endl;
+    out << "VECTORS Velocity float" << std::endl;
+    for (int j = 1; j < NY + 1; j += jskip)
+        for (int i = 1; i < NX + 1; i++)
+            if (i == 0 || i % iskip == 0) {
+                const long long c = static_cast<long long>(i - 1) * NY + (j - 1);
+                out << std::fixed << std::setprecision(3) << ustar[c] << " " << vstar[c] << " "
+                    << 0.0 << std::endl;
+            }
+}
+
+}  // namespace mesobasic

@@ -1,0 +1,21 @@
+// VtkWriter.hpp -- legacy ASCII VTK files in the reference's exact format:
+// LBfluid3D::writeVTKFile (LBfluid3D.cpp:355-422) for 3-D density and
+// LBfluid2D::writeVTKFile (LBfluid2D.cpp:396-487) for 2-D density plus the
+// Shan-Chen velocity.  Values are printed fixed with 3 decimals, x fastest.
+#pragma once
+
+#include <string>
+#include <vector>
+
+namespace mesobasic {
+
+// rho: interior array (x slowest, z fastest) of NX*NY*NZ values.
+void write_vtk_3d(const std::string& dir, const std::string& tag, int step, int NX, int NY,
+                  int NZ, int iskip, int jskip, int kskip, const std::vector<double>& rho);
+
+// rho, ustar, vstar: interior arrays (x slowest) of NX*NY values.
+void write_vtk_2d(const std::string& dir, const std::string& tag, int step, int NX, int NY,
+                  int iskip, int jskip, const std::vector<double>& rho,
+                  const std::vector<double>& ustar, const std::vector<double>& vstar);
+
+}  // namespace mesobasic

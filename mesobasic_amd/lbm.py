@@ -56,6 +56,7 @@ def load():
         "lbm_step_timed": (ctypes.c_int, [P, ctypes.c_int, _D]),
         "lbm_get_fields": (ctypes.c_int, [P, ctypes.c_int, _D, _D, _D, _D]),
         "lbm_get_f": (ctypes.c_int, [P, _D]),
+        "lbm_get_force": (ctypes.c_int, [P, _D, _D, _D]),
         "lbm_set_f": (ctypes.c_int, [P, _D]),
         "lbm_conservation": (ctypes.c_int, [P, _D]),
         "lbm_steps_done": (ctypes.c_int64, [P]),
@@ -70,6 +71,7 @@ def load():
         "lbm_fill_droplet": (ctypes.c_int, [ctypes.c_int] * 3 + [ctypes.c_double] * 4 + [_D]),
         "lbm_abi_version": (ctypes.c_int, []),
         "lbm_probe_exp": (ctypes.c_int, [_D, _D, ctypes.c_int64]),
+        "lbm_probe_tile_map": (ctypes.c_int, [ctypes.c_int] * 3 + [ctypes.POINTER(ctypes.c_int)] * 3),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -116,6 +118,18 @@ def probe_exp(x):
     y = np.empty_like(x)
     _check(lib.lbm_probe_exp(_dptr(x), _dptr(y), x.size))
     return y
+
+
+def probe_tile_map(nty, ntz, ncx):
+    """Block -> (ty, tz, cx) map of the fused kernel (host evaluation)."""
+    lib = load()
+    nb = nty * ntz * ncx
+    out = [np.empty(nb, dtype=np.int32) for _ in range(3)]
+    P = ctypes.POINTER(ctypes.c_int)
+    rc = lib.lbm_probe_tile_map(nty, ntz, ncx, *(a.ctypes.data_as(P) for a in out))
+    if rc != 0:
+        raise ValueError("bad tile grid")
+    return out
 
 
 class Lattice:
@@ -196,6 +210,12 @@ class Lattice:
         _check(self._lib.lbm_get_fields(self._h, kind, _dptr(rho), _dptr(u), _dptr(v), _dptr(w)))
         sh = self.shape
         return dict(rho=rho.reshape(sh), u=u.reshape(sh), v=v.reshape(sh), w=w.reshape(sh))
+
+    def force(self):
+        fx, fy, fz = (np.empty(self.n) for _ in range(3))
+        _check(self._lib.lbm_get_force(self._h, _dptr(fx), _dptr(fy), _dptr(fz)))
+        sh = self.shape
+        return fx.reshape(sh), fy.reshape(sh), fz.reshape(sh)
 
     def get_f(self):
         f = np.empty(self.n * self.q)

@@ -72,8 +72,18 @@ CASES = [
 ]
 
 
+# lattices the fused kernel takes (ny % 16 == 0, nz % 32 == 0); nx = 70 spans
+# two full 32-plane chunks and a partial one
+FUSED_CASES = [
+    (19, (6, 16, 32), 1.0, -5.5, 3, False, 30),
+    (19, (5, 32, 32), 0.7, -5.0, 3, True, 30),
+    (19, (70, 16, 32), 1.0, -5.5, 3, False, 12),
+    (19, (3, 16, 64), 1.2, -5.5, 2, True, 20),
+]
+
+
 @pytest.mark.parametrize("kernel", KERNELS)
-@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("case", CASES + FUSED_CASES)
 def test_matches_oracle_bitwise(case, kernel):
     q, shape, tau, G, order, walls, steps = case
     print(f"case {case} kernel {kernel}", flush=True)
@@ -98,6 +108,21 @@ def test_matches_oracle_bitwise(case, kernel):
         for k in ("rho", "u", "v", "w"):
             assert same_bits(cur[k], mom[k]), k
     lat.close()
+
+
+@pytest.mark.parametrize("shape", [(40, 48, 64), (128, 32, 128)])
+def test_fused_equals_two_pass_bitwise(shape):
+    # (128, 32, 128): 2 x 4 tiles x 4 chunks, so the XCD-ordered tile map is used
+    rho0 = mb.droplet_rho(shape, 10.0)
+    a = mb.Lattice(19, shape, 1.0, -5.5, kernel=mb.KERNEL_TWO_PASS)
+    b = mb.Lattice(19, shape, 1.0, -5.5, kernel=mb.KERNEL_FUSED)
+    assert b.kernel == mb.KERNEL_FUSED
+    for lat in (a, b):
+        lat.set_rho(rho0); lat.init(); lat.step(25)
+    assert same_bits(a.get_f(), b.get_f())
+    fa, fb = a.fields(), b.fields()
+    for k in ("rho", "u", "v", "w"):
+        assert same_bits(fa[k], fb[k]), k
 
 
 def test_set_f_roundtrip_and_restart():

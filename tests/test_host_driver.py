@@ -1,0 +1,157 @@
+"""The C++ drop-in driver (mesobasic_amd/host): input.dat grammar on the CPU,
+and on the GPU a full run whose VTK output matches the oracle's fields to
+the reference writer's 3 decimals, in the reference's file format."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+MESO = os.path.join(ROOT, "mesobasic_amd", "host", "meso_amd")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def built():
+    if not os.path.exists(MESO):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "mesobasic_amd", "host")], check=True)
+
+
+def _params(tmp_path, text):
+    p = tmp_path / "input.dat"
+    p.write_text(text)
+    out = subprocess.run([MESO, "--print-params", str(p)], check=True, capture_output=True,
+                         text=True).stdout
+    kv = {}
+    secs = []
+    for line in out.splitlines():
+        if line.startswith("section "):
+            secs.append(line.split()[1])
+        elif " = " in line:
+            k, v = line.split(" = ", 1)
+            kv[k] = v
+    return secs, kv, out
+
+
+def test_getpot_grammar(tmp_path):
+    secs, kv, out = _params(tmp_path, """
+# leading comment
+[Domain]
+  nx = 64   # trailing comment
+  ny=32
+  nz =16
+[Time]
+  nstep = 2.0e3
+[LBApp]
+  app = true
+  G = -5.5
+  [./inner]
+    a = 1
+  [../other]
+    b = 2
+[]
+  root = 3
+[Domain]
+  nx = 65
+""")
+    assert secs == ["Domain/", "Time/", "LBApp/", "LBApp/inner/", "LBApp/other/"]
+    assert kv["Domain/nx"] == "65"          # a later assignment wins
+    assert kv["Domain/ny"] == "32" and kv["Domain/nz"] == "16"
+    assert kv["LBApp/inner/a"] == "1" and kv["LBApp/other/b"] == "2"
+    assert kv["root"] == "3"
+    assert "nstep 2000 numOutputs 1" in out  # ints convert through double
+
+
+def test_unknown_type_is_reported(tmp_path):
+    (tmp_path / "input.dat").write_text("[LBApp]\n app = true\n type = mcmp2D\n")
+    r = subprocess.run([MESO], cwd=tmp_path, capture_output=True, text=True)
+    assert r.returncode != 0
+
+
+def _read_vtk(path):
+    lines = open(path).read().splitlines()
+    hdr = lines[:12]
+    vals = []
+    i = 12
+    while i < len(lines) and lines[i].strip() and not lines[i].startswith("VECTORS"):
+        vals.append(float(lines[i]))
+        i += 1
+    return hdr, np.array(vals), lines
+
+
+@pytest.mark.gpu
+def test_scmp3d_run_matches_oracle(tmp_path):
+    nx, ny, nz, nstep = 12, 10, 8, 40
+    (tmp_path / "input.dat").write_text(f"""
+[Domain]
+  nx = {nx}
+  ny = {ny}
+  nz = {nz}
+[Time]
+  nstep = {nstep}
+[Output]
+  numOutputs = 2
+[LBApp]
+  app = true
+  type = scmp3D
+  G = -5.5
+  rhoAi = 0.7
+  rhoAi_noise = 0.02
+  tauA = 1.0
+  seed = 9
+""")
+    r = subprocess.run([MESO], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    files = sorted(os.listdir(tmp_path / "vtkoutput"))
+    assert files == ["rhoA_0.vtk", "rhoA_20.vtk", "rhoA_40.vtk"]
+    o = oracle.Oracle(19, (nx, ny, nz), 1.0, -5.5)
+    o.set_rho(oracle.noise_rho((nx, ny, nz), 9))
+    o.init()
+    o.step(nstep)
+    ref = o.fields()["rho"]          # the r array the reference would print
+    hdr, vals, _ = _read_vtk(tmp_path / "vtkoutput" / "rhoA_40.vtk")
+    assert hdr[5] == f"DIMENSIONS   {nx}   {ny}   {nz}"
+    assert hdr[9] == f"POINT_DATA {nx * ny * nz}"
+    expect = [f"{ref[i, j, k]:.3f}" for k in range(nz) for j in range(ny) for i in range(nx)]
+    got = open(tmp_path / "vtkoutput" / "rhoA_40.vtk").read().splitlines()[12:]
+    assert got == expect
+
+
+@pytest.mark.gpu
+def test_scmp2d_run_writes_rho_and_velocity(tmp_path):
+    nx, ny, nstep = 20, 16, 30
+    (tmp_path / "input.dat").write_text(f"""
+[Domain]
+  nx = {nx}
+  ny = {ny}
+[Time]
+  nstep = {nstep}
+[Output]
+  numOutputs = 1
+[LBApp]
+  app = true
+  type = scmp2D
+  stencil = D2Q9
+  G = -5.0
+  rhoAi = 0.7
+  rhoAi_noise = 0.02
+  seed = 5
+""")
+    r = subprocess.run([MESO], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    o = oracle.Oracle(9, (nx, ny), 1.0, -5.0)
+    o.set_rho(oracle.noise_rho((nx, ny), 5))
+    o.init()
+    o.step(nstep)
+    fl = o.fields()
+    fx, fy, _ = o.force()
+    lines = open(tmp_path / "vtkoutput" / f"rhoA_{nstep}.vtk").read().splitlines()
+    rho_lines = lines[12:12 + nx * ny]
+    assert rho_lines == [f"{fl['rho'][i, j]:.3f}" for j in range(ny) for i in range(nx)]
+    assert lines[12 + nx * ny + 1] == "VECTORS Velocity float"
+    us = fl["u"] + 0.5 * fx / fl["rho"]
+    vs = fl["v"] + 0.5 * fy / fl["rho"]
+    vel = lines[12 + nx * ny + 2:]
+    assert vel == [f"{us[i, j]:.3f} {vs[i, j]:.3f} 0.000" for j in range(ny) for i in range(nx)]
