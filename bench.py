@@ -89,7 +89,15 @@ def main():
     ap.add_argument("--workload", default="d3q19_fp64_256_droplet", choices=sorted(WORKLOADS))
     ap.add_argument("--kernel", default="auto", choices=["auto", "two_pass", "fused"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run the streaming copy kernel over MIB MiB (for counter scaling)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        nbytes = args.copy_calibration * 2**20
+        ms = mb.lbm.probe_copy(nbytes, reps=3)
+        print(json.dumps({"copy_bytes": nbytes, "ms": ms,
+                          "GBs_read_plus_write": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(the lattice would shard along x; see DESIGN.md).", file=sys.stderr)

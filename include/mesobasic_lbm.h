@@ -139,6 +139,14 @@ int lbm_set_stream(lbm_handle* h, void* stream);
 void* lbm_get_stream(const lbm_handle* h);
 int lbm_synchronize(lbm_handle* h);
 
+/* Launch shape of the fused kernel: tile shape id (0: 8x64, 1: 16x32,
+ * 2: 4x64, 3: 8x32 nodes of the y-z cross-section; -1 = default) and x
+ * planes per workgroup.  Results do not depend on it (the FP64 path stays
+ * bit-identical); only speed does.  No reference counterpart: the
+ * reference has no launch shapes. */
+int lbm_set_tuning(lbm_handle* h, int shape, int chunk);
+int lbm_get_tuning(const lbm_handle* h, int* shape, int* chunk);
+
 /* Device bytes held by the handle, and the kernel it resolved to. */
 int64_t lbm_device_bytes(const lbm_handle* h);
 int lbm_kernel_in_use(const lbm_handle* h);

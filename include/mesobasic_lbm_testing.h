@@ -20,6 +20,11 @@ int lbm_probe_exp(const double* x, double* y, int64_t n);
  * chunk cx[b].  Arrays of nty*ntz*ncx ints.  No device work. */
 int lbm_probe_tile_map(int nty, int ntz, int ncx, int* ty, int* tz, int* cx);
 
+/* Streaming copy kernel over `bytes` (multiple of 16) between two device
+ * buffers it allocates, run `reps` times; *ms = average time per copy.  The
+ * reference point for scaling HBM traffic counters. */
+int lbm_probe_copy(int64_t bytes, int reps, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

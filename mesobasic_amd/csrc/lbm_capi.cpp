@@ -13,6 +13,7 @@
 #include <string>
 #include <vector>
 
+#include "lbm_fused.h"
 #include "lbm_internal.h"
 #include "lbm_lowp.h"
 
@@ -73,6 +74,8 @@ struct lbm_handle {
     bool prev_valid = false;         // buf[1-cur] holds f(N-1)
     bool ref_is_rho0 = false;        // reference arrays still hold rho0, u = 0
     int kernel = LBM_KERNEL_TWO_PASS;
+    int fused_shape = -1;            // tile shape of the fused kernel
+    int fused_chunk = 64;            // x planes per workgroup of the fused kernel
     int64_t bytes = 0;
 };
 
@@ -99,7 +102,8 @@ int run_step(lbm_handle* h)
         auto* Pout = static_cast<double*>(h->buf[dst]);
         const int mode = h->first_step_stored ? 1 : 0;
         if (h->kernel == LBM_KERNEL_FUSED && mode == 0) {
-            HIP_OK(lbm::fp64_fused_step(h->a, h->g, Pin, h->kind[src], Pout, h->stream));
+            HIP_OK(lbm::fp64_fused_step(h->a, h->g, Pin, h->kind[src], Pout, h->fused_shape,
+                                        h->fused_chunk, h->stream));
         } else {
             if (!h->psi) return fail(LBM_ESTATE, "psi field not allocated");
             if (mode == 1) HIP_OK(lbm::fp64_psi_from_rho(h->rho0, h->psi, h->g.n, h->stream));
@@ -111,7 +115,8 @@ int run_step(lbm_handle* h)
         if (!h->psi) return fail(LBM_ESTATE, "psi field not allocated");
         HIP_OK(lbm::lowp_step(h->d.precision, h->a, h->g, h->d.rho_shift, h->buf[src],
                               h->kind[src], h->buf[dst], h->first_step_stored ? h->rho0 : nullptr,
-                              reinterpret_cast<float*>(h->psi), h->kernel, h->stream));
+                              reinterpret_cast<float*>(h->psi), h->kernel, h->fused_shape,
+                              h->fused_chunk, h->stream));
     }
     h->kind[dst] = lbm::kHoldsPost;
     h->cur = dst;
@@ -206,6 +211,7 @@ int lbm_create(const lbm_desc* desc, lbm_handle** out)
             return fail(LBM_EINVAL, "requested kernel does not support this lattice/precision");
         }
     }
+    h->fused_shape = lbm::fused::default_shape(h->g, desc->q);
     const int64_t pop_bytes = h->g.ps * desc->q * h->elem;
     auto cleanup = [&](int code) {
         lbm_destroy(h);
@@ -431,6 +437,28 @@ int lbm_conservation(lbm_handle* h, double out[4])
 }
 
 int64_t lbm_steps_done(const lbm_handle* h) { return h ? h->steps : -1; }
+
+int lbm_set_tuning(lbm_handle* h, int shape, int chunk)
+{
+    if (!h) return fail(LBM_EINVAL, "lbm_set_tuning: NULL handle");
+    if (h->kernel != LBM_KERNEL_FUSED) return fail(LBM_EINVAL, "lbm_set_tuning: handle does not use the fused kernel");
+    if (shape < 0) shape = lbm::fused::default_shape(h->g, h->d.q);
+    if (!lbm::fused::supported(h->g, h->d.q, shape))
+        return fail(LBM_EINVAL, "tile shape %d does not divide the %d x %d cross-section", shape,
+                    h->d.ny, h->d.nz);
+    if (chunk < 1) return fail(LBM_EINVAL, "chunk must be >= 1");
+    h->fused_shape = shape;
+    h->fused_chunk = chunk;
+    return LBM_OK;
+}
+
+int lbm_get_tuning(const lbm_handle* h, int* shape, int* chunk)
+{
+    if (!h) return fail(LBM_EINVAL, "lbm_get_tuning: NULL handle");
+    if (shape) *shape = h->fused_shape;
+    if (chunk) *chunk = h->fused_chunk;
+    return LBM_OK;
+}
 
 int lbm_set_stream(lbm_handle* h, void* stream)
 {

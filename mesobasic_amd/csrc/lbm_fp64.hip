@@ -26,6 +26,8 @@
 #include "lbm_internal.h"
 #include "lbm_stencil.h"
 
+#include <algorithm>
+
 namespace lbm {
 namespace {
 
@@ -436,5 +438,59 @@ extern "C" int lbm_probe_exp(const double* x, double* y, int64_t n)
     }
     if (!rc && hipMemcpy(y, d + n, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) rc = -3;
     (void)hipFree(d);
+    return rc;
+}
+
+// Plain streaming copy of `bytes` (a multiple of 16) between two fresh
+// device buffers, `reps` times: the reference point that scales HBM traffic
+// counters (FETCH_SIZE / WRITE_SIZE) to bytes.
+namespace {
+// Each thread moves 4 x 16 B with all four loads issued before any store.
+__global__ void __launch_bounds__(256) k_probe_copy(const uint4* __restrict__ src,
+                                                    uint4* __restrict__ dst, int64_t n16)
+{
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * 1024 + threadIdx.x;
+    uint4 v[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int64_t i = base + k * 256;
+        if (i < n16) v[k] = src[i];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int64_t i = base + k * 256;
+        if (i < n16) dst[i] = v[k];
+    }
+}
+}  // namespace
+
+extern "C" int lbm_probe_copy(int64_t bytes, int reps, double* ms)
+{
+    if (bytes <= 0 || bytes % 16 != 0 || reps < 1) return -1;
+    void* a = nullptr;
+    void* b = nullptr;
+    if (hipMalloc(&a, bytes) != hipSuccess) return -2;
+    if (hipMalloc(&b, bytes) != hipSuccess) { (void)hipFree(a); return -2; }
+    int rc = 0;
+    if (hipMemset(a, 1, bytes) != hipSuccess) rc = -3;
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const int64_t n16 = bytes / 16;
+    const unsigned grid = static_cast<unsigned>((n16 + 1023) / 1024);
+    if (!rc) {
+        (void)hipEventRecord(e0, nullptr);
+        for (int r = 0; r < reps; r++)
+            k_probe_copy<<<grid, 256>>>(static_cast<const uint4*>(a), static_cast<uint4*>(b), n16);
+        (void)hipEventRecord(e1, nullptr);
+        if (hipEventSynchronize(e1) != hipSuccess) rc = -3;
+        float t = 0.f;
+        (void)hipEventElapsedTime(&t, e0, e1);
+        if (ms) *ms = t / reps;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
     return rc;
 }

@@ -8,21 +8,46 @@
 
 namespace lbm {
 
-bool fp64_fused_supported(const StepArgs& a, const Grid& g) { return fused::supported(g, a.q); }
+bool fp64_fused_supported(const StepArgs& a, const Grid& g) { return fused::default_shape(g, a.q) >= 0; }
+
+namespace {
+
+template <bool W, bool T1, typename TL>
+void launch(const double* Pin, int kind, double* Pout, const Grid& g, const StepArgs& a, int chunk,
+            hipStream_t s)
+{
+    const int ncx = (g.nx + chunk - 1) / chunk;
+    const unsigned nb = static_cast<unsigned>((g.ny / TL::TY) * (g.nz / TL::TZ) * ncx);
+    fused::k_fused<StF64, double, W, true, T1, TL><<<nb, TL::NT, 0, s>>>(Pin, kind, Pout, g, a.tau,
+                                                                          a.G, 0.0, chunk);
+}
+
+template <bool W, bool T1>
+void launch_shape(int shape, const double* Pin, int kind, double* Pout, const Grid& g,
+                  const StepArgs& a, int chunk, hipStream_t s)
+{
+    using namespace fused;
+    switch (shape) {
+    case 0: launch<W, T1, Tile<kShapeTY[0], kShapeTZ[0]>>(Pin, kind, Pout, g, a, chunk, s); break;
+    case 1: launch<W, T1, Tile<kShapeTY[1], kShapeTZ[1]>>(Pin, kind, Pout, g, a, chunk, s); break;
+    case 2: launch<W, T1, Tile<kShapeTY[2], kShapeTZ[2]>>(Pin, kind, Pout, g, a, chunk, s); break;
+    default: launch<W, T1, Tile<kShapeTY[3], kShapeTZ[3]>>(Pin, kind, Pout, g, a, chunk, s); break;
+    }
+}
+
+}  // namespace
 
 hipError_t fp64_fused_step(const StepArgs& a, const Grid& g, const double* Pin, int kind,
-                           double* Pout, hipStream_t s)
+                           double* Pout, int shape, int chunk, hipStream_t s)
 {
-    if (!fused::supported(g, a.q)) return hipErrorNotSupported;
-    const int ncx = (g.nx + fused::kChunkX - 1) / fused::kChunkX;
-    const unsigned nb = static_cast<unsigned>((g.ny / fused::TY) * (g.nz / fused::TZ) * ncx);
-    const bool tau1 = (a.tau == 1.0);
+    if (!fused::supported(g, a.q, shape) || chunk < 1) return hipErrorNotSupported;
+    const bool tau1 = (a.tau == 1.0);   // x / 1.0 == x: skip 19 divides, same bits
     if (a.walls) {
-        if (tau1) fused::k_fused<StF64, double, true, true, true><<<nb, fused::NT, 0, s>>>(Pin, kind, Pout, g, a.tau, a.G, 0.0);
-        else fused::k_fused<StF64, double, true, true, false><<<nb, fused::NT, 0, s>>>(Pin, kind, Pout, g, a.tau, a.G, 0.0);
+        if (tau1) launch_shape<true, true>(shape, Pin, kind, Pout, g, a, chunk, s);
+        else launch_shape<true, false>(shape, Pin, kind, Pout, g, a, chunk, s);
     } else {
-        if (tau1) fused::k_fused<StF64, double, false, true, true><<<nb, fused::NT, 0, s>>>(Pin, kind, Pout, g, a.tau, a.G, 0.0);
-        else fused::k_fused<StF64, double, false, true, false><<<nb, fused::NT, 0, s>>>(Pin, kind, Pout, g, a.tau, a.G, 0.0);
+        if (tau1) launch_shape<false, true>(shape, Pin, kind, Pout, g, a, chunk, s);
+        else launch_shape<false, false>(shape, Pin, kind, Pout, g, a, chunk, s);
     }
     return hipGetLastError();
 }

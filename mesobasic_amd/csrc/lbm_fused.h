@@ -32,39 +32,59 @@
 namespace lbm {
 namespace fused {
 
-constexpr int TY = 16;           // tile rows (y)
-constexpr int TZ = 32;           // tile columns (z, contiguous)
-constexpr int NT = TY * TZ;      // threads per workgroup: one per tile node
-constexpr int LY = TY + 2;       // LDS plane rows incl. ring
-constexpr int LZ = TZ + 2;       // LDS plane columns incl. ring
-constexpr int NRING = 2 * LZ + 2 * TY;  // ring nodes per plane
-constexpr int kChunkX = 32;      // planes per workgroup march
+// Tile of TY rows (y) by TZ columns (z, contiguous); one thread per tile
+// node.  LDS holds psi planes of (TY+2) x (TZ+2) including the ring.
+template <int TY_, int TZ_> struct Tile {
+    static constexpr int TY = TY_;
+    static constexpr int TZ = TZ_;
+    static constexpr int NT = TY * TZ;
+    static constexpr int LY = TY + 2;
+    static constexpr int LZ = TZ + 2;
+    static constexpr int NRING = 2 * LZ + 2 * TY;
+    static_assert(NRING <= NT, "ring must fit in one pass of the workgroup");
+    static_assert(NT % 64 == 0 && NT <= 1024, "whole wavefronts");
+};
 
-static_assert(NRING <= NT, "ring must fit in one pass of the workgroup");
+// Tile shapes instantiated for the launcher and the tuner (index = id).
+constexpr int kNumShapes = 4;
+constexpr int kShapeTY[kNumShapes] = {8, 16, 4, 8};
+constexpr int kShapeTZ[kNumShapes] = {64, 32, 64, 32};
 
-inline bool supported(const Grid& g, int q)
+inline bool supported(const Grid& g, int q, int shape)
 {
-    return q == 19 && g.ny % TY == 0 && g.nz % TZ == 0 && g.nx >= 1 && g.ny >= TY && g.nz >= TZ;
+    if (q != 19 || shape < 0 || shape >= kNumShapes) return false;
+    const int ty = kShapeTY[shape], tz = kShapeTZ[shape];
+    return g.ny % ty == 0 && g.nz % tz == 0 && g.nx >= 1;
+}
+
+// Default tile shape for a lattice: the first (largest-tile) shape that
+// divides it, or -1.
+inline int default_shape(const Grid& g, int q)
+{
+    for (int s = 0; s < kNumShapes; s++)
+        if (supported(g, q, s)) return s;
+    return -1;
 }
 
 // Ring slot r -> (row, col) in LDS coordinates (row 0 and LY-1 are the
 // y-ring rows, col 0 and LZ-1 the z-ring columns).
-__device__ __forceinline__ void ring_slot(int r, int& row, int& col)
+template <typename TL> __device__ __forceinline__ void ring_slot(int r, int& row, int& col)
 {
-    if (r < LZ) { row = 0; col = r; }
-    else if (r < 2 * LZ) { row = LY - 1; col = r - LZ; }
-    else if (r < 2 * LZ + TY) { row = 1 + (r - 2 * LZ); col = 0; }
-    else { row = 1 + (r - 2 * LZ - TY); col = LZ - 1; }
+    if (r < TL::LZ) { row = 0; col = r; }
+    else if (r < 2 * TL::LZ) { row = TL::LY - 1; col = r - TL::LZ; }
+    else if (r < 2 * TL::LZ + TL::TY) { row = 1 + (r - 2 * TL::LZ); col = 0; }
+    else { row = 1 + (r - 2 * TL::LZ - TL::TY); col = TL::LZ - 1; }
 }
 
 __device__ __forceinline__ int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
-// Pull f for node (x, y, z) (all coordinates already wrapped into range).
-// Offsets are 32-bit node indices added to per-population base pointers that
+// Pull the stored values for node (x, y, z) (coordinates already wrapped
+// into range): f itself in the FP64/FP32 formats, d = f - w*rho_shift in the
+// 16-bit ones.  Offsets are 32-bit node indices added to per-population base pointers that
 // are uniform across the wavefront, so each gather needs one VGPR of address.
 template <typename SP, typename CT, bool WALLS>
 __device__ __forceinline__ void pull(const typename SP::T* __restrict__ P, const Grid& g, int kind,
-                                     int x, int y, int z, const CT (&shift)[19], CT (&f)[19])
+                                     int x, int y, int z, CT (&f)[19])
 {
     using S = St<19>;
     const uint32_t ny = static_cast<uint32_t>(g.ny), nz = static_cast<uint32_t>(g.nz);
@@ -73,7 +93,7 @@ __device__ __forceinline__ void pull(const typename SP::T* __restrict__ P, const
     if (kind == kHoldsF) {
         static_for<19>([&](auto I) {
             constexpr int n = decltype(I)::value;
-            f[n] = SP::load(P[n * g.ps + c], shift[n]);
+            f[n] = SP::d(P[n * g.ps + c]);
         });
         return;
     }
@@ -97,19 +117,19 @@ __device__ __forceinline__ void pull(const typename SP::T* __restrict__ P, const
         if constexpr (S::ez[n] == 1) o += dzm;
         if constexpr (S::ez[n] == -1) o += dzp;
         const typename SP::T* base = P + n * g.ps;
-        f[n] = SP::load(base[o], shift[n]);
+        f[n] = SP::d(base[o]);
     });
     if constexpr (WALLS) {
         if (z == 1) {
             static_for<S::nwall>([&](auto I) {
                 constexpr int m = decltype(I)::value;
-                f[S::wall_up[m]] = SP::load(P[S::wall_dn[m] * g.ps + c], shift[S::wall_dn[m]]);
+                f[S::wall_up[m]] = SP::d(P[S::wall_dn[m] * g.ps + c]);
             });
         }
         if (z == g.nz - 2) {
             static_for<S::nwall>([&](auto I) {
                 constexpr int m = decltype(I)::value;
-                f[S::wall_dn[m]] = SP::load(P[S::wall_up[m] * g.ps + c], shift[S::wall_up[m]]);
+                f[S::wall_dn[m]] = SP::d(P[S::wall_up[m] * g.ps + c]);
             });
         }
     }
@@ -124,22 +144,30 @@ template <> struct Psi<float> {
     static __device__ __forceinline__ float of(float rho) { return 1.0f - __expf(-rho); }
 };
 
-template <typename CT>
-__device__ __forceinline__ CT rho_of(const CT (&f)[19])
+// rho of pulled values: sum f (reference order), or rs + sum d.
+template <bool EXACT, typename CT>
+__device__ __forceinline__ CT rho_of(const CT (&f)[19], CT rs)
 {
     CT r = CT(0);
     static_for<19>([&](auto I) { r = r + f[decltype(I)::value]; });
+    if constexpr (!EXACT) r = r + rs;
     return r;
 }
 
-// Collision of one node from its pulled f and the LDS psi window.  Writes
-// the post-collision values at node c.
-template <typename SP, typename CT, bool EXACT, bool TAU1>
+// Collision of one node from its pulled values and the LDS psi window;
+// writes the post-collision values at node c.
+//
+// EXACT (FP64): the reference's formulas on f, operation for operation.
+// Otherwise the shifted frame d = f - w*rs (rs = rho_shift, 0 for FP32
+// storage): rho = rs + sum d, j = sum d e (sum w e = 0), and the collision
+// d' = d - (d - d_eq)/tau with d_eq = feq - w*rs = w*(rho*poly - rs), so no
+// large f values are ever formed in FP32.
+template <typename SP, typename CT, bool EXACT, bool TAU1, int LZ>
 __device__ __forceinline__ void collide_store(typename SP::T* __restrict__ Pout, const Grid& g,
-                                              int64_t c, const CT (&f)[19],
+                                              uint32_t c, const CT (&f)[19],
                                               const CT* __restrict__ pm, const CT* __restrict__ p0,
                                               const CT* __restrict__ pp, int li, CT tau, CT G,
-                                              const CT (&shift)[19])
+                                              CT rs)
 {
     using S = St<19>;
     // moments, LBfluid3D.cpp:223-233
@@ -151,6 +179,7 @@ __device__ __forceinline__ void collide_store(typename SP::T* __restrict__ Pout,
         acc_e<S::ey[n]>(sy, f[n]);
         acc_e<S::ez[n]>(sz, f[n]);
     });
+    if constexpr (!EXACT) rho = rho + rs;
     CT u, v, w, irho = CT(0);
     if constexpr (EXACT) {
         u = sx / rho; v = sy / rho; w = sz / rho;
@@ -187,12 +216,17 @@ __device__ __forceinline__ void collide_store(typename SP::T* __restrict__ Pout,
     static_for<19>([&](auto I) {
         constexpr int n = decltype(I)::value;
         const CT ev = edot<S::ex[n], S::ey[n], S::ez[n]>(ueq, veq, weq);
-        const CT feq = (((CT(1) + CT(3) * ev) + (CT(4.5) * ev) * ev) - usq15) * (rho * static_cast<CT>(S::w[n]));
+        const CT poly = ((CT(1) + CT(3) * ev) + (CT(4.5) * ev) * ev) - usq15;
         CT out;
-        if constexpr (TAU1) out = f[n] - (f[n] - feq);        // x / 1.0 == x exactly
-        else if constexpr (EXACT) out = f[n] - (f[n] - feq) / tau;
-        else out = f[n] - (f[n] - feq) * itau;
-        Pout[n * g.ps + c] = SP::store(out, shift[n]);
+        if constexpr (EXACT) {
+            const CT feq = poly * (rho * static_cast<CT>(S::w[n]));
+            if constexpr (TAU1) out = f[n] - (f[n] - feq);   // x / 1.0 == x exactly
+            else out = f[n] - (f[n] - feq) / tau;
+        } else {
+            const CT deq = static_cast<CT>(S::w[n]) * (rho * poly - rs);
+            out = f[n] - (f[n] - deq) * itau;
+        }
+        Pout[n * g.ps + c] = SP::of_d(out);
     });
 }
 
@@ -223,74 +257,100 @@ __host__ __device__ inline void tile_of_block(int b, int nty, int ntz, int ncx, 
     cx = t / ntz;
 }
 
-template <typename SP, typename CT, bool WALLS, bool EXACT, bool TAU1>
-__global__ void __launch_bounds__(NT, 1)
+// Plane pipeline of one workgroup.  At march step x the workgroup
+//   1. issues the pulls of plane x+2 for its tile nodes into registers,
+//   2. collides plane x from the registers pulled two steps earlier, with
+//      psi of planes x-1, x, x+1 from LDS, and stores it,
+//   3. pulls the ring nodes of plane x+2, sums rho of tile and ring and
+//      writes psi(x+2) into LDS,
+//   4. waits at the one barrier of the step.
+// The loads of step 1 are in flight during the collision of step 2; the
+// ring pulls come after the collision so that their registers are not live
+// at the same time as the collision's.  Four
+// LDS psi slots rotate so that step 3 overwrites the slot of plane x-2,
+// which no wavefront reads after the previous barrier.
+// Occupancy target: FP64 keeps three planes of 19 doubles in registers
+// (2 wavefronts per SIMD); the FP32-compute forms fit 4.
+template <typename CT> struct Occ { static constexpr int waves = 2; };
+template <> struct Occ<float> { static constexpr int waves = 4; };
+
+template <typename SP, typename CT, bool WALLS, bool EXACT, bool TAU1, typename TL>
+__global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(Occ<CT>::waves)))
     k_fused(const typename SP::T* __restrict__ Pin, int kind, typename SP::T* __restrict__ Pout,
-            Grid g, CT tau, CT G, CT rho_shift)
+            Grid g, CT tau, CT G, CT rho_shift, int chunk)
 {
-    __shared__ CT psi_lds[3][LY * LZ];
+    constexpr int TY = TL::TY, TZ = TL::TZ, LZ = TL::LZ, PL = TL::LY * TL::LZ;
+    __shared__ CT psi_lds[4 * PL];
 
     const int nty = g.ny / TY, ntz = g.nz / TZ;
-    const int ncx = (g.nx + kChunkX - 1) / kChunkX;
+    const int ncx = (g.nx + chunk - 1) / chunk;
     int ty, tz, cx;
     tile_of_block(blockIdx.x, nty, ntz, ncx, ty, tz, cx);
     const int y0 = ty * TY, z0 = tz * TZ;
-    const int x0 = cx * kChunkX;
-    const int x1 = min(g.nx, x0 + kChunkX);
+    const int x0 = cx * chunk;
+    const int x1 = min(g.nx, x0 + chunk);
 
     const int tid = threadIdx.x;
     const int ly = tid / TZ, lz = tid - (tid / TZ) * TZ;  // tile node of this thread
     const int yi = y0 + ly, zi = z0 + lz;
     const int li = (ly + 1) * LZ + (lz + 1);              // its LDS slot
 
-    // ring node of this thread (if any)
-    int rrow = 0, rcol = 0, ry = 0, rz = 0;
-    const bool has_ring = tid < NRING;
+    int rrow = 0, rcol = 0, ry = 0, rz = 0;               // ring node of this thread
+    const bool has_ring = tid < TL::NRING;
     if (has_ring) {
-        ring_slot(tid, rrow, rcol);
+        ring_slot<TL>(tid, rrow, rcol);
         ry = wrap(y0 + rrow - 1, g.ny);
         rz = wrap(z0 + rcol - 1, g.nz);
     }
+    const int lr = rrow * LZ + rcol;
 
-    CT shift[19];
-    static_for<19>([&](auto I) {
-        constexpr int n = decltype(I)::value;
-        shift[n] = static_cast<CT>(St<19>::w[n]) * rho_shift;
-    });
-
-    // psi of plane xp (tile + ring) into LDS slot s; the tile's f into fdst.
-    auto load_plane = [&](int xp, int s, CT (&fdst)[19]) {
-        if (has_ring) {  // ring first: its registers are free again before fdst fills
+    // LDS slot of plane x: (x - x0 + 1) mod 4
+    auto slot = [&](int x) { return ((x - x0 + 1) & 3) * PL; };
+    auto xw = [&](int x) { return x >= g.nx ? x - g.nx : (x < 0 ? x + g.nx : x); };
+    auto psi_of = [&](const CT (&v)[19]) { return Psi<CT>::of(rho_of<EXACT>(v, rho_shift)); };
+    auto ring_psi = [&](int xp, int s) {
+        if (has_ring) {
             CT fr[19];
-            pull<SP, CT, WALLS>(Pin, g, kind, xp, ry, rz, shift, fr);
-            psi_lds[s][rrow * LZ + rcol] = Psi<CT>::of(rho_of(fr));
+            pull<SP, CT, WALLS>(Pin, g, kind, xp, ry, rz, fr);
+            psi_lds[s + lr] = psi_of(fr);
         }
-        pull<SP, CT, WALLS>(Pin, g, kind, xp, yi, zi, shift, fdst);
-        psi_lds[s][li] = Psi<CT>::of(rho_of(fdst));
     };
 
-    CT fa[19], fb[19];
-    // warm-up: planes x0-1 and x0
-    load_plane(wrap(x0 - 1, g.nx), 0, fb);
-    load_plane(x0, 1, fa);
+    CT fa[19], fb[19], fc[19];
+    // prologue: psi of planes x0-1, x0, x0+1; values of x0 (fa) and x0+1 (fb)
+    ring_psi(xw(x0 - 1), slot(x0 - 1));
+    pull<SP, CT, WALLS>(Pin, g, kind, xw(x0 - 1), yi, zi, fc);
+    psi_lds[slot(x0 - 1) + li] = psi_of(fc);
+    ring_psi(x0, slot(x0));
+    pull<SP, CT, WALLS>(Pin, g, kind, x0, yi, zi, fa);
+    psi_lds[slot(x0) + li] = psi_of(fa);
+    ring_psi(xw(x0 + 1), slot(x0 + 1));
+    pull<SP, CT, WALLS>(Pin, g, kind, xw(x0 + 1), yi, zi, fb);
+    psi_lds[slot(x0 + 1) + li] = psi_of(fb);
     __syncthreads();
 
-    int sm = 0, s0 = 1, sp = 2;  // LDS slots of planes x-1, x, x+1
-    auto step = [&](int x, CT (&fc)[19], CT (&fn)[19]) {
-        load_plane(x + 1 == g.nx ? 0 : x + 1, sp, fn);
+    auto step = [&](int x, CT (&fcur)[19], CT (&fnn)[19]) {
+        const int x2 = xw(x + 2);
+        pull<SP, CT, WALLS>(Pin, g, kind, x2, yi, zi, fnn);   // in flight during the collision
+        const uint32_t c = (static_cast<uint32_t>(x) * static_cast<uint32_t>(g.ny) +
+                            static_cast<uint32_t>(yi)) * static_cast<uint32_t>(g.nz) +
+                           static_cast<uint32_t>(zi);
+        collide_store<SP, CT, EXACT, TAU1, LZ>(Pout, g, c, fcur, psi_lds + slot(x - 1),
+                                               psi_lds + slot(x), psi_lds + slot(x + 1), li, tau, G,
+                                               rho_shift);
+        ring_psi(x2, slot(x + 2));
+        psi_lds[slot(x + 2) + li] = psi_of(fnn);
         __syncthreads();
-        const int64_t c = (static_cast<int64_t>(x) * g.ny + yi) * g.nz + zi;
-        collide_store<SP, CT, EXACT, TAU1>(Pout, g, c, fc, psi_lds[sm], psi_lds[s0], psi_lds[sp], li,
-                                           tau, G, shift);
-        __syncthreads();
-        const int t = sm; sm = s0; s0 = sp; sp = t;
     };
+    // three register sets rotate: (collide, next, pulled) = (a,b,c) -> (b,c,a) -> (c,a,b)
     int x = x0;
-    for (; x + 1 < x1; x += 2) {
-        step(x, fa, fb);
+    for (; x + 2 < x1; x += 3) {
+        step(x, fa, fc);
         step(x + 1, fb, fa);
+        step(x + 2, fc, fb);
     }
-    if (x < x1) step(x, fa, fb);
+    if (x < x1) step(x, fa, fc);
+    if (x + 1 < x1) step(x + 1, fb, fa);
 }
 
 }  // namespace fused

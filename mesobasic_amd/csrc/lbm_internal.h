@@ -41,8 +41,9 @@ hipError_t fp64_collide(const StepArgs& a, const Grid& g, const double* Pin, int
                         hipStream_t s);
 // One fused pass (psi halo recomputed on chip).  Returns hipErrorNotSupported
 // when the geometry does not suit it.
+// shape: tile shape id (fused::kShapeTY/TZ); chunk: x planes per workgroup.
 hipError_t fp64_fused_step(const StepArgs& a, const Grid& g, const double* Pin, int kind,
-                           double* Pout, hipStream_t s);
+                           double* Pout, int shape, int chunk, hipStream_t s);
 bool fp64_fused_supported(const StepArgs& a, const Grid& g);
 // Shan-Chen force from a psi field (FP64, reference arithmetic).
 hipError_t fp64_force(const StepArgs& a, const Grid& g, const double* psi, double* fx, double* fy,

@@ -312,11 +312,40 @@ inline unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + kBlock
 
 int lowp_resolve_kernel(int, const StepArgs& a, const Grid& g, int requested)
 {
-    const bool fused_ok = fused::supported(g, a.q);
+    const bool fused_ok = fused::default_shape(g, a.q) >= 0;
     if (requested == LBM_KERNEL_FUSED) return fused_ok ? LBM_KERNEL_FUSED : -1;
     if (requested == LBM_KERNEL_TWO_PASS) return LBM_KERNEL_TWO_PASS;
     return fused_ok ? LBM_KERNEL_FUSED : LBM_KERNEL_TWO_PASS;
 }
+
+namespace {
+
+template <typename SP, bool W, typename TL>
+void launch_fused(const void* Pin, int kind, void* Pout, const Grid& g, const StepArgs& a, float sh,
+                  int chunk, hipStream_t s)
+{
+    using T = typename SP::T;
+    const int ncx = (g.nx + chunk - 1) / chunk;
+    const unsigned nb = static_cast<unsigned>((g.ny / TL::TY) * (g.nz / TL::TZ) * ncx);
+    fused::k_fused<SP, float, W, false, false, TL><<<nb, TL::NT, 0, s>>>(
+        static_cast<const T*>(Pin), kind, static_cast<T*>(Pout), g, (float)a.tau, (float)a.G, sh,
+        chunk);
+}
+
+template <typename SP, bool W>
+void launch_fused_shape(int shape, const void* Pin, int kind, void* Pout, const Grid& g,
+                        const StepArgs& a, float sh, int chunk, hipStream_t s)
+{
+    using namespace fused;
+    switch (shape) {
+    case 0: launch_fused<SP, W, Tile<kShapeTY[0], kShapeTZ[0]>>(Pin, kind, Pout, g, a, sh, chunk, s); break;
+    case 1: launch_fused<SP, W, Tile<kShapeTY[1], kShapeTZ[1]>>(Pin, kind, Pout, g, a, sh, chunk, s); break;
+    case 2: launch_fused<SP, W, Tile<kShapeTY[2], kShapeTZ[2]>>(Pin, kind, Pout, g, a, sh, chunk, s); break;
+    default: launch_fused<SP, W, Tile<kShapeTY[3], kShapeTZ[3]>>(Pin, kind, Pout, g, a, sh, chunk, s); break;
+    }
+}
+
+}  // namespace
 
 hipError_t lowp_init(int precision, const StepArgs& a, const Grid& g, double rho_shift,
                      const double* rho0, void* P, float* psi, hipStream_t s)
@@ -333,23 +362,17 @@ hipError_t lowp_init(int precision, const StepArgs& a, const Grid& g, double rho
 
 hipError_t lowp_step(int precision, const StepArgs& a, const Grid& g, double rho_shift,
                      const void* Pin, int kind, void* Pout, const double* rho0, float* psi,
-                     int kernel, hipStream_t s)
+                     int kernel, int shape, int chunk, hipStream_t s)
 {
     const float sh = static_cast<float>(rho_shift);
-    if (kernel == LBM_KERNEL_FUSED && rho0 == nullptr && fused::supported(g, a.q)) {
-        const int ncx = (g.nx + fused::kChunkX - 1) / fused::kChunkX;
-        const unsigned nb = static_cast<unsigned>((g.ny / fused::TY) * (g.nz / fused::TZ) * ncx);
+    if (kernel == LBM_KERNEL_FUSED && rho0 == nullptr) {
+        if (!fused::supported(g, a.q, shape) || chunk < 1) return hipErrorNotSupported;
+        // FP32 stores f itself: the shifted frame then has rs = 0
+        const float rs = (precision == LBM_FP32) ? 0.f : sh;
         auto go = [&](auto sptag) {
             using SP = decltype(sptag);
-            using T = typename SP::T;
-            const T* in = static_cast<const T*>(Pin);
-            T* out = static_cast<T*>(Pout);
-            if (a.walls)
-                fused::k_fused<SP, float, true, false, false><<<nb, fused::NT, 0, s>>>(
-                    in, kind, out, g, (float)a.tau, (float)a.G, sh);
-            else
-                fused::k_fused<SP, float, false, false, false><<<nb, fused::NT, 0, s>>>(
-                    in, kind, out, g, (float)a.tau, (float)a.G, sh);
+            if (a.walls) launch_fused_shape<SP, true>(shape, Pin, kind, Pout, g, a, rs, chunk, s);
+            else launch_fused_shape<SP, false>(shape, Pin, kind, Pout, g, a, rs, chunk, s);
         };
         if (precision == LBM_FP32) go(StF32{});
         else if (precision == LBM_FP16S) go(StF16{});

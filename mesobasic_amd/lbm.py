@@ -65,12 +65,15 @@ def load():
         "lbm_synchronize": (ctypes.c_int, [P]),
         "lbm_device_bytes": (ctypes.c_int64, [P]),
         "lbm_kernel_in_use": (ctypes.c_int, [P]),
+        "lbm_set_tuning": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),
+        "lbm_get_tuning": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
         "lbm_last_error": (ctypes.c_char_p, []),
         "lbm_fill_noise": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_int64, ctypes.c_double,
                                           ctypes.c_double, _D]),
         "lbm_fill_droplet": (ctypes.c_int, [ctypes.c_int] * 3 + [ctypes.c_double] * 4 + [_D]),
         "lbm_abi_version": (ctypes.c_int, []),
         "lbm_probe_exp": (ctypes.c_int, [_D, _D, ctypes.c_int64]),
+        "lbm_probe_copy": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, _D]),
         "lbm_probe_tile_map": (ctypes.c_int, [ctypes.c_int] * 3 + [ctypes.POINTER(ctypes.c_int)] * 3),
     }
     for name, (res, args) in sig.items():
@@ -118,6 +121,14 @@ def probe_exp(x):
     y = np.empty_like(x)
     _check(lib.lbm_probe_exp(_dptr(x), _dptr(y), x.size))
     return y
+
+
+def probe_copy(nbytes, reps=5):
+    """Runs the calibration copy kernel; returns ms per copy."""
+    lib = load()
+    ms = ctypes.c_double()
+    _check(lib.lbm_probe_copy(int(nbytes), int(reps), ctypes.byref(ms)))
+    return ms.value
 
 
 def probe_tile_map(nty, ntz, ncx):
@@ -185,6 +196,15 @@ class Lattice:
     @property
     def stream(self):
         return self._lib.lbm_get_stream(self._h)
+
+    def set_tuning(self, shape=-1, chunk=64):
+        _check(self._lib.lbm_set_tuning(self._h, int(shape), int(chunk)))
+
+    @property
+    def tuning(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        _check(self._lib.lbm_get_tuning(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def set_rho(self, rho0):
         r = np.ascontiguousarray(rho0, dtype=np.float64).reshape(-1)

@@ -125,6 +125,24 @@ def test_fused_equals_two_pass_bitwise(shape):
         assert same_bits(fa[k], fb[k]), k
 
 
+def test_fused_every_tile_shape_and_chunk_bitwise():
+    # results must not depend on the launch shape
+    shape = (24, 32, 64)
+    rho0 = mb.noise_rho(shape, 17)
+    ref = mb.Lattice(19, shape, 0.8, -5.5, kernel=mb.KERNEL_TWO_PASS, walls=True)
+    ref.set_rho(rho0); ref.init(); ref.step(9)
+    want = ref.get_f()
+    for tile in range(4):
+        for chunk in (1, 2, 5, 64):
+            print(f"tile {tile} chunk {chunk}", flush=True)
+            lat = mb.Lattice(19, shape, 0.8, -5.5, kernel=mb.KERNEL_FUSED, walls=True)
+            lat.set_tuning(tile, chunk)
+            assert lat.tuning == (tile, chunk)
+            lat.set_rho(rho0); lat.init(); lat.step(9)
+            assert same_bits(lat.get_f(), want), (tile, chunk)
+            lat.close()
+
+
 def test_set_f_roundtrip_and_restart():
     shape = (8, 6, 5)
     rho0 = mb.noise_rho(shape, 4)
