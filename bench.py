@@ -89,6 +89,8 @@ def main():
     ap.add_argument("--workload", default="d3q19_fp64_256_droplet", choices=sorted(WORKLOADS))
     ap.add_argument("--kernel", default="auto", choices=["auto", "two_pass", "fused"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tuning", default="", metavar="SHAPE,CHUNK[,ORDER[,NTSTORE]]",
+                    help="fused-kernel launch configuration (default: the library's choice)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the streaming copy kernel over MIB MiB (for counter scaling)")
     args = ap.parse_args()
@@ -108,6 +110,8 @@ def main():
               "fused": mb.KERNEL_FUSED}[args.kernel]
     n = int(np.prod(shape))
     lat = mb.Lattice(q, shape, tau, G, precision=prec, kernel=kernel)
+    if args.tuning:
+        lat.set_tuning(*(int(v) for v in args.tuning.split(",")))
     lat.set_rho(initial_rho(shape, init))
     lat.init()
     print(f"# workload {args.workload} kernel {KERNEL_NAMES.get(lat.kernel)} "
@@ -146,6 +150,7 @@ def main():
         "config": {"workload": args.workload, "stencil": "D3Q19" if q == 19 else "D2Q9",
                    "lattice": list(shape), "tau": tau, "G": G, "init": list(init),
                    "boundary": "periodic", "kernel": kname,
+                   "tuning": list(lat.tuning) if lat.kernel == mb.KERNEL_FUSED else None,
                    "wall_s_timed_region": round(wall, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

@@ -139,13 +139,24 @@ int lbm_set_stream(lbm_handle* h, void* stream);
 void* lbm_get_stream(const lbm_handle* h);
 int lbm_synchronize(lbm_handle* h);
 
-/* Launch shape of the fused kernel: tile shape id (0: 8x64, 1: 16x32,
- * 2: 4x64, 3: 8x32 nodes of the y-z cross-section; -1 = default) and x
- * planes per workgroup.  Results do not depend on it (the FP64 path stays
- * bit-identical); only speed does.  No reference counterpart: the
- * reference has no launch shapes. */
-int lbm_set_tuning(lbm_handle* h, int shape, int chunk);
-int lbm_get_tuning(const lbm_handle* h, int* shape, int* chunk);
+/* Launch configuration of the fused kernel.  Results do not depend on it
+ * (the FP64 path stays bit-identical to the reference); only speed does.
+ * No reference counterpart: the reference has no launch configuration.
+ *   shape     tile of the y-z cross-section per workgroup: 0: 1x512, 1: 2x256,
+ *             2: 1x256 (whole z rows, nz must equal 512 / 256 / 256),
+ *             3: 8x64, 4: 4x64, 5: 8x32; -1 = first that fits
+ *   chunk     x planes marched per workgroup
+ *   order     block -> tile order: 0 plain, 1 contiguous runs per XCD,
+ *             2 y-tiles of one column per XCD
+ *   nt_store  1 = stores carry the non-temporal hint */
+typedef struct lbm_tuning {
+    int shape;
+    int chunk;
+    int order;
+    int nt_store;
+} lbm_tuning;
+int lbm_set_tuning(lbm_handle* h, const lbm_tuning* t);
+int lbm_get_tuning(const lbm_handle* h, lbm_tuning* t);
 
 /* Device bytes held by the handle, and the kernel it resolved to. */
 int64_t lbm_device_bytes(const lbm_handle* h);

@@ -15,10 +15,11 @@ extern "C" {
 /* y[i] = the device exp() used by the FP64 kernels, for n host values x. */
 int lbm_probe_exp(const double* x, double* y, int64_t n);
 
-/* Host evaluation of the fused kernel's block -> tile map for a grid of
- * nty x ntz tiles and ncx x-chunks: block b owns tile (ty[b], tz[b]) of
- * chunk cx[b].  Arrays of nty*ntz*ncx ints.  No device work. */
-int lbm_probe_tile_map(int nty, int ntz, int ncx, int* ty, int* tz, int* cx);
+/* Host evaluation of the fused kernel's block -> tile map (block order
+ * `order`, as lbm_tuning.order) for a grid of nty x ntz tiles and ncx
+ * x-chunks: block b owns tile (ty[b], tz[b]) of chunk cx[b].  Arrays of
+ * nty*ntz*ncx ints.  No device work. */
+int lbm_probe_tile_map(int nty, int ntz, int ncx, int order, int* ty, int* tz, int* cx);
 
 /* Streaming copy kernel over `bytes` (multiple of 16) between two device
  * buffers it allocates, run `reps` times; *ms = average time per copy.  The

@@ -74,8 +74,7 @@ struct lbm_handle {
     bool prev_valid = false;         // buf[1-cur] holds f(N-1)
     bool ref_is_rho0 = false;        // reference arrays still hold rho0, u = 0
     int kernel = LBM_KERNEL_TWO_PASS;
-    int fused_shape = -1;            // tile shape of the fused kernel
-    int fused_chunk = 64;            // x planes per workgroup of the fused kernel
+    lbm::FusedTuning tune{-1, 64, 2, 1};  // fused kernel launch configuration
     int64_t bytes = 0;
 };
 
@@ -102,8 +101,7 @@ int run_step(lbm_handle* h)
         auto* Pout = static_cast<double*>(h->buf[dst]);
         const int mode = h->first_step_stored ? 1 : 0;
         if (h->kernel == LBM_KERNEL_FUSED && mode == 0) {
-            HIP_OK(lbm::fp64_fused_step(h->a, h->g, Pin, h->kind[src], Pout, h->fused_shape,
-                                        h->fused_chunk, h->stream));
+            HIP_OK(lbm::fp64_fused_step(h->a, h->g, Pin, h->kind[src], Pout, h->tune, h->stream));
         } else {
             if (!h->psi) return fail(LBM_ESTATE, "psi field not allocated");
             if (mode == 1) HIP_OK(lbm::fp64_psi_from_rho(h->rho0, h->psi, h->g.n, h->stream));
@@ -115,8 +113,7 @@ int run_step(lbm_handle* h)
         if (!h->psi) return fail(LBM_ESTATE, "psi field not allocated");
         HIP_OK(lbm::lowp_step(h->d.precision, h->a, h->g, h->d.rho_shift, h->buf[src],
                               h->kind[src], h->buf[dst], h->first_step_stored ? h->rho0 : nullptr,
-                              reinterpret_cast<float*>(h->psi), h->kernel, h->fused_shape,
-                              h->fused_chunk, h->stream));
+                              reinterpret_cast<float*>(h->psi), h->kernel, h->tune, h->stream));
     }
     h->kind[dst] = lbm::kHoldsPost;
     h->cur = dst;
@@ -211,7 +208,7 @@ int lbm_create(const lbm_desc* desc, lbm_handle** out)
             return fail(LBM_EINVAL, "requested kernel does not support this lattice/precision");
         }
     }
-    h->fused_shape = lbm::fused::default_shape(h->g, desc->q);
+    h->tune.shape = lbm::fused::default_shape(h->g, desc->q);
     const int64_t pop_bytes = h->g.ps * desc->q * h->elem;
     auto cleanup = [&](int code) {
         lbm_destroy(h);
@@ -438,25 +435,31 @@ int lbm_conservation(lbm_handle* h, double out[4])
 
 int64_t lbm_steps_done(const lbm_handle* h) { return h ? h->steps : -1; }
 
-int lbm_set_tuning(lbm_handle* h, int shape, int chunk)
+int lbm_set_tuning(lbm_handle* h, const lbm_tuning* t)
 {
-    if (!h) return fail(LBM_EINVAL, "lbm_set_tuning: NULL handle");
-    if (h->kernel != LBM_KERNEL_FUSED) return fail(LBM_EINVAL, "lbm_set_tuning: handle does not use the fused kernel");
-    if (shape < 0) shape = lbm::fused::default_shape(h->g, h->d.q);
+    if (!h || !t) return fail(LBM_EINVAL, "lbm_set_tuning: NULL argument");
+    if (h->kernel != LBM_KERNEL_FUSED)
+        return fail(LBM_EINVAL, "lbm_set_tuning: handle does not use the fused kernel");
+    const int shape = t->shape < 0 ? lbm::fused::default_shape(h->g, h->d.q) : t->shape;
     if (!lbm::fused::supported(h->g, h->d.q, shape))
-        return fail(LBM_EINVAL, "tile shape %d does not divide the %d x %d cross-section", shape,
-                    h->d.ny, h->d.nz);
-    if (chunk < 1) return fail(LBM_EINVAL, "chunk must be >= 1");
-    h->fused_shape = shape;
-    h->fused_chunk = chunk;
+        return fail(LBM_EINVAL, "tile shape %d does not fit the %d x %d cross-section", shape, h->d.ny,
+                    h->d.nz);
+    if (t->chunk < 1) return fail(LBM_EINVAL, "chunk must be >= 1");
+    if (t->order < 0 || t->order > 2) return fail(LBM_EINVAL, "order must be 0, 1 or 2");
+    h->tune.shape = shape;
+    h->tune.chunk = t->chunk;
+    h->tune.order = t->order;
+    h->tune.nts = t->nt_store ? 1 : 0;
     return LBM_OK;
 }
 
-int lbm_get_tuning(const lbm_handle* h, int* shape, int* chunk)
+int lbm_get_tuning(const lbm_handle* h, lbm_tuning* t)
 {
-    if (!h) return fail(LBM_EINVAL, "lbm_get_tuning: NULL handle");
-    if (shape) *shape = h->fused_shape;
-    if (chunk) *chunk = h->fused_chunk;
+    if (!h || !t) return fail(LBM_EINVAL, "lbm_get_tuning: NULL argument");
+    t->shape = h->tune.shape;
+    t->chunk = h->tune.chunk;
+    t->order = h->tune.order;
+    t->nt_store = h->tune.nts;
     return LBM_OK;
 }
 

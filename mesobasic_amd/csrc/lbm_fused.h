@@ -33,43 +33,61 @@ namespace lbm {
 namespace fused {
 
 // Tile of TY rows (y) by TZ columns (z, contiguous); one thread per tile
-// node.  LDS holds psi planes of (TY+2) x (TZ+2) including the ring.
-template <int TY_, int TZ_> struct Tile {
+// node.  LDS holds psi planes of (TY+2) rows including the y ring.
+//
+// ZF ("full rows"): TZ equals the lattice's z extent, so the tile's rows
+// wrap periodically within the tile and there is no z ring at all; every
+// cache line a row pull touches belongs to the tile itself.  Otherwise the
+// plane carries a one-column z ring on each side, whose pulls touch cache
+// lines of the z-neighbour tiles.
+template <int TY_, int TZ_, bool ZF_> struct Tile {
     static constexpr int TY = TY_;
     static constexpr int TZ = TZ_;
+    static constexpr bool ZF = ZF_;
     static constexpr int NT = TY * TZ;
     static constexpr int LY = TY + 2;
-    static constexpr int LZ = TZ + 2;
-    static constexpr int NRING = 2 * LZ + 2 * TY;
-    static_assert(NRING <= NT, "ring must fit in one pass of the workgroup");
+    static constexpr int LZ = ZF ? TZ : TZ + 2;
+    static constexpr int NRING = ZF ? 2 * TZ : 2 * LZ + 2 * TY;
+    static constexpr int RPT = (NRING + NT - 1) / NT;   // ring nodes per thread
+    static_assert(RPT <= 2, "at most two ring nodes per thread");
     static_assert(NT % 64 == 0 && NT <= 1024, "whole wavefronts");
 };
 
 // Tile shapes instantiated for the launcher and the tuner (index = id).
-constexpr int kNumShapes = 4;
-constexpr int kShapeTY[kNumShapes] = {8, 16, 4, 8};
-constexpr int kShapeTZ[kNumShapes] = {64, 32, 64, 32};
+constexpr int kNumShapes = 6;
+constexpr int kShapeTY[kNumShapes] = {1, 2, 1, 8, 4, 8};
+constexpr int kShapeTZ[kNumShapes] = {512, 256, 256, 64, 64, 32};
+constexpr bool kShapeZF[kNumShapes] = {true, true, true, false, false, false};
 
 inline bool supported(const Grid& g, int q, int shape)
 {
     if (q != 19 || shape < 0 || shape >= kNumShapes) return false;
     const int ty = kShapeTY[shape], tz = kShapeTZ[shape];
-    return g.ny % ty == 0 && g.nz % tz == 0 && g.nx >= 1;
+    if (g.ny % ty != 0 || g.nx < 1) return false;
+    return kShapeZF[shape] ? g.nz == tz : g.nz % tz == 0;
 }
 
-// Default tile shape for a lattice: the first (largest-tile) shape that
-// divides it, or -1.
+// Default tile shape for a lattice: the first that fits in order of
+// measured speed on MI355X (8x64, 4x64, 8x32, then the full-row tiles), or
+// -1.  (Full-row tiles avoid the z ring but pull three rows per row, and
+// measured slower.)
 inline int default_shape(const Grid& g, int q)
 {
-    for (int s = 0; s < kNumShapes; s++)
+    constexpr int pref[kNumShapes] = {3, 4, 5, 0, 1, 2};
+    for (int s : pref)
         if (supported(g, q, s)) return s;
     return -1;
 }
 
-// Ring slot r -> (row, col) in LDS coordinates (row 0 and LY-1 are the
-// y-ring rows, col 0 and LZ-1 the z-ring columns).
+// Ring slot r -> (row, col) in LDS coordinates.  Rows 0 and LY-1 are the y
+// ring; without ZF, columns 0 and LZ-1 are the z ring.
 template <typename TL> __device__ __forceinline__ void ring_slot(int r, int& row, int& col)
 {
+    if (TL::ZF) {
+        if (r < TL::TZ) { row = 0; col = r; }
+        else { row = TL::LY - 1; col = r - TL::TZ; }
+        return;
+    }
     if (r < TL::LZ) { row = 0; col = r; }
     else if (r < 2 * TL::LZ) { row = TL::LY - 1; col = r - TL::LZ; }
     else if (r < 2 * TL::LZ + TL::TY) { row = 1 + (r - 2 * TL::LZ); col = 0; }
@@ -162,12 +180,12 @@ __device__ __forceinline__ CT rho_of(const CT (&f)[19], CT rs)
 // storage): rho = rs + sum d, j = sum d e (sum w e = 0), and the collision
 // d' = d - (d - d_eq)/tau with d_eq = feq - w*rs = w*(rho*poly - rs), so no
 // large f values are ever formed in FP32.
-template <typename SP, typename CT, bool EXACT, bool TAU1, int LZ>
+template <typename SP, typename CT, bool EXACT, bool TAU1, int LZ, bool NTS>
 __device__ __forceinline__ void collide_store(typename SP::T* __restrict__ Pout, const Grid& g,
                                               uint32_t c, const CT (&f)[19],
                                               const CT* __restrict__ pm, const CT* __restrict__ p0,
-                                              const CT* __restrict__ pp, int li, CT tau, CT G,
-                                              CT rs)
+                                              const CT* __restrict__ pp, int li, int dzm, int dzp,
+                                              CT tau, CT G, CT rs)
 {
     using S = St<19>;
     // moments, LBfluid3D.cpp:223-233
@@ -194,7 +212,8 @@ __device__ __forceinline__ void collide_store(typename SP::T* __restrict__ Pout,
         constexpr int n = decltype(I)::value;
         if constexpr (n > 0) {
             const CT* pl = S::ex[n] < 0 ? pm : (S::ex[n] > 0 ? pp : p0);
-            const CT t = static_cast<CT>(S::w[n]) * pl[li + S::ey[n] * LZ + S::ez[n]];
+            const int dz = S::ez[n] < 0 ? dzm : (S::ez[n] > 0 ? dzp : 0);
+            const CT t = static_cast<CT>(S::w[n]) * pl[li + S::ey[n] * LZ + dz];
             acc_e<S::ex[n]>(gx, t);
             acc_e<S::ey[n]>(gy, t);
             acc_e<S::ez[n]>(gz, t);
@@ -226,30 +245,49 @@ __device__ __forceinline__ void collide_store(typename SP::T* __restrict__ Pout,
             const CT deq = static_cast<CT>(S::w[n]) * (rho * poly - rs);
             out = f[n] - (f[n] - deq) * itau;
         }
-        Pout[n * g.ps + c] = SP::of_d(out);
+        if constexpr (NTS) SP::put_nt(Pout + n * g.ps + c, SP::of_d(out));
+        else Pout[n * g.ps + c] = SP::of_d(out);
     });
 }
 
-// Block index -> tile.  Workgroups are dispatched to the 8 XCDs round robin
-// (block b runs on XCD b % 8); consecutive y-tiles of one (z, x-chunk)
-// column get block indices 8 apart so that they share an L2.  With
-// b = 8*j + g, (j, g) -> (ty = j % nty, t = (j / nty)*8 + g) is a bijection
-// onto [0, nty) x [0, ntz*ncx) exactly when ntz*ncx is a multiple of 8;
-// otherwise the plain row-major order is used.
-__host__ __device__ constexpr bool xcd_remap(int nty, int ntz, int ncx)
+// Block index -> tile.  Workgroups are dispatched to the 8 XCDs round
+// robin (block b runs on XCD b % 8), each XCD with its own L2; all XCDs
+// share the memory-side cache.  With linear tile L = (cx*ntz + tz)*nty + ty
+// and N = nty*ntz*ncx blocks, `order` picks
+//   0  L = b: consecutive y-tiles on different XCDs, all XCDs on the same
+//      planes at once (ring rows shared through the memory-side cache);
+//   1  b = 8j + g -> L = g*(N/8) + j (needs N % 8 == 0): each XCD walks a
+//      contiguous run of y-tiles (ring rows shared through its L2);
+//   2  b = 8j + g -> ty = j % nty, t = (j / nty)*8 + g (needs ntz*ncx % 8
+//      == 0): y-tiles of one column on one XCD, the 8 XCDs on 8 columns of
+//      the same chunk.
+// Each is a bijection of [0, N) when its condition holds; otherwise order 0.
+__host__ __device__ inline int effective_order(int order, int nty, int ntz, int ncx)
 {
-    return nty > 1 && (ntz * ncx) % 8 == 0;
+    if (order == 1 && (nty * ntz * ncx) % 8 == 0) return 1;
+    if (order == 2 && nty > 1 && (ntz * ncx) % 8 == 0) return 2;
+    return 0;
 }
 
-__host__ __device__ inline void tile_of_block(int b, int nty, int ntz, int ncx, int& ty, int& tz,
-                                              int& cx)
+__host__ __device__ inline void tile_of_block(int b, int nty, int ntz, int ncx, int order, int& ty,
+                                              int& tz, int& cx)
 {
+    const int n = nty * ntz * ncx;
     int t;
-    if (xcd_remap(nty, ntz, ncx)) {
-        const int g = b & 7, j = b >> 3;
+    switch (effective_order(order, nty, ntz, ncx)) {
+    case 1: {
+        const int l = (b & 7) * (n >> 3) + (b >> 3);
+        ty = l % nty;
+        t = l / nty;
+        break;
+    }
+    case 2: {
+        const int j = b >> 3;
         ty = j % nty;
-        t = (j / nty) * 8 + g;
-    } else {
+        t = (j / nty) * 8 + (b & 7);
+        break;
+    }
+    default:
         ty = b % nty;
         t = b / nty;
     }
@@ -257,27 +295,15 @@ __host__ __device__ inline void tile_of_block(int b, int nty, int ntz, int ncx, 
     cx = t / ntz;
 }
 
-// Plane pipeline of one workgroup.  At march step x the workgroup
-//   1. issues the pulls of plane x+2 for its tile nodes into registers,
-//   2. collides plane x from the registers pulled two steps earlier, with
-//      psi of planes x-1, x, x+1 from LDS, and stores it,
-//   3. pulls the ring nodes of plane x+2, sums rho of tile and ring and
-//      writes psi(x+2) into LDS,
-//   4. waits at the one barrier of the step.
-// The loads of step 1 are in flight during the collision of step 2; the
-// ring pulls come after the collision so that their registers are not live
-// at the same time as the collision's.  Four
-// LDS psi slots rotate so that step 3 overwrites the slot of plane x-2,
-// which no wavefront reads after the previous barrier.
 // Occupancy target: FP64 keeps three planes of 19 doubles in registers
 // (2 wavefronts per SIMD); the FP32-compute forms fit 4.
 template <typename CT> struct Occ { static constexpr int waves = 2; };
 template <> struct Occ<float> { static constexpr int waves = 4; };
 
-template <typename SP, typename CT, bool WALLS, bool EXACT, bool TAU1, typename TL>
+template <typename SP, typename CT, bool WALLS, bool EXACT, bool TAU1, typename TL, bool NTS>
 __global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(Occ<CT>::waves)))
     k_fused(const typename SP::T* __restrict__ Pin, int kind, typename SP::T* __restrict__ Pout,
-            Grid g, CT tau, CT G, CT rho_shift, int chunk)
+            Grid g, CT tau, CT G, CT rho_shift, int chunk, int order)
 {
     constexpr int TY = TL::TY, TZ = TL::TZ, LZ = TL::LZ, PL = TL::LY * TL::LZ;
     __shared__ CT psi_lds[4 * PL];
@@ -285,7 +311,7 @@ __global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(Occ
     const int nty = g.ny / TY, ntz = g.nz / TZ;
     const int ncx = (g.nx + chunk - 1) / chunk;
     int ty, tz, cx;
-    tile_of_block(blockIdx.x, nty, ntz, ncx, ty, tz, cx);
+    tile_of_block(blockIdx.x, nty, ntz, ncx, order, ty, tz, cx);
     const int y0 = ty * TY, z0 = tz * TZ;
     const int x0 = cx * chunk;
     const int x1 = min(g.nx, x0 + chunk);
@@ -293,26 +319,40 @@ __global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(Occ
     const int tid = threadIdx.x;
     const int ly = tid / TZ, lz = tid - (tid / TZ) * TZ;  // tile node of this thread
     const int yi = y0 + ly, zi = z0 + lz;
-    const int li = (ly + 1) * LZ + (lz + 1);              // its LDS slot
+    constexpr int c0 = TL::ZF ? 0 : 1;                    // LDS column of z0
+    const int li = (ly + 1) * LZ + (lz + c0);             // its LDS slot
+    // LDS offsets of the z-1 / z+1 neighbours (periodic within a full row)
+    const int dzm = (TL::ZF && lz == 0) ? TZ - 1 : -1;
+    const int dzp = (TL::ZF && lz == TZ - 1) ? 1 - TZ : 1;
 
-    int rrow = 0, rcol = 0, ry = 0, rz = 0;               // ring node of this thread
-    const bool has_ring = tid < TL::NRING;
-    if (has_ring) {
-        ring_slot<TL>(tid, rrow, rcol);
-        ry = wrap(y0 + rrow - 1, g.ny);
-        rz = wrap(z0 + rcol - 1, g.nz);
+    // ring nodes of this thread: slots tid and tid + NT
+    int ry[2] = {0, 0}, rz[2] = {0, 0}, lr[2] = {0, 0};
+    bool has_ring[2] = {false, false};
+#pragma unroll
+    for (int k = 0; k < TL::RPT; k++) {
+        const int r = tid + k * TL::NT;
+        if (r < TL::NRING) {
+            int rrow, rcol;
+            ring_slot<TL>(r, rrow, rcol);
+            has_ring[k] = true;
+            ry[k] = wrap(y0 + rrow - 1, g.ny);
+            rz[k] = wrap(z0 + rcol - c0, g.nz);
+            lr[k] = rrow * LZ + rcol;
+        }
     }
-    const int lr = rrow * LZ + rcol;
 
     // LDS slot of plane x: (x - x0 + 1) mod 4
     auto slot = [&](int x) { return ((x - x0 + 1) & 3) * PL; };
     auto xw = [&](int x) { return x >= g.nx ? x - g.nx : (x < 0 ? x + g.nx : x); };
     auto psi_of = [&](const CT (&v)[19]) { return Psi<CT>::of(rho_of<EXACT>(v, rho_shift)); };
     auto ring_psi = [&](int xp, int s) {
-        if (has_ring) {
-            CT fr[19];
-            pull<SP, CT, WALLS>(Pin, g, kind, xp, ry, rz, fr);
-            psi_lds[s + lr] = psi_of(fr);
+#pragma unroll
+        for (int k = 0; k < TL::RPT; k++) {
+            if (has_ring[k]) {
+                CT fr[19];
+                pull<SP, CT, WALLS>(Pin, g, kind, xp, ry[k], rz[k], fr);
+                psi_lds[s + lr[k]] = psi_of(fr);
+            }
         }
     };
 
@@ -331,14 +371,21 @@ __global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(Occ
 
     auto step = [&](int x, CT (&fcur)[19], CT (&fnn)[19]) {
         const int x2 = xw(x + 2);
-        pull<SP, CT, WALLS>(Pin, g, kind, x2, yi, zi, fnn);   // in flight during the collision
+        // pulls of plane x+2, in flight during the collision of plane x
+        pull<SP, CT, WALLS>(Pin, g, kind, x2, yi, zi, fnn);
+        CT fr[TL::RPT][19];
+#pragma unroll
+        for (int k = 0; k < TL::RPT; k++)
+            if (has_ring[k]) pull<SP, CT, WALLS>(Pin, g, kind, x2, ry[k], rz[k], fr[k]);
         const uint32_t c = (static_cast<uint32_t>(x) * static_cast<uint32_t>(g.ny) +
                             static_cast<uint32_t>(yi)) * static_cast<uint32_t>(g.nz) +
                            static_cast<uint32_t>(zi);
-        collide_store<SP, CT, EXACT, TAU1, LZ>(Pout, g, c, fcur, psi_lds + slot(x - 1),
-                                               psi_lds + slot(x), psi_lds + slot(x + 1), li, tau, G,
-                                               rho_shift);
-        ring_psi(x2, slot(x + 2));
+        collide_store<SP, CT, EXACT, TAU1, LZ, NTS>(Pout, g, c, fcur, psi_lds + slot(x - 1),
+                                               psi_lds + slot(x), psi_lds + slot(x + 1), li, dzm,
+                                               dzp, tau, G, rho_shift);
+#pragma unroll
+        for (int k = 0; k < TL::RPT; k++)
+            if (has_ring[k]) psi_lds[slot(x + 2) + lr[k]] = psi_of(fr[k]);
         psi_lds[slot(x + 2) + li] = psi_of(fnn);
         __syncthreads();
     };

@@ -24,6 +24,15 @@ enum BufKind : int {
     kHoldsPost = 1 // post-collision values (f is gathered from neighbours)
 };
 
+// Launch configuration of the fused kernel (see lbm_fused.h): tile shape
+// id, x planes per workgroup, block -> tile order, non-temporal stores.
+struct FusedTuning {
+    int shape;
+    int chunk;
+    int order;
+    int nts;
+};
+
 struct StepArgs {
     int q;
     int walls;
@@ -41,9 +50,8 @@ hipError_t fp64_collide(const StepArgs& a, const Grid& g, const double* Pin, int
                         hipStream_t s);
 // One fused pass (psi halo recomputed on chip).  Returns hipErrorNotSupported
 // when the geometry does not suit it.
-// shape: tile shape id (fused::kShapeTY/TZ); chunk: x planes per workgroup.
 hipError_t fp64_fused_step(const StepArgs& a, const Grid& g, const double* Pin, int kind,
-                           double* Pout, int shape, int chunk, hipStream_t s);
+                           double* Pout, const FusedTuning& t, hipStream_t s);
 bool fp64_fused_supported(const StepArgs& a, const Grid& g);
 // Shan-Chen force from a psi field (FP64, reference arithmetic).
 hipError_t fp64_force(const StepArgs& a, const Grid& g, const double* psi, double* fx, double* fy,

@@ -15,7 +15,7 @@ hipError_t lowp_init(int precision, const StepArgs& a, const Grid& g, double rho
 // n-float scratch field (unused by the fused kernel).
 hipError_t lowp_step(int precision, const StepArgs& a, const Grid& g, double rho_shift,
                      const void* Pin, int kind, void* Pout, const double* rho0, float* psi,
-                     int kernel, int shape, int chunk, hipStream_t s);
+                     int kernel, const FusedTuning& t, hipStream_t s);
 hipError_t lowp_moments(int precision, const StepArgs& a, const Grid& g, double rho_shift,
                         const void* P, int kind, double* rho, double* u, double* v, double* w,
                         hipStream_t s);

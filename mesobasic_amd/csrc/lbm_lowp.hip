@@ -320,28 +320,30 @@ int lowp_resolve_kernel(int, const StepArgs& a, const Grid& g, int requested)
 
 namespace {
 
-template <typename SP, bool W, typename TL>
-void launch_fused(const void* Pin, int kind, void* Pout, const Grid& g, const StepArgs& a, float sh,
-                  int chunk, hipStream_t s)
+template <typename SP, bool W, typename TL, bool NTS>
+void launch_fused(const void* Pin, int kind, void* Pout, const Grid& g, const StepArgs& a, float rs,
+                  const FusedTuning& t, hipStream_t s)
 {
     using T = typename SP::T;
-    const int ncx = (g.nx + chunk - 1) / chunk;
+    const int ncx = (g.nx + t.chunk - 1) / t.chunk;
     const unsigned nb = static_cast<unsigned>((g.ny / TL::TY) * (g.nz / TL::TZ) * ncx);
-    fused::k_fused<SP, float, W, false, false, TL><<<nb, TL::NT, 0, s>>>(
-        static_cast<const T*>(Pin), kind, static_cast<T*>(Pout), g, (float)a.tau, (float)a.G, sh,
-        chunk);
+    fused::k_fused<SP, float, W, false, false, TL, NTS><<<nb, TL::NT, 0, s>>>(
+        static_cast<const T*>(Pin), kind, static_cast<T*>(Pout), g, (float)a.tau, (float)a.G, rs,
+        t.chunk, t.order);
 }
 
-template <typename SP, bool W>
-void launch_fused_shape(int shape, const void* Pin, int kind, void* Pout, const Grid& g,
-                        const StepArgs& a, float sh, int chunk, hipStream_t s)
+template <typename SP, bool W, bool NTS>
+void launch_fused_shape(const void* Pin, int kind, void* Pout, const Grid& g, const StepArgs& a,
+                        float rs, const FusedTuning& t, hipStream_t s)
 {
     using namespace fused;
-    switch (shape) {
-    case 0: launch_fused<SP, W, Tile<kShapeTY[0], kShapeTZ[0]>>(Pin, kind, Pout, g, a, sh, chunk, s); break;
-    case 1: launch_fused<SP, W, Tile<kShapeTY[1], kShapeTZ[1]>>(Pin, kind, Pout, g, a, sh, chunk, s); break;
-    case 2: launch_fused<SP, W, Tile<kShapeTY[2], kShapeTZ[2]>>(Pin, kind, Pout, g, a, sh, chunk, s); break;
-    default: launch_fused<SP, W, Tile<kShapeTY[3], kShapeTZ[3]>>(Pin, kind, Pout, g, a, sh, chunk, s); break;
+    switch (t.shape) {
+    case 0: launch_fused<SP, W, Tile<kShapeTY[0], kShapeTZ[0], kShapeZF[0]>, NTS>(Pin, kind, Pout, g, a, rs, t, s); break;
+    case 1: launch_fused<SP, W, Tile<kShapeTY[1], kShapeTZ[1], kShapeZF[1]>, NTS>(Pin, kind, Pout, g, a, rs, t, s); break;
+    case 2: launch_fused<SP, W, Tile<kShapeTY[2], kShapeTZ[2], kShapeZF[2]>, NTS>(Pin, kind, Pout, g, a, rs, t, s); break;
+    case 3: launch_fused<SP, W, Tile<kShapeTY[3], kShapeTZ[3], kShapeZF[3]>, NTS>(Pin, kind, Pout, g, a, rs, t, s); break;
+    case 4: launch_fused<SP, W, Tile<kShapeTY[4], kShapeTZ[4], kShapeZF[4]>, NTS>(Pin, kind, Pout, g, a, rs, t, s); break;
+    default: launch_fused<SP, W, Tile<kShapeTY[5], kShapeTZ[5], kShapeZF[5]>, NTS>(Pin, kind, Pout, g, a, rs, t, s); break;
     }
 }
 
@@ -362,17 +364,22 @@ hipError_t lowp_init(int precision, const StepArgs& a, const Grid& g, double rho
 
 hipError_t lowp_step(int precision, const StepArgs& a, const Grid& g, double rho_shift,
                      const void* Pin, int kind, void* Pout, const double* rho0, float* psi,
-                     int kernel, int shape, int chunk, hipStream_t s)
+                     int kernel, const FusedTuning& t, hipStream_t s)
 {
     const float sh = static_cast<float>(rho_shift);
     if (kernel == LBM_KERNEL_FUSED && rho0 == nullptr) {
-        if (!fused::supported(g, a.q, shape) || chunk < 1) return hipErrorNotSupported;
+        if (!fused::supported(g, a.q, t.shape) || t.chunk < 1) return hipErrorNotSupported;
         // FP32 stores f itself: the shifted frame then has rs = 0
         const float rs = (precision == LBM_FP32) ? 0.f : sh;
         auto go = [&](auto sptag) {
             using SP = decltype(sptag);
-            if (a.walls) launch_fused_shape<SP, true>(shape, Pin, kind, Pout, g, a, rs, chunk, s);
-            else launch_fused_shape<SP, false>(shape, Pin, kind, Pout, g, a, rs, chunk, s);
+            if (a.walls) {
+                if (t.nts) launch_fused_shape<SP, true, true>(Pin, kind, Pout, g, a, rs, t, s);
+                else launch_fused_shape<SP, true, false>(Pin, kind, Pout, g, a, rs, t, s);
+            } else {
+                if (t.nts) launch_fused_shape<SP, false, true>(Pin, kind, Pout, g, a, rs, t, s);
+                else launch_fused_shape<SP, false, false>(Pin, kind, Pout, g, a, rs, t, s);
+            }
         };
         if (precision == LBM_FP32) go(StF32{});
         else if (precision == LBM_FP16S) go(StF16{});

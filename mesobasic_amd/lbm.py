@@ -35,6 +35,11 @@ class Desc(ctypes.Structure):
                 ("rho_shift", ctypes.c_double), ("kernel", ctypes.c_int)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("shape", ctypes.c_int), ("chunk", ctypes.c_int), ("order", ctypes.c_int),
+                ("nt_store", ctypes.c_int)]
+
+
 _lib = None
 
 
@@ -65,8 +70,8 @@ def load():
         "lbm_synchronize": (ctypes.c_int, [P]),
         "lbm_device_bytes": (ctypes.c_int64, [P]),
         "lbm_kernel_in_use": (ctypes.c_int, [P]),
-        "lbm_set_tuning": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),
-        "lbm_get_tuning": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+        "lbm_set_tuning": (ctypes.c_int, [P, ctypes.POINTER(Tuning)]),
+        "lbm_get_tuning": (ctypes.c_int, [P, ctypes.POINTER(Tuning)]),
         "lbm_last_error": (ctypes.c_char_p, []),
         "lbm_fill_noise": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_int64, ctypes.c_double,
                                           ctypes.c_double, _D]),
@@ -74,7 +79,7 @@ def load():
         "lbm_abi_version": (ctypes.c_int, []),
         "lbm_probe_exp": (ctypes.c_int, [_D, _D, ctypes.c_int64]),
         "lbm_probe_copy": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, _D]),
-        "lbm_probe_tile_map": (ctypes.c_int, [ctypes.c_int] * 3 + [ctypes.POINTER(ctypes.c_int)] * 3),
+        "lbm_probe_tile_map": (ctypes.c_int, [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_int)] * 3),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -131,13 +136,13 @@ def probe_copy(nbytes, reps=5):
     return ms.value
 
 
-def probe_tile_map(nty, ntz, ncx):
+def probe_tile_map(nty, ntz, ncx, order=2):
     """Block -> (ty, tz, cx) map of the fused kernel (host evaluation)."""
     lib = load()
     nb = nty * ntz * ncx
     out = [np.empty(nb, dtype=np.int32) for _ in range(3)]
     P = ctypes.POINTER(ctypes.c_int)
-    rc = lib.lbm_probe_tile_map(nty, ntz, ncx, *(a.ctypes.data_as(P) for a in out))
+    rc = lib.lbm_probe_tile_map(nty, ntz, ncx, order, *(a.ctypes.data_as(P) for a in out))
     if rc != 0:
         raise ValueError("bad tile grid")
     return out
@@ -197,14 +202,15 @@ class Lattice:
     def stream(self):
         return self._lib.lbm_get_stream(self._h)
 
-    def set_tuning(self, shape=-1, chunk=64):
-        _check(self._lib.lbm_set_tuning(self._h, int(shape), int(chunk)))
+    def set_tuning(self, shape=-1, chunk=64, order=2, nt_store=1):
+        t = Tuning(int(shape), int(chunk), int(order), int(nt_store))
+        _check(self._lib.lbm_set_tuning(self._h, ctypes.byref(t)))
 
     @property
     def tuning(self):
-        a, b = ctypes.c_int(), ctypes.c_int()
-        _check(self._lib.lbm_get_tuning(self._h, ctypes.byref(a), ctypes.byref(b)))
-        return a.value, b.value
+        t = Tuning()
+        _check(self._lib.lbm_get_tuning(self._h, ctypes.byref(t)))
+        return t.shape, t.chunk, t.order, t.nt_store
 
     def set_rho(self, rho0):
         r = np.ascontiguousarray(rho0, dtype=np.float64).reshape(-1)

@@ -79,6 +79,8 @@ FUSED_CASES = [
     (19, (5, 32, 32), 0.7, -5.0, 3, True, 30),
     (19, (70, 16, 32), 1.0, -5.5, 3, False, 12),
     (19, (3, 16, 64), 1.2, -5.5, 2, True, 20),
+    (19, (5, 6, 256), 1.0, -5.5, 3, False, 15),
+    (19, (3, 2, 512), 0.9, -5.0, 3, True, 12),
 ]
 
 
@@ -125,22 +127,30 @@ def test_fused_equals_two_pass_bitwise(shape):
         assert same_bits(fa[k], fb[k]), k
 
 
-def test_fused_every_tile_shape_and_chunk_bitwise():
-    # results must not depend on the launch shape
-    shape = (24, 32, 64)
+@pytest.mark.parametrize("shape", [(24, 32, 64), (6, 4, 256), (4, 3, 512)])
+def test_fused_every_tile_shape_and_chunk_bitwise(shape):
+    # results must not depend on the launch shape; (6, 4, 256) and (4, 3, 512)
+    # take the full-row tiles, (24, 32, 64) the z-tiled ones
     rho0 = mb.noise_rho(shape, 17)
     ref = mb.Lattice(19, shape, 0.8, -5.5, kernel=mb.KERNEL_TWO_PASS, walls=True)
     ref.set_rho(rho0); ref.init(); ref.step(9)
     want = ref.get_f()
-    for tile in range(4):
-        for chunk in (1, 2, 5, 64):
-            print(f"tile {tile} chunk {chunk}", flush=True)
+    tried = 0
+    for tile in range(6):
+        for chunk, order, nts in ((1, 0, 0), (2, 1, 1), (5, 2, 1), (64, 2, 0)):
             lat = mb.Lattice(19, shape, 0.8, -5.5, kernel=mb.KERNEL_FUSED, walls=True)
-            lat.set_tuning(tile, chunk)
-            assert lat.tuning == (tile, chunk)
+            try:
+                lat.set_tuning(tile, chunk, order, nts)
+            except mb.LbmError:
+                lat.close()
+                break           # this tile shape does not fit the lattice
+            print(f"tile {tile} chunk {chunk} order {order} nts {nts}", flush=True)
+            assert lat.tuning == (tile, chunk, order, nts)
             lat.set_rho(rho0); lat.init(); lat.step(9)
             assert same_bits(lat.get_f(), want), (tile, chunk)
             lat.close()
+            tried += 1
+    assert tried >= 4
 
 
 def test_set_f_roundtrip_and_restart():
