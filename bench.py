@@ -73,6 +73,23 @@ def cpu_baseline(q, shape, tau, G, init, budget_s=15.0):
                       f"threads), {'x'.join(map(str, shape))} lattice, {k} timed steps after 1 warm-up"}
 
 
+def time_workload(name, steps=10, warmup=2):
+    """MLUPS of one workload (fused kernel, library defaults)."""
+    q, shape, prec, tau, G, init = WORKLOADS[name]
+    lat = mb.Lattice(q, shape, tau, G, precision=prec)
+    lat.set_rho(initial_rho(shape, init))
+    lat.init()
+    lat.step_timed(warmup)
+    ms = lat.step_timed(steps) / steps
+    n = int(np.prod(shape))
+    bpl = 2 * q * BYTES_PER_POP[prec]
+    out = {"mlups": round(n / (ms * 1e-3) / 1e6, 1), "ms_per_step": round(ms, 4),
+           "hbm_frac": round(bpl * n / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "kernel": KERNEL_NAMES.get(lat.kernel)}
+    lat.close()
+    return out
+
+
 def load_traffic(kernel_name, workload):
     path = os.path.join(ROOT, "profiles", f"traffic_{kernel_name}_{workload}.json")
     if os.path.exists(path):
@@ -91,14 +108,19 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tuning", default="", metavar="SHAPE,CHUNK[,ORDER[,NTSTORE]]",
                     help="fused-kernel launch configuration (default: the library's choice)")
+    ap.add_argument("--no-precision-forms", action="store_true",
+                    help="skip the 512^3 FP32/FP16/BF16 side measurements")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the streaming copy kernel over MIB MiB (for counter scaling)")
     args = ap.parse_args()
     if args.copy_calibration:
         nbytes = args.copy_calibration * 2**20
-        ms = mb.lbm.probe_copy(nbytes, reps=3)
-        print(json.dumps({"copy_bytes": nbytes, "ms": ms,
-                          "GBs_read_plus_write": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
+        res = {"copy_bytes": nbytes}
+        for mode, name in ((0, "copy_4x16"), (1, "copy_8x16"), (2, "copy_8x16_nt"), (3, "read_only")):
+            ms = mb.lbm.probe_copy(nbytes, reps=3, mode=mode)
+            moved = nbytes * (1 if mode == 3 else 2)
+            res[name] = {"ms": round(ms, 4), "GBs": round(moved / (ms * 1e-3) / 1e9, 1)}
+        print(json.dumps(res), flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -160,6 +182,11 @@ def main():
         "mass_after": cons[0],
     }
     lat.close()
+    if not args.no_precision_forms and args.workload == "d3q19_fp64_256_droplet":
+        forms = {k: time_workload(f"d3q19_{k}_512_noise") for k in ("fp64", "fp32", "fp16", "bf16")}
+        forms["speedup_fp16_vs_fp32"] = round(forms["fp16"]["mlups"] / forms["fp32"]["mlups"], 3)
+        forms["speedup_fp16_vs_fp64"] = round(forms["fp16"]["mlups"] / forms["fp64"]["mlups"], 3)
+        out["precision_forms_512"] = forms
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(q, shape, tau, G, init)
     print(json.dumps(out), flush=True)

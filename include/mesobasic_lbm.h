@@ -147,7 +147,8 @@ int lbm_synchronize(lbm_handle* h);
  *             3: 8x64, 4: 4x64, 5: 8x32; -1 = first that fits
  *   chunk     x planes marched per workgroup
  *   order     block -> tile order: 0 plain, 1 contiguous runs per XCD,
- *             2 y-tiles of one column per XCD
+ *             2 y-tiles of one column per XCD, 3 a y-band x all z-tiles
+ *             per XCD
  *   nt_store  1 = stores carry the non-temporal hint */
 typedef struct lbm_tuning {
     int shape;

@@ -21,10 +21,12 @@ int lbm_probe_exp(const double* x, double* y, int64_t n);
  * nty*ntz*ncx ints.  No device work. */
 int lbm_probe_tile_map(int nty, int ntz, int ncx, int order, int* ty, int* tz, int* cx);
 
-/* Streaming copy kernel over `bytes` (multiple of 16) between two device
- * buffers it allocates, run `reps` times; *ms = average time per copy.  The
- * reference point for scaling HBM traffic counters. */
-int lbm_probe_copy(int64_t bytes, int reps, double* ms);
+/* Streaming reference kernels over `bytes` (multiple of 16) on two device
+ * buffers it allocates, run `reps` times; *ms = average time per run.
+ * mode 0: copy (4 x 16 B per thread), 1: copy (8 x 16 B), 2: copy with
+ * non-temporal loads/stores, 3: read only.  The reference points for scaling
+ * HBM traffic counters and for the achievable bandwidth. */
+int lbm_probe_copy(int64_t bytes, int reps, int mode, double* ms);
 
 #ifdef __cplusplus
 }

@@ -74,7 +74,7 @@ struct lbm_handle {
     bool prev_valid = false;         // buf[1-cur] holds f(N-1)
     bool ref_is_rho0 = false;        // reference arrays still hold rho0, u = 0
     int kernel = LBM_KERNEL_TWO_PASS;
-    lbm::FusedTuning tune{-1, 64, 2, 1};  // fused kernel launch configuration
+    lbm::FusedTuning tune{-1, 64, 3, 1};  // fused kernel launch configuration
     int64_t bytes = 0;
 };
 
@@ -445,7 +445,7 @@ int lbm_set_tuning(lbm_handle* h, const lbm_tuning* t)
         return fail(LBM_EINVAL, "tile shape %d does not fit the %d x %d cross-section", shape, h->d.ny,
                     h->d.nz);
     if (t->chunk < 1) return fail(LBM_EINVAL, "chunk must be >= 1");
-    if (t->order < 0 || t->order > 2) return fail(LBM_EINVAL, "order must be 0, 1 or 2");
+    if (t->order < 0 || t->order > 3) return fail(LBM_EINVAL, "order must be 0, 1, 2 or 3");
     h->tune.shape = shape;
     h->tune.chunk = t->chunk;
     h->tune.order = t->order;

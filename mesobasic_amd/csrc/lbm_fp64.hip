@@ -441,32 +441,59 @@ extern "C" int lbm_probe_exp(const double* x, double* y, int64_t n)
     return rc;
 }
 
-// Plain streaming copy of `bytes` (a multiple of 16) between two fresh
-// device buffers, `reps` times: the reference point that scales HBM traffic
-// counters (FETCH_SIZE / WRITE_SIZE) to bytes.
+// Streaming reference kernels over `bytes` (a multiple of 16): the points
+// that calibrate HBM traffic counters (FETCH_SIZE / WRITE_SIZE) and bound
+// the achievable bandwidth.  mode 0: copy, 4 x 16 B per thread; 1: copy,
+// 8 x 16 B per thread; 2: as 1 with non-temporal loads and stores;
+// 3: read-only (sum of 8 x 16 B per thread into one word per block).
 namespace {
-// Each thread moves 4 x 16 B with all four loads issued before any store.
-__global__ void __launch_bounds__(256) k_probe_copy(const uint4* __restrict__ src,
-                                                    uint4* __restrict__ dst, int64_t n16)
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+template <int K, bool NT>
+__global__ void __launch_bounds__(256) k_probe_copy(const u32x4* __restrict__ src,
+                                                    u32x4* __restrict__ dst, int64_t n16)
 {
-    const int64_t base = static_cast<int64_t>(blockIdx.x) * 1024 + threadIdx.x;
-    uint4 v[4];
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * (256 * K) + threadIdx.x;
+    u32x4 v[K];
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < K; k++) {
         const int64_t i = base + k * 256;
-        if (i < n16) v[k] = src[i];
+        if (i < n16) {
+            if constexpr (NT) v[k] = __builtin_nontemporal_load(src + i);
+            else v[k] = src[i];
+        }
     }
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < K; k++) {
         const int64_t i = base + k * 256;
-        if (i < n16) dst[i] = v[k];
+        if (i < n16) {
+            if constexpr (NT) __builtin_nontemporal_store(v[k], dst + i);
+            else dst[i] = v[k];
+        }
     }
+}
+
+__global__ void __launch_bounds__(256) k_probe_read(const u32x4* __restrict__ src,
+                                                    uint32_t* __restrict__ out, int64_t n16)
+{
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * (256 * 8) + threadIdx.x;
+    uint32_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const int64_t i = base + k * 256;
+        if (i < n16) {
+            const u32x4 v = src[i];
+            acc ^= v.x ^ v.y ^ v.z ^ v.w;
+        }
+    }
+    // one word per block (so the loads are not dead); lane 0 of each wave
+    if ((threadIdx.x & 63) == 0) out[blockIdx.x * 4 + (threadIdx.x >> 6)] = acc;
 }
 }  // namespace
 
-extern "C" int lbm_probe_copy(int64_t bytes, int reps, double* ms)
+extern "C" int lbm_probe_copy(int64_t bytes, int reps, int mode, double* ms)
 {
-    if (bytes <= 0 || bytes % 16 != 0 || reps < 1) return -1;
+    if (bytes <= 0 || bytes % 16 != 0 || reps < 1 || mode < 0 || mode > 3) return -1;
     void* a = nullptr;
     void* b = nullptr;
     if (hipMalloc(&a, bytes) != hipSuccess) return -2;
@@ -477,11 +504,19 @@ extern "C" int lbm_probe_copy(int64_t bytes, int reps, double* ms)
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
     const int64_t n16 = bytes / 16;
-    const unsigned grid = static_cast<unsigned>((n16 + 1023) / 1024);
+    const int per = (mode == 0) ? 4 : 8;
+    const unsigned grid = static_cast<unsigned>((n16 + 256 * per - 1) / (256 * per));
+    // the read-only kernel writes 4 words per block: b has room (grid*16 <= bytes)
     if (!rc) {
         (void)hipEventRecord(e0, nullptr);
-        for (int r = 0; r < reps; r++)
-            k_probe_copy<<<grid, 256>>>(static_cast<const uint4*>(a), static_cast<uint4*>(b), n16);
+        for (int r = 0; r < reps; r++) {
+            const u32x4* src = static_cast<const u32x4*>(a);
+            u32x4* dst = static_cast<u32x4*>(b);
+            if (mode == 0) k_probe_copy<4, false><<<grid, 256>>>(src, dst, n16);
+            else if (mode == 1) k_probe_copy<8, false><<<grid, 256>>>(src, dst, n16);
+            else if (mode == 2) k_probe_copy<8, true><<<grid, 256>>>(src, dst, n16);
+            else k_probe_read<<<grid, 256>>>(src, static_cast<uint32_t*>(b), n16);
+        }
         (void)hipEventRecord(e1, nullptr);
         if (hipEventSynchronize(e1) != hipSuccess) rc = -3;
         float t = 0.f;

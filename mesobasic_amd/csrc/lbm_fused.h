@@ -261,11 +261,16 @@ __device__ __forceinline__ void collide_store(typename SP::T* __restrict__ Pout,
 //   2  b = 8j + g -> ty = j % nty, t = (j / nty)*8 + g (needs ntz*ncx % 8
 //      == 0): y-tiles of one column on one XCD, the 8 XCDs on 8 columns of
 //      the same chunk.
+//   3  b = 8k + g -> ty = g*(nty/8) + (k / ntz) % (nty/8), tz = k % ntz,
+//      cx = k / (ntz*nty/8) (needs nty % 8 == 0): XCD g owns a band of
+//      nty/8 y-tiles and walks all z-tiles of the band before the next
+//      chunk, so both the y and the z neighbours of a tile run on its XCD.
 // Each is a bijection of [0, N) when its condition holds; otherwise order 0.
 __host__ __device__ inline int effective_order(int order, int nty, int ntz, int ncx)
 {
     if (order == 1 && (nty * ntz * ncx) % 8 == 0) return 1;
     if (order == 2 && nty > 1 && (ntz * ncx) % 8 == 0) return 2;
+    if (order == 3 && nty % 8 == 0) return 3;
     return 0;
 }
 
@@ -286,6 +291,14 @@ __host__ __device__ inline void tile_of_block(int b, int nty, int ntz, int ncx, 
         ty = j % nty;
         t = (j / nty) * 8 + (b & 7);
         break;
+    }
+    case 3: {
+        const int k = b >> 3, band = nty >> 3;
+        tz = k % ntz;
+        const int r = k / ntz;
+        ty = (b & 7) * band + r % band;
+        cx = r / band;
+        return;
     }
     default:
         ty = b % nty;

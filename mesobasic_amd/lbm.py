@@ -78,7 +78,7 @@ def load():
         "lbm_fill_droplet": (ctypes.c_int, [ctypes.c_int] * 3 + [ctypes.c_double] * 4 + [_D]),
         "lbm_abi_version": (ctypes.c_int, []),
         "lbm_probe_exp": (ctypes.c_int, [_D, _D, ctypes.c_int64]),
-        "lbm_probe_copy": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, _D]),
+        "lbm_probe_copy": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, ctypes.c_int, _D]),
         "lbm_probe_tile_map": (ctypes.c_int, [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_int)] * 3),
     }
     for name, (res, args) in sig.items():
@@ -128,11 +128,12 @@ def probe_exp(x):
     return y
 
 
-def probe_copy(nbytes, reps=5):
-    """Runs the calibration copy kernel; returns ms per copy."""
+def probe_copy(nbytes, reps=5, mode=0):
+    """Runs a streaming reference kernel (0-2 copy variants, 3 read-only);
+    returns ms per run."""
     lib = load()
     ms = ctypes.c_double()
-    _check(lib.lbm_probe_copy(int(nbytes), int(reps), ctypes.byref(ms)))
+    _check(lib.lbm_probe_copy(int(nbytes), int(reps), int(mode), ctypes.byref(ms)))
     return ms.value
 
 
@@ -202,7 +203,7 @@ class Lattice:
     def stream(self):
         return self._lib.lbm_get_stream(self._h)
 
-    def set_tuning(self, shape=-1, chunk=64, order=2, nt_store=1):
+    def set_tuning(self, shape=-1, chunk=64, order=3, nt_store=1):
         t = Tuning(int(shape), int(chunk), int(order), int(nt_store))
         _check(self._lib.lbm_set_tuning(self._h, ctypes.byref(t)))
 

@@ -24,7 +24,10 @@ def load_fixture(name):
     return meta, data
 
 
-FIXTURES = sorted(f[:-4] for f in os.listdir(GOLDEN) if f.endswith(".npz"))
+# small cases with full fields; "full_*" files hold hashes and norms of the
+# full-size runs instead
+FIXTURES = sorted(f[:-4] for f in os.listdir(GOLDEN) if f.endswith(".npz") and not f.startswith("full_"))
+FULL_FIXTURES = sorted(f[:-4] for f in os.listdir(GOLDEN) if f.endswith(".npz") and f.startswith("full_"))
 
 
 def same_bits(a, b):

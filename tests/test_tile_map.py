@@ -7,7 +7,7 @@ import pytest
 from mesobasic_amd import lbm
 
 
-@pytest.mark.parametrize("order", [0, 1, 2])
+@pytest.mark.parametrize("order", [0, 1, 2, 3])
 @pytest.mark.parametrize("nty,ntz,ncx", [(16, 8, 8), (32, 16, 16), (2, 4, 4), (8, 1, 3), (3, 2, 2),
                                          (1, 5, 7), (24, 3, 8), (5, 8, 1), (7, 7, 7)])
 def test_tile_map_is_a_bijection(nty, ntz, ncx, order):
@@ -44,8 +44,18 @@ def test_tile_map_y_interleave_keeps_columns_on_one_xcd():
             assert set(ty[sel & (tz == z) & (cx == x)]) == set(range(nty))
 
 
+def test_tile_map_band_order_keeps_neighbours_on_one_xcd():
+    nty, ntz, ncx = 64, 8, 8
+    ty, tz, cx = lbm.probe_tile_map(nty, ntz, ncx, 3)
+    b = np.arange(nty * ntz * ncx)
+    for g in range(8):
+        sel = b % 8 == g
+        assert set(ty[sel]) == set(range(8 * g, 8 * g + 8))
+        assert set(tz[sel]) == set(range(ntz))
+
+
 def test_tile_map_exhaustive_small_grids():
-    for order in (0, 1, 2):
+    for order in (0, 1, 2, 3):
         for nty in range(1, 13):
             for ntz in range(1, 9):
                 for ncx in range(1, 9):
