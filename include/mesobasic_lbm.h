@@ -177,6 +177,70 @@ int lbm_fill_droplet(int nx, int ny, int nz, double R, double W, double rl,
 
 int lbm_abi_version(void);
 
+/* ---- batched and serving form -------------------------------------------
+ * Many independent Shan-Chen lattices ("jobs") of different sizes,
+ * stencils and parameters advanced together: one persistent launch per
+ * stencil class per step over device-resident job tables, a fixed number of
+ * steps per tick captured in a hipGraph, jobs admitted against an HBM
+ * budget and retired as they finish (continuous batching).  FP32 arithmetic
+ * with FP32, FP16 or BF16 storage of f - w*rho_shift.  A job's arithmetic
+ * does not depend on the other jobs in the batch.  No counterpart in the
+ * reference, whose driver runs one lattice per process (MesoExecute.cpp:
+ * 107-152); each job follows the reference step order of its stencil
+ * (D2Q9: LBfluid2D.cpp:227-312, D3Q19: scmp3D.cpp:90-119). */
+
+typedef struct lbm_batch lbm_batch;
+
+typedef struct lbm_batch_desc {
+    int precision;          /* LBM_FP32, LBM_FP16S or LBM_BF16S */
+    double rho_shift;       /* storage shift (0 is used for LBM_FP32) */
+    int64_t hbm_budget;     /* bytes the jobs may hold; 0 = 90% of free HBM */
+    int steps_per_tick;     /* steps per tick (rounded up to even) */
+    int use_graph;          /* 1 = replay each tick as a captured hipGraph */
+    int autotune;           /* 1 = time launch configurations on the first ticks */
+} lbm_batch_desc;
+
+typedef struct lbm_job_desc {
+    int q;                  /* 9 or 19 */
+    int nx, ny, nz;         /* nz = 1 for q = 9 */
+    double tau, G;
+    int64_t steps;          /* steps to run; the job finishes after them */
+} lbm_job_desc;
+
+int lbm_batch_create(const lbm_batch_desc* desc, lbm_batch** out);
+void lbm_batch_destroy(lbm_batch* b);
+
+/* Device bytes a job would hold. */
+int64_t lbm_batch_job_bytes(const lbm_batch* b, const lbm_job_desc* job);
+
+/* Admits a job with initial density rho0 (host, nx*ny*nz doubles, zero
+ * velocity).  Returns LBM_ENOMEM, with nothing allocated, if the job does
+ * not fit the remaining budget: the caller keeps it queued and retries
+ * after jobs finish.  The job starts with the next tick. */
+int lbm_batch_submit(lbm_batch* b, const lbm_job_desc* job, const double* rho0, int64_t* job_id);
+
+/* Runs one tick (steps_per_tick steps of every unfinished job; a job stops
+ * exactly at its step count), waits for it, and reports how many jobs are
+ * finished and not yet collected. */
+int lbm_batch_tick(lbm_batch* b, int* n_finished);
+
+/* Ids of finished, uncollected jobs (up to max). */
+int lbm_batch_finished(lbm_batch* b, int64_t* ids, int max, int* n);
+
+/* Density and velocity (moments of the final populations, FP64) of a
+ * finished job into host arrays (any may be NULL); frees the job. */
+int lbm_batch_collect(lbm_batch* b, int64_t job_id, double* rho, double* u, double* v, double* w);
+
+/* Jobs admitted and not collected; device bytes they hold; steps run. */
+int lbm_batch_active(const lbm_batch* b);
+int64_t lbm_batch_bytes_in_use(const lbm_batch* b);
+int64_t lbm_batch_steps_run(const lbm_batch* b);
+
+/* Launch configuration the batch uses (after autotuning): D2Q9 rows per
+ * work item and whether D2Q9 jobs use the row-marching kernel (1) or the
+ * two-pass kernels (0). */
+int lbm_batch_tuning(const lbm_batch* b, int* d2q9_chunk, int* d2q9_rows_kernel);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,59 @@
+// lbm_batch.h -- device tables and launchers of the batched engine: many
+// independent Shan-Chen lattices ("jobs") of different sizes and parameters
+// advanced together, one launch per stencil per step.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lbm {
+namespace batch {
+
+// One job, as the kernels see it (device table entry).
+struct Job {
+    void* P[2];          // population buffers; the input of launch parity p is P[p ^ phase]
+    float* psi;          // psi scratch (two-pass stencils)
+    const float* rho0;   // initial density for the LBfluid2D-order first step (nullptr: none)
+    int nx, ny, nz;      // nz = 1 for D2Q9
+    int phase;
+    int64_t n;           // nodes
+    int64_t ps;          // population plane stride (elements)
+    float tau, G, rs;    // relaxation time, coupling, storage shift
+    int first;           // 1 until the first step has run (device-updated)
+    int64_t remaining;   // steps still to run (device-updated, 0 = idle)
+};
+
+// Unit of work of one launch: a range of x planes (D2Q9 row kernel) or of
+// nodes (two-pass kernels) of one job.
+struct Item {
+    int job;
+    int x0, x1;          // row kernel: planes [x0, x1)
+    int64_t c0, c1;      // two-pass kernels: nodes [c0, c1)
+};
+
+// Launch parameters: job and item tables, the ticket counters of the
+// persistent loop (two, alternating by launch parity) and their count.
+struct Tables {
+    Job* jobs;
+    const Item* items;
+    const int* nitems;   // device: number of valid items (tables change between graph replays)
+    unsigned* tickets;   // [2]
+};
+
+// prec: LBM_FP32 / LBM_FP16S / LBM_BF16S (FP32 compute in every case).
+hipError_t d2q9_rows_step(int prec, const Tables& t, int parity, int threads, int grid,
+                          hipStream_t s);
+hipError_t twopass_psi(int prec, int q, const Tables& t, int parity, int grid, hipStream_t s);
+hipError_t twopass_collide(int prec, int q, const Tables& t, int parity, int grid, hipStream_t s);
+// After a step: remaining -= 1 and first = 0 for every stepped job.
+hipError_t advance_counters(Job* jobs, int njobs, hipStream_t s);
+// Initial equilibrium of one job from rho0 (device FP32): writes the input
+// buffer of the job's next launch so that its first pull yields feq.
+hipError_t init_job(int prec, int q, const Job& j, int buf, hipStream_t s);
+// Moments of a job's current populations into device double arrays.
+hipError_t job_moments(int prec, int q, const Job& j, int buf, double* rho, double* u, double* v,
+                       double* w, hipStream_t s);
+hipError_t zero_u32(unsigned* p, int n, hipStream_t s);
+
+}  // namespace batch
+}  // namespace lbm

@@ -1,0 +1,473 @@
+// lbm_batch.hip -- kernels of the batched engine (lbm_batch.h).
+//
+// Every job holds post-collision populations in the shifted frame
+// d = f - w*rs (16-bit or FP32 storage, FP32 arithmetic) and advances by the
+// same step as the single-lattice FP32/16-bit forms: pull f(x) = f*(x - e)
+// with periodic wrap, rho = rs + sum d, j = sum d e, psi = 1 - exp(-rho),
+// Shan-Chen force (scmp3D.cpp:141-179), BGK collision with velocity shift
+// (LBfluid3D.cpp:259-276) written for d:  d' = d - (d - d_eq)/tau,
+// d_eq = w*(rho*poly - rs).  A job's first step follows the reference's
+// step order for its stencil: D2Q9 jobs collide with the stored rho0 and
+// u = 0 (LBfluid2D order, LBfluid2D.cpp:235-249 after setFtoFeq), D3Q19 jobs
+// with the moments of f0 (scmp3D order, scmp3D.cpp:90-119).
+//
+// Launches are persistent: a fixed grid whose workgroups take items from a
+// ticket counter until the table is exhausted, so uneven jobs balance.  The
+// counter used by launch parity p is tickets[p]; each launch clears the other
+// one, which the next launch (parity p^1) then starts from.
+#include "../../include/mesobasic_lbm.h"
+#include "lbm_batch.h"
+#include "lbm_stencil.h"
+#include "lbm_storage.h"
+
+namespace lbm {
+namespace batch {
+namespace {
+
+__device__ __forceinline__ float psi_of(float rho) { return 1.0f - __expf(-rho); }
+
+__device__ __forceinline__ int wrapi(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+// Takes the next item of this launch for the whole workgroup; -1 when done.
+__device__ __forceinline__ int next_item(const Tables& t, int parity, int* slot)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) *slot = static_cast<int>(atomicAdd(&t.tickets[parity], 1u));
+    __syncthreads();
+    const int it = *slot;
+    return it < *t.nitems ? it : -1;
+}
+
+__device__ __forceinline__ void reset_other(const Tables& t, int parity)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) t.tickets[parity ^ 1] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// D2Q9: one workgroup marches a job along x with one thread per node of a
+// whole y row (periodic in y within the row, so there is no halo ring); the
+// psi of rows x-1 .. x+2 sits in four LDS slots, and the row x+2 pulls are in
+// flight during the collision of row x.
+
+template <typename SP>
+__device__ __forceinline__ void pull_row9(const typename SP::T* __restrict__ P, int64_t ps, int nx,
+                                          int ny, int x, int y, float (&d)[9])
+{
+    using S = St<9>;
+    const int xm = x == 0 ? nx - 1 : x - 1, xp = x == nx - 1 ? 0 : x + 1;
+    const int ym = y == 0 ? ny - 1 : y - 1, yp = y == ny - 1 ? 0 : y + 1;
+    const int xs[3] = {xm, x, xp};
+    const int ys[3] = {ym, y, yp};
+    static_for<9>([&](auto I) {
+        constexpr int n = decltype(I)::value;
+        const int64_t src = static_cast<int64_t>(xs[1 - S::ex[n]]) * ny + ys[1 - S::ey[n]];
+        d[n] = SP::d(P[n * ps + src]);
+    });
+}
+
+template <typename SP, int T>
+__global__ void __launch_bounds__(T) k_d2q9_rows(Tables t, int parity)
+{
+    using S = St<9>;
+    __shared__ float psi_lds[4 * T];
+    __shared__ int slot_s;
+    reset_other(t, parity);
+    const int y = threadIdx.x;
+    for (int it = next_item(t, parity, &slot_s); it >= 0; it = next_item(t, parity, &slot_s)) {
+        const Item item = t.items[it];
+        Job& job = t.jobs[item.job];
+        if (job.remaining <= 0) continue;                 // uniform across the workgroup
+        const int p = parity ^ job.phase;
+        const auto* Pin = static_cast<const typename SP::T*>(job.P[p]);
+        auto* Pout = static_cast<typename SP::T*>(job.P[p ^ 1]);
+        const int nx = job.nx, ny = job.ny;
+        const int64_t ps = job.ps;
+        const float tau = job.tau, G = job.G, rs = job.rs, itau = 1.0f / tau;
+        const bool first = job.first != 0;
+        const float* rho0 = job.rho0;
+        const bool active = y < ny;
+        const int ym = y == 0 ? ny - 1 : y - 1, yp = y == ny - 1 ? 0 : y + 1;
+        const int x0 = item.x0, x1 = item.x1;
+        auto slot = [&](int x) { return ((x - x0 + 1) & 3) * T; };
+        auto xw = [&](int x) { return x >= nx ? x - nx : (x < 0 ? x + nx : x); };
+        auto rho_at = [&](int x, const float (&d)[9]) {
+            if (first && rho0) return rho0[static_cast<int64_t>(x) * ny + y];
+            float r = 0.f;
+            static_for<9>([&](auto I) { r += d[decltype(I)::value]; });
+            return r + rs;
+        };
+        float fa[9], fb[9], fc[9];
+        if (active) {
+            pull_row9<SP>(Pin, ps, nx, ny, xw(x0 - 1), y, fc);
+            psi_lds[slot(x0 - 1) + y] = psi_of(rho_at(xw(x0 - 1), fc));
+            pull_row9<SP>(Pin, ps, nx, ny, x0, y, fa);
+            psi_lds[slot(x0) + y] = psi_of(rho_at(x0, fa));
+            pull_row9<SP>(Pin, ps, nx, ny, xw(x0 + 1), y, fb);
+            psi_lds[slot(x0 + 1) + y] = psi_of(rho_at(xw(x0 + 1), fb));
+        }
+        __syncthreads();
+        auto step = [&](int x, float (&fcur)[9], float (&fnn)[9]) {
+            const int x2 = xw(x + 2);
+            if (active) {
+                pull_row9<SP>(Pin, ps, nx, ny, x2, y, fnn);
+                const float* pm = psi_lds + slot(x - 1);
+                const float* p0 = psi_lds + slot(x);
+                const float* pp = psi_lds + slot(x + 1);
+                float rho, jx = 0.f, jy = 0.f;
+                float r = 0.f;
+                static_for<9>([&](auto I) {
+                    constexpr int n = decltype(I)::value;
+                    r += fcur[n];
+                    acc_e<S::ex[n]>(jx, fcur[n]);
+                    acc_e<S::ey[n]>(jy, fcur[n]);
+                });
+                float ux, uy;
+                if (first && rho0) {
+                    rho = rho0[static_cast<int64_t>(x) * ny + y];
+                    ux = 0.f; uy = 0.f;
+                } else {
+                    rho = r + rs;
+                    const float ir = 1.0f / rho;
+                    ux = jx * ir; uy = jy * ir;
+                }
+                const int yy[3] = {ym, y, yp};
+                float gx = 0.f, gy = 0.f;
+                static_for<9>([&](auto I) {
+                    constexpr int n = decltype(I)::value;
+                    if constexpr (n > 0) {
+                        const float* pl = S::ex[n] < 0 ? pm : (S::ex[n] > 0 ? pp : p0);
+                        const float tt = static_cast<float>(S::w[n]) * pl[yy[1 + S::ey[n]]];
+                        acc_e<S::ex[n]>(gx, tt);
+                        acc_e<S::ey[n]>(gy, tt);
+                    }
+                });
+                const float k = (-G * p0[y]) * tau / rho;
+                const float ue = ux + k * gx, ve = uy + k * gy;
+                const float usq15 = 1.5f * (ue * ue + ve * ve);
+                const int64_t c = static_cast<int64_t>(x) * ny + y;
+                static_for<9>([&](auto I) {
+                    constexpr int n = decltype(I)::value;
+                    const float ev = edot<S::ex[n], S::ey[n], 0>(ue, ve, 0.f);
+                    const float poly = 1.0f + 3.0f * ev + 4.5f * ev * ev - usq15;
+                    const float deq = static_cast<float>(S::w[n]) * (rho * poly - rs);
+                    SP::put_nt(Pout + n * ps + c, SP::of_d(fcur[n] - (fcur[n] - deq) * itau));
+                });
+                psi_lds[slot(x + 2) + y] = psi_of(rho_at(x2, fnn));
+            }
+            __syncthreads();
+        };
+        int x = x0;
+        for (; x + 2 < x1; x += 3) {
+            step(x, fa, fc);
+            step(x + 1, fb, fa);
+            step(x + 2, fc, fb);
+        }
+        if (x < x1) step(x, fa, fc);
+        if (x + 1 < x1) step(x + 1, fb, fa);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Two-pass kernels for any job (D2Q9 or D3Q19, any size): items are node
+// ranges; pass 1 writes psi, pass 2 collides.
+
+template <int Q> struct Pos {
+    int x, y, z;
+};
+
+template <int Q>
+__device__ __forceinline__ void decompose(int64_t c, const Job& j, int& x, int& y, int& z)
+{
+    const int64_t t = c / j.nz;
+    z = static_cast<int>(c - t * j.nz);
+    x = static_cast<int>(t / j.ny);
+    y = static_cast<int>(t - static_cast<int64_t>(x) * j.ny);
+}
+
+template <int Q, typename SP>
+__device__ __forceinline__ void pull_node(const typename SP::T* __restrict__ P, const Job& j, int x,
+                                          int y, int z, float (&d)[Q])
+{
+    using S = St<Q>;
+    const int xs[3] = {x == 0 ? j.nx - 1 : x - 1, x, x == j.nx - 1 ? 0 : x + 1};
+    const int ys[3] = {y == 0 ? j.ny - 1 : y - 1, y, y == j.ny - 1 ? 0 : y + 1};
+    const int zs[3] = {z == 0 ? j.nz - 1 : z - 1, z, z == j.nz - 1 ? 0 : z + 1};
+    static_for<Q>([&](auto I) {
+        constexpr int n = decltype(I)::value;
+        const int64_t src = (static_cast<int64_t>(xs[1 - S::ex[n]]) * j.ny + ys[1 - S::ey[n]]) * j.nz +
+                            zs[1 - S::ez[n]];
+        d[n] = SP::d(P[n * j.ps + src]);
+    });
+}
+
+constexpr int kTP = 256;
+
+template <int Q, typename SP>
+__global__ void __launch_bounds__(kTP) k_tp_psi(Tables t, int parity)
+{
+    __shared__ int slot_s;
+    reset_other(t, parity);
+    for (int it = next_item(t, parity, &slot_s); it >= 0; it = next_item(t, parity, &slot_s)) {
+        const Item item = t.items[it];
+        const Job& job = t.jobs[item.job];
+        if (job.remaining <= 0) continue;
+        const auto* Pin = static_cast<const typename SP::T*>(job.P[parity ^ job.phase]);
+        for (int64_t c = item.c0 + threadIdx.x; c < item.c1; c += kTP) {
+            float rho;
+            if (job.first && job.rho0) {
+                rho = job.rho0[c];
+            } else {
+                int x, y, z;
+                decompose<Q>(c, job, x, y, z);
+                float d[Q];
+                pull_node<Q, SP>(Pin, job, x, y, z, d);
+                float r = 0.f;
+                static_for<Q>([&](auto I) { r += d[decltype(I)::value]; });
+                rho = r + job.rs;
+            }
+            job.psi[c] = psi_of(rho);
+        }
+    }
+}
+
+template <int Q, typename SP>
+__global__ void __launch_bounds__(kTP) k_tp_collide(Tables t, int parity)
+{
+    using S = St<Q>;
+    __shared__ int slot_s;
+    reset_other(t, parity);
+    for (int it = next_item(t, parity, &slot_s); it >= 0; it = next_item(t, parity, &slot_s)) {
+        const Item item = t.items[it];
+        const Job& job = t.jobs[item.job];
+        if (job.remaining <= 0) continue;
+        const int p = parity ^ job.phase;
+        const auto* Pin = static_cast<const typename SP::T*>(job.P[p]);
+        auto* Pout = static_cast<typename SP::T*>(job.P[p ^ 1]);
+        const float tau = job.tau, G = job.G, rs = job.rs, itau = 1.0f / tau;
+        const float* psi = job.psi;
+        for (int64_t c = item.c0 + threadIdx.x; c < item.c1; c += kTP) {
+            int x, y, z;
+            decompose<Q>(c, job, x, y, z);
+            float d[Q];
+            pull_node<Q, SP>(Pin, job, x, y, z, d);
+            float r = 0.f, jx = 0.f, jy = 0.f, jz = 0.f;
+            static_for<Q>([&](auto I) {
+                constexpr int n = decltype(I)::value;
+                r += d[n];
+                acc_e<S::ex[n]>(jx, d[n]);
+                acc_e<S::ey[n]>(jy, d[n]);
+                acc_e<S::ez[n]>(jz, d[n]);
+            });
+            float rho, ux, uy, uz;
+            if (job.first && job.rho0) {
+                rho = job.rho0[c];
+                ux = uy = uz = 0.f;
+            } else {
+                rho = r + rs;
+                const float ir = 1.0f / rho;
+                ux = jx * ir; uy = jy * ir; uz = jz * ir;
+            }
+            const int xs[3] = {x == 0 ? job.nx - 1 : x - 1, x, x == job.nx - 1 ? 0 : x + 1};
+            const int ys[3] = {y == 0 ? job.ny - 1 : y - 1, y, y == job.ny - 1 ? 0 : y + 1};
+            const int zs[3] = {z == 0 ? job.nz - 1 : z - 1, z, z == job.nz - 1 ? 0 : z + 1};
+            float gx = 0.f, gy = 0.f, gz = 0.f;
+            static_for<Q>([&](auto I) {
+                constexpr int n = decltype(I)::value;
+                if constexpr (n > 0) {
+                    const int64_t nb = (static_cast<int64_t>(xs[1 + S::ex[n]]) * job.ny + ys[1 + S::ey[n]]) *
+                                           job.nz + zs[1 + S::ez[n]];
+                    const float tt = static_cast<float>(S::w[n]) * psi[nb];
+                    acc_e<S::ex[n]>(gx, tt);
+                    acc_e<S::ey[n]>(gy, tt);
+                    acc_e<S::ez[n]>(gz, tt);
+                }
+            });
+            const float k = (-G * psi[c]) * tau / rho;
+            const float ue = ux + k * gx, ve = uy + k * gy, we = (S::D == 3) ? uz + k * gz : 0.f;
+            const float usq15 = 1.5f * (ue * ue + ve * ve + we * we);
+            static_for<Q>([&](auto I) {
+                constexpr int n = decltype(I)::value;
+                const float ev = edot<S::ex[n], S::ey[n], S::ez[n]>(ue, ve, we);
+                const float poly = 1.0f + 3.0f * ev + 4.5f * ev * ev - usq15;
+                const float deq = static_cast<float>(S::w[n]) * (rho * poly - rs);
+                SP::put_nt(Pout + n * job.ps + c, SP::of_d(d[n] - (d[n] - deq) * itau));
+            });
+        }
+    }
+}
+
+__global__ void k_advance(Job* jobs, int njobs)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < njobs && jobs[i].remaining > 0) {
+        jobs[i].remaining -= 1;
+        jobs[i].first = 0;
+    }
+}
+
+// Initial equilibrium: psi(rho0) into job.psi, then d_eq of each node
+// scattered to the upstream slots so that the first pull reads it back.
+template <int Q>
+__global__ void __launch_bounds__(kTP) k_init_psi(Job j)
+{
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kTP + threadIdx.x;
+    if (c < j.n) j.psi[c] = psi_of(j.rho0[c]);
+}
+
+template <int Q, typename SP>
+__global__ void __launch_bounds__(kTP) k_init_feq(Job j, int buf)
+{
+    using S = St<Q>;
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kTP + threadIdx.x;
+    if (c >= j.n) return;
+    auto* P = static_cast<typename SP::T*>(j.P[buf]);
+    int x, y, z;
+    decompose<Q>(c, j, x, y, z);
+    const int xs[3] = {x == 0 ? j.nx - 1 : x - 1, x, x == j.nx - 1 ? 0 : x + 1};
+    const int ys[3] = {y == 0 ? j.ny - 1 : y - 1, y, y == j.ny - 1 ? 0 : y + 1};
+    const int zs[3] = {z == 0 ? j.nz - 1 : z - 1, z, z == j.nz - 1 ? 0 : z + 1};
+    float gx = 0.f, gy = 0.f, gz = 0.f;
+    static_for<Q>([&](auto I) {
+        constexpr int n = decltype(I)::value;
+        if constexpr (n > 0) {
+            const int64_t nb = (static_cast<int64_t>(xs[1 + S::ex[n]]) * j.ny + ys[1 + S::ey[n]]) * j.nz +
+                               zs[1 + S::ez[n]];
+            const float tt = static_cast<float>(S::w[n]) * j.psi[nb];
+            acc_e<S::ex[n]>(gx, tt);
+            acc_e<S::ey[n]>(gy, tt);
+            acc_e<S::ez[n]>(gz, tt);
+        }
+    });
+    const float rho = j.rho0[c];
+    const float k = (-j.G * j.psi[c]) * j.tau / rho;
+    const float ue = k * gx, ve = k * gy, we = (S::D == 3) ? k * gz : 0.f;
+    const float usq15 = 1.5f * (ue * ue + ve * ve + we * we);
+    static_for<Q>([&](auto I) {
+        constexpr int n = decltype(I)::value;
+        const float ev = edot<S::ex[n], S::ey[n], S::ez[n]>(ue, ve, we);
+        const float poly = 1.0f + 3.0f * ev + 4.5f * ev * ev - usq15;
+        const float deq = static_cast<float>(S::w[n]) * (rho * poly - j.rs);
+        // slot read by the pull at node c: upstream node c - e_n
+        const int64_t up = (static_cast<int64_t>(xs[1 - S::ex[n]]) * j.ny + ys[1 - S::ey[n]]) * j.nz +
+                           zs[1 - S::ez[n]];
+        P[n * j.ps + up] = SP::of_d(deq);
+    });
+}
+
+template <int Q, typename SP>
+__global__ void __launch_bounds__(kTP) k_job_moments(Job j, int buf, double* __restrict__ rho,
+                                                     double* __restrict__ u, double* __restrict__ v,
+                                                     double* __restrict__ w)
+{
+    using S = St<Q>;
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kTP + threadIdx.x;
+    if (c >= j.n) return;
+    const auto* P = static_cast<const typename SP::T*>(j.P[buf]);
+    int x, y, z;
+    decompose<Q>(c, j, x, y, z);
+    float d[Q];
+    pull_node<Q, SP>(P, j, x, y, z, d);
+    double r = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+    static_for<Q>([&](auto I) {
+        constexpr int n = decltype(I)::value;
+        const double fv = static_cast<double>(d[n]) + S::w[n] * static_cast<double>(j.rs);
+        r += fv;
+        acc_e<S::ex[n]>(sx, fv);
+        acc_e<S::ey[n]>(sy, fv);
+        acc_e<S::ez[n]>(sz, fv);
+    });
+    rho[c] = r;
+    u[c] = sx / r;
+    v[c] = sy / r;
+    w[c] = (S::D == 3) ? sz / r : 0.0;
+}
+
+__global__ void k_zero(unsigned* p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+template <typename F> void by_prec(int prec, F&& f)
+{
+    if (prec == LBM_FP32) f(StF32{});
+    else if (prec == LBM_FP16S) f(StF16{});
+    else f(StBF16{});
+}
+
+inline unsigned blocks(int64_t n) { return static_cast<unsigned>((n + kTP - 1) / kTP); }
+
+}  // namespace
+
+hipError_t d2q9_rows_step(int prec, const Tables& t, int parity, int threads, int grid,
+                          hipStream_t s)
+{
+    if (threads < 1 || threads > 1024) return hipErrorInvalidValue;   // one thread per row node
+    by_prec(prec, [&](auto sp) {
+        using SP = decltype(sp);
+        if (threads <= 64) k_d2q9_rows<SP, 64><<<grid, 64, 0, s>>>(t, parity);
+        else if (threads <= 128) k_d2q9_rows<SP, 128><<<grid, 128, 0, s>>>(t, parity);
+        else if (threads <= 256) k_d2q9_rows<SP, 256><<<grid, 256, 0, s>>>(t, parity);
+        else if (threads <= 512) k_d2q9_rows<SP, 512><<<grid, 512, 0, s>>>(t, parity);
+        else k_d2q9_rows<SP, 1024><<<grid, 1024, 0, s>>>(t, parity);
+    });
+    return hipGetLastError();
+}
+
+hipError_t twopass_psi(int prec, int q, const Tables& t, int parity, int grid, hipStream_t s)
+{
+    by_prec(prec, [&](auto sp) {
+        using SP = decltype(sp);
+        if (q == 9) k_tp_psi<9, SP><<<grid, kTP, 0, s>>>(t, parity);
+        else k_tp_psi<19, SP><<<grid, kTP, 0, s>>>(t, parity);
+    });
+    return hipGetLastError();
+}
+
+hipError_t twopass_collide(int prec, int q, const Tables& t, int parity, int grid, hipStream_t s)
+{
+    by_prec(prec, [&](auto sp) {
+        using SP = decltype(sp);
+        if (q == 9) k_tp_collide<9, SP><<<grid, kTP, 0, s>>>(t, parity);
+        else k_tp_collide<19, SP><<<grid, kTP, 0, s>>>(t, parity);
+    });
+    return hipGetLastError();
+}
+
+hipError_t advance_counters(Job* jobs, int njobs, hipStream_t s)
+{
+    if (njobs > 0) k_advance<<<(njobs + 255) / 256, 256, 0, s>>>(jobs, njobs);
+    return hipGetLastError();
+}
+
+hipError_t init_job(int prec, int q, const Job& j, int buf, hipStream_t s)
+{
+    if (q == 9) k_init_psi<9><<<blocks(j.n), kTP, 0, s>>>(j);
+    else k_init_psi<19><<<blocks(j.n), kTP, 0, s>>>(j);
+    by_prec(prec, [&](auto sp) {
+        using SP = decltype(sp);
+        if (q == 9) k_init_feq<9, SP><<<blocks(j.n), kTP, 0, s>>>(j, buf);
+        else k_init_feq<19, SP><<<blocks(j.n), kTP, 0, s>>>(j, buf);
+    });
+    return hipGetLastError();
+}
+
+hipError_t job_moments(int prec, int q, const Job& j, int buf, double* rho, double* u, double* v,
+                       double* w, hipStream_t s)
+{
+    by_prec(prec, [&](auto sp) {
+        using SP = decltype(sp);
+        if (q == 9) k_job_moments<9, SP><<<blocks(j.n), kTP, 0, s>>>(j, buf, rho, u, v, w);
+        else k_job_moments<19, SP><<<blocks(j.n), kTP, 0, s>>>(j, buf, rho, u, v, w);
+    });
+    return hipGetLastError();
+}
+
+hipError_t zero_u32(unsigned* p, int n, hipStream_t s)
+{
+    k_zero<<<(n + 255) / 256, 256, 0, s>>>(p, n);
+    return hipGetLastError();
+}
+
+}  // namespace batch
+}  // namespace lbm

@@ -1,0 +1,531 @@
+// lbm_batch_capi.cpp -- host side of the batched / serving form: job
+// admission against an HBM budget, device job and work tables, ticks of
+// steps (captured once in a hipGraph and replayed), retirement and the
+// launch-configuration tuner.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "../../include/mesobasic_lbm.h"
+#include "lbm_batch.h"
+#include "lbm_internal.h"
+
+using lbm::batch::Item;
+using lbm::batch::Job;
+using lbm::batch::Tables;
+
+namespace {
+
+#define BHIP(expr)                                                                           \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return lbm::set_error(LBM_EHIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_),  \
+                                  __FILE__, __LINE__);                                       \
+    } while (0)
+
+// Kernel classes of one step: D2Q9 row-marching kernels by row width, and the
+// two-pass kernels (psi then collide) of each stencil.
+enum Cls { kRow64, kRow128, kRow256, kRow512, kRow1024, kTp9Psi, kTp9Col, kTp19Psi, kTp19Col, kNumCls };
+constexpr int kRowWidth[5] = {64, 128, 256, 512, 1024};
+constexpr int64_t kTpItem = 8192;   // nodes per two-pass work item
+
+struct HostJob {
+    int64_t id;
+    lbm_job_desc d;
+    int64_t n, ps;
+    void* P[2] = {nullptr, nullptr};
+    float* psi = nullptr;
+    float* rho0 = nullptr;
+    int64_t bytes = 0;
+    int64_t remaining;
+    int first = 1;
+    bool finished = false;
+};
+
+int64_t plane_stride(int64_t n, int elem)
+{
+    const int64_t per = 256 / elem;
+    int64_t ps = (n + per - 1) / per * per;
+    if ((ps * elem) % 4096 == 0) ps += 512 / elem;
+    return ps;
+}
+
+int elem_bytes(int prec) { return prec == LBM_FP32 ? 4 : 2; }
+
+}  // namespace
+
+struct lbm_batch {
+    lbm_batch_desc d{};
+    hipStream_t stream = nullptr;
+    int cus = 256;
+    std::map<int64_t, std::unique_ptr<HostJob>> jobs;   // admitted, not collected
+    std::vector<HostJob*> order;                        // device table order
+    int64_t next_id = 1;
+    int64_t in_use = 0;
+    int64_t steps_run = 0;
+    // device tables
+    Job* d_jobs = nullptr;
+    int cap_jobs = 0;
+    Item* d_items[kNumCls] = {};
+    int cap_items[kNumCls] = {};
+    int* d_nitems = nullptr;      // [kNumCls]
+    unsigned* d_tickets = nullptr; // [kNumCls][2]
+    int nitems[kNumCls] = {};
+    unsigned class_mask = 0;      // classes with items; only these are launched
+    bool dirty = true;
+    // launch configuration
+    int chunk = 32;               // D2Q9 rows per row-kernel item
+    int rows_kernel = 1;          // D2Q9 jobs on the row kernel (1) or two-pass (0)
+    int tune_left = 0;            // ticks left to try candidate configurations
+    std::vector<std::pair<int, int>> candidates;
+    std::vector<double> cand_rate;
+    int cand_idx = 0;
+    // tick graph
+    hipGraphExec_t graph = nullptr;
+    bool graph_valid = false;
+    // pinned staging for per-tick counters
+    Job* h_jobs = nullptr;
+};
+
+namespace {
+
+int row_class(int ny)
+{
+    for (int k = 0; k < 5; k++)
+        if (ny <= kRowWidth[k]) return k;
+    return -1;
+}
+
+bool use_rows(const lbm_batch* b, const HostJob& j)
+{
+    return j.d.q == 9 && b->rows_kernel && row_class(j.d.ny) >= 0;
+}
+
+int ensure_items(lbm_batch* b, int cls, int n)
+{
+    if (b->cap_items[cls] >= n) return LBM_OK;
+    const int cap = std::max(n, 2 * b->cap_items[cls] + 64);
+    if (b->d_items[cls]) (void)hipFree(b->d_items[cls]);
+    b->d_items[cls] = nullptr;
+    b->cap_items[cls] = 0;
+    BHIP(hipMalloc(&b->d_items[cls], sizeof(Item) * cap));
+    b->cap_items[cls] = cap;
+    b->graph_valid = false;   // kernel arguments hold the table pointers
+    return LBM_OK;
+}
+
+int ensure_jobs(lbm_batch* b, int n)
+{
+    if (b->cap_jobs >= n) return LBM_OK;
+    const int cap = std::max(n, 2 * b->cap_jobs + 64);
+    if (b->d_jobs) (void)hipFree(b->d_jobs);
+    if (b->h_jobs) (void)hipHostFree(b->h_jobs);
+    b->d_jobs = nullptr;
+    b->h_jobs = nullptr;
+    b->cap_jobs = 0;
+    BHIP(hipMalloc(&b->d_jobs, sizeof(Job) * cap));
+    BHIP(hipHostMalloc(&b->h_jobs, sizeof(Job) * cap, hipHostMallocDefault));
+    static_assert(sizeof(Job) % 4 == 0, "job table is cleared as 32-bit words");
+    // unused table entries are cleared: the per-step counter kernel walks the
+    // whole capacity
+    BHIP(lbm::batch::zero_u32(reinterpret_cast<unsigned*>(b->d_jobs),
+                              static_cast<int>(sizeof(Job) * cap / 4), b->stream));
+    b->cap_jobs = cap;
+    b->graph_valid = false;
+    return LBM_OK;
+}
+
+Job device_view(const HostJob& j, int prec, double rs)
+{
+    Job v{};
+    v.P[0] = j.P[0];
+    v.P[1] = j.P[1];
+    v.psi = j.psi;
+    v.rho0 = (j.d.q == 9) ? j.rho0 : nullptr;   // LBfluid2D-order first step
+    v.nx = j.d.nx;
+    v.ny = j.d.ny;
+    v.nz = j.d.q == 9 ? 1 : j.d.nz;
+    v.phase = 0;   // jobs join at tick boundaries, where the launch parity is 0
+    v.n = j.n;
+    v.ps = j.ps;
+    v.tau = static_cast<float>(j.d.tau);
+    v.G = static_cast<float>(j.d.G);
+    v.rs = static_cast<float>(prec == LBM_FP32 ? 0.0 : rs);
+    v.first = j.first;
+    v.remaining = j.remaining;
+    return v;
+}
+
+// Rebuilds and uploads the job and item tables (between ticks only).
+int upload_tables(lbm_batch* b)
+{
+    b->order.clear();
+    for (auto& kv : b->jobs)
+        if (!kv.second->finished) b->order.push_back(kv.second.get());
+    int rc = ensure_jobs(b, static_cast<int>(b->order.size()) + 1);
+    if (rc) return rc;
+    std::vector<Item> items[kNumCls];
+    for (int ji = 0; ji < static_cast<int>(b->order.size()); ji++) {
+        const HostJob& j = *b->order[ji];
+        b->h_jobs[ji] = device_view(j, b->d.precision, b->d.rho_shift);
+        if (use_rows(b, j)) {
+            const int cls = row_class(j.d.ny);
+            for (int x0 = 0; x0 < j.d.nx; x0 += b->chunk) {
+                Item it{};
+                it.job = ji;
+                it.x0 = x0;
+                it.x1 = std::min(j.d.nx, x0 + b->chunk);
+                items[cls].push_back(it);
+            }
+        } else {
+            const int cp = j.d.q == 9 ? kTp9Psi : kTp19Psi;
+            for (int64_t c0 = 0; c0 < j.n; c0 += kTpItem) {
+                Item it{};
+                it.job = ji;
+                it.c0 = c0;
+                it.c1 = std::min(j.n, c0 + kTpItem);
+                items[cp].push_back(it);
+                items[cp + 1].push_back(it);
+            }
+        }
+    }
+    // largest jobs first, so that the long items start early
+    unsigned mask = 0;
+    for (int c = 0; c < kNumCls; c++) {
+        std::stable_sort(items[c].begin(), items[c].end(), [&](const Item& a, const Item& z) {
+            return b->order[a.job]->n > b->order[z.job]->n;
+        });
+        rc = ensure_items(b, c, static_cast<int>(items[c].size()) + 1);
+        if (rc) return rc;
+        b->nitems[c] = static_cast<int>(items[c].size());
+        if (!items[c].empty()) mask |= 1u << c;
+        if (!items[c].empty())
+            BHIP(hipMemcpyAsync(b->d_items[c], items[c].data(), sizeof(Item) * items[c].size(),
+                                hipMemcpyHostToDevice, b->stream));
+    }
+    if (!b->order.empty())
+        BHIP(hipMemcpyAsync(b->d_jobs, b->h_jobs, sizeof(Job) * b->order.size(),
+                            hipMemcpyHostToDevice, b->stream));
+    BHIP(hipMemcpyAsync(b->d_nitems, b->nitems, sizeof(int) * kNumCls, hipMemcpyHostToDevice,
+                        b->stream));
+    BHIP(hipStreamSynchronize(b->stream));
+    if (mask != b->class_mask) b->graph_valid = false;   // the launch sequence changes
+    b->class_mask = mask;
+    b->dirty = false;
+    return LBM_OK;
+}
+
+int persistent_grid(const lbm_batch* b, int threads)
+{
+    // enough workgroups to fill every CU at the kernel's occupancy
+    const int per_cu = std::max(1, 2048 / threads);
+    return b->cus * std::min(per_cu, 8);
+}
+
+// One step of every job: row kernels, two-pass kernels, counters.  Only the
+// classes that hold items are launched.
+int launch_step(lbm_batch* b, int parity)
+{
+    for (int k = 0; k < 5; k++) {
+        if (!(b->class_mask & (1u << k))) continue;
+        Tables t{b->d_jobs, b->d_items[k], b->d_nitems + k, b->d_tickets + 2 * k};
+        BHIP(lbm::batch::d2q9_rows_step(b->d.precision, t, parity, kRowWidth[k],
+                                        persistent_grid(b, kRowWidth[k]), b->stream));
+    }
+    for (int q : {9, 19}) {
+        const int cp = q == 9 ? kTp9Psi : kTp19Psi;
+        if (!(b->class_mask & (1u << cp))) continue;
+        Tables tp{b->d_jobs, b->d_items[cp], b->d_nitems + cp, b->d_tickets + 2 * cp};
+        Tables tc{b->d_jobs, b->d_items[cp + 1], b->d_nitems + cp + 1, b->d_tickets + 2 * (cp + 1)};
+        BHIP(lbm::batch::twopass_psi(b->d.precision, q, tp, parity, persistent_grid(b, 256), b->stream));
+        BHIP(lbm::batch::twopass_collide(b->d.precision, q, tc, parity, persistent_grid(b, 256),
+                                         b->stream));
+    }
+    BHIP(lbm::batch::advance_counters(b->d_jobs, b->cap_jobs, b->stream));
+    return LBM_OK;
+}
+
+// Captures the K steps of a tick into a graph (when graphs are on and the
+// launch sequence changed since the last capture).
+int prepare_tick(lbm_batch* b)
+{
+    const int K = b->d.steps_per_tick;
+    if (b->d.use_graph) {
+        if (!b->graph_valid) {
+            if (b->graph) (void)hipGraphExecDestroy(b->graph);
+            b->graph = nullptr;
+            hipGraph_t g = nullptr;
+            BHIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            int rc = LBM_OK;
+            for (int s = 0; s < K && rc == LBM_OK; s++) rc = launch_step(b, s & 1);
+            hipError_t e = hipStreamEndCapture(b->stream, &g);
+            if (rc) return rc;
+            if (e != hipSuccess) return lbm::set_error(LBM_EHIP, "graph capture: %s", hipGetErrorString(e));
+            e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) return lbm::set_error(LBM_EHIP, "graph instantiate: %s", hipGetErrorString(e));
+            b->graph_valid = true;
+        }
+    }
+    return LBM_OK;
+}
+
+int run_tick(lbm_batch* b)
+{
+    const int K = b->d.steps_per_tick;
+    if (b->d.use_graph) {
+        BHIP(hipGraphLaunch(b->graph, b->stream));
+    } else {
+        for (int s = 0; s < K; s++) {
+            int rc = launch_step(b, s & 1);
+            if (rc) return rc;
+        }
+    }
+    return LBM_OK;
+}
+
+void free_job(lbm_batch* b, HostJob* j)
+{
+    for (void* p : j->P)
+        if (p) (void)hipFree(p);
+    if (j->psi) (void)hipFree(j->psi);
+    if (j->rho0) (void)hipFree(j->rho0);
+    b->in_use -= j->bytes;
+}
+
+int64_t job_bytes(const lbm_batch* b, const lbm_job_desc& d)
+{
+    const int64_t n = int64_t(d.nx) * d.ny * (d.q == 9 ? 1 : d.nz);
+    const int elem = elem_bytes(b->d.precision);
+    return 2 * plane_stride(n, elem) * d.q * elem + 8 * n;   // populations, psi, rho0
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_batch_create(const lbm_batch_desc* desc, lbm_batch** out)
+{
+    if (!desc || !out) return lbm::set_error(LBM_EINVAL, "lbm_batch_create: NULL argument");
+    *out = nullptr;
+    if (desc->precision != LBM_FP32 && desc->precision != LBM_FP16S && desc->precision != LBM_BF16S)
+        return lbm::set_error(LBM_EINVAL, "batch precision must be LBM_FP32, LBM_FP16S or LBM_BF16S");
+    if (desc->steps_per_tick < 1) return lbm::set_error(LBM_EINVAL, "steps_per_tick must be >= 1");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return lbm::set_error(LBM_ENODEV, "lbm_batch_create: no HIP device");
+    auto b = std::make_unique<lbm_batch>();
+    b->d = *desc;
+    b->d.steps_per_tick += b->d.steps_per_tick & 1;   // even: every tick starts at parity 0
+    int dev = 0;
+    BHIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    BHIP(hipGetDeviceProperties(&prop, dev));
+    b->cus = prop.multiProcessorCount;
+    if (b->d.hbm_budget <= 0) {
+        size_t free_b = 0, total_b = 0;
+        BHIP(hipMemGetInfo(&free_b, &total_b));
+        b->d.hbm_budget = static_cast<int64_t>(0.9 * static_cast<double>(free_b));
+    }
+    BHIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    BHIP(hipMalloc(&b->d_nitems, sizeof(int) * kNumCls));
+    BHIP(hipMalloc(&b->d_tickets, sizeof(unsigned) * 2 * kNumCls));
+    BHIP(lbm::batch::zero_u32(b->d_tickets, 2 * kNumCls, b->stream));
+    BHIP(hipStreamSynchronize(b->stream));
+    if (b->d.autotune) {
+        for (int rk : {1, 0})
+            for (int ch : {16, 32, 64})
+                if (rk || ch == 32) b->candidates.emplace_back(rk, ch);
+        b->cand_rate.assign(b->candidates.size(), 0.0);
+        b->tune_left = static_cast<int>(b->candidates.size());
+        b->rows_kernel = b->candidates[0].first;
+        b->chunk = b->candidates[0].second;
+    }
+    *out = b.release();
+    return LBM_OK;
+}
+
+void lbm_batch_destroy(lbm_batch* b)
+{
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (auto& kv : b->jobs) free_job(b, kv.second.get());
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    if (b->d_jobs) (void)hipFree(b->d_jobs);
+    if (b->h_jobs) (void)hipHostFree(b->h_jobs);
+    for (Item* p : b->d_items)
+        if (p) (void)hipFree(p);
+    if (b->d_nitems) (void)hipFree(b->d_nitems);
+    if (b->d_tickets) (void)hipFree(b->d_tickets);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int64_t lbm_batch_job_bytes(const lbm_batch* b, const lbm_job_desc* job)
+{
+    return (b && job) ? job_bytes(b, *job) : -1;
+}
+
+int lbm_batch_submit(lbm_batch* b, const lbm_job_desc* job, const double* rho0, int64_t* job_id)
+{
+    if (!b || !job || !rho0) return lbm::set_error(LBM_EINVAL, "lbm_batch_submit: NULL argument");
+    if (job->q != 9 && job->q != 19) return lbm::set_error(LBM_EINVAL, "job q must be 9 or 19");
+    const int nz = job->q == 9 ? 1 : job->nz;
+    if (job->nx < 3 || job->ny < 3 || nz < 1 || (job->q == 19 && nz < 3))
+        return lbm::set_error(LBM_EINVAL, "job lattice too small (%d x %d x %d)", job->nx, job->ny, nz);
+    if (!(job->tau > 0.5) || !std::isfinite(job->G) || job->steps < 0)
+        return lbm::set_error(LBM_EINVAL, "job needs tau > 0.5, finite G, steps >= 0");
+    const int64_t n = int64_t(job->nx) * job->ny * nz;
+    if (n >= (int64_t(1) << 31)) return lbm::set_error(LBM_EINVAL, "job too large");
+    const int64_t bytes = job_bytes(b, *job);
+    if (b->in_use + bytes > b->d.hbm_budget)
+        return lbm::set_error(LBM_ENOMEM, "job needs %lld B, %lld B of the %lld B budget left",
+                              (long long)bytes, (long long)(b->d.hbm_budget - b->in_use),
+                              (long long)b->d.hbm_budget);
+    auto j = std::make_unique<HostJob>();
+    j->id = b->next_id++;
+    j->d = *job;
+    j->d.nz = nz;
+    j->n = n;
+    j->ps = plane_stride(n, elem_bytes(b->d.precision));
+    j->remaining = job->steps;
+    j->bytes = bytes;
+    const int64_t pop = j->ps * job->q * elem_bytes(b->d.precision);
+    auto undo = [&](int code) {
+        free_job(b, j.get());
+        b->in_use += j->bytes;   // free_job subtracted what was never added
+        return code;
+    };
+    for (int k = 0; k < 2; k++)
+        if (hipMalloc(&j->P[k], pop) != hipSuccess)
+            return undo(lbm::set_error(LBM_ENOMEM, "hipMalloc of job populations failed"));
+    if (hipMalloc(&j->psi, n * 4) != hipSuccess || hipMalloc(&j->rho0, n * 4) != hipSuccess)
+        return undo(lbm::set_error(LBM_ENOMEM, "hipMalloc of job fields failed"));
+    std::vector<float> r32(static_cast<size_t>(n));
+    for (int64_t c = 0; c < n; c++) r32[c] = static_cast<float>(rho0[c]);
+    if (hipMemcpyAsync(j->rho0, r32.data(), n * 4, hipMemcpyHostToDevice, b->stream) != hipSuccess)
+        return undo(lbm::set_error(LBM_EHIP, "upload of rho0 failed"));
+    Job v = device_view(*j, b->d.precision, b->d.rho_shift);
+    v.rho0 = j->rho0;   // the initialisation reads it for both stencils
+    if (lbm::batch::init_job(b->d.precision, job->q, v, 0, b->stream) != hipSuccess ||
+        hipStreamSynchronize(b->stream) != hipSuccess)
+        return undo(lbm::set_error(LBM_EHIP, "job initialisation failed"));
+    b->in_use += bytes;
+    *job_id = j->id;
+    if (job->steps == 0) j->finished = true;
+    b->jobs[j->id] = std::move(j);
+    b->dirty = true;
+    return LBM_OK;
+}
+
+int lbm_batch_tick(lbm_batch* b, int* n_finished)
+{
+    if (!b) return lbm::set_error(LBM_EINVAL, "lbm_batch_tick: NULL batch");
+    if (b->dirty) {
+        int rc = upload_tables(b);
+        if (rc) return rc;
+    }
+    if (!b->order.empty()) {
+        int rc = prepare_tick(b);
+        if (rc) return rc;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    int64_t nodes = 0;
+    for (HostJob* j : b->order) nodes += j->n * std::min<int64_t>(j->remaining, b->d.steps_per_tick);
+    if (!b->order.empty()) {
+        int rc = run_tick(b);
+        if (rc) return rc;
+        BHIP(hipMemcpyAsync(b->h_jobs, b->d_jobs, sizeof(Job) * b->order.size(),
+                            hipMemcpyDeviceToHost, b->stream));
+    }
+    BHIP(hipStreamSynchronize(b->stream));
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    b->steps_run += b->d.steps_per_tick;
+    for (int ji = 0; ji < static_cast<int>(b->order.size()); ji++) {
+        HostJob* j = b->order[ji];
+        j->remaining = b->h_jobs[ji].remaining;
+        j->first = b->h_jobs[ji].first;
+        if (j->remaining == 0 && !j->finished) {
+            j->finished = true;
+            b->dirty = true;   // finished jobs leave the work tables
+        }
+    }
+    // tuner: one candidate configuration per tick, then the fastest
+    if (b->tune_left > 0 && nodes > 0) {
+        b->cand_rate[b->cand_idx] = nodes / std::max(secs, 1e-9);
+        b->tune_left--;
+        b->cand_idx++;
+        if (b->tune_left > 0) {
+            b->rows_kernel = b->candidates[b->cand_idx].first;
+            b->chunk = b->candidates[b->cand_idx].second;
+        } else {
+            const size_t best = std::max_element(b->cand_rate.begin(), b->cand_rate.end()) -
+                                b->cand_rate.begin();
+            b->rows_kernel = b->candidates[best].first;
+            b->chunk = b->candidates[best].second;
+        }
+        b->dirty = true;
+        b->graph_valid = false;
+    }
+    int nf = 0;
+    for (auto& kv : b->jobs) nf += kv.second->finished ? 1 : 0;
+    if (n_finished) *n_finished = nf;
+    return LBM_OK;
+}
+
+int lbm_batch_finished(lbm_batch* b, int64_t* ids, int max, int* n)
+{
+    if (!b || !n) return lbm::set_error(LBM_EINVAL, "lbm_batch_finished: NULL argument");
+    int k = 0;
+    for (auto& kv : b->jobs)
+        if (kv.second->finished && k < max && ids) ids[k++] = kv.first;
+    *n = k;
+    return LBM_OK;
+}
+
+int lbm_batch_collect(lbm_batch* b, int64_t job_id, double* rho, double* u, double* v, double* w)
+{
+    if (!b) return lbm::set_error(LBM_EINVAL, "lbm_batch_collect: NULL batch");
+    auto it = b->jobs.find(job_id);
+    if (it == b->jobs.end()) return lbm::set_error(LBM_EINVAL, "unknown job %lld", (long long)job_id);
+    HostJob* j = it->second.get();
+    if (!j->finished) return lbm::set_error(LBM_ESTATE, "job %lld has not finished", (long long)job_id);
+    const int64_t n = j->n;
+    double* d = nullptr;
+    BHIP(hipMalloc(&d, 4 * n * 8));
+    Job view = device_view(*j, b->d.precision, b->d.rho_shift);
+    const int buf = static_cast<int>(j->d.steps & 1);   // the buffer the last step wrote
+    hipError_t e = lbm::batch::job_moments(b->d.precision, j->d.q, view, buf, d, d + n, d + 2 * n,
+                                           d + 3 * n, b->stream);
+    double* outs[4] = {rho, u, v, w};
+    for (int k = 0; k < 4 && e == hipSuccess; k++)
+        if (outs[k]) e = hipMemcpyAsync(outs[k], d + k * n, n * 8, hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return lbm::set_error(LBM_EHIP, "collect: %s", hipGetErrorString(e));
+    free_job(b, j);
+    b->jobs.erase(it);
+    return LBM_OK;
+}
+
+int lbm_batch_active(const lbm_batch* b) { return b ? static_cast<int>(b->jobs.size()) : -1; }
+
+int64_t lbm_batch_bytes_in_use(const lbm_batch* b) { return b ? b->in_use : -1; }
+
+int64_t lbm_batch_steps_run(const lbm_batch* b) { return b ? b->steps_run : -1; }
+
+int lbm_batch_tuning(const lbm_batch* b, int* d2q9_chunk, int* d2q9_rows_kernel)
+{
+    if (!b) return lbm::set_error(LBM_EINVAL, "lbm_batch_tuning: NULL batch");
+    if (d2q9_chunk) *d2q9_chunk = b->chunk;
+    if (d2q9_rows_kernel) *d2q9_rows_kernel = b->rows_kernel;
+    return LBM_OK;
+}
+
+}  // extern "C"

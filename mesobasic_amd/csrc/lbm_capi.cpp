@@ -18,10 +18,12 @@
 #include "lbm_lowp.h"
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
 
-int fail(int code, const char* fmt, ...)
+namespace lbm {
+// Records the message returned by lbm_last_error() on this thread.
+int set_error(int code, const char* fmt, ...)
 {
     char buf[512];
     va_list ap;
@@ -31,6 +33,11 @@ int fail(int code, const char* fmt, ...)
     g_err = buf;
     return code;
 }
+}  // namespace lbm
+
+namespace {
+
+#define fail lbm::set_error
 
 #define HIP_OK(expr)                                                                  \
     do {                                                                              \

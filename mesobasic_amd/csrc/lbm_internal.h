@@ -7,6 +7,9 @@
 
 namespace lbm {
 
+// Sets the calling thread's lbm_last_error() message; returns code.
+int set_error(int code, const char* fmt, ...);
+
 // Lattice geometry on the device.  Node index c = (x*ny + y)*nz + z
 // (nz = 1 in 2-D), the interior order of the reference arrays without the
 // ghost shell; population n of node c lives at n*ps + c (structure of

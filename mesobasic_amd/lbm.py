@@ -35,6 +35,17 @@ class Desc(ctypes.Structure):
                 ("rho_shift", ctypes.c_double), ("kernel", ctypes.c_int)]
 
 
+class BatchDesc(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("rho_shift", ctypes.c_double),
+                ("hbm_budget", ctypes.c_int64), ("steps_per_tick", ctypes.c_int),
+                ("use_graph", ctypes.c_int), ("autotune", ctypes.c_int)]
+
+
+class JobDesc(ctypes.Structure):
+    _fields_ = [("q", ctypes.c_int), ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("tau", ctypes.c_double), ("G", ctypes.c_double), ("steps", ctypes.c_int64)]
+
+
 class Tuning(ctypes.Structure):
     _fields_ = [("shape", ctypes.c_int), ("chunk", ctypes.c_int), ("order", ctypes.c_int),
                 ("nt_store", ctypes.c_int)]
@@ -78,6 +89,18 @@ def load():
         "lbm_fill_droplet": (ctypes.c_int, [ctypes.c_int] * 3 + [ctypes.c_double] * 4 + [_D]),
         "lbm_abi_version": (ctypes.c_int, []),
         "lbm_probe_exp": (ctypes.c_int, [_D, _D, ctypes.c_int64]),
+        "lbm_batch_create": (ctypes.c_int, [ctypes.POINTER(BatchDesc), ctypes.POINTER(P)]),
+        "lbm_batch_destroy": (None, [P]),
+        "lbm_batch_job_bytes": (ctypes.c_int64, [P, ctypes.POINTER(JobDesc)]),
+        "lbm_batch_submit": (ctypes.c_int, [P, ctypes.POINTER(JobDesc), _D, ctypes.POINTER(ctypes.c_int64)]),
+        "lbm_batch_tick": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int)]),
+        "lbm_batch_finished": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int64), ctypes.c_int,
+                                              ctypes.POINTER(ctypes.c_int)]),
+        "lbm_batch_collect": (ctypes.c_int, [P, ctypes.c_int64, _D, _D, _D, _D]),
+        "lbm_batch_active": (ctypes.c_int, [P]),
+        "lbm_batch_bytes_in_use": (ctypes.c_int64, [P]),
+        "lbm_batch_steps_run": (ctypes.c_int64, [P]),
+        "lbm_batch_tuning": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
         "lbm_probe_copy": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, ctypes.c_int, _D]),
         "lbm_probe_tile_map": (ctypes.c_int, [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_int)] * 3),
     }
@@ -258,3 +281,87 @@ class Lattice:
         out = np.empty(4)
         _check(self._lib.lbm_conservation(self._h, _dptr(out)))
         return out
+
+
+class Batch:
+    """Many independent lattices advanced together (the batched / serving
+    form of the C ABI).  Jobs are admitted against an HBM budget with
+    submit(), advanced by tick(), and read back and freed by collect()."""
+
+    def __init__(self, precision=FP16S, rho_shift=0.7, hbm_budget=0, steps_per_tick=20,
+                 use_graph=True, autotune=False):
+        lib = load()
+        if isinstance(precision, str):
+            precision = PRECISIONS[precision]
+        d = BatchDesc(int(precision), float(rho_shift), int(hbm_budget), int(steps_per_tick),
+                      int(bool(use_graph)), int(bool(autotune)))
+        h = ctypes.c_void_p()
+        _check(lib.lbm_batch_create(ctypes.byref(d), ctypes.byref(h)))
+        self._h, self._lib = h, lib
+        self.shapes = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.lbm_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @staticmethod
+    def _desc(q, shape, tau, G, steps):
+        shape = tuple(int(s) for s in shape)
+        nz = shape[2] if len(shape) == 3 else 1
+        return JobDesc(int(q), shape[0], shape[1], nz, float(tau), float(G), int(steps))
+
+    def job_bytes(self, q, shape, tau=1.0, G=-5.0, steps=1):
+        d = self._desc(q, shape, tau, G, steps)
+        return self._lib.lbm_batch_job_bytes(self._h, ctypes.byref(d))
+
+    def submit(self, q, shape, tau, G, steps, rho0):
+        """Returns the job id, or None when the job does not fit the budget."""
+        d = self._desc(q, shape, tau, G, steps)
+        r = np.ascontiguousarray(rho0, dtype=np.float64).reshape(-1)
+        jid = ctypes.c_int64()
+        rc = self._lib.lbm_batch_submit(self._h, ctypes.byref(d), _dptr(r), ctypes.byref(jid))
+        if rc == -2:
+            return None
+        _check(rc)
+        self.shapes[jid.value] = tuple(int(s) for s in shape)
+        return jid.value
+
+    def tick(self):
+        n = ctypes.c_int()
+        _check(self._lib.lbm_batch_tick(self._h, ctypes.byref(n)))
+        return n.value
+
+    def finished(self, max_ids=1 << 16):
+        ids = (ctypes.c_int64 * max_ids)()
+        n = ctypes.c_int()
+        _check(self._lib.lbm_batch_finished(self._h, ids, max_ids, ctypes.byref(n)))
+        return [ids[i] for i in range(n.value)]
+
+    def collect(self, job_id):
+        shape = self.shapes.pop(job_id)
+        n = int(np.prod(shape))
+        rho, u, v, w = (np.empty(n) for _ in range(4))
+        _check(self._lib.lbm_batch_collect(self._h, int(job_id), _dptr(rho), _dptr(u), _dptr(v), _dptr(w)))
+        return dict(rho=rho.reshape(shape), u=u.reshape(shape), v=v.reshape(shape), w=w.reshape(shape))
+
+    @property
+    def active(self):
+        return self._lib.lbm_batch_active(self._h)
+
+    @property
+    def bytes_in_use(self):
+        return self._lib.lbm_batch_bytes_in_use(self._h)
+
+    @property
+    def steps_run(self):
+        return self._lib.lbm_batch_steps_run(self._h)
+
+    @property
+    def tuning(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        _check(self._lib.lbm_batch_tuning(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return dict(d2q9_chunk=a.value, d2q9_rows_kernel=b.value)

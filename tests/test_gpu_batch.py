@@ -1,0 +1,107 @@
+"""Batched / serving form: many jobs of different sizes, stencils and
+parameters in one batch.  Checks that a job's result does not depend on the
+other jobs, on graph capture, on the launch configuration the tuner picks,
+or on where a tick boundary falls; that it stops exactly at its step count;
+that admission respects the HBM budget; and that results agree with the
+FP64 oracle to the 16-bit storage tolerance."""
+import numpy as np
+import pytest
+
+import mesobasic_amd as mb
+from mesobasic_amd.lbm import Batch
+import oracle
+from conftest import rel_l2, same_bits
+
+pytestmark = pytest.mark.gpu
+
+
+def _jobs(rng, n, q9_share=0.7):
+    out = []
+    for _ in range(n):
+        if rng.random() < q9_share:
+            shape = (int(rng.integers(20, 140)), int(rng.integers(20, 140)))
+            q = 9
+        else:
+            shape = tuple(int(v) for v in rng.integers(8, 28, 3))
+            q = 19
+        out.append(dict(q=q, shape=shape, tau=float(rng.uniform(0.6, 1.5)), G=float(rng.uniform(-6.0, -4.5)),
+                        steps=int(rng.integers(5, 60)), seed=int(rng.integers(1 << 30))))
+    return out
+
+
+def _run(batch, jobs):
+    ids = {}
+    for k, j in enumerate(jobs):
+        rho0 = mb.noise_rho(j["shape"], j["seed"])
+        jid = batch.submit(j["q"], j["shape"], j["tau"], j["G"], j["steps"], rho0)
+        assert jid is not None
+        ids[jid] = k
+    out = {}
+    while len(out) < len(jobs):
+        batch.tick()
+        for jid in batch.finished():
+            out[ids[jid]] = batch.collect(jid)
+    return out
+
+
+def test_alone_equals_in_the_mix_bitwise():
+    rng = np.random.default_rng(5)
+    jobs = _jobs(rng, 24)
+    mixed = _run(Batch(precision=mb.FP16S, steps_per_tick=8), jobs)
+    for k in (0, 3, 7, 11, 19):
+        alone = _run(Batch(precision=mb.FP16S, steps_per_tick=6), [jobs[k]])[0]
+        for f in ("rho", "u", "v", "w"):
+            assert same_bits(alone[f], mixed[k][f]), (k, f, jobs[k])
+
+
+@pytest.mark.parametrize("prec", [mb.FP32, mb.FP16S, mb.BF16S])
+def test_graph_tuner_and_tick_length_do_not_change_results(prec):
+    rng = np.random.default_rng(6)
+    jobs = _jobs(rng, 10)
+    a = _run(Batch(precision=prec, steps_per_tick=4, use_graph=False), jobs)
+    b = _run(Batch(precision=prec, steps_per_tick=10, use_graph=True, autotune=True), jobs)
+    for k in range(len(jobs)):
+        assert same_bits(a[k]["rho"], b[k]["rho"]), (k, jobs[k])
+        assert same_bits(a[k]["u"], b[k]["u"]), (k, jobs[k])
+
+
+@pytest.mark.parametrize("q,shape,steps", [(9, (64, 48), 37), (9, (40, 200), 20), (19, (20, 18, 16), 23)])
+def test_job_matches_fp64_oracle(q, shape, steps):
+    tau, G = 0.9, -5.0
+    rho0 = mb.noise_rho(shape, 3)
+    o = oracle.Oracle(q, shape, tau, G)
+    o.set_rho(rho0); o.init(); o.step(steps)
+    ref = o.moments()
+    for prec, tol in ((mb.FP32, 2e-6), (mb.FP16S, 1e-4)):
+        res = _run(Batch(precision=prec, steps_per_tick=6), [dict(q=q, shape=shape, tau=tau, G=G,
+                                                                  steps=steps, seed=3)])[0]
+        err = rel_l2(res["rho"], ref["rho"])
+        print(f"q {q} {shape} prec {prec}: rel L2 rho {err:.2e}", flush=True)
+        assert err < tol
+
+
+def test_admission_by_budget():
+    b = Batch(precision=mb.FP16S, steps_per_tick=4, hbm_budget=1)
+    assert b.submit(9, (32, 32), 1.0, -5.0, 4, np.full((32, 32), 0.7)) is None
+    need = Batch(precision=mb.FP16S).job_bytes(9, (32, 32))
+    b = Batch(precision=mb.FP16S, steps_per_tick=4, hbm_budget=int(2.5 * need))
+    ids = [b.submit(9, (32, 32), 1.0, -5.0, 4, mb.noise_rho((32, 32), s)) for s in range(3)]
+    assert ids[0] is not None and ids[1] is not None and ids[2] is None
+    assert b.bytes_in_use == 2 * need
+    b.tick()
+    for jid in b.finished():
+        b.collect(jid)
+    assert b.bytes_in_use == 0
+    assert b.submit(9, (32, 32), 1.0, -5.0, 4, mb.noise_rho((32, 32), 2)) is not None
+
+
+def test_large_rows_take_the_two_pass_path():
+    # rows wider than the largest row-kernel workgroup (1024) go to the
+    # two-pass kernels; both paths must agree with the oracle
+    shape = (12, 1100)
+    rho0 = mb.noise_rho(shape, 8)
+    o = oracle.Oracle(9, shape, 1.0, -5.0)
+    o.set_rho(rho0); o.init(); o.step(10)
+    res = _run(Batch(precision=mb.FP32, steps_per_tick=4), [dict(q=9, shape=shape, tau=1.0, G=-5.0,
+                                                                steps=10, seed=8)])[0]
+    assert rel_l2(res["rho"], o.moments()["rho"]) < 2e-6
