@@ -110,6 +110,12 @@ def main():
                     help="fused-kernel launch configuration (default: the library's choice)")
     ap.add_argument("--no-precision-forms", action="store_true",
                     help="skip the 512^3 FP32/FP16/BF16 side measurements")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched D2Q9 (512 lattices) side measurement")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="skip the Poisson-arrival serving side measurement")
+    ap.add_argument("--serve-requests", type=int, default=200,
+                    help="requests in the serving stream (Poisson, 200/s)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the streaming copy kernel over MIB MiB (for counter scaling)")
     args = ap.parse_args()
@@ -187,6 +193,24 @@ def main():
         forms["speedup_fp16_vs_fp32"] = round(forms["fp16"]["mlups"] / forms["fp32"]["mlups"], 3)
         forms["speedup_fp16_vs_fp64"] = round(forms["fp16"]["mlups"] / forms["fp64"]["mlups"], 3)
         out["precision_forms_512"] = forms
+    if args.workload == "d3q19_fp64_256_droplet":
+        from mesobasic_amd import serve
+        if not args.no_batched:
+            r = serve.run_batched(serve.make_batched_jobs(512), steps=2000)
+            r["config"] = ("512 D2Q9 lattices, NX, NY uniform in {64..512}, G in [-6, -4.5], "
+                           "tau in [0.6, 1.5], noisy rho 0.7, 2000 steps, FP32 compute / FP16 "
+                           "storage, graph-captured ticks of 20 steps, tuner on")
+            r["roofline_frac"] = round(r["aggregate_mlups"] * 1e6 * 36 / (HBM_PEAK_GBS * 1e9), 4)
+            out["batched_d2q9_512"] = r
+            print(f"# batched: {r['aggregate_mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
+        if not args.no_serving:
+            r = serve.run_stream(serve.make_requests(args.serve_requests, 200.0, 7))
+            r["config"] = (f"{args.serve_requests} requests, Poisson 200/s (seed 7), 70% D2Q9 "
+                           "128..1024 per side, 30% D3Q19 64..192 per side, steps U[500, 5000], "
+                           "G U[-6, -4.5], droplet or noisy init, FP16 storage, HBM budget 24 GiB")
+            out["serving"] = r
+            print(f"# serving: p50 {r['p50_s']:.2f} s p99 {r['p99_s']:.2f} s "
+                  f"{r['aggregate_mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(q, shape, tau, G, init)
     print(json.dumps(out), flush=True)

@@ -1,0 +1,176 @@
+"""Serving form: continuous batching of independent Shan-Chen simulations on
+one GPU, on top of the batched C ABI (lbm_batch_*).
+
+    python -m mesobasic_amd.serve [--requests 200] [--rate 200] [--seed 7]
+           [--budget-gb 24] [--max-active 0] [--steps-per-tick 20]
+    python -m mesobasic_amd.serve --batched [--jobs 512] [--steps 2000]
+
+Request stream: seeded Poisson arrivals at `rate` requests/s; 70% D2Q9 with
+side lengths uniform in [128, 1024], 30% D3Q19 with side lengths uniform in
+[64, 192] (each side drawn on its own, so lattices need not be square);
+step counts uniform in [500, 5000]; G uniform in [-6.0, -4.5]; tau 1.0;
+half the requests start from a droplet (R = 3/16 of the shortest side,
+width 2, 2.0 / 0.15), half from noisy-uniform rho = 0.7 +- 0.01.
+
+Loop (host side; all lattice state stays on the device): at each tick
+boundary, requests that have arrived are admitted in arrival order while
+the HBM budget (and the optional cap on active jobs) allows -- a request
+that does not fit waits, and later ones wait behind it, so large requests
+are not starved; then one tick (steps_per_tick steps of every active job,
+one graph replay) runs, and finished jobs are read back and freed, which
+makes room for the next admissions.  Latency = completion - arrival, on
+the host's monotonic clock; aggregate MLUPS = sum(nodes * steps) / wall
+time from the first arrival to the last completion.
+
+--batched runs the batched configuration instead: `jobs` D2Q9 lattices
+with NX and NY uniform in {64..512}, G uniform in [-6.0, -4.5], tau uniform
+in [0.6, 1.5], noisy rho = 0.7 +- 0.01, `steps` steps each, all submitted
+before the clock starts, the tuner on; reports aggregate MLUPS.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import time
+
+import numpy as np
+
+from .lbm import FP16S, PRECISIONS, Batch, droplet_rho, noise_rho
+
+
+def make_requests(n, rate=200.0, seed=7):
+    """The seeded request stream (list of dicts sorted by arrival time, s)."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=n))
+    reqs = []
+    for k in range(n):
+        if rng.random() < 0.7:
+            q, shape = 9, (int(rng.integers(128, 1025)), int(rng.integers(128, 1025)))
+        else:
+            q, shape = 19, tuple(int(v) for v in rng.integers(64, 193, size=3))
+        reqs.append(dict(id=k, arrival=float(t[k]), q=q, shape=shape,
+                         steps=int(rng.integers(500, 5001)), tau=1.0,
+                         G=float(rng.uniform(-6.0, -4.5)),
+                         init="droplet" if rng.random() < 0.5 else "noise",
+                         seed=int(rng.integers(1 << 31))))
+    return reqs
+
+
+def initial_rho(req):
+    if req["init"] == "droplet":
+        return droplet_rho(req["shape"], 3.0 * min(req["shape"]) / 16.0)
+    return noise_rho(req["shape"], req["seed"])
+
+
+def run_stream(reqs, precision=FP16S, budget_bytes=24 << 30, max_active=0, steps_per_tick=20,
+               keep_results=False, clock=time.perf_counter):
+    """Serves `reqs` in real time; returns a summary dict (and the per-request
+    fields when keep_results)."""
+    b = Batch(precision=precision, hbm_budget=budget_bytes, steps_per_tick=steps_per_tick,
+              use_graph=True, autotune=False)
+    pending = list(reqs)            # not yet admitted, arrival order
+    live = {}                       # job id -> request
+    done = {}
+    results = {}
+    work = 0
+    ticks = 0
+    max_live = 0
+    peak_bytes = 0
+    t0 = clock()
+    while pending or live:
+        now = clock() - t0
+        # admissions, in arrival order, no overtaking
+        while pending and pending[0]["arrival"] <= now and (max_active <= 0 or len(live) < max_active):
+            r = pending[0]
+            jid = b.submit(r["q"], r["shape"], r["tau"], r["G"], r["steps"], initial_rho(r))
+            if jid is None:
+                if not live:
+                    raise RuntimeError(f"request {r['id']} does not fit the HBM budget on its own")
+                break
+            live[jid] = r
+            pending.pop(0)
+        max_live = max(max_live, len(live))
+        peak_bytes = max(peak_bytes, b.bytes_in_use)
+        if not live:
+            time.sleep(max(0.0, pending[0]["arrival"] - (clock() - t0)))
+            continue
+        b.tick()
+        ticks += 1
+        for jid in b.finished():
+            r = live.pop(jid)
+            out = b.collect(jid)
+            done[r["id"]] = clock() - t0 - r["arrival"]
+            work += int(np.prod(r["shape"])) * r["steps"]
+            if keep_results:
+                results[r["id"]] = out
+    wall = clock() - t0 - reqs[0]["arrival"]
+    lat = np.array([done[r["id"]] for r in reqs])
+    summary = dict(requests=len(reqs), p50_s=float(np.percentile(lat, 50)),
+                   p99_s=float(np.percentile(lat, 99)), mean_s=float(lat.mean()),
+                   aggregate_mlups=work / wall / 1e6, wall_s=wall, ticks=ticks,
+                   max_active=max_live, peak_hbm_bytes=int(peak_bytes))
+    b.close()
+    return (summary, results) if keep_results else summary
+
+
+def make_batched_jobs(n=512, seed=11):
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for k in range(n):
+        jobs.append(dict(id=k, q=9, shape=(int(rng.integers(64, 513)), int(rng.integers(64, 513))),
+                         tau=float(rng.uniform(0.6, 1.5)), G=float(rng.uniform(-6.0, -4.5)),
+                         seed=int(rng.integers(1 << 31))))
+    return jobs
+
+
+def run_batched(jobs, steps=2000, precision=FP16S, steps_per_tick=20, autotune=True,
+                clock=time.perf_counter):
+    """All jobs submitted, then ticked to completion; returns a summary."""
+    b = Batch(precision=precision, steps_per_tick=steps_per_tick, use_graph=True, autotune=autotune)
+    ids = []
+    for j in jobs:
+        ids.append(b.submit(9, j["shape"], j["tau"], j["G"], steps, noise_rho(j["shape"], j["seed"])))
+    assert all(i is not None for i in ids)
+    nodes = sum(int(np.prod(j["shape"])) for j in jobs)
+    t0 = clock()
+    n_ticks = 0
+    while b.active:
+        b.tick()
+        n_ticks += 1
+        if len(b.finished()) == len(jobs):
+            break
+    wall = clock() - t0
+    finite = 0
+    for jid in b.finished():
+        finite += bool(np.isfinite(b.collect(jid)["rho"]).all())
+    summary = dict(jobs=len(jobs), steps=steps, nodes=nodes, wall_s=wall, ticks=n_ticks,
+                   aggregate_mlups=nodes * steps / wall / 1e6, tuning=b.tuning,
+                   finite_results=finite)
+    b.close()
+    return summary
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--batched", action="store_true")
+    ap.add_argument("--jobs", type=int, default=512)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--requests", type=int, default=200)
+    ap.add_argument("--rate", type=float, default=200.0)
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--budget-gb", type=float, default=24.0)
+    ap.add_argument("--max-active", type=int, default=0)
+    ap.add_argument("--steps-per-tick", type=int, default=20)
+    ap.add_argument("--precision", default="fp16", choices=["fp32", "fp16", "bf16"])
+    args = ap.parse_args()
+    prec = PRECISIONS[args.precision]
+    if args.batched:
+        s = run_batched(make_batched_jobs(args.jobs), args.steps, prec, args.steps_per_tick)
+    else:
+        s = run_stream(make_requests(args.requests, args.rate, args.seed), prec,
+                       int(args.budget_gb * (1 << 30)), args.max_active, args.steps_per_tick)
+    print(json.dumps(s), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -105,3 +105,32 @@ def test_large_rows_take_the_two_pass_path():
     res = _run(Batch(precision=mb.FP32, steps_per_tick=4), [dict(q=9, shape=shape, tau=1.0, G=-5.0,
                                                                 steps=10, seed=8)])[0]
     assert rel_l2(res["rho"], o.moments()["rho"]) < 2e-6
+
+
+def test_serving_stream_admits_as_memory_frees_and_matches_alone():
+    from mesobasic_amd import serve
+    rng = np.random.default_rng(9)
+    reqs = []
+    for k in range(14):
+        q = 9 if k % 3 else 19
+        shape = (int(rng.integers(24, 90)), int(rng.integers(24, 90))) if q == 9 else \
+            tuple(int(v) for v in rng.integers(10, 24, 3))
+        reqs.append(dict(id=k, arrival=0.004 * max(0, k - 3), q=q, shape=shape, steps=int(rng.integers(20, 90)),
+                         tau=1.0, G=float(rng.uniform(-6.0, -4.5)),
+                         init="droplet" if k % 2 else "noise", seed=100 + k))
+    need = max(Batch(precision=mb.FP16S).job_bytes(r["q"], r["shape"]) for r in reqs)
+    summary, res = serve.run_stream(reqs, precision=mb.FP16S, budget_bytes=3 * need, steps_per_tick=8,
+                                    keep_results=True)
+    print(summary, flush=True)
+    # the first four arrive together: at least three fit the budget at once
+    assert summary["requests"] == len(reqs) and summary["max_active"] >= 3
+    assert summary["peak_hbm_bytes"] <= 3 * need
+    assert summary["p50_s"] <= summary["p99_s"]
+    for r in reqs:
+        b = Batch(precision=mb.FP16S, steps_per_tick=4)
+        jid = b.submit(r["q"], r["shape"], r["tau"], r["G"], r["steps"], serve.initial_rho(r))
+        while not b.finished():
+            b.tick()
+        alone = b.collect(jid)
+        assert same_bits(alone["rho"], res[r["id"]]["rho"]), r
+        assert same_bits(alone["u"], res[r["id"]]["u"]), r

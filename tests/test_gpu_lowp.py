@@ -4,8 +4,14 @@ tolerance.  Tolerances: relative L2 of the density field, set at about 3x
 the values measured on MI355X (two-pass and fused agree to within 2x):
 
   case                                FP32     FP16     BF16
-  noise 0.7+-0.01, G -5, 50 steps     1.5e-6   2.2e-5   7.8e-5
-  droplet R 8 (2.0/0.15), 300 steps   7.0e-6   1.5e-3   3.2e-2
+  noise 0.7+-0.01, G -5, 50 steps     1.6e-6   2.2e-5   7.4e-5
+  droplet R 8 (2.0/0.15), 300 steps   7.6e-6   2.4e-3   1.2e-1
+  z walls, 0.9/0.5 droplet, 50 steps  1.9e-6   7.9e-4   -
+  D2Q9 noise, 50 steps                2.3e-6   2.8e-5   2.1e-4
+
+BF16 keeps 8 significant bits, so a droplet whose densities (2.0 / 0.15)
+lie far from the single storage shift (0.7) is not resolved by it; the
+FP16 form is the one meant for such cases.
 
 Spinodal growth amplifies differences, so longer noisy runs are compared
 through size-independent quantities instead (coexistence densities and mass
@@ -20,7 +26,8 @@ from conftest import rel_l2
 pytestmark = pytest.mark.gpu
 
 TOL_NOISE = {mb.FP32: 5e-6, mb.FP16S: 7e-5, mb.BF16S: 3e-4}
-TOL_DROP = {mb.FP32: 3e-5, mb.FP16S: 5e-3, mb.BF16S: 1e-1}
+TOL_DROP = {mb.FP32: 3e-5, mb.FP16S: 7e-3, mb.BF16S: 3e-1}
+TOL_WALLS = {mb.FP32: 6e-6, mb.FP16S: 2.5e-3}
 
 
 @pytest.fixture(scope="module")
@@ -93,4 +100,4 @@ def test_lowp_walls(prec):
         lat.set_rho(rho0); lat.init(); lat.step(steps)
         err = rel_l2(lat.fields(mb.FIELDS_CURRENT)["rho"], ref["rho"])
         print(f"walls prec {prec} kernel {kernel}: rel L2 rho = {err:.3e}", flush=True)
-        assert err < 10 * TOL_NOISE[prec]
+        assert err < TOL_WALLS[prec]
