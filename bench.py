@@ -35,11 +35,15 @@ WORKLOADS = {
     "d3q19_fp32_512_noise": (19, (512, 512, 512), mb.FP32, 1.0, -5.0, ("noise", 1)),
     "d3q19_fp16_512_noise": (19, (512, 512, 512), mb.FP16S, 1.0, -5.0, ("noise", 1)),
     "d3q19_bf16_512_noise": (19, (512, 512, 512), mb.BF16S, 1.0, -5.0, ("noise", 1)),
+    "d3q19_fp16n_512_noise": (19, (512, 512, 512), mb.FP16N, 1.0, -5.0, ("noise", 1)),
+    "d3q19_bf16n_512_noise": (19, (512, 512, 512), mb.BF16N, 1.0, -5.0, ("noise", 1)),
     "d2q9_fp64_256_noise": (9, (256, 256), mb.FP64, 1.0, -5.0, ("noise", 1)),
 }
-BYTES_PER_POP = {mb.FP64: 8, mb.FP32: 4, mb.FP16S: 2, mb.BF16S: 2}
+BYTES_PER_POP = {mb.FP64: 8, mb.FP32: 4, mb.FP16S: 2, mb.BF16S: 2, mb.FP16N: 2, mb.BF16N: 2}
 DTYPE = {mb.FP64: "f64", mb.FP32: "f32", mb.FP16S: "f32 compute / f16 storage",
-         mb.BF16S: "f32 compute / bf16 storage"}
+         mb.BF16S: "f32 compute / bf16 storage",
+         mb.FP16N: "f32 compute / f16 node-shifted storage",
+         mb.BF16N: "f32 compute / bf16 node-shifted storage"}
 KERNEL_NAMES = {mb.KERNEL_TWO_PASS: "two_pass", mb.KERNEL_FUSED: "fused"}
 
 
@@ -189,9 +193,11 @@ def main():
     }
     lat.close()
     if not args.no_precision_forms and args.workload == "d3q19_fp64_256_droplet":
-        forms = {k: time_workload(f"d3q19_{k}_512_noise") for k in ("fp64", "fp32", "fp16", "bf16")}
-        forms["speedup_fp16_vs_fp32"] = round(forms["fp16"]["mlups"] / forms["fp32"]["mlups"], 3)
-        forms["speedup_fp16_vs_fp64"] = round(forms["fp16"]["mlups"] / forms["fp64"]["mlups"], 3)
+        forms = {k: time_workload(f"d3q19_{k}_512_noise")
+                 for k in ("fp64", "fp32", "fp16", "bf16", "fp16n", "bf16n")}
+        for k in ("fp16", "fp16n"):
+            forms[f"speedup_{k}_vs_fp32"] = round(forms[k]["mlups"] / forms["fp32"]["mlups"], 3)
+            forms[f"speedup_{k}_vs_fp64"] = round(forms[k]["mlups"] / forms["fp64"]["mlups"], 3)
         out["precision_forms_512"] = forms
     if args.workload == "d3q19_fp64_256_droplet":
         from mesobasic_amd import serve

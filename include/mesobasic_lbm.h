@@ -37,7 +37,13 @@ enum lbm_precision {
     LBM_FP64 = 0,      /* FP64 compute and storage: the drop-in parity path */
     LBM_FP32 = 1,      /* FP32 compute and storage */
     LBM_FP16S = 2,     /* FP32 compute, FP16 storage of f_i - w_i*rho_shift */
-    LBM_BF16S = 3      /* FP32 compute, BF16 storage of f_i - w_i*rho_shift */
+    LBM_BF16S = 3,     /* FP32 compute, BF16 storage of f_i - w_i*rho_shift */
+    /* Node-shifted 16-bit storage: f_i - w_i*R(x), where R(x) is the density
+     * that the node x reading the value had after the previous step (kept
+     * as an FP32 field, 4 B per node, double-buffered).  Resolves densities
+     * far from any single shift (droplets, walls); rho_shift is unused. */
+    LBM_FP16N = 4,     /* FP32 compute, FP16 node-shifted storage */
+    LBM_BF16N = 5      /* FP32 compute, BF16 node-shifted storage */
 };
 
 enum lbm_kernel {

@@ -41,6 +41,9 @@ SOURCES = [
     ("lbm_fp64.hip", ["-ffp-contract=off"]),
     ("lbm_fp64_fused.hip", ["-ffp-contract=off"]),
     ("lbm_lowp.hip", ["-ffp-contract=fast"]),
+    ("lbm_lowp_fused_f32.hip", ["-ffp-contract=fast"]),
+    ("lbm_lowp_fused_f16.hip", ["-ffp-contract=off"]),
+    ("lbm_lowp_fused_bf16.hip", ["-ffp-contract=off"]),
     ("lbm_batch.hip", ["-ffp-contract=off"]),
     ("lbm_capi.cpp", ["-ffp-contract=off"]),
     ("lbm_batch_capi.cpp", ["-ffp-contract=off"]),

@@ -69,6 +69,7 @@ struct lbm_handle {
     int kind[2] = {lbm::kHoldsF, lbm::kHoldsF};
     int cur = 0;
     double* psi = nullptr;    // FP64 two-pass density field
+    float* rref[2] = {nullptr, nullptr};  // node-shift reference density of buf[0] / buf[1]
     double* rho0 = nullptr;   // initial density (reference r array at N=0)
     double* tmp = nullptr;    // staging for fields / f / partials
     int64_t tmp_bytes = 0;
@@ -120,7 +121,8 @@ int run_step(lbm_handle* h)
         if (!h->psi) return fail(LBM_ESTATE, "psi field not allocated");
         HIP_OK(lbm::lowp_step(h->d.precision, h->a, h->g, h->d.rho_shift, h->buf[src],
                               h->kind[src], h->buf[dst], h->first_step_stored ? h->rho0 : nullptr,
-                              reinterpret_cast<float*>(h->psi), h->kernel, h->tune, h->stream));
+                              reinterpret_cast<float*>(h->psi), h->rref[src], h->rref[dst],
+                              h->kernel, h->tune, h->stream));
     }
     h->kind[dst] = lbm::kHoldsPost;
     h->cur = dst;
@@ -142,7 +144,8 @@ int moments_to_host(lbm_handle* h, int b, double* rho, double* u, double* v, dou
                                  d, d + n, d + 2 * n, d + 3 * n, h->stream));
     else
         HIP_OK(lbm::lowp_moments(h->d.precision, h->a, h->g, h->d.rho_shift, h->buf[b],
-                                 h->kind[b], d, d + n, d + 2 * n, d + 3 * n, h->stream));
+                                 h->kind[b], h->rref[b], d, d + n, d + 2 * n, d + 3 * n,
+                                 h->stream));
     double* outs[4] = {rho, u, v, w};
     for (int k = 0; k < 4; k++)
         if (outs[k])
@@ -166,7 +169,7 @@ int check_desc(const lbm_desc* d)
         const int last = d->q == 19 ? d->nz : d->ny;
         if (last < 3) return fail(LBM_EINVAL, "walls need >= 3 nodes on the wall axis (got %d)", last);
     }
-    if (d->precision < LBM_FP64 || d->precision > LBM_BF16S)
+    if (d->precision < LBM_FP64 || d->precision > LBM_BF16N)
         return fail(LBM_EINVAL, "unknown precision %d", d->precision);
     if (d->kernel < LBM_KERNEL_AUTO || d->kernel > LBM_KERNEL_FUSED)
         return fail(LBM_EINVAL, "unknown kernel %d", d->kernel);
@@ -239,6 +242,12 @@ int lbm_create(const lbm_desc* desc, lbm_handle** out)
     if (hipMalloc(&h->rho0, h->g.n * 8) != hipSuccess)
         return cleanup(fail(LBM_ENOMEM, "hipMalloc for rho0 failed"));
     h->bytes += h->g.n * 8;
+    if (lbm::lowp_node_shift(desc->precision)) {
+        for (int b = 0; b < 2; b++)
+            if (hipMalloc(&h->rref[b], h->g.n * 4) != hipSuccess)
+                return cleanup(fail(LBM_ENOMEM, "hipMalloc for the node-shift density failed"));
+        h->bytes += 2 * h->g.n * 4;
+    }
     if (hipStreamCreateWithFlags(&h->own, hipStreamNonBlocking) != hipSuccess)
         return cleanup(fail(LBM_EHIP, "hipStreamCreate failed"));
     h->stream = h->own;
@@ -255,6 +264,8 @@ void lbm_destroy(lbm_handle* h)
     for (void* b : h->buf)
         if (b) hipFree(b);
     if (h->psi) hipFree(h->psi);
+    for (float* r : h->rref)
+        if (r) hipFree(r);
     if (h->rho0) hipFree(h->rho0);
     if (h->tmp) hipFree(h->tmp);
     if (h->ev0) hipEventDestroy(h->ev0);
@@ -283,7 +294,7 @@ int lbm_init_equilibrium(lbm_handle* h)
                                  psi, h->rho0, 2, h->stream));
     } else {
         HIP_OK(lbm::lowp_init(h->d.precision, h->a, h->g, h->d.rho_shift, h->rho0, h->buf[0],
-                              reinterpret_cast<float*>(h->psi), h->stream));
+                              reinterpret_cast<float*>(h->psi), h->rref[0], h->stream));
     }
     HIP_OK(hipStreamSynchronize(h->stream));
     h->cur = 0;
@@ -364,7 +375,8 @@ int lbm_get_force(lbm_handle* h, double* fx, double* fy, double* fz)
                                      rho, rho + n, rho + 2 * n, rho + 3 * n, h->stream));
         else
             HIP_OK(lbm::lowp_moments(h->d.precision, h->a, h->g, h->d.rho_shift, h->buf[b],
-                                     h->kind[b], rho, rho + n, rho + 2 * n, rho + 3 * n, h->stream));
+                                     h->kind[b], h->rref[b], rho, rho + n, rho + 2 * n, rho + 3 * n,
+                                     h->stream));
     }
     HIP_OK(lbm::fp64_psi_from_rho(rho, psi, n, h->stream));
     HIP_OK(lbm::fp64_force(h->a, h->g, psi, F, F + n, F + 2 * n, h->stream));
@@ -388,7 +400,7 @@ int lbm_get_f(lbm_handle* h, double* f)
                                h->kind[h->cur], h->tmp, h->stream));
     else
         HIP_OK(lbm::lowp_get_f(h->d.precision, h->a, h->g, h->d.rho_shift, h->buf[h->cur],
-                               h->kind[h->cur], h->tmp, h->stream));
+                               h->kind[h->cur], h->rref[h->cur], h->tmp, h->stream));
     HIP_OK(hipMemcpyAsync(f, h->tmp, n * 8, hipMemcpyDeviceToHost, h->stream));
     HIP_OK(hipStreamSynchronize(h->stream));
     return LBM_OK;
@@ -405,7 +417,7 @@ int lbm_set_f(lbm_handle* h, const double* f)
         HIP_OK(lbm::fp64_set_f(h->a, h->g, h->tmp, static_cast<double*>(h->buf[0]), h->stream));
     else
         HIP_OK(lbm::lowp_set_f(h->d.precision, h->a, h->g, h->d.rho_shift, h->tmp, h->buf[0],
-                               h->stream));
+                               h->rref[0], h->stream));
     HIP_OK(hipStreamSynchronize(h->stream));
     h->cur = 0;
     h->kind[0] = lbm::kHoldsF;
@@ -430,7 +442,8 @@ int lbm_conservation(lbm_handle* h, double out[4])
                                       h->kind[h->cur], h->tmp, &nb, h->stream));
     else
         HIP_OK(lbm::lowp_conservation(h->d.precision, h->a, h->g, h->d.rho_shift,
-                                      h->buf[h->cur], h->kind[h->cur], h->tmp, &nb, h->stream));
+                                      h->buf[h->cur], h->kind[h->cur], h->rref[h->cur], h->tmp,
+                                      &nb, h->stream));
     std::vector<double> part(static_cast<size_t>(nb) * 4);
     HIP_OK(hipMemcpyAsync(part.data(), h->tmp, part.size() * 8, hipMemcpyDeviceToHost, h->stream));
     HIP_OK(hipStreamSynchronize(h->stream));

@@ -1,17 +1,23 @@
 // lbm_lowp.hip -- reduced-precision Shan-Chen step: FP32 arithmetic with
-// FP32 storage of f, or FP16 / BF16 storage of the shifted populations
-// d_i = f_i - w_i*rho_shift ("DDF shifting", Lehmann et al. 2022,
-// "Accuracy and performance of the lattice Boltzmann method with 64-bit,
-// 32-bit, and customized 16-bit number formats").  Same step structure and
-// formulas as the FP64 path (lbm_fp64.hip, which cites the reference);
-// only the number formats differ, so results are compared with the FP64
-// oracle by tolerance, not bit for bit.
+// FP32 storage of f, or FP16 / BF16 storage of shifted populations
+// d_i = f_i - w_i*R.  With a constant R (= rho_shift) this is "DDF
+// shifting" (Lehmann et al. 2022, "Accuracy and performance of the lattice
+// Boltzmann method with 64-bit, 32-bit, and customized 16-bit number
+// formats"); the node-shift forms use the reading node's density after the
+// previous step instead (lbm_fused_lp.h describes the scheme).  Same step
+// structure and formulas as the FP64 path (lbm_fp64.hip, which cites the
+// reference); only the number formats differ, so results are compared with
+// the FP64 oracle by tolerance, not bit for bit.
+//
+// This file holds the two-pass kernels (any stencil and lattice), the
+// initialisation, read-back and reductions; the fused D3Q19 kernel is
+// launched from lbm_lowp_fused_*.hip.
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 
 #include "../../include/mesobasic_lbm.h"
 #include "lbm_lowp.h"
-#include "lbm_fused.h"
+#include "lbm_fused_lp.h"
 #include "lbm_stencil.h"
 #include "lbm_storage.h"
 
@@ -61,106 +67,119 @@ __device__ __forceinline__ int64_t at(const Nbr& nb, int dx, int dy, int dz)
     return nb.xy[dx + 1][dy + 1] + nb.z[dz + 1];
 }
 
-template <int Q> struct Shift {
-    float s[Q];
-    __device__ __forceinline__ explicit Shift(float rho_shift)
+// Reference density of the stored values: the node-shift field, or the
+// constant shift.
+template <bool NS> struct Ref {
+    const float* r;
+    float rs;
+    __device__ __forceinline__ float operator()(int64_t c) const
     {
-#pragma unroll
-        for (int n = 0; n < Q; n++) s[n] = static_cast<float>(St<Q>::w[n]) * rho_shift;
+        if constexpr (NS) return r[c];
+        else return rs;
     }
 };
 
-template <int Q, bool WALLS, typename SP>
+// Scale of the stored values (FP16 node shift: 2^10, see lbm_fused_lp.h).
+template <typename SP, bool NS> constexpr float kScale = fused::kStoreScale<SP, NS>;
+
+// f at node c (coordinates p) from the stored values: gathered from the
+// upstream neighbours (kHoldsPost) or in place (kHoldsF), plus w_n*R(c).
+// Bounce-back slots were written by c itself for its wall-side neighbour,
+// so they carry that neighbour's reference density.
+template <int Q, bool WALLS, typename SP, bool NS>
 __device__ __forceinline__ void load_f(const typename SP::T* __restrict__ P, const Grid& g, int kind,
-                                       const Coord& p, const Nbr& nb, int64_t c,
-                                       const Shift<Q>& sh, float (&f)[Q])
+                                       const Coord& p, const Nbr& nb, int64_t c, const Ref<NS>& R,
+                                       float (&f)[Q])
 {
     using S = St<Q>;
+    const float Rc = R(c);
     if (kind == kHoldsF) {
-#pragma unroll
-        for (int n = 0; n < Q; n++) f[n] = SP::load(P[n * g.ps + c], sh.s[n]);
+        static_for<Q>([&](auto I) {
+            constexpr int n = decltype(I)::value;
+            f[n] = fmaf(static_cast<float>(S::w[n]), Rc, SP::d(P[n * g.ps + c]) * (1.0f / kScale<SP, NS>));
+        });
         return;
     }
-#pragma unroll
-    for (int n = 0; n < Q; n++)
-        f[n] = SP::load(P[n * g.ps + at(nb, -S::ex[n], -S::ey[n], -S::ez[n])], sh.s[n]);
+    static_for<Q>([&](auto I) {
+        constexpr int n = decltype(I)::value;
+        f[n] = fmaf(static_cast<float>(S::w[n]), Rc,
+                    SP::d(P[n * g.ps + at(nb, -S::ex[n], -S::ey[n], -S::ez[n])]) *
+                        (1.0f / kScale<SP, NS>));
+    });
     if constexpr (WALLS) {
         const int last = (S::D == 3) ? p.z : p.y;
         const int nl = (S::D == 3) ? g.nz : g.ny;
-        if (last == 1) {
-#pragma unroll
-            for (int m = 0; m < S::nwall; m++)
-                f[S::wall_up[m]] = SP::load(P[S::wall_dn[m] * g.ps + c], sh.s[S::wall_dn[m]]);
-        }
-        if (last == nl - 2) {
-#pragma unroll
-            for (int m = 0; m < S::nwall; m++)
-                f[S::wall_dn[m]] = SP::load(P[S::wall_up[m] * g.ps + c], sh.s[S::wall_up[m]]);
-        }
+        auto bounce = [&](auto M, auto from_dn) {
+            constexpr int m = decltype(M)::value;
+            constexpr bool DN = decltype(from_dn)::value;
+            constexpr int src = DN ? S::wall_dn[m] : S::wall_up[m];
+            constexpr int dst = DN ? S::wall_up[m] : S::wall_dn[m];
+            const float Rn = R(at(nb, S::ex[src], S::ey[src], S::ez[src]));
+            f[dst] = fmaf(static_cast<float>(S::w[src]), Rn,
+                          SP::d(P[src * g.ps + c]) * (1.0f / kScale<SP, NS>));
+        };
+        if (last == 1) static_for<S::nwall>([&](auto M) { bounce(M, std::true_type{}); });
+        if (last == nl - 2) static_for<S::nwall>([&](auto M) { bounce(M, std::false_type{}); });
     }
-}
-
-template <int Q>
-__device__ __forceinline__ void moments(const float (&f)[Q], float& rho, float& sx, float& sy, float& sz)
-{
-    using S = St<Q>;
-    rho = 0.f; sx = 0.f; sy = 0.f; sz = 0.f;
-    static_for<Q>([&](auto I) {
-        constexpr int n = decltype(I)::value;
-        rho += f[n];
-        acc_e<S::ex[n]>(sx, f[n]);
-        acc_e<S::ey[n]>(sy, f[n]);
-        acc_e<S::ez[n]>(sz, f[n]);
-    });
 }
 
 __device__ __forceinline__ float psi_of(float rho) { return 1.0f - __expf(-rho); }
 
-template <int Q, bool WALLS, typename SP>
+// Pass 1: psi at every node (from rho0 when given), and with the node shift
+// the density of the gathered f (the reference density of this step's
+// output).  P == nullptr: initialisation (rho0 only).
+template <int Q, bool WALLS, typename SP, bool NS>
 __global__ void __launch_bounds__(kBlock)
     k_psi(const typename SP::T* __restrict__ P, int kind, const double* __restrict__ rho0,
-          float* __restrict__ psi, Grid g, float rho_shift)
+          float* __restrict__ psi, Grid g, Ref<NS> rin, float* __restrict__ rout)
 {
     const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     if (c >= g.n) return;
-    if (rho0) {
-        psi[c] = psi_of(static_cast<float>(rho0[c]));
-        return;
+    float rho = rho0 ? static_cast<float>(rho0[c]) : 0.f;
+    float rf = rho;
+    if (P && (!rho0 || NS)) {
+        const Coord p = decompose(static_cast<uint32_t>(c), g);
+        const Nbr nb = neighbours(p, g);
+        float f[Q];
+        load_f<Q, WALLS, SP, NS>(P, g, kind, p, nb, c, rin, f);
+        rf = 0.f;
+        static_for<Q>([&](auto I) { rf += f[decltype(I)::value]; });
+        if (!rho0) rho = rf;
     }
-    const Coord p = decompose(static_cast<uint32_t>(c), g);
-    const Nbr nb = neighbours(p, g);
-    const Shift<Q> sh(rho_shift);
-    float f[Q];
-    load_f<Q, WALLS, SP>(P, g, kind, p, nb, c, sh, f);
-    float rho = 0.f;
-#pragma unroll
-    for (int n = 0; n < Q; n++) rho += f[n];
     psi[c] = psi_of(rho);
+    if constexpr (NS) rout[c] = rf;
 }
 
-// MODE 0: moments of f; 1: rho = rho0, u = 0; 2: initial equilibrium.
-template <int Q, bool WALLS, int MODE, typename SP>
+// Pass 2.  MODE 0: moments of f; 1: rho = rho0, u = 0; 2: initial
+// equilibrium (written as f held at its own node).
+template <int Q, bool WALLS, int MODE, typename SP, bool NS>
 __global__ void __launch_bounds__(kBlock)
     k_collide(const typename SP::T* __restrict__ Pin, int kind, typename SP::T* __restrict__ Pout,
               const float* __restrict__ psi, const double* __restrict__ rho0, Grid g, float tau,
-              float G, float rho_shift)
+              float G, Ref<NS> rin, Ref<NS> rout)
 {
     using S = St<Q>;
     const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     if (c >= g.n) return;
     const Coord p = decompose(static_cast<uint32_t>(c), g);
     const Nbr nb = neighbours(p, g);
-    const Shift<Q> sh(rho_shift);
     float f[Q];
     float rho, u = 0.f, v = 0.f, w = 0.f;
     if constexpr (MODE == 0) {
-        load_f<Q, WALLS, SP>(Pin, g, kind, p, nb, c, sh, f);
-        float sx, sy, sz;
-        moments<Q>(f, rho, sx, sy, sz);
+        load_f<Q, WALLS, SP, NS>(Pin, g, kind, p, nb, c, rin, f);
+        float sx = 0.f, sy = 0.f, sz = 0.f;
+        rho = 0.f;
+        static_for<Q>([&](auto I) {
+            constexpr int n = decltype(I)::value;
+            rho += f[n];
+            acc_e<S::ex[n]>(sx, f[n]);
+            acc_e<S::ey[n]>(sy, f[n]);
+            acc_e<S::ez[n]>(sz, f[n]);
+        });
         const float ir = 1.0f / rho;
         u = sx * ir; v = sy * ir; w = sz * ir;
     } else {
-        if constexpr (MODE == 1) load_f<Q, WALLS, SP>(Pin, g, kind, p, nb, c, sh, f);
+        if constexpr (MODE == 1) load_f<Q, WALLS, SP, NS>(Pin, g, kind, p, nb, c, rin, f);
         rho = static_cast<float>(rho0[c]);
     }
     const float pc = psi[c];
@@ -183,15 +202,21 @@ __global__ void __launch_bounds__(kBlock)
         const float ev = edot<S::ex[n], S::ey[n], S::ez[n]>(ueq, veq, weq);
         const float feq = (1.0f + 3.0f * ev + 4.5f * ev * ev - usq15) * (rho * static_cast<float>(S::w[n]));
         float out;
-        if constexpr (MODE == 2) out = feq;
-        else out = f[n] - (f[n] - feq) * omega;
-        Pout[n * g.ps + c] = SP::store(out, sh.s[n]);
+        float Rd;   // reference density of the node that reads this value
+        if constexpr (MODE == 2) {
+            out = feq;
+            Rd = rout(c);
+        } else {
+            out = f[n] - (f[n] - feq) * omega;
+            Rd = rout(at(nb, S::ex[n], S::ey[n], S::ez[n]));
+        }
+        Pout[n * g.ps + c] = SP::of_d(fmaf(-static_cast<float>(S::w[n]), Rd, out) * kScale<SP, NS>);
     });
 }
 
-template <int Q, bool WALLS, typename SP>
+template <int Q, bool WALLS, typename SP, bool NS>
 __global__ void __launch_bounds__(kBlock)
-    k_moments(const typename SP::T* __restrict__ P, int kind, Grid g, float rho_shift,
+    k_moments(const typename SP::T* __restrict__ P, int kind, Grid g, Ref<NS> rin,
               double* __restrict__ rho, double* __restrict__ u, double* __restrict__ v,
               double* __restrict__ w)
 {
@@ -199,10 +224,9 @@ __global__ void __launch_bounds__(kBlock)
     if (c >= g.n) return;
     const Coord p = decompose(static_cast<uint32_t>(c), g);
     const Nbr nb = neighbours(p, g);
-    const Shift<Q> sh(rho_shift);
     float f[Q];
-    load_f<Q, WALLS, SP>(P, g, kind, p, nb, c, sh, f);
-    // FP64 sums of the stored values, so the reported moments carry no
+    load_f<Q, WALLS, SP, NS>(P, g, kind, p, nb, c, rin, f);
+    // FP64 sums of the rebuilt values, so the reported moments carry no
     // extra FP32 rounding.
     double r = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
     static_for<Q>([&](auto I) {
@@ -219,43 +243,52 @@ __global__ void __launch_bounds__(kBlock)
     w[c] = (St<Q>::D == 3) ? sz / r : 0.0;
 }
 
-template <int Q, bool WALLS, typename SP>
+template <int Q, bool WALLS, typename SP, bool NS>
 __global__ void __launch_bounds__(kBlock)
-    k_get_f(const typename SP::T* __restrict__ P, int kind, Grid g, float rho_shift,
+    k_get_f(const typename SP::T* __restrict__ P, int kind, Grid g, Ref<NS> rin,
             double* __restrict__ aos)
 {
     const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     if (c >= g.n) return;
     const Coord p = decompose(static_cast<uint32_t>(c), g);
     const Nbr nb = neighbours(p, g);
-    const Shift<Q> sh(rho_shift);
     float f[Q];
-    load_f<Q, WALLS, SP>(P, g, kind, p, nb, c, sh, f);
+    load_f<Q, WALLS, SP, NS>(P, g, kind, p, nb, c, rin, f);
 #pragma unroll
     for (int n = 0; n < Q; n++) aos[c * Q + n] = f[n];
 }
 
-template <int Q, typename SP>
+template <int Q, typename SP, bool NS>
 __global__ void __launch_bounds__(kBlock)
-    k_set_f(const double* __restrict__ aos, Grid g, float rho_shift, typename SP::T* __restrict__ P)
+    k_set_f(const double* __restrict__ aos, Grid g, float rs, typename SP::T* __restrict__ P,
+            float* __restrict__ rout)
 {
     const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     if (c >= g.n) return;
-    const Shift<Q> sh(rho_shift);
+    float f[Q];
 #pragma unroll
-    for (int n = 0; n < Q; n++) P[n * g.ps + c] = SP::store(static_cast<float>(aos[c * Q + n]), sh.s[n]);
+    for (int n = 0; n < Q; n++) f[n] = static_cast<float>(aos[c * Q + n]);
+    float R = rs;
+    if constexpr (NS) {
+        R = 0.f;
+        static_for<Q>([&](auto I) { R += f[decltype(I)::value]; });
+        rout[c] = R;
+    }
+    static_for<Q>([&](auto I) {
+        constexpr int n = decltype(I)::value;
+        P[n * g.ps + c] = SP::of_d(fmaf(-static_cast<float>(St<Q>::w[n]), R, f[n]) * kScale<SP, NS>);
+    });
 }
 
 constexpr int kConsBlock = 256;
 constexpr int kConsPerThread = 16;
 
-template <int Q, bool WALLS, typename SP>
+template <int Q, bool WALLS, typename SP, bool NS>
 __global__ void __launch_bounds__(kConsBlock)
-    k_conservation(const typename SP::T* __restrict__ P, int kind, Grid g, float rho_shift,
+    k_conservation(const typename SP::T* __restrict__ P, int kind, Grid g, Ref<NS> rin,
                    double* __restrict__ part)
 {
     __shared__ double red[4][kConsBlock];
-    const Shift<Q> sh(rho_shift);
     double m = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
     const int64_t base = static_cast<int64_t>(blockIdx.x) * kConsBlock * kConsPerThread;
     for (int i = 0; i < kConsPerThread; i++) {
@@ -264,7 +297,7 @@ __global__ void __launch_bounds__(kConsBlock)
         const Coord p = decompose(static_cast<uint32_t>(c), g);
         const Nbr nb = neighbours(p, g);
         float f[Q];
-        load_f<Q, WALLS, SP>(P, g, kind, p, nb, c, sh, f);
+        load_f<Q, WALLS, SP, NS>(P, g, kind, p, nb, c, rin, f);
         static_for<Q>([&](auto I) {
             constexpr int n = decltype(I)::value;
             const double fv = f[n];
@@ -287,18 +320,25 @@ __global__ void __launch_bounds__(kConsBlock)
 
 inline unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
 
+// Runs CALL with Q, W (walls), SP (storage) and NS (node shift) bound as
+// compile-time names.
 #define LOWP_DISPATCH(prec, a, CALL)                                                    \
     do {                                                                               \
-        auto go_ = [&](auto qtag, auto wtag, auto sptag) {                             \
+        auto go_ = [&](auto qtag, auto wtag, auto sptag, auto nstag) {                 \
             constexpr int Q = decltype(qtag)::value;                                   \
             constexpr bool W = decltype(wtag)::value;                                  \
+            constexpr bool NS = decltype(nstag)::value;                                \
             using SP = decltype(sptag);                                                \
             CALL;                                                                      \
         };                                                                             \
         auto pick_sp_ = [&](auto qtag, auto wtag) {                                    \
-            if ((prec) == LBM_FP32) go_(qtag, wtag, StF32{});                          \
-            else if ((prec) == LBM_FP16S) go_(qtag, wtag, StF16{});                    \
-            else go_(qtag, wtag, StBF16{});                                            \
+            switch (prec) {                                                            \
+            case LBM_FP32: go_(qtag, wtag, StF32{}, std::false_type{}); break;         \
+            case LBM_FP16S: go_(qtag, wtag, StF16{}, std::false_type{}); break;        \
+            case LBM_BF16S: go_(qtag, wtag, StBF16{}, std::false_type{}); break;       \
+            case LBM_FP16N: go_(qtag, wtag, StF16{}, std::true_type{}); break;         \
+            default: go_(qtag, wtag, StBF16{}, std::true_type{}); break;               \
+            }                                                                          \
         };                                                                             \
         auto pick_w_ = [&](auto qtag) {                                                \
             if ((a).walls) pick_sp_(qtag, std::true_type{});                           \
@@ -307,6 +347,11 @@ inline unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + kBlock
         if ((a).q == 9) pick_w_(std::integral_constant<int, 9>{});                     \
         else pick_w_(std::integral_constant<int, 19>{});                               \
     } while (0)
+
+float shift_of(int precision, double rho_shift)
+{
+    return precision == LBM_FP32 ? 0.f : static_cast<float>(rho_shift);
+}
 
 }  // namespace
 
@@ -318,133 +363,99 @@ int lowp_resolve_kernel(int, const StepArgs& a, const Grid& g, int requested)
     return fused_ok ? LBM_KERNEL_FUSED : LBM_KERNEL_TWO_PASS;
 }
 
-namespace {
-
-template <typename SP, bool W, typename TL, bool NTS>
-void launch_fused(const void* Pin, int kind, void* Pout, const Grid& g, const StepArgs& a, float rs,
-                  const FusedTuning& t, hipStream_t s)
-{
-    using T = typename SP::T;
-    const int ncx = (g.nx + t.chunk - 1) / t.chunk;
-    const unsigned nb = static_cast<unsigned>((g.ny / TL::TY) * (g.nz / TL::TZ) * ncx);
-    fused::k_fused<SP, float, W, false, false, TL, NTS><<<nb, TL::NT, 0, s>>>(
-        static_cast<const T*>(Pin), kind, static_cast<T*>(Pout), g, (float)a.tau, (float)a.G, rs,
-        t.chunk, t.order);
-}
-
-template <typename SP, bool W, bool NTS>
-void launch_fused_shape(const void* Pin, int kind, void* Pout, const Grid& g, const StepArgs& a,
-                        float rs, const FusedTuning& t, hipStream_t s)
-{
-    using namespace fused;
-    switch (t.shape) {
-    case 0: launch_fused<SP, W, Tile<kShapeTY[0], kShapeTZ[0], kShapeZF[0]>, NTS>(Pin, kind, Pout, g, a, rs, t, s); break;
-    case 1: launch_fused<SP, W, Tile<kShapeTY[1], kShapeTZ[1], kShapeZF[1]>, NTS>(Pin, kind, Pout, g, a, rs, t, s); break;
-    case 2: launch_fused<SP, W, Tile<kShapeTY[2], kShapeTZ[2], kShapeZF[2]>, NTS>(Pin, kind, Pout, g, a, rs, t, s); break;
-    case 3: launch_fused<SP, W, Tile<kShapeTY[3], kShapeTZ[3], kShapeZF[3]>, NTS>(Pin, kind, Pout, g, a, rs, t, s); break;
-    case 4: launch_fused<SP, W, Tile<kShapeTY[4], kShapeTZ[4], kShapeZF[4]>, NTS>(Pin, kind, Pout, g, a, rs, t, s); break;
-    default: launch_fused<SP, W, Tile<kShapeTY[5], kShapeTZ[5], kShapeZF[5]>, NTS>(Pin, kind, Pout, g, a, rs, t, s); break;
-    }
-}
-
-}  // namespace
-
 hipError_t lowp_init(int precision, const StepArgs& a, const Grid& g, double rho_shift,
-                     const double* rho0, void* P, float* psi, hipStream_t s)
+                     const double* rho0, void* P, float* psi, float* rout, hipStream_t s)
 {
-    const float sh = static_cast<float>(rho_shift);
+    const float rs = shift_of(precision, rho_shift);
     LOWP_DISPATCH(precision, a, ({
         using T = typename SP::T;
-        k_psi<Q, W, SP><<<blocks_for(g.n), kBlock, 0, s>>>(nullptr, kHoldsF, rho0, psi, g, sh);
-        k_collide<Q, W, 2, SP><<<blocks_for(g.n), kBlock, 0, s>>>(
-            nullptr, kHoldsF, static_cast<T*>(P), psi, rho0, g, (float)a.tau, (float)a.G, sh);
+        const Ref<NS> ro{rout, rs};
+        k_psi<Q, W, SP, NS><<<blocks_for(g.n), kBlock, 0, s>>>(nullptr, kHoldsF, rho0, psi, g, ro,
+                                                                 rout);
+        k_collide<Q, W, 2, SP, NS><<<blocks_for(g.n), kBlock, 0, s>>>(
+            nullptr, kHoldsF, static_cast<T*>(P), psi, rho0, g, (float)a.tau, (float)a.G, ro, ro);
     }));
     return hipGetLastError();
 }
 
 hipError_t lowp_step(int precision, const StepArgs& a, const Grid& g, double rho_shift,
                      const void* Pin, int kind, void* Pout, const double* rho0, float* psi,
-                     int kernel, const FusedTuning& t, hipStream_t s)
+                     const float* rin, float* rout, int kernel, const FusedTuning& t,
+                     hipStream_t s)
 {
-    const float sh = static_cast<float>(rho_shift);
+    const float rs = shift_of(precision, rho_shift);
+    if (lowp_node_shift(precision) && (!rin || !rout)) return hipErrorInvalidValue;
     if (kernel == LBM_KERNEL_FUSED && rho0 == nullptr) {
         if (!fused::supported(g, a.q, t.shape) || t.chunk < 1) return hipErrorNotSupported;
-        // FP32 stores f itself: the shifted frame then has rs = 0
-        const float rs = (precision == LBM_FP32) ? 0.f : sh;
-        auto go = [&](auto sptag) {
-            using SP = decltype(sptag);
-            if (a.walls) {
-                if (t.nts) launch_fused_shape<SP, true, true>(Pin, kind, Pout, g, a, rs, t, s);
-                else launch_fused_shape<SP, true, false>(Pin, kind, Pout, g, a, rs, t, s);
-            } else {
-                if (t.nts) launch_fused_shape<SP, false, true>(Pin, kind, Pout, g, a, rs, t, s);
-                else launch_fused_shape<SP, false, false>(Pin, kind, Pout, g, a, rs, t, s);
-            }
-        };
-        if (precision == LBM_FP32) go(StF32{});
-        else if (precision == LBM_FP16S) go(StF16{});
-        else go(StBF16{});
-        return hipGetLastError();
+        switch (precision) {
+        case LBM_FP32: return lowp_fused_launch<0, false>(Pin, kind, Pout, g, a, rs, rin, rout, t, s);
+        case LBM_FP16S: return lowp_fused_launch<1, false>(Pin, kind, Pout, g, a, rs, rin, rout, t, s);
+        case LBM_BF16S: return lowp_fused_launch<2, false>(Pin, kind, Pout, g, a, rs, rin, rout, t, s);
+        case LBM_FP16N: return lowp_fused_launch<1, true>(Pin, kind, Pout, g, a, rs, rin, rout, t, s);
+        default: return lowp_fused_launch<2, true>(Pin, kind, Pout, g, a, rs, rin, rout, t, s);
+        }
     }
     LOWP_DISPATCH(precision, a, ({
         using T = typename SP::T;
         const T* in = static_cast<const T*>(Pin);
         T* out = static_cast<T*>(Pout);
-        k_psi<Q, W, SP><<<blocks_for(g.n), kBlock, 0, s>>>(in, kind, rho0, psi, g, sh);
+        const Ref<NS> ri{rin, rs}, ro{rout, rs};
+        k_psi<Q, W, SP, NS><<<blocks_for(g.n), kBlock, 0, s>>>(in, kind, rho0, psi, g, ri, rout);
         if (rho0)
-            k_collide<Q, W, 1, SP><<<blocks_for(g.n), kBlock, 0, s>>>(in, kind, out, psi, rho0, g,
-                                                                        (float)a.tau, (float)a.G, sh);
+            k_collide<Q, W, 1, SP, NS><<<blocks_for(g.n), kBlock, 0, s>>>(
+                in, kind, out, psi, rho0, g, (float)a.tau, (float)a.G, ri, ro);
         else
-            k_collide<Q, W, 0, SP><<<blocks_for(g.n), kBlock, 0, s>>>(in, kind, out, psi, rho0, g,
-                                                                        (float)a.tau, (float)a.G, sh);
+            k_collide<Q, W, 0, SP, NS><<<blocks_for(g.n), kBlock, 0, s>>>(
+                in, kind, out, psi, rho0, g, (float)a.tau, (float)a.G, ri, ro);
     }));
     return hipGetLastError();
 }
 
 hipError_t lowp_moments(int precision, const StepArgs& a, const Grid& g, double rho_shift,
-                        const void* P, int kind, double* rho, double* u, double* v, double* w,
-                        hipStream_t s)
+                        const void* P, int kind, const float* rin, double* rho, double* u,
+                        double* v, double* w, hipStream_t s)
 {
-    const float sh = static_cast<float>(rho_shift);
+    const float rs = shift_of(precision, rho_shift);
     LOWP_DISPATCH(precision, a, ({
-        k_moments<Q, W, SP><<<blocks_for(g.n), kBlock, 0, s>>>(
-            static_cast<const typename SP::T*>(P), kind, g, sh, rho, u, v, w);
+        k_moments<Q, W, SP, NS><<<blocks_for(g.n), kBlock, 0, s>>>(
+            static_cast<const typename SP::T*>(P), kind, g, Ref<NS>{rin, rs}, rho, u, v, w);
     }));
     return hipGetLastError();
 }
 
 hipError_t lowp_get_f(int precision, const StepArgs& a, const Grid& g, double rho_shift,
-                      const void* P, int kind, double* aos, hipStream_t s)
+                      const void* P, int kind, const float* rin, double* aos, hipStream_t s)
 {
-    const float sh = static_cast<float>(rho_shift);
+    const float rs = shift_of(precision, rho_shift);
     LOWP_DISPATCH(precision, a, ({
-        k_get_f<Q, W, SP><<<blocks_for(g.n), kBlock, 0, s>>>(static_cast<const typename SP::T*>(P),
-                                                              kind, g, sh, aos);
+        k_get_f<Q, W, SP, NS><<<blocks_for(g.n), kBlock, 0, s>>>(
+            static_cast<const typename SP::T*>(P), kind, g, Ref<NS>{rin, rs}, aos);
     }));
     return hipGetLastError();
 }
 
 hipError_t lowp_set_f(int precision, const StepArgs& a, const Grid& g, double rho_shift,
-                      const double* aos, void* P, hipStream_t s)
+                      const double* aos, void* P, float* rout, hipStream_t s)
 {
-    const float sh = static_cast<float>(rho_shift);
+    const float rs = shift_of(precision, rho_shift);
     LOWP_DISPATCH(precision, a, ({
-        k_set_f<Q, SP><<<blocks_for(g.n), kBlock, 0, s>>>(aos, g, sh, static_cast<typename SP::T*>(P));
+        k_set_f<Q, SP, NS><<<blocks_for(g.n), kBlock, 0, s>>>(aos, g, rs,
+                                                               static_cast<typename SP::T*>(P), rout);
     }));
     return hipGetLastError();
 }
 
 hipError_t lowp_conservation(int precision, const StepArgs& a, const Grid& g, double rho_shift,
-                             const void* P, int kind, double* partials, int* nblocks,
-                             hipStream_t s)
+                             const void* P, int kind, const float* rin, double* partials,
+                             int* nblocks, hipStream_t s)
 {
     const int64_t per = static_cast<int64_t>(kConsBlock) * kConsPerThread;
     const int nb = static_cast<int>((g.n + per - 1) / per);
     *nblocks = nb;
-    const float sh = static_cast<float>(rho_shift);
+    const float rs = shift_of(precision, rho_shift);
     LOWP_DISPATCH(precision, a, ({
-        k_conservation<Q, W, SP><<<nb, kConsBlock, 0, s>>>(static_cast<const typename SP::T*>(P),
-                                                             kind, g, sh, partials);
+        k_conservation<Q, W, SP, NS><<<nb, kConsBlock, 0, s>>>(
+            static_cast<const typename SP::T*>(P), kind, g, Ref<NS>{rin, rs}, partials);
     }));
     return hipGetLastError();
 }

@@ -14,10 +14,11 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmesobasic_lbm.so")
 
-FP64, FP32, FP16S, BF16S = 0, 1, 2, 3
+FP64, FP32, FP16S, BF16S, FP16N, BF16N = 0, 1, 2, 3, 4, 5
 KERNEL_AUTO, KERNEL_TWO_PASS, KERNEL_FUSED = 0, 1, 2
 FIELDS_REFERENCE, FIELDS_CURRENT = 0, 1
-PRECISIONS = {"fp64": FP64, "fp32": FP32, "fp16": FP16S, "bf16": BF16S}
+PRECISIONS = {"fp64": FP64, "fp32": FP32, "fp16": FP16S, "bf16": BF16S, "fp16n": FP16N,
+              "bf16n": BF16N}
 
 _D = ctypes.POINTER(ctypes.c_double)
 

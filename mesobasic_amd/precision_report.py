@@ -5,7 +5,8 @@
 
 Part 1 (spinodal decomposition, D3Q19 N^3 periodic, tau 1, G -5.0, rho =
 0.7 +- 0.01 seeded noise): runs FP64 (bit-identical to the reference), FP32,
-FP16 and BF16 storage for the same steps and reports, per form,
+and FP16 and BF16 storage with the constant shift (0.7) and with the node
+shift, for the same steps, and reports, per form,
   - relative L2 of the density field against FP64 (at several step counts;
     spinodal growth is unstable, so small differences grow),
   - coexistence densities: mean density of the liquid (rho > midpoint) and
@@ -27,9 +28,11 @@ import time
 import numpy as np
 
 from . import lbm
-from .lbm import BF16S, FP16S, FP32, FP64, Lattice, droplet_rho, noise_rho
+from .lbm import BF16N, BF16S, FP16N, FP16S, FP32, FP64, Lattice, droplet_rho, noise_rho
 
-NAMES = {FP64: "FP64", FP32: "FP32", FP16S: "FP16 (shifted)", BF16S: "BF16 (shifted)"}
+NAMES = {FP64: "FP64", FP32: "FP32", FP16S: "FP16 (constant shift 0.7)",
+         BF16S: "BF16 (constant shift 0.7)", FP16N: "FP16 (node shift)", BF16N: "BF16 (node shift)"}
+FORMS = (FP32, FP16S, BF16S, FP16N, BF16N)
 
 
 def rel_l2(a, b):
@@ -109,7 +112,7 @@ def main():
     print("|---|" + "---|" * len(cps) + "---|---|---|---|")
     print(f"| FP64 | " + " | ".join("0" for _ in cps) + f" | {rl:.5f} | {rv:.5f} | {drift64:.2e} | {s64:.1f} |",
           flush=True)
-    for prec in (FP32, FP16S, BF16S):
+    for prec in FORMS:
         got, drift, secs = run_spinodal(prec, args.size, args.steps, cps)
         l, v = coexistence(got[args.steps])
         errs = " | ".join(f"{rel_l2(got[c], ref[c]):.2e}" for c in cps)
@@ -123,7 +126,7 @@ def main():
     print("| form | rho rel L2 | radius | Laplace dp | liquid rho | vapour rho |")
     print("|---|---|---|---|---|---|")
     print(f"| FP64 | 0 | {m['R']:.3f} | {m['dp']:.5e} | {m['rho_l']:.5f} | {m['rho_v']:.5f} |", flush=True)
-    for prec in (FP32, FP16S, BF16S):
+    for prec in FORMS:
         got = run_droplet(prec, args.droplet_size, args.droplet_steps)
         g = droplet_metrics(got, -5.5)
         print(f"| {NAMES[prec]} | {rel_l2(got, ref):.2e} | {g['R']:.3f} | {g['dp']:.5e} | "
