@@ -242,6 +242,15 @@ int lbm_batch_active(const lbm_batch* b);
 int64_t lbm_batch_bytes_in_use(const lbm_batch* b);
 int64_t lbm_batch_steps_run(const lbm_batch* b);
 
+/* Scheduling: with max_nodes > 0, a tick advances only the unfinished jobs
+ * with the least remaining work (nodes x steps left), taken in that order
+ * while their node count stays within max_nodes (always at least one job);
+ * the others stay resident and wait.  Shortest-remaining-work-first lowers
+ * the latency of small requests when more work is admitted than the GPU
+ * needs to stay busy.  0 (default): every unfinished job, every tick.
+ * Results do not depend on the schedule. */
+int lbm_batch_set_work_limit(lbm_batch* b, int64_t max_nodes);
+
 /* Launch configuration the batch uses (after autotuning): D2Q9 rows per
  * work item and whether D2Q9 jobs use the row-marching kernel (1) or the
  * two-pass kernels (0). */

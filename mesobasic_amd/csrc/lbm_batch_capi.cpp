@@ -69,6 +69,7 @@ struct lbm_batch {
     int64_t next_id = 1;
     int64_t in_use = 0;
     int64_t steps_run = 0;
+    int64_t work_limit = 0;       // max nodes advanced per step (0: all)
     // device tables
     Job* d_jobs = nullptr;
     int cap_jobs = 0;
@@ -168,6 +169,19 @@ int upload_tables(lbm_batch* b)
     b->order.clear();
     for (auto& kv : b->jobs)
         if (!kv.second->finished) b->order.push_back(kv.second.get());
+    if (b->work_limit > 0 && !b->order.empty()) {
+        // shortest remaining work first, within the node limit; a running
+        // job's remaining work only shrinks, so the set changes only when a
+        // job is submitted or finishes (both mark the tables dirty)
+        std::stable_sort(b->order.begin(), b->order.end(), [](const HostJob* a, const HostJob* z) {
+            return a->n * a->remaining < z->n * z->remaining;
+        });
+        int64_t nodes = 0;
+        size_t keep = 0;
+        while (keep < b->order.size() && (keep == 0 || nodes + b->order[keep]->n <= b->work_limit))
+            nodes += b->order[keep++]->n;
+        b->order.resize(keep);
+    }
     int rc = ensure_jobs(b, static_cast<int>(b->order.size()) + 1);
     if (rc) return rc;
     std::vector<Item> items[kNumCls];
@@ -519,6 +533,14 @@ int lbm_batch_active(const lbm_batch* b) { return b ? static_cast<int>(b->jobs.s
 int64_t lbm_batch_bytes_in_use(const lbm_batch* b) { return b ? b->in_use : -1; }
 
 int64_t lbm_batch_steps_run(const lbm_batch* b) { return b ? b->steps_run : -1; }
+
+int lbm_batch_set_work_limit(lbm_batch* b, int64_t max_nodes)
+{
+    if (!b || max_nodes < 0) return lbm::set_error(LBM_EINVAL, "lbm_batch_set_work_limit: bad argument");
+    b->work_limit = max_nodes;
+    b->dirty = true;
+    return LBM_OK;
+}
 
 int lbm_batch_tuning(const lbm_batch* b, int* d2q9_chunk, int* d2q9_rows_kernel)
 {

@@ -16,7 +16,9 @@ int precision_of(const std::string& s)
     if (s == "fp32") return LBM_FP32;
     if (s == "fp16") return LBM_FP16S;
     if (s == "bf16") return LBM_BF16S;
-    throw std::runtime_error("LBApp/precision must be fp64, fp32, fp16 or bf16 (got " + s + ")");
+    if (s == "fp16n") return LBM_FP16N;
+    if (s == "bf16n") return LBM_BF16N;
+    throw std::runtime_error("LBApp/precision must be fp64, fp32, fp16, bf16, fp16n or bf16n (got " + s + ")");
 }
 
 int kernel_of(const std::string& s)

@@ -102,6 +102,7 @@ def load():
         "lbm_batch_bytes_in_use": (ctypes.c_int64, [P]),
         "lbm_batch_steps_run": (ctypes.c_int64, [P]),
         "lbm_batch_tuning": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+        "lbm_batch_set_work_limit": (ctypes.c_int, [P, ctypes.c_int64]),
         "lbm_probe_copy": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, ctypes.c_int, _D]),
         "lbm_probe_tile_map": (ctypes.c_int, [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_int)] * 3),
     }
@@ -290,7 +291,7 @@ class Batch:
     submit(), advanced by tick(), and read back and freed by collect()."""
 
     def __init__(self, precision=FP16S, rho_shift=0.7, hbm_budget=0, steps_per_tick=20,
-                 use_graph=True, autotune=False):
+                 use_graph=True, autotune=False, work_limit=0):
         lib = load()
         if isinstance(precision, str):
             precision = PRECISIONS[precision]
@@ -300,6 +301,8 @@ class Batch:
         _check(lib.lbm_batch_create(ctypes.byref(d), ctypes.byref(h)))
         self._h, self._lib = h, lib
         self.shapes = {}
+        if work_limit:
+            _check(lib.lbm_batch_set_work_limit(h, int(work_limit)))
 
     def close(self):
         if getattr(self, "_h", None):

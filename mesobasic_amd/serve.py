@@ -63,11 +63,12 @@ def initial_rho(req):
 
 
 def run_stream(reqs, precision=FP16S, budget_bytes=24 << 30, max_active=0, steps_per_tick=20,
-               keep_results=False, clock=time.perf_counter):
+               work_limit=0, keep_results=False, clock=time.perf_counter):
     """Serves `reqs` in real time; returns a summary dict (and the per-request
-    fields when keep_results)."""
+    fields when keep_results).  work_limit > 0: each tick advances the
+    admitted jobs with the least remaining work, up to that many nodes."""
     b = Batch(precision=precision, hbm_budget=budget_bytes, steps_per_tick=steps_per_tick,
-              use_graph=True, autotune=False)
+              use_graph=True, autotune=False, work_limit=work_limit)
     pending = list(reqs)            # not yet admitted, arrival order
     live = {}                       # job id -> request
     done = {}
@@ -140,12 +141,12 @@ def run_batched(jobs, steps=2000, precision=FP16S, steps_per_tick=20, autotune=T
         if len(b.finished()) == len(jobs):
             break
     wall = clock() - t0
-    finite = 0
-    for jid in b.finished():
-        finite += bool(np.isfinite(b.collect(jid)["rho"]).all())
+    index = {jid: k for k, jid in enumerate(ids)}
+    nonfinite = sorted(index[jid] for jid in b.finished()
+                       if not np.isfinite(b.collect(jid)["rho"]).all())
     summary = dict(jobs=len(jobs), steps=steps, nodes=nodes, wall_s=wall, ticks=n_ticks,
                    aggregate_mlups=nodes * steps / wall / 1e6, tuning=b.tuning,
-                   finite_results=finite)
+                   finite_results=len(jobs) - len(nonfinite), nonfinite_jobs=nonfinite)
     b.close()
     return summary
 
@@ -161,6 +162,8 @@ def main():
     ap.add_argument("--budget-gb", type=float, default=24.0)
     ap.add_argument("--max-active", type=int, default=0)
     ap.add_argument("--steps-per-tick", type=int, default=20)
+    ap.add_argument("--work-limit", type=int, default=0,
+                    help="nodes advanced per step, least remaining work first (0: all)")
     ap.add_argument("--precision", default="fp16", choices=["fp32", "fp16", "bf16"])
     args = ap.parse_args()
     prec = PRECISIONS[args.precision]
@@ -168,7 +171,8 @@ def main():
         s = run_batched(make_batched_jobs(args.jobs), args.steps, prec, args.steps_per_tick)
     else:
         s = run_stream(make_requests(args.requests, args.rate, args.seed), prec,
-                       int(args.budget_gb * (1 << 30)), args.max_active, args.steps_per_tick)
+                       int(args.budget_gb * (1 << 30)), args.max_active, args.steps_per_tick,
+                       args.work_limit)
     print(json.dumps(s), flush=True)
 
 

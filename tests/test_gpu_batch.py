@@ -107,7 +107,8 @@ def test_large_rows_take_the_two_pass_path():
     assert rel_l2(res["rho"], o.moments()["rho"]) < 2e-6
 
 
-def test_serving_stream_admits_as_memory_frees_and_matches_alone():
+@pytest.mark.parametrize("work_limit", [0, 3000])
+def test_serving_stream_admits_as_memory_frees_and_matches_alone(work_limit):
     from mesobasic_amd import serve
     rng = np.random.default_rng(9)
     reqs = []
@@ -120,7 +121,7 @@ def test_serving_stream_admits_as_memory_frees_and_matches_alone():
                          init="droplet" if k % 2 else "noise", seed=100 + k))
     need = max(Batch(precision=mb.FP16S).job_bytes(r["q"], r["shape"]) for r in reqs)
     summary, res = serve.run_stream(reqs, precision=mb.FP16S, budget_bytes=3 * need, steps_per_tick=8,
-                                    keep_results=True)
+                                    work_limit=work_limit, keep_results=True)
     print(summary, flush=True)
     # the first four arrive together: at least three fit the budget at once
     assert summary["requests"] == len(reqs) and summary["max_active"] >= 3
