@@ -53,9 +53,22 @@ namespace fused {
 // Wall variants use PkFloat for both: their bounce-back values carry a
 // density correction that is not a stored value.
 template <typename SP> struct Load {
-    typename SP::T v[19];
-    template <int n> __device__ __forceinline__ void load(const typename SP::T* p) { v[n] = *p; }
-    template <int n> __device__ __forceinline__ float get() const { return SP::d(v[n]); }
+    // 16-bit values zero-extended into 32-bit registers: the compiler would
+    // otherwise pack pairs of them, which waits for the loads at once
+    uint32_t v[19];
+    template <int n> __device__ __forceinline__ void load(const typename SP::T* p)
+    {
+        v[n] = *reinterpret_cast<const unsigned short*>(p);
+    }
+    template <int n> __device__ __forceinline__ float get() const
+    {
+        return SP::d(__builtin_bit_cast(typename SP::T, static_cast<unsigned short>(v[n])));
+    }
+};
+template <> struct Load<StF32> {
+    float v[19];
+    template <int n> __device__ __forceinline__ void load(const float* p) { v[n] = *p; }
+    template <int n> __device__ __forceinline__ float get() const { return v[n]; }
 };
 template <typename SP> struct Held;
 template <> struct Held<StF16> {
@@ -65,8 +78,9 @@ template <> struct Held<StF16> {
     {
 #pragma unroll
         for (int k = 0; k < 10; k++) {
-            v[k][0] = __builtin_bit_cast(_Float16, l.v[2 * k]);
-            if (2 * k + 1 < 19) v[k][1] = __builtin_bit_cast(_Float16, l.v[2 * k + 1]);
+            v[k][0] = __builtin_bit_cast(_Float16, static_cast<unsigned short>(l.v[2 * k]));
+            if (2 * k + 1 < 19)
+                v[k][1] = __builtin_bit_cast(_Float16, static_cast<unsigned short>(l.v[2 * k + 1]));
         }
     }
     template <int n> __device__ __forceinline__ float get() const
@@ -80,8 +94,8 @@ template <> struct Held<StBF16> {
     {
 #pragma unroll
         for (int k = 0; k < 10; k++) {
-            const uint32_t lo = __builtin_bit_cast(unsigned short, l.v[2 * k]);
-            const uint32_t hi = (2 * k + 1 < 19) ? __builtin_bit_cast(unsigned short, l.v[2 * k + 1]) : 0u;
+            const uint32_t lo = l.v[2 * k];
+            const uint32_t hi = (2 * k + 1 < 19) ? l.v[2 * k + 1] : 0u;
             v[k] = lo | (hi << 16);
         }
     }

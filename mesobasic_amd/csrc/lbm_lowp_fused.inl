@@ -20,7 +20,7 @@ void launch(const void* Pin, int kind, void* Pout, const Grid& g, const StepArgs
     using T = typename SP::T;
     const int ncx = (g.nx + t.chunk - 1) / t.chunk;
     const unsigned nb = static_cast<unsigned>((g.ny / TL::TY) * (g.nz / TL::TZ) * ncx);
-    if constexpr (!NS) {
+    if constexpr (!NS && !T1) {
         // constant shift (FP32 storage: shift 0, f itself): the shared march
         // kernel in its FP32-arithmetic form, converting at load
         fused::k_fused<SP, float, W, false, false, TL, NTS><<<nb, TL::NT, 0, s>>>(
@@ -71,9 +71,8 @@ template <typename SP, bool NS>
 hipError_t run(const void* Pin, int kind, void* Pout, const Grid& g, const StepArgs& a, float rs,
                const float* rin, float* rout, const FusedTuning& t, hipStream_t s)
 {
-    // tau = 1 has its own node-shift kernel (a waiting plane keeps only its
-    // momentum); the constant-shift kernel does not specialise it
-    if (NS && a.tau == 1.0) by_flags<SP, NS, NS>(Pin, kind, Pout, g, a, rs, rin, rout, t, s);
+    // tau = 1 has its own kernel (a waiting plane keeps only its momentum)
+    if (a.tau == 1.0) by_flags<SP, NS, true>(Pin, kind, Pout, g, a, rs, rin, rout, t, s);
     else by_flags<SP, NS, false>(Pin, kind, Pout, g, a, rs, rin, rout, t, s);
     return hipGetLastError();
 }
