@@ -25,6 +25,8 @@
 // reciprocals instead of dividing.
 #pragma once
 
+#include <type_traits>
+
 #include "lbm_exp.h"
 #include "lbm_internal.h"
 #include "lbm_stencil.h"
@@ -62,6 +64,9 @@ constexpr bool kShapeZF[kNumShapes] = {true, true, true, false, false, false};
 inline bool supported(const Grid& g, int q, int shape)
 {
     if (q != 19 || shape < 0 || shape >= kNumShapes) return false;
+    // element offsets within a population plane are 32-bit byte offsets
+    // (elem() below): planes of at most 2^29 nodes, for 8-byte values
+    if (g.ps >= (int64_t(1) << 29)) return false;
     const int ty = kShapeTY[shape], tz = kShapeTZ[shape];
     if (g.ny % ty != 0 || g.nx < 1) return false;
     return kShapeZF[shape] ? g.nz == tz : g.nz % tz == 0;
@@ -96,6 +101,16 @@ template <typename TL> __device__ __forceinline__ void ring_slot(int r, int& row
 
 __device__ __forceinline__ int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
+// Element i of an array addressed as a 32-bit byte offset from a base that is
+// uniform across the wavefront: the access then takes the global "saddr" form
+// (64-bit base in SGPRs plus one 32-bit VGPR), with no 64-bit address
+// arithmetic per gather.  Callers keep i * sizeof(T) below 2^32 (supported()).
+template <typename T> __device__ __forceinline__ T* elem(T* base, uint32_t i)
+{
+    using B = std::conditional_t<std::is_const_v<T>, const char, char>;
+    return reinterpret_cast<T*>(reinterpret_cast<B*>(base) + i * static_cast<uint32_t>(sizeof(T)));
+}
+
 // Pull the stored values for node (x, y, z) (coordinates already wrapped
 // into range): f itself in the FP64/FP32 formats, d = f - w*rho_shift in the
 // 16-bit ones.  Offsets are 32-bit node indices added to per-population base pointers that
@@ -111,7 +126,7 @@ __device__ __forceinline__ void pull(const typename SP::T* __restrict__ P, const
     if (kind == kHoldsF) {
         static_for<19>([&](auto I) {
             constexpr int n = decltype(I)::value;
-            f[n] = SP::d(P[n * g.ps + c]);
+            f[n] = SP::d(*elem(P + n * g.ps, c));
         });
         return;
     }
@@ -135,19 +150,19 @@ __device__ __forceinline__ void pull(const typename SP::T* __restrict__ P, const
         if constexpr (S::ez[n] == 1) o += dzm;
         if constexpr (S::ez[n] == -1) o += dzp;
         const typename SP::T* base = P + n * g.ps;
-        f[n] = SP::d(base[o]);
+        f[n] = SP::d(*elem(base, o));
     });
     if constexpr (WALLS) {
         if (z == 1) {
             static_for<S::nwall>([&](auto I) {
                 constexpr int m = decltype(I)::value;
-                f[S::wall_up[m]] = SP::d(P[S::wall_dn[m] * g.ps + c]);
+                f[S::wall_up[m]] = SP::d(*elem(P + S::wall_dn[m] * g.ps, c));
             });
         }
         if (z == g.nz - 2) {
             static_for<S::nwall>([&](auto I) {
                 constexpr int m = decltype(I)::value;
-                f[S::wall_dn[m]] = SP::d(P[S::wall_up[m] * g.ps + c]);
+                f[S::wall_dn[m]] = SP::d(*elem(P + S::wall_up[m] * g.ps, c));
             });
         }
     }
@@ -245,8 +260,8 @@ __device__ __forceinline__ void collide_store(typename SP::T* __restrict__ Pout,
             const CT deq = static_cast<CT>(S::w[n]) * (rho * poly - rs);
             out = f[n] - (f[n] - deq) * itau;
         }
-        if constexpr (NTS) SP::put_nt(Pout + n * g.ps + c, SP::of_d(out));
-        else Pout[n * g.ps + c] = SP::of_d(out);
+        if constexpr (NTS) SP::put_nt(elem(Pout + n * g.ps, c), SP::of_d(out));
+        else *elem(Pout + n * g.ps, c) = SP::of_d(out);
     });
 }
 

@@ -185,11 +185,11 @@ __device__ __forceinline__ float pull_lp(const typename SP::T* __restrict__ P,
                         static_cast<uint32_t>(y)) * static_cast<uint32_t>(g.nz) +
                        static_cast<uint32_t>(z);
     float R = rs;
-    if constexpr (NS) R = rin[c];
+    if constexpr (NS) R = *elem(rin, c);
     if (kind == kHoldsF) {
         static_for<19>([&](auto I) {
             constexpr int n = decltype(I)::value;
-            d.template load<n>(P + n * g.ps + c);
+            d.template load<n>(elem(P + n * g.ps, c));
         });
         return R;
     }
@@ -197,7 +197,7 @@ __device__ __forceinline__ float pull_lp(const typename SP::T* __restrict__ P,
     static_for<19>([&](auto I) {
         constexpr int n = decltype(I)::value;
         const uint32_t up = c + o.template to<-S::ex[n], -S::ey[n], -S::ez[n]>();
-        d.template load<n>(P + n * g.ps + up);
+        d.template load<n>(elem(P + n * g.ps, up));
     });
     if constexpr (WALLS) {
         auto bounce = [&](auto M, auto from_dn) {
@@ -205,9 +205,9 @@ __device__ __forceinline__ float pull_lp(const typename SP::T* __restrict__ P,
             constexpr bool DN = decltype(from_dn)::value;
             constexpr int src = DN ? S::wall_dn[m] : S::wall_up[m];
             constexpr int dst = DN ? S::wall_up[m] : S::wall_dn[m];
-            float v = SP::d(P[src * g.ps + c]);
+            float v = SP::d(*elem(P + src * g.ps, c));
             if constexpr (NS) {
-                const float Rn = rin[c + o.template to<S::ex[src], S::ey[src], S::ez[src]>()];
+                const float Rn = *elem(rin, c + o.template to<S::ex[src], S::ey[src], S::ez[src]>());
                 v = fmaf(kStoreScale<SP, NS> * static_cast<float>(S::w[src]), Rn - R, v);
             }
             d.v[dst] = v;
@@ -303,7 +303,7 @@ __global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(Occ
     auto own_win = [&](int xu, const LD& dv, float R) {
         const float rho = put_win(slot(xu) + li, dv, R);
         if constexpr (NS) {
-            if (xu >= x0 && xu < x1) rout[static_cast<uint32_t>(xu) * plane + cyz] = rho;
+            if (xu >= x0 && xu < x1) *elem(rout, static_cast<uint32_t>(xu) * plane + cyz) = rho;
         }
     };
 
@@ -381,8 +381,8 @@ __global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(Occ
             if constexpr (T1) v = fmaf(sw * rho, pm1, fmaf(-sw, rnb, sw * rho));
             else v = fmaf(d.template get<n>(), omt, fmaf((om * sw) * rho, pm1, fmaf(-sw, rnb, sw * t)));
             typename SP::T* base = Pout + n * g.ps;
-            if constexpr (NTS) SP::put_nt(base + c, SP::of_d(v));
-            else base[c] = SP::of_d(v);
+            if constexpr (NTS) SP::put_nt(elem(base, c), SP::of_d(v));
+            else *elem(base, c) = SP::of_d(v);
         });
     };
 
