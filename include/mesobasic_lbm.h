@@ -198,7 +198,7 @@ int lbm_abi_version(void);
 typedef struct lbm_batch lbm_batch;
 
 typedef struct lbm_batch_desc {
-    int precision;          /* LBM_FP32, LBM_FP16S or LBM_BF16S */
+    int precision;          /* LBM_FP32, LBM_FP16S, LBM_BF16S, LBM_FP16N or LBM_BF16N */
     double rho_shift;       /* storage shift (0 is used for LBM_FP32) */
     int64_t hbm_budget;     /* bytes the jobs may hold; 0 = 90% of free HBM */
     int steps_per_tick;     /* steps per tick (rounded up to even) */

@@ -13,6 +13,7 @@ namespace batch {
 struct Job {
     void* P[2];          // population buffers; the input of launch parity p is P[p ^ phase]
     float* psi;          // psi scratch (two-pass stencils)
+    float* R[2];         // node shift: reference density of P[0] / P[1] (nullptr otherwise)
     const float* rho0;   // initial density for the LBfluid2D-order first step (nullptr: none)
     int nx, ny, nz;      // nz = 1 for D2Q9
     int phase;
@@ -40,7 +41,8 @@ struct Tables {
     unsigned* tickets;   // [2]
 };
 
-// prec: LBM_FP32 / LBM_FP16S / LBM_BF16S (FP32 compute in every case).
+// prec: LBM_FP32 / LBM_FP16S / LBM_BF16S / LBM_FP16N / LBM_BF16N (FP32
+// compute in every case).
 hipError_t d2q9_rows_step(int prec, const Tables& t, int parity, int threads, int grid,
                           hipStream_t s);
 hipError_t twopass_psi(int prec, int q, const Tables& t, int parity, int grid, hipStream_t s);

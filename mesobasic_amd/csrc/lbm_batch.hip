@@ -1,20 +1,30 @@
 // lbm_batch.hip -- kernels of the batched engine (lbm_batch.h).
 //
-// Every job holds post-collision populations in the shifted frame
-// d = f - w*rs (16-bit or FP32 storage, FP32 arithmetic) and advances by the
-// same step as the single-lattice FP32/16-bit forms: pull f(x) = f*(x - e)
-// with periodic wrap, rho = rs + sum d, j = sum d e, psi = 1 - exp(-rho),
-// Shan-Chen force (scmp3D.cpp:141-179), BGK collision with velocity shift
-// (LBfluid3D.cpp:259-276) written for d:  d' = d - (d - d_eq)/tau,
-// d_eq = w*(rho*poly - rs).  A job's first step follows the reference's
-// step order for its stencil: D2Q9 jobs collide with the stored rho0 and
-// u = 0 (LBfluid2D order, LBfluid2D.cpp:235-249 after setFtoFeq), D3Q19 jobs
-// with the moments of f0 (scmp3D order, scmp3D.cpp:90-119).
+// Every job holds post-collision populations as stored values d relative to
+// a reference density R of the node that will read them, scaled by S:
+// f = d/S + w*R.  Constant shift: R = rs (rs = 0 for FP32 storage), S = 1.
+// Node shift (FP16N / BF16N): R is the reading node's density after the
+// previous step, an FP32 field double-buffered with the populations (job.R),
+// and FP16 values are stored times S = 2^10 (the scheme of the
+// single-lattice kernels, lbm_fused_lp.h).  Each step is the single-lattice
+// FP32-compute step: pull f(x) = f*(x - e) with periodic wrap, rho = R +
+// sum d/S, j = sum d e/S, psi = 1 - exp(-rho), Shan-Chen force
+// (scmp3D.cpp:141-179), BGK collision with velocity shift
+// (LBfluid3D.cpp:259-276).  A job's first step follows the reference's step
+// order for its stencil: D2Q9 jobs collide with the stored rho0 and u = 0
+// (LBfluid2D order, LBfluid2D.cpp:235-249 after setFtoFeq), D3Q19 jobs with
+// the moments of f0 (scmp3D order, scmp3D.cpp:90-119).
 //
 // Launches are persistent: a fixed grid whose workgroups take items from a
 // ticket counter until the table is exhausted, so uneven jobs balance.  The
 // counter used by launch parity p is tickets[p]; each launch clears the other
 // one, which the next launch (parity p^1) then starts from.
+//
+// The row-marching and two-pass kernels evaluate the same expressions in the
+// same order (shared helpers below; the file is compiled without FP
+// contraction), so a job's bits do not depend on which of them runs.
+#include <type_traits>
+
 #include "../../include/mesobasic_lbm.h"
 #include "lbm_batch.h"
 #include "lbm_stencil.h"
@@ -25,6 +35,41 @@ namespace batch {
 namespace {
 
 __device__ __forceinline__ float psi_of(float rho) { return 1.0f - __expf(-rho); }
+
+// Storage scale of the stored values.
+template <typename SP, bool NS>
+constexpr float kScale = (NS && std::is_same_v<SP, StF16>) ? 1024.0f : 1.0f;
+
+// Reference density of buffer b at node c.
+template <bool NS> __device__ __forceinline__ float ref(const Job& j, int b, int64_t c)
+{
+    if constexpr (NS) return j.R[b][c];
+    else return j.rs;
+}
+
+// Density of pulled values: R + (sum d)/S, d summed in population order.
+template <int Q, typename SP, bool NS>
+__device__ __forceinline__ float pulled_rho(const float (&d)[Q], float R)
+{
+    float r = 0.f;
+    static_for<Q>([&](auto I) { r += d[decltype(I)::value]; });
+    return R + r * (1.0f / kScale<SP, NS>);
+}
+
+// Stored value of population n after the collision: the post-collision f
+// relative to the destination's reference density Rw, scaled.  dn is the
+// pulled stored value (relative to the reader's Rr); deq the equilibrium in
+// the reader's frame.
+template <typename SP, bool NS>
+__device__ __forceinline__ float post_value(float dn, float w, float rho, float poly, float Rr, float Rw,
+                                            float itau)
+{
+    constexpr float S = kScale<SP, NS>;
+    const float deq = (S * w) * (rho * poly - Rr);
+    const float out = dn - (dn - deq) * itau;
+    if constexpr (NS) return out + (S * w) * (Rr - Rw);
+    else return out;
+}
 
 __device__ __forceinline__ int wrapi(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
@@ -46,8 +91,8 @@ __device__ __forceinline__ void reset_other(const Tables& t, int parity)
 // ---------------------------------------------------------------------------
 // D2Q9: one workgroup marches a job along x with one thread per node of a
 // whole y row (periodic in y within the row, so there is no halo ring); the
-// psi of rows x-1 .. x+2 sits in four LDS slots, and the row x+2 pulls are in
-// flight during the collision of row x.
+// psi (and, with the node shift, rho) of rows x-1 .. x+2 sits in four LDS
+// slots, and the row x+2 pulls are in flight during the collision of row x.
 
 template <typename SP>
 __device__ __forceinline__ void pull_row9(const typename SP::T* __restrict__ P, int64_t ps, int nx,
@@ -65,11 +110,12 @@ __device__ __forceinline__ void pull_row9(const typename SP::T* __restrict__ P, 
     });
 }
 
-template <typename SP, int T>
+template <typename SP, bool NS, int T>
 __global__ void __launch_bounds__(T) k_d2q9_rows(Tables t, int parity)
 {
     using S = St<9>;
     __shared__ float psi_lds[4 * T];
+    __shared__ float rho_lds[NS ? 4 * T : 1];
     __shared__ int slot_s;
     reset_other(t, parity);
     const int y = threadIdx.x;
@@ -80,9 +126,10 @@ __global__ void __launch_bounds__(T) k_d2q9_rows(Tables t, int parity)
         const int p = parity ^ job.phase;
         const auto* Pin = static_cast<const typename SP::T*>(job.P[p]);
         auto* Pout = static_cast<typename SP::T*>(job.P[p ^ 1]);
+        float* Rout = NS ? job.R[p ^ 1] : nullptr;
         const int nx = job.nx, ny = job.ny;
         const int64_t ps = job.ps;
-        const float tau = job.tau, G = job.G, rs = job.rs, itau = 1.0f / tau;
+        const float tau = job.tau, G = job.G, itau = 1.0f / tau;
         const bool first = job.first != 0;
         const float* rho0 = job.rho0;
         const bool active = y < ny;
@@ -90,45 +137,57 @@ __global__ void __launch_bounds__(T) k_d2q9_rows(Tables t, int parity)
         const int x0 = item.x0, x1 = item.x1;
         auto slot = [&](int x) { return ((x - x0 + 1) & 3) * T; };
         auto xw = [&](int x) { return x >= nx ? x - nx : (x < 0 ? x + nx : x); };
-        auto rho_at = [&](int x, const float (&d)[9]) {
-            if (first && rho0) return rho0[static_cast<int64_t>(x) * ny + y];
-            float r = 0.f;
-            static_for<9>([&](auto I) { r += d[decltype(I)::value]; });
-            return r + rs;
+        // window entries of pulled row xu (unwrapped): psi, and rho with the
+        // node shift (also written to the output reference field for the
+        // rows of this item)
+        auto put = [&](int xu, const float (&d)[9], float R) {
+            const int xr = xw(xu);
+            const int64_t c = static_cast<int64_t>(xr) * ny + y;
+            const float rp = pulled_rho<9, SP, NS>(d, R);
+            psi_lds[slot(xu) + y] = psi_of((first && rho0) ? rho0[c] : rp);
+            if constexpr (NS) {
+                rho_lds[slot(xu) + y] = rp;
+                if (xu >= x0 && xu < x1) Rout[c] = rp;
+            }
+        };
+        auto pullR = [&](int xr, float (&d)[9]) {
+            pull_row9<SP>(Pin, ps, nx, ny, xr, y, d);
+            return ref<NS>(job, p, static_cast<int64_t>(xr) * ny + y);
         };
         float fa[9], fb[9], fc[9];
+        float Ra = 0.f, Rb = 0.f, Rc = 0.f;
         if (active) {
-            pull_row9<SP>(Pin, ps, nx, ny, xw(x0 - 1), y, fc);
-            psi_lds[slot(x0 - 1) + y] = psi_of(rho_at(xw(x0 - 1), fc));
-            pull_row9<SP>(Pin, ps, nx, ny, x0, y, fa);
-            psi_lds[slot(x0) + y] = psi_of(rho_at(x0, fa));
-            pull_row9<SP>(Pin, ps, nx, ny, xw(x0 + 1), y, fb);
-            psi_lds[slot(x0 + 1) + y] = psi_of(rho_at(xw(x0 + 1), fb));
+            Rc = pullR(xw(x0 - 1), fc);
+            put(x0 - 1, fc, Rc);
+            Ra = pullR(x0, fa);
+            put(x0, fa, Ra);
+            Rb = pullR(xw(x0 + 1), fb);
+            put(x0 + 1, fb, Rb);
         }
         __syncthreads();
-        auto step = [&](int x, float (&fcur)[9], float (&fnn)[9]) {
+        auto step = [&](int x, float (&fcur)[9], float Rcur, float (&fnn)[9], float& Rnn) {
             const int x2 = xw(x + 2);
             if (active) {
-                pull_row9<SP>(Pin, ps, nx, ny, x2, y, fnn);
+                Rnn = pullR(x2, fnn);
                 const float* pm = psi_lds + slot(x - 1);
                 const float* p0 = psi_lds + slot(x);
                 const float* pp = psi_lds + slot(x + 1);
-                float rho, jx = 0.f, jy = 0.f;
-                float r = 0.f;
+                float jx = 0.f, jy = 0.f;
                 static_for<9>([&](auto I) {
                     constexpr int n = decltype(I)::value;
-                    r += fcur[n];
                     acc_e<S::ex[n]>(jx, fcur[n]);
                     acc_e<S::ey[n]>(jy, fcur[n]);
                 });
-                float ux, uy;
+                const int64_t c = static_cast<int64_t>(x) * ny + y;
+                float rho, ux, uy;
                 if (first && rho0) {
-                    rho = rho0[static_cast<int64_t>(x) * ny + y];
+                    rho = rho0[c];
                     ux = 0.f; uy = 0.f;
                 } else {
-                    rho = r + rs;
+                    rho = pulled_rho<9, SP, NS>(fcur, Rcur);
                     const float ir = 1.0f / rho;
-                    ux = jx * ir; uy = jy * ir;
+                    const float js = 1.0f / kScale<SP, NS>;
+                    ux = (jx * js) * ir; uy = (jy * js) * ir;
                 }
                 const int yy[3] = {ym, y, yp};
                 float gx = 0.f, gy = 0.f;
@@ -144,36 +203,37 @@ __global__ void __launch_bounds__(T) k_d2q9_rows(Tables t, int parity)
                 const float k = (-G * p0[y]) * tau / rho;
                 const float ue = ux + k * gx, ve = uy + k * gy;
                 const float usq15 = 1.5f * (ue * ue + ve * ve);
-                const int64_t c = static_cast<int64_t>(x) * ny + y;
                 static_for<9>([&](auto I) {
                     constexpr int n = decltype(I)::value;
                     const float ev = edot<S::ex[n], S::ey[n], 0>(ue, ve, 0.f);
                     const float poly = 1.0f + 3.0f * ev + 4.5f * ev * ev - usq15;
-                    const float deq = static_cast<float>(S::w[n]) * (rho * poly - rs);
-                    SP::put_nt(Pout + n * ps + c, SP::of_d(fcur[n] - (fcur[n] - deq) * itau));
+                    float Rw = 0.f;
+                    if constexpr (NS) {
+                        const float* rl = rho_lds + slot(x + S::ex[n]);
+                        Rw = rl[yy[1 + S::ey[n]]];
+                    }
+                    const float v = post_value<SP, NS>(fcur[n], static_cast<float>(S::w[n]), rho, poly,
+                                                       Rcur, Rw, itau);
+                    SP::put_nt(Pout + n * ps + c, SP::of_d(v));
                 });
-                psi_lds[slot(x + 2) + y] = psi_of(rho_at(x2, fnn));
+                put(x + 2, fnn, Rnn);
             }
             __syncthreads();
         };
         int x = x0;
         for (; x + 2 < x1; x += 3) {
-            step(x, fa, fc);
-            step(x + 1, fb, fa);
-            step(x + 2, fc, fb);
+            step(x, fa, Ra, fc, Rc);
+            step(x + 1, fb, Rb, fa, Ra);
+            step(x + 2, fc, Rc, fb, Rb);
         }
-        if (x < x1) step(x, fa, fc);
-        if (x + 1 < x1) step(x + 1, fb, fa);
+        if (x < x1) step(x, fa, Ra, fc, Rc);
+        if (x + 1 < x1) step(x + 1, fb, Rb, fa, Ra);
     }
 }
 
 // ---------------------------------------------------------------------------
 // Two-pass kernels for any job (D2Q9 or D3Q19, any size): items are node
-// ranges; pass 1 writes psi, pass 2 collides.
-
-template <int Q> struct Pos {
-    int x, y, z;
-};
+// ranges; pass 1 writes psi (and the node-shift reference), pass 2 collides.
 
 template <int Q>
 __device__ __forceinline__ void decompose(int64_t c, const Job& j, int& x, int& y, int& z)
@@ -202,7 +262,7 @@ __device__ __forceinline__ void pull_node(const typename SP::T* __restrict__ P, 
 
 constexpr int kTP = 256;
 
-template <int Q, typename SP>
+template <int Q, typename SP, bool NS>
 __global__ void __launch_bounds__(kTP) k_tp_psi(Tables t, int parity)
 {
     __shared__ int slot_s;
@@ -211,26 +271,25 @@ __global__ void __launch_bounds__(kTP) k_tp_psi(Tables t, int parity)
         const Item item = t.items[it];
         const Job& job = t.jobs[item.job];
         if (job.remaining <= 0) continue;
-        const auto* Pin = static_cast<const typename SP::T*>(job.P[parity ^ job.phase]);
+        const int p = parity ^ job.phase;
+        const auto* Pin = static_cast<const typename SP::T*>(job.P[p]);
         for (int64_t c = item.c0 + threadIdx.x; c < item.c1; c += kTP) {
-            float rho;
-            if (job.first && job.rho0) {
-                rho = job.rho0[c];
-            } else {
+            const bool from_rho0 = job.first && job.rho0;
+            float rp = 0.f;
+            if (!from_rho0 || NS) {
                 int x, y, z;
                 decompose<Q>(c, job, x, y, z);
                 float d[Q];
                 pull_node<Q, SP>(Pin, job, x, y, z, d);
-                float r = 0.f;
-                static_for<Q>([&](auto I) { r += d[decltype(I)::value]; });
-                rho = r + job.rs;
+                rp = pulled_rho<Q, SP, NS>(d, ref<NS>(job, p, c));
             }
-            job.psi[c] = psi_of(rho);
+            job.psi[c] = psi_of(from_rho0 ? job.rho0[c] : rp);
+            if constexpr (NS) job.R[p ^ 1][c] = rp;
         }
     }
 }
 
-template <int Q, typename SP>
+template <int Q, typename SP, bool NS>
 __global__ void __launch_bounds__(kTP) k_tp_collide(Tables t, int parity)
 {
     using S = St<Q>;
@@ -243,17 +302,17 @@ __global__ void __launch_bounds__(kTP) k_tp_collide(Tables t, int parity)
         const int p = parity ^ job.phase;
         const auto* Pin = static_cast<const typename SP::T*>(job.P[p]);
         auto* Pout = static_cast<typename SP::T*>(job.P[p ^ 1]);
-        const float tau = job.tau, G = job.G, rs = job.rs, itau = 1.0f / tau;
+        const float tau = job.tau, G = job.G, itau = 1.0f / tau;
         const float* psi = job.psi;
         for (int64_t c = item.c0 + threadIdx.x; c < item.c1; c += kTP) {
             int x, y, z;
             decompose<Q>(c, job, x, y, z);
             float d[Q];
             pull_node<Q, SP>(Pin, job, x, y, z, d);
-            float r = 0.f, jx = 0.f, jy = 0.f, jz = 0.f;
+            const float Rr = ref<NS>(job, p, c);
+            float jx = 0.f, jy = 0.f, jz = 0.f;
             static_for<Q>([&](auto I) {
                 constexpr int n = decltype(I)::value;
-                r += d[n];
                 acc_e<S::ex[n]>(jx, d[n]);
                 acc_e<S::ey[n]>(jy, d[n]);
                 acc_e<S::ez[n]>(jz, d[n]);
@@ -263,20 +322,24 @@ __global__ void __launch_bounds__(kTP) k_tp_collide(Tables t, int parity)
                 rho = job.rho0[c];
                 ux = uy = uz = 0.f;
             } else {
-                rho = r + rs;
+                rho = pulled_rho<Q, SP, NS>(d, Rr);
                 const float ir = 1.0f / rho;
-                ux = jx * ir; uy = jy * ir; uz = jz * ir;
+                const float js = 1.0f / kScale<SP, NS>;
+                ux = (jx * js) * ir; uy = (jy * js) * ir; uz = (jz * js) * ir;
             }
             const int xs[3] = {x == 0 ? job.nx - 1 : x - 1, x, x == job.nx - 1 ? 0 : x + 1};
             const int ys[3] = {y == 0 ? job.ny - 1 : y - 1, y, y == job.ny - 1 ? 0 : y + 1};
             const int zs[3] = {z == 0 ? job.nz - 1 : z - 1, z, z == job.nz - 1 ? 0 : z + 1};
+            auto nb = [&](auto N) {
+                constexpr int n = decltype(N)::value;
+                return (static_cast<int64_t>(xs[1 + S::ex[n]]) * job.ny + ys[1 + S::ey[n]]) * job.nz +
+                       zs[1 + S::ez[n]];
+            };
             float gx = 0.f, gy = 0.f, gz = 0.f;
             static_for<Q>([&](auto I) {
                 constexpr int n = decltype(I)::value;
                 if constexpr (n > 0) {
-                    const int64_t nb = (static_cast<int64_t>(xs[1 + S::ex[n]]) * job.ny + ys[1 + S::ey[n]]) *
-                                           job.nz + zs[1 + S::ez[n]];
-                    const float tt = static_cast<float>(S::w[n]) * psi[nb];
+                    const float tt = static_cast<float>(S::w[n]) * psi[nb(I)];
                     acc_e<S::ex[n]>(gx, tt);
                     acc_e<S::ey[n]>(gy, tt);
                     acc_e<S::ez[n]>(gz, tt);
@@ -289,8 +352,11 @@ __global__ void __launch_bounds__(kTP) k_tp_collide(Tables t, int parity)
                 constexpr int n = decltype(I)::value;
                 const float ev = edot<S::ex[n], S::ey[n], S::ez[n]>(ue, ve, we);
                 const float poly = 1.0f + 3.0f * ev + 4.5f * ev * ev - usq15;
-                const float deq = static_cast<float>(S::w[n]) * (rho * poly - rs);
-                SP::put_nt(Pout + n * job.ps + c, SP::of_d(d[n] - (d[n] - deq) * itau));
+                float Rw = 0.f;
+                if constexpr (NS) Rw = job.R[p ^ 1][nb(I)];
+                const float v = post_value<SP, NS>(d[n], static_cast<float>(S::w[n]), rho, poly, Rr, Rw,
+                                                   itau);
+                SP::put_nt(Pout + n * job.ps + c, SP::of_d(v));
             });
         }
     }
@@ -305,16 +371,20 @@ __global__ void k_advance(Job* jobs, int njobs)
     }
 }
 
-// Initial equilibrium: psi(rho0) into job.psi, then d_eq of each node
-// scattered to the upstream slots so that the first pull reads it back.
-template <int Q>
-__global__ void __launch_bounds__(kTP) k_init_psi(Job j)
+// Initial equilibrium: psi(rho0) into job.psi (and R = rho0 for the node
+// shift), then the equilibrium of each node, relative to that node's
+// reference, scattered to the upstream slots so that its first pull reads
+// it back.
+template <int Q, bool NS>
+__global__ void __launch_bounds__(kTP) k_init_psi(Job j, int buf)
 {
     const int64_t c = static_cast<int64_t>(blockIdx.x) * kTP + threadIdx.x;
-    if (c < j.n) j.psi[c] = psi_of(j.rho0[c]);
+    if (c >= j.n) return;
+    j.psi[c] = psi_of(j.rho0[c]);
+    if constexpr (NS) j.R[buf][c] = j.rho0[c];
 }
 
-template <int Q, typename SP>
+template <int Q, typename SP, bool NS>
 __global__ void __launch_bounds__(kTP) k_init_feq(Job j, int buf)
 {
     using S = St<Q>;
@@ -339,6 +409,7 @@ __global__ void __launch_bounds__(kTP) k_init_feq(Job j, int buf)
         }
     });
     const float rho = j.rho0[c];
+    const float Rr = ref<NS>(j, buf, c);
     const float k = (-j.G * j.psi[c]) * j.tau / rho;
     const float ue = k * gx, ve = k * gy, we = (S::D == 3) ? k * gz : 0.f;
     const float usq15 = 1.5f * (ue * ue + ve * ve + we * we);
@@ -346,7 +417,7 @@ __global__ void __launch_bounds__(kTP) k_init_feq(Job j, int buf)
         constexpr int n = decltype(I)::value;
         const float ev = edot<S::ex[n], S::ey[n], S::ez[n]>(ue, ve, we);
         const float poly = 1.0f + 3.0f * ev + 4.5f * ev * ev - usq15;
-        const float deq = static_cast<float>(S::w[n]) * (rho * poly - j.rs);
+        const float deq = (kScale<SP, NS> * static_cast<float>(S::w[n])) * (rho * poly - Rr);
         // slot read by the pull at node c: upstream node c - e_n
         const int64_t up = (static_cast<int64_t>(xs[1 - S::ex[n]]) * j.ny + ys[1 - S::ey[n]]) * j.nz +
                            zs[1 - S::ez[n]];
@@ -354,7 +425,7 @@ __global__ void __launch_bounds__(kTP) k_init_feq(Job j, int buf)
     });
 }
 
-template <int Q, typename SP>
+template <int Q, typename SP, bool NS>
 __global__ void __launch_bounds__(kTP) k_job_moments(Job j, int buf, double* __restrict__ rho,
                                                      double* __restrict__ u, double* __restrict__ v,
                                                      double* __restrict__ w)
@@ -367,10 +438,11 @@ __global__ void __launch_bounds__(kTP) k_job_moments(Job j, int buf, double* __r
     decompose<Q>(c, j, x, y, z);
     float d[Q];
     pull_node<Q, SP>(P, j, x, y, z, d);
+    const double R = ref<NS>(j, buf, c);
     double r = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
     static_for<Q>([&](auto I) {
         constexpr int n = decltype(I)::value;
-        const double fv = static_cast<double>(d[n]) + S::w[n] * static_cast<double>(j.rs);
+        const double fv = static_cast<double>(d[n]) / kScale<SP, NS> + S::w[n] * R;
         r += fv;
         acc_e<S::ex[n]>(sx, fv);
         acc_e<S::ey[n]>(sy, fv);
@@ -388,11 +460,16 @@ __global__ void k_zero(unsigned* p, int n)
     if (i < n) p[i] = 0u;
 }
 
+// Runs f(storage tag, node-shift tag) for a batch precision.
 template <typename F> void by_prec(int prec, F&& f)
 {
-    if (prec == LBM_FP32) f(StF32{});
-    else if (prec == LBM_FP16S) f(StF16{});
-    else f(StBF16{});
+    switch (prec) {
+    case LBM_FP32: f(StF32{}, std::false_type{}); break;
+    case LBM_FP16S: f(StF16{}, std::false_type{}); break;
+    case LBM_BF16S: f(StBF16{}, std::false_type{}); break;
+    case LBM_FP16N: f(StF16{}, std::true_type{}); break;
+    default: f(StBF16{}, std::true_type{}); break;
+    }
 }
 
 inline unsigned blocks(int64_t n) { return static_cast<unsigned>((n + kTP - 1) / kTP); }
@@ -403,33 +480,36 @@ hipError_t d2q9_rows_step(int prec, const Tables& t, int parity, int threads, in
                           hipStream_t s)
 {
     if (threads < 1 || threads > 1024) return hipErrorInvalidValue;   // one thread per row node
-    by_prec(prec, [&](auto sp) {
+    by_prec(prec, [&](auto sp, auto ns) {
         using SP = decltype(sp);
-        if (threads <= 64) k_d2q9_rows<SP, 64><<<grid, 64, 0, s>>>(t, parity);
-        else if (threads <= 128) k_d2q9_rows<SP, 128><<<grid, 128, 0, s>>>(t, parity);
-        else if (threads <= 256) k_d2q9_rows<SP, 256><<<grid, 256, 0, s>>>(t, parity);
-        else if (threads <= 512) k_d2q9_rows<SP, 512><<<grid, 512, 0, s>>>(t, parity);
-        else k_d2q9_rows<SP, 1024><<<grid, 1024, 0, s>>>(t, parity);
+        constexpr bool NS = decltype(ns)::value;
+        if (threads <= 64) k_d2q9_rows<SP, NS, 64><<<grid, 64, 0, s>>>(t, parity);
+        else if (threads <= 128) k_d2q9_rows<SP, NS, 128><<<grid, 128, 0, s>>>(t, parity);
+        else if (threads <= 256) k_d2q9_rows<SP, NS, 256><<<grid, 256, 0, s>>>(t, parity);
+        else if (threads <= 512) k_d2q9_rows<SP, NS, 512><<<grid, 512, 0, s>>>(t, parity);
+        else k_d2q9_rows<SP, NS, 1024><<<grid, 1024, 0, s>>>(t, parity);
     });
     return hipGetLastError();
 }
 
 hipError_t twopass_psi(int prec, int q, const Tables& t, int parity, int grid, hipStream_t s)
 {
-    by_prec(prec, [&](auto sp) {
+    by_prec(prec, [&](auto sp, auto ns) {
         using SP = decltype(sp);
-        if (q == 9) k_tp_psi<9, SP><<<grid, kTP, 0, s>>>(t, parity);
-        else k_tp_psi<19, SP><<<grid, kTP, 0, s>>>(t, parity);
+        constexpr bool NS = decltype(ns)::value;
+        if (q == 9) k_tp_psi<9, SP, NS><<<grid, kTP, 0, s>>>(t, parity);
+        else k_tp_psi<19, SP, NS><<<grid, kTP, 0, s>>>(t, parity);
     });
     return hipGetLastError();
 }
 
 hipError_t twopass_collide(int prec, int q, const Tables& t, int parity, int grid, hipStream_t s)
 {
-    by_prec(prec, [&](auto sp) {
+    by_prec(prec, [&](auto sp, auto ns) {
         using SP = decltype(sp);
-        if (q == 9) k_tp_collide<9, SP><<<grid, kTP, 0, s>>>(t, parity);
-        else k_tp_collide<19, SP><<<grid, kTP, 0, s>>>(t, parity);
+        constexpr bool NS = decltype(ns)::value;
+        if (q == 9) k_tp_collide<9, SP, NS><<<grid, kTP, 0, s>>>(t, parity);
+        else k_tp_collide<19, SP, NS><<<grid, kTP, 0, s>>>(t, parity);
     });
     return hipGetLastError();
 }
@@ -442,12 +522,16 @@ hipError_t advance_counters(Job* jobs, int njobs, hipStream_t s)
 
 hipError_t init_job(int prec, int q, const Job& j, int buf, hipStream_t s)
 {
-    if (q == 9) k_init_psi<9><<<blocks(j.n), kTP, 0, s>>>(j);
-    else k_init_psi<19><<<blocks(j.n), kTP, 0, s>>>(j);
-    by_prec(prec, [&](auto sp) {
+    by_prec(prec, [&](auto sp, auto ns) {
         using SP = decltype(sp);
-        if (q == 9) k_init_feq<9, SP><<<blocks(j.n), kTP, 0, s>>>(j, buf);
-        else k_init_feq<19, SP><<<blocks(j.n), kTP, 0, s>>>(j, buf);
+        constexpr bool NS = decltype(ns)::value;
+        if (q == 9) {
+            k_init_psi<9, NS><<<blocks(j.n), kTP, 0, s>>>(j, buf);
+            k_init_feq<9, SP, NS><<<blocks(j.n), kTP, 0, s>>>(j, buf);
+        } else {
+            k_init_psi<19, NS><<<blocks(j.n), kTP, 0, s>>>(j, buf);
+            k_init_feq<19, SP, NS><<<blocks(j.n), kTP, 0, s>>>(j, buf);
+        }
     });
     return hipGetLastError();
 }
@@ -455,10 +539,11 @@ hipError_t init_job(int prec, int q, const Job& j, int buf, hipStream_t s)
 hipError_t job_moments(int prec, int q, const Job& j, int buf, double* rho, double* u, double* v,
                        double* w, hipStream_t s)
 {
-    by_prec(prec, [&](auto sp) {
+    by_prec(prec, [&](auto sp, auto ns) {
         using SP = decltype(sp);
-        if (q == 9) k_job_moments<9, SP><<<blocks(j.n), kTP, 0, s>>>(j, buf, rho, u, v, w);
-        else k_job_moments<19, SP><<<blocks(j.n), kTP, 0, s>>>(j, buf, rho, u, v, w);
+        constexpr bool NS = decltype(ns)::value;
+        if (q == 9) k_job_moments<9, SP, NS><<<blocks(j.n), kTP, 0, s>>>(j, buf, rho, u, v, w);
+        else k_job_moments<19, SP, NS><<<blocks(j.n), kTP, 0, s>>>(j, buf, rho, u, v, w);
     });
     return hipGetLastError();
 }

@@ -42,6 +42,7 @@ struct HostJob {
     void* P[2] = {nullptr, nullptr};
     float* psi = nullptr;
     float* rho0 = nullptr;
+    float* R[2] = {nullptr, nullptr};   // node shift: reference densities of P[0] / P[1]
     int64_t bytes = 0;
     int64_t remaining;
     int first = 1;
@@ -57,6 +58,8 @@ int64_t plane_stride(int64_t n, int elem)
 }
 
 int elem_bytes(int prec) { return prec == LBM_FP32 ? 4 : 2; }
+
+bool node_shift(int prec) { return prec == LBM_FP16N || prec == LBM_BF16N; }
 
 }  // namespace
 
@@ -148,6 +151,8 @@ Job device_view(const HostJob& j, int prec, double rs)
     v.P[0] = j.P[0];
     v.P[1] = j.P[1];
     v.psi = j.psi;
+    v.R[0] = j.R[0];
+    v.R[1] = j.R[1];
     v.rho0 = (j.d.q == 9) ? j.rho0 : nullptr;   // LBfluid2D-order first step
     v.nx = j.d.nx;
     v.ny = j.d.ny;
@@ -157,7 +162,7 @@ Job device_view(const HostJob& j, int prec, double rs)
     v.ps = j.ps;
     v.tau = static_cast<float>(j.d.tau);
     v.G = static_cast<float>(j.d.G);
-    v.rs = static_cast<float>(prec == LBM_FP32 ? 0.0 : rs);
+    v.rs = static_cast<float>((prec == LBM_FP32 || node_shift(prec)) ? 0.0 : rs);
     v.first = j.first;
     v.remaining = j.remaining;
     return v;
@@ -310,6 +315,8 @@ void free_job(lbm_batch* b, HostJob* j)
         if (p) (void)hipFree(p);
     if (j->psi) (void)hipFree(j->psi);
     if (j->rho0) (void)hipFree(j->rho0);
+    for (float* r : j->R)
+        if (r) (void)hipFree(r);
     b->in_use -= j->bytes;
 }
 
@@ -317,7 +324,8 @@ int64_t job_bytes(const lbm_batch* b, const lbm_job_desc& d)
 {
     const int64_t n = int64_t(d.nx) * d.ny * (d.q == 9 ? 1 : d.nz);
     const int elem = elem_bytes(b->d.precision);
-    return 2 * plane_stride(n, elem) * d.q * elem + 8 * n;   // populations, psi, rho0
+    // populations, psi, rho0, and the node-shift reference densities
+    return 2 * plane_stride(n, elem) * d.q * elem + 8 * n + (node_shift(b->d.precision) ? 8 * n : 0);
 }
 
 }  // namespace
@@ -328,8 +336,10 @@ int lbm_batch_create(const lbm_batch_desc* desc, lbm_batch** out)
 {
     if (!desc || !out) return lbm::set_error(LBM_EINVAL, "lbm_batch_create: NULL argument");
     *out = nullptr;
-    if (desc->precision != LBM_FP32 && desc->precision != LBM_FP16S && desc->precision != LBM_BF16S)
-        return lbm::set_error(LBM_EINVAL, "batch precision must be LBM_FP32, LBM_FP16S or LBM_BF16S");
+    if (desc->precision != LBM_FP32 && desc->precision != LBM_FP16S && desc->precision != LBM_BF16S &&
+        !node_shift(desc->precision))
+        return lbm::set_error(LBM_EINVAL,
+                              "batch precision must be LBM_FP32, LBM_FP16S, LBM_BF16S, LBM_FP16N or LBM_BF16N");
     if (desc->steps_per_tick < 1) return lbm::set_error(LBM_EINVAL, "steps_per_tick must be >= 1");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
@@ -421,6 +431,10 @@ int lbm_batch_submit(lbm_batch* b, const lbm_job_desc* job, const double* rho0, 
             return undo(lbm::set_error(LBM_ENOMEM, "hipMalloc of job populations failed"));
     if (hipMalloc(&j->psi, n * 4) != hipSuccess || hipMalloc(&j->rho0, n * 4) != hipSuccess)
         return undo(lbm::set_error(LBM_ENOMEM, "hipMalloc of job fields failed"));
+    if (node_shift(b->d.precision))
+        for (int k = 0; k < 2; k++)
+            if (hipMalloc(&j->R[k], n * 4) != hipSuccess)
+                return undo(lbm::set_error(LBM_ENOMEM, "hipMalloc of job reference density failed"));
     std::vector<float> r32(static_cast<size_t>(n));
     for (int64_t c = 0; c < n; c++) r32[c] = static_cast<float>(rho0[c]);
     if (hipMemcpyAsync(j->rho0, r32.data(), n * 4, hipMemcpyHostToDevice, b->stream) != hipSuccess)

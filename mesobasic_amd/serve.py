@@ -164,7 +164,7 @@ def main():
     ap.add_argument("--steps-per-tick", type=int, default=20)
     ap.add_argument("--work-limit", type=int, default=0,
                     help="nodes advanced per step, least remaining work first (0: all)")
-    ap.add_argument("--precision", default="fp16", choices=["fp32", "fp16", "bf16"])
+    ap.add_argument("--precision", default="fp16", choices=["fp32", "fp16", "bf16", "fp16n", "bf16n"])
     args = ap.parse_args()
     prec = PRECISIONS[args.precision]
     if args.batched:

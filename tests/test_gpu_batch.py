@@ -44,17 +44,18 @@ def _run(batch, jobs):
     return out
 
 
-def test_alone_equals_in_the_mix_bitwise():
+@pytest.mark.parametrize("prec", [mb.FP16S, mb.FP16N])
+def test_alone_equals_in_the_mix_bitwise(prec):
     rng = np.random.default_rng(5)
     jobs = _jobs(rng, 24)
-    mixed = _run(Batch(precision=mb.FP16S, steps_per_tick=8), jobs)
+    mixed = _run(Batch(precision=prec, steps_per_tick=8), jobs)
     for k in (0, 3, 7, 11, 19):
-        alone = _run(Batch(precision=mb.FP16S, steps_per_tick=6), [jobs[k]])[0]
+        alone = _run(Batch(precision=prec, steps_per_tick=6), [jobs[k]])[0]
         for f in ("rho", "u", "v", "w"):
             assert same_bits(alone[f], mixed[k][f]), (k, f, jobs[k])
 
 
-@pytest.mark.parametrize("prec", [mb.FP32, mb.FP16S, mb.BF16S])
+@pytest.mark.parametrize("prec", [mb.FP32, mb.FP16S, mb.BF16S, mb.FP16N, mb.BF16N])
 def test_graph_tuner_and_tick_length_do_not_change_results(prec):
     rng = np.random.default_rng(6)
     jobs = _jobs(rng, 10)
@@ -72,7 +73,7 @@ def test_job_matches_fp64_oracle(q, shape, steps):
     o = oracle.Oracle(q, shape, tau, G)
     o.set_rho(rho0); o.init(); o.step(steps)
     ref = o.moments()
-    for prec, tol in ((mb.FP32, 2e-6), (mb.FP16S, 1e-4)):
+    for prec, tol in ((mb.FP32, 2e-6), (mb.FP16S, 1e-4), (mb.FP16N, 1e-4), (mb.BF16N, 5e-4)):
         res = _run(Batch(precision=prec, steps_per_tick=6), [dict(q=q, shape=shape, tau=tau, G=G,
                                                                   steps=steps, seed=3)])[0]
         err = rel_l2(res["rho"], ref["rho"])
@@ -135,3 +136,25 @@ def test_serving_stream_admits_as_memory_frees_and_matches_alone(work_limit):
         alone = b.collect(jid)
         assert same_bits(alone["rho"], res[r["id"]]["rho"]), r
         assert same_bits(alone["u"], res[r["id"]]["u"]), r
+
+
+@pytest.mark.parametrize("q,shape", [(9, (96, 80)), (19, (24, 24, 32))])
+def test_node_shift_resolves_droplet_jobs(q, shape):
+    # a droplet (2.0 / 0.15) is far from any single shift: the node-shifted
+    # batch forms stay close to the FP64 oracle, the constant shift does not
+    steps, tau, G = 200, 1.0, -5.5
+    rho0 = mb.droplet_rho(shape, min(shape) * 3.0 / 16.0)
+    o = oracle.Oracle(q, shape, tau, G)
+    o.set_rho(rho0); o.init(); o.step(steps)
+    ref = o.moments()["rho"]
+    err = {}
+    for prec in (mb.FP16S, mb.FP16N, mb.BF16N):
+        b = Batch(precision=prec, steps_per_tick=8)
+        jid = b.submit(q, shape, tau, G, steps, rho0)
+        while not b.finished():
+            b.tick()
+        err[prec] = rel_l2(b.collect(jid)["rho"], ref)
+    print(f"q {q} {shape}: rel L2 FP16S {err[mb.FP16S]:.2e} FP16N {err[mb.FP16N]:.2e} "
+          f"BF16N {err[mb.BF16N]:.2e}", flush=True)
+    assert err[mb.FP16N] < 3e-4 and err[mb.BF16N] < 3e-3
+    assert err[mb.FP16N] < 0.1 * err[mb.FP16S]
