@@ -111,6 +111,38 @@ template <typename T> __device__ __forceinline__ T* elem(T* base, uint32_t i)
     return reinterpret_cast<T*>(reinterpret_cast<B*>(base) + i * static_cast<uint32_t>(sizeof(T)));
 }
 
+// 32-bit offsets from node c = (x, y, z) to its +-1 neighbours along each
+// axis, periodic.  With keep = 0 (a buffer read in place) every offset is 0,
+// so in-place reads and pulls share one address path: a merged branch would
+// turn each gather's 32-bit offset into a 64-bit one (no SGPR-base address
+// form, two extra VALU operations and VGPRs per gather).
+struct NbrOff {
+    uint32_t xm, xp, ym, yp, zm, zp;
+    __device__ __forceinline__ NbrOff(const Grid& g, int x, int y, int z, uint32_t keep = ~0u)
+    {
+        const uint32_t ny = static_cast<uint32_t>(g.ny), nz = static_cast<uint32_t>(g.nz);
+        const uint32_t plane = ny * nz;
+        xm = keep & ((x == 0) ? (static_cast<uint32_t>(g.nx) - 1u) * plane : 0u - plane);
+        xp = keep & ((x == g.nx - 1) ? 0u - (static_cast<uint32_t>(g.nx) - 1u) * plane : plane);
+        ym = keep & ((y == 0) ? (ny - 1u) * nz : 0u - nz);
+        yp = keep & ((y == g.ny - 1) ? 0u - (ny - 1u) * nz : nz);
+        zm = keep & ((z == 0) ? nz - 1u : 0u - 1u);
+        zp = keep & ((z == g.nz - 1) ? 0u - (nz - 1u) : 1u);
+    }
+    // offset of node c + (EX, EY, EZ)
+    template <int EX, int EY, int EZ> __device__ __forceinline__ uint32_t to() const
+    {
+        uint32_t o = 0u;
+        if constexpr (EX == 1) o += xp;
+        if constexpr (EX == -1) o += xm;
+        if constexpr (EY == 1) o += yp;
+        if constexpr (EY == -1) o += ym;
+        if constexpr (EZ == 1) o += zp;
+        if constexpr (EZ == -1) o += zm;
+        return o;
+    }
+};
+
 // Pull the stored values for node (x, y, z) (coordinates already wrapped
 // into range): f itself in the FP64/FP32 formats, d = f - w*rho_shift in the
 // 16-bit ones.  Offsets are 32-bit node indices added to per-population base pointers that
@@ -123,35 +155,14 @@ __device__ __forceinline__ void pull(const typename SP::T* __restrict__ P, const
     const uint32_t ny = static_cast<uint32_t>(g.ny), nz = static_cast<uint32_t>(g.nz);
     const uint32_t c = (static_cast<uint32_t>(x) * ny + static_cast<uint32_t>(y)) * nz +
                        static_cast<uint32_t>(z);
-    if (kind == kHoldsF) {
-        static_for<19>([&](auto I) {
-            constexpr int n = decltype(I)::value;
-            f[n] = SP::d(*elem(P + n * g.ps, c));
-        });
-        return;
-    }
-    // offsets of the x-1 / x+1 planes, y-1 / y+1 rows and z-1 / z+1 columns,
-    // with periodic wrap
-    const uint32_t plane = ny * nz;
-    const uint32_t dxm = (x == 0) ? (static_cast<uint32_t>(g.nx) - 1u) * plane : 0u - plane;
-    const uint32_t dxp = (x == g.nx - 1) ? 0u - (static_cast<uint32_t>(g.nx) - 1u) * plane : plane;
-    const uint32_t dym = (y == 0) ? (ny - 1u) * nz : 0u - nz;
-    const uint32_t dyp = (y == g.ny - 1) ? 0u - (ny - 1u) * nz : nz;
-    const uint32_t dzm = (z == 0) ? nz - 1u : 0u - 1u;
-    const uint32_t dzp = (z == g.nz - 1) ? 0u - (nz - 1u) : 1u;
+    // in place (kHoldsF): zero offsets; pulled: the upstream node x - e_n
+    const NbrOff o(g, x, y, z, kind == kHoldsF ? 0u : ~0u);
     static_for<19>([&](auto I) {
         constexpr int n = decltype(I)::value;
-        // upstream node x - e_n
-        uint32_t o = c;
-        if constexpr (S::ex[n] == 1) o += dxm;
-        if constexpr (S::ex[n] == -1) o += dxp;
-        if constexpr (S::ey[n] == 1) o += dym;
-        if constexpr (S::ey[n] == -1) o += dyp;
-        if constexpr (S::ez[n] == 1) o += dzm;
-        if constexpr (S::ez[n] == -1) o += dzp;
-        const typename SP::T* base = P + n * g.ps;
-        f[n] = SP::d(*elem(base, o));
+        const uint32_t up = c + o.template to<-S::ex[n], -S::ey[n], -S::ez[n]>();
+        f[n] = SP::d(*elem(P + n * g.ps, up));
     });
+    if (kind == kHoldsF) return;
     if constexpr (WALLS) {
         if (z == 1) {
             static_for<S::nwall>([&](auto I) {

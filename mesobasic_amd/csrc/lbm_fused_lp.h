@@ -141,35 +141,6 @@ using HeldT = std::conditional_t<T1, HeldJ, std::conditional_t<WALLS, PkFloat<SP
 template <typename SP, bool NS>
 constexpr float kStoreScale = (NS && std::is_same_v<SP, StF16>) ? 1024.0f : 1.0f;
 
-// 32-bit offsets from node c = (x, y, z) to its +-1 neighbours along each
-// axis, periodic.
-struct NbrOff {
-    uint32_t xm, xp, ym, yp, zm, zp;
-    __device__ __forceinline__ NbrOff(const Grid& g, int x, int y, int z)
-    {
-        const uint32_t ny = static_cast<uint32_t>(g.ny), nz = static_cast<uint32_t>(g.nz);
-        const uint32_t plane = ny * nz;
-        xm = (x == 0) ? (static_cast<uint32_t>(g.nx) - 1u) * plane : 0u - plane;
-        xp = (x == g.nx - 1) ? 0u - (static_cast<uint32_t>(g.nx) - 1u) * plane : plane;
-        ym = (y == 0) ? (ny - 1u) * nz : 0u - nz;
-        yp = (y == g.ny - 1) ? 0u - (ny - 1u) * nz : nz;
-        zm = (z == 0) ? nz - 1u : 0u - 1u;
-        zp = (z == g.nz - 1) ? 0u - (nz - 1u) : 1u;
-    }
-    // offset of node c + (EX, EY, EZ)
-    template <int EX, int EY, int EZ> __device__ __forceinline__ uint32_t to() const
-    {
-        uint32_t o = 0u;
-        if constexpr (EX == 1) o += xp;
-        if constexpr (EX == -1) o += xm;
-        if constexpr (EY == 1) o += yp;
-        if constexpr (EY == -1) o += ym;
-        if constexpr (EZ == 1) o += zp;
-        if constexpr (EZ == -1) o += zm;
-        return o;
-    }
-};
-
 // Pulls the stored values of node (x, y, z) (wrapped coordinates), which
 // are relative to the node's own reference density, and returns that
 // density R(x) (rin[c] with the node shift, rs otherwise).  Bounce-back slots
@@ -186,19 +157,14 @@ __device__ __forceinline__ float pull_lp(const typename SP::T* __restrict__ P,
                        static_cast<uint32_t>(z);
     float R = rs;
     if constexpr (NS) R = *elem(rin, c);
-    if (kind == kHoldsF) {
-        static_for<19>([&](auto I) {
-            constexpr int n = decltype(I)::value;
-            d.template load<n>(elem(P + n * g.ps, c));
-        });
-        return R;
-    }
-    const NbrOff o(g, x, y, z);
+    // in place (kHoldsF): zero offsets; pulled: the upstream node x - e_n
+    const NbrOff o(g, x, y, z, kind == kHoldsF ? 0u : ~0u);
     static_for<19>([&](auto I) {
         constexpr int n = decltype(I)::value;
         const uint32_t up = c + o.template to<-S::ex[n], -S::ey[n], -S::ez[n]>();
         d.template load<n>(elem(P + n * g.ps, up));
     });
+    if (kind == kHoldsF) return R;
     if constexpr (WALLS) {
         auto bounce = [&](auto M, auto from_dn) {
             constexpr int m = decltype(M)::value;
