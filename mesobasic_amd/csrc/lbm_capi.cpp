@@ -218,7 +218,9 @@ int lbm_create(const lbm_desc* desc, lbm_handle** out)
             return fail(LBM_EINVAL, "requested kernel does not support this lattice/precision");
         }
     }
-    h->tune.shape = lbm::fused::default_shape(h->g, desc->q);
+    h->tune.shape = desc->precision == LBM_FP64
+                        ? lbm::fused::default_shape(h->g, desc->q)
+                        : lbm::lowp_default_shape(desc->precision, h->a, h->g);
     const int64_t pop_bytes = h->g.ps * desc->q * h->elem;
     auto cleanup = [&](int code) {
         lbm_destroy(h);
@@ -460,8 +462,12 @@ int lbm_set_tuning(lbm_handle* h, const lbm_tuning* t)
     if (!h || !t) return fail(LBM_EINVAL, "lbm_set_tuning: NULL argument");
     if (h->kernel != LBM_KERNEL_FUSED)
         return fail(LBM_EINVAL, "lbm_set_tuning: handle does not use the fused kernel");
-    const int shape = t->shape < 0 ? lbm::fused::default_shape(h->g, h->d.q) : t->shape;
-    if (!lbm::fused::supported(h->g, h->d.q, shape))
+    const bool fp64 = h->d.precision == LBM_FP64;
+    const int shape = t->shape >= 0 ? t->shape
+                      : fp64    ? lbm::fused::default_shape(h->g, h->d.q)
+                                : lbm::lowp_default_shape(h->d.precision, h->a, h->g);
+    if (fp64 ? !lbm::fused::supported(h->g, h->d.q, shape)
+             : !lbm::lowp_shape_ok(h->d.precision, h->a, h->g, shape))
         return fail(LBM_EINVAL, "tile shape %d does not fit the %d x %d cross-section", shape, h->d.ny,
                     h->d.nz);
     if (t->chunk < 1) return fail(LBM_EINVAL, "chunk must be >= 1");

@@ -84,6 +84,30 @@ inline int default_shape(const Grid& g, int q)
     return -1;
 }
 
+// Pair tile shapes (tuning ids kNumShapes + i): two z-adjacent nodes per
+// thread, TY x TZ nodes per tile, 16-bit storage only (lbm_fused_pair.h).
+constexpr int kNumPairShapes = 2;
+constexpr int kPairTY[kNumPairShapes] = {8, 4};
+constexpr int kPairTZ[kNumPairShapes] = {128, 128};
+
+inline bool is_pair_shape(int shape) { return shape >= kNumShapes && shape < kNumShapes + kNumPairShapes; }
+
+inline bool pair_supported(const Grid& g, int q, int shape)
+{
+    if (q != 19 || !is_pair_shape(shape)) return false;
+    if (g.ps >= (int64_t(1) << 29)) return false;
+    const int i = shape - kNumShapes;
+    return g.nx >= 1 && g.ny % kPairTY[i] == 0 && g.nz % kPairTZ[i] == 0;
+}
+
+// Default pair tile for a lattice, in order of measured speed, or -1.
+inline int default_pair_shape(const Grid& g, int q)
+{
+    for (int i = 0; i < kNumPairShapes; i++)
+        if (pair_supported(g, q, kNumShapes + i)) return kNumShapes + i;
+    return -1;
+}
+
 // Ring slot r -> (row, col) in LDS coordinates.  Rows 0 and LY-1 are the y
 // ring; without ZF, columns 0 and LZ-1 are the z ring.
 template <typename TL> __device__ __forceinline__ void ring_slot(int r, int& row, int& col)

@@ -193,30 +193,110 @@ template <typename PK> __device__ __forceinline__ float sum19(const PK& d)
 
 __device__ __forceinline__ float psi_lp(float rho) { return 1.0f - __expf(-rho); }
 
+// Collision of one node from its stored values (or, tau = 1, its momentum)
+// and the LDS window, split in two so that a thread can interleave the
+// outputs of two nodes: lp_prep() forms the per-node scalars, lp_out<n>()
+// the stored value of population n.  Every fused reduced-precision kernel
+// calls these, so a node's result does not depend on the kernel or its
+// launch configuration.
+//
+// LpNode: where the node sits in the window -- the slots of planes x-1, x,
+// x+1, its index within a plane, and the window offsets of its z-1 / z+1
+// neighbours.
+struct LpNode {
+    const int (&sl)[3];
+    int li, dzm, dzp;
+    // window index of node + e_n
+    template <int n, int LZ> __device__ __forceinline__ int at() const
+    {
+        using S = St<19>;
+        const int dz = S::ez[n] < 0 ? dzm : (S::ez[n] > 0 ? dzp : 0);
+        return sl[1 + S::ex[n]] + li + S::ey[n] * LZ + dz;
+    }
+};
+struct LpColl {
+    float rho, ue, ve, we, usq15, t;
+};
+
+template <typename SP, bool NS, bool T1, int LZ, typename V>
+__device__ __forceinline__ LpColl lp_prep(const V& d, const LpNode& nd, const float* wpsi,
+                                          const float* wrho, float R, float tau, float G, float om,
+                                          float omt)
+{
+    using S = St<19>;
+    LpColl k;
+    k.rho = wrho[nd.sl[1] + nd.li];
+    // momentum of f equals that of d (sum_n w_n e_n = 0)
+    float jx, jy, jz;
+    if constexpr (T1) {
+        jx = d.jx; jy = d.jy; jz = d.jz;
+    } else {
+        jx = 0.f; jy = 0.f; jz = 0.f;
+        static_for<19>([&](auto I) {
+            constexpr int n = decltype(I)::value;
+            const float dn = d.template get<n>();
+            acc_e<S::ex[n]>(jx, dn);
+            acc_e<S::ey[n]>(jy, dn);
+            acc_e<S::ez[n]>(jz, dn);
+        });
+    }
+    // Shan-Chen force: sum_n w_n e_n psi(x + e_n)
+    float gx = 0.f, gy = 0.f, gz = 0.f;
+    static_for<19>([&](auto I) {
+        constexpr int n = decltype(I)::value;
+        if constexpr (n > 0) {
+            const float p = static_cast<float>(S::w[n]) * wpsi[nd.template at<n, LZ>()];
+            acc_e<S::ex[n]>(gx, p);
+            acc_e<S::ey[n]>(gy, p);
+            acc_e<S::ez[n]>(gz, p);
+        }
+    });
+    const float ir = __builtin_amdgcn_rcpf(k.rho);
+    const float jr = ir * (1.0f / kStoreScale<SP, NS>);   // j is summed from scaled values
+    const float kf = (tau * ir) * ((-G) * wpsi[nd.sl[1] + nd.li]);
+    k.ue = fmaf(kf, gx, jx * jr);
+    k.ve = fmaf(kf, gy, jy * jr);
+    k.we = fmaf(kf, gz, jz * jr);
+    k.usq15 = 1.5f * fmaf(k.ue, k.ue, fmaf(k.ve, k.ve, k.we * k.we));
+    k.t = fmaf(om, k.rho, omt * R);
+    return k;
+}
+
+template <typename SP, bool NS, bool T1, int LZ, int n, typename V>
+__device__ __forceinline__ typename SP::T lp_out(const LpColl& k, const V& d, const LpNode& nd,
+                                                 const float* wrho, float rs, float om, float omt)
+{
+    using S = St<19>;
+    constexpr float wn = static_cast<float>(S::w[n]);
+    constexpr float sw = kStoreScale<SP, NS> * wn;
+    const float ev = edot<S::ex[n], S::ey[n], S::ez[n]>(k.ue, k.ve, k.we);
+    const float pm1 = fmaf(fmaf(4.5f, ev, 3.0f), ev, -k.usq15);
+    float rnb;
+    if constexpr (NS) rnb = (n == 0) ? k.rho : wrho[nd.template at<n, LZ>()];
+    else rnb = rs;
+    float v;
+    if constexpr (T1) v = fmaf(sw * k.rho, pm1, fmaf(-sw, rnb, sw * k.rho));
+    else v = fmaf(d.template get<n>(), omt, fmaf((om * sw) * k.rho, pm1, fmaf(-sw, rnb, sw * k.t)));
+    return SP::of_d(v);
+}
+
 // Occupancy target: 4 wavefronts per SIMD (128 VGPRs); the wall variants
 // hold their values as floats and run at 2.
 template <bool WALLS> struct OccLp { static constexpr int waves = WALLS ? 2 : 4; };
 
+// One tile of the march (TY x TZ nodes at y0, z0; planes [x0, x1)), with
+// the LDS window wpsi / wrho of 4 * TL::LY * TL::LZ floats each.
 template <typename SP, bool WALLS, bool NS, bool T1, typename TL, bool NTS>
-__global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(OccLp<WALLS>::waves)))
-    k_fused_lp(const typename SP::T* __restrict__ Pin, int kind, typename SP::T* __restrict__ Pout,
-               Grid g, float tau, float G, float rs, const float* __restrict__ rin,
-               float* __restrict__ rout, int chunk, int order)
+__device__ __forceinline__ void lp_tile(const typename SP::T* __restrict__ Pin, int kind,
+                                        typename SP::T* __restrict__ Pout, const Grid& g, float tau,
+                                        float G, float rs, const float* __restrict__ rin,
+                                        float* __restrict__ rout, int y0, int z0, int x0, int x1,
+                                        float* __restrict__ wpsi, float* __restrict__ wrho)
 {
     using S = St<19>;
     using LD = LoadT<SP, WALLS>;
     using HD = HeldT<SP, WALLS, T1>;
     constexpr int TY = TL::TY, TZ = TL::TZ, LZ = TL::LZ, PL = TL::LY * TL::LZ;
-    // window of psi and rho for planes x-1 .. x+2 (slot (x - x0 + 1) mod 4)
-    __shared__ float wpsi[4 * PL], wrho[4 * PL];
-
-    const int nty = g.ny / TY, ntz = g.nz / TZ;
-    const int ncx = (g.nx + chunk - 1) / chunk;
-    int ty, tz, cx;
-    tile_of_block(blockIdx.x, nty, ntz, ncx, order, ty, tz, cx);
-    const int y0 = ty * TY, z0 = tz * TZ;
-    const int x0 = cx * chunk;
-    const int x1 = min(g.nx, x0 + chunk);
 
     const int tid = threadIdx.x;
     const int ly = tid / TZ, lz = tid - (tid / TZ) * TZ;
@@ -295,60 +375,15 @@ __global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(Occ
 
     auto collide = [&](int x, const HD& d, float R) {
         const int sl[3] = {slot(x - 1), slot(x), slot(x + 1)};
-        // window index of node x + e_n
-        auto at = [&](auto N) -> int {
-            constexpr int n = decltype(N)::value;
-            const int dz = S::ez[n] < 0 ? dzm : (S::ez[n] > 0 ? dzp : 0);
-            return sl[1 + S::ex[n]] + li + S::ey[n] * LZ + dz;
-        };
-        const float rho = wrho[sl[1] + li];
-        // momentum of f equals that of d (sum_n w_n e_n = 0)
-        float jx, jy, jz;
-        if constexpr (T1) {
-            jx = d.jx; jy = d.jy; jz = d.jz;
-        } else {
-            jx = 0.f; jy = 0.f; jz = 0.f;
-            static_for<19>([&](auto I) {
-                constexpr int n = decltype(I)::value;
-                const float dn = d.template get<n>();
-                acc_e<S::ex[n]>(jx, dn);
-                acc_e<S::ey[n]>(jy, dn);
-                acc_e<S::ez[n]>(jz, dn);
-            });
-        }
-        // Shan-Chen force: sum_n w_n e_n psi(x + e_n)
-        float gx = 0.f, gy = 0.f, gz = 0.f;
-        static_for<19>([&](auto I) {
-            constexpr int n = decltype(I)::value;
-            if constexpr (n > 0) {
-                const float p = static_cast<float>(S::w[n]) * wpsi[at(I)];
-                acc_e<S::ex[n]>(gx, p);
-                acc_e<S::ey[n]>(gy, p);
-                acc_e<S::ez[n]>(gz, p);
-            }
-        });
-        const float ir = __builtin_amdgcn_rcpf(rho);
-        const float jr = ir * (1.0f / kStoreScale<SP, NS>);   // j is summed from scaled values
-        const float kf = (tau * ir) * ((-G) * wpsi[sl[1] + li]);
-        const float ue = fmaf(kf, gx, jx * jr), ve = fmaf(kf, gy, jy * jr), we = fmaf(kf, gz, jz * jr);
-        const float usq15 = 1.5f * fmaf(ue, ue, fmaf(ve, ve, we * we));
-        const float t = fmaf(om, rho, omt * R);
+        const LpNode nd{sl, li, dzm, dzp};
+        const LpColl k = lp_prep<SP, NS, T1, LZ>(d, nd, wpsi, wrho, R, tau, G, om, omt);
         const uint32_t c = static_cast<uint32_t>(x) * plane + cyz;
         static_for<19>([&](auto I) {
             constexpr int n = decltype(I)::value;
-            constexpr float wn = static_cast<float>(S::w[n]);
-            constexpr float sw = kStoreScale<SP, NS> * wn;
-            const float ev = edot<S::ex[n], S::ey[n], S::ez[n]>(ue, ve, we);
-            const float pm1 = fmaf(fmaf(4.5f, ev, 3.0f), ev, -usq15);
-            float rnb;
-            if constexpr (NS) rnb = (n == 0) ? rho : wrho[at(I)];
-            else rnb = rs;
-            float v;
-            if constexpr (T1) v = fmaf(sw * rho, pm1, fmaf(-sw, rnb, sw * rho));
-            else v = fmaf(d.template get<n>(), omt, fmaf((om * sw) * rho, pm1, fmaf(-sw, rnb, sw * t)));
+            const typename SP::T v = lp_out<SP, NS, T1, LZ, n>(k, d, nd, wrho, rs, om, omt);
             typename SP::T* base = Pout + n * g.ps;
-            if constexpr (NTS) SP::put_nt(elem(base, c), SP::of_d(v));
-            else *elem(base, c) = SP::of_d(v);
+            if constexpr (NTS) SP::put_nt(elem(base, c), v);
+            else *elem(base, c) = v;
         });
     };
 
@@ -377,6 +412,26 @@ __global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(Occ
     }
     if (x < x1) step(x, fa, Ra, fc, Rc);
     if (x + 1 < x1) step(x + 1, fb, Rb, fa, Ra);
+}
+
+// The one-node march kernel: one tile per workgroup.
+template <typename SP, bool WALLS, bool NS, bool T1, typename TL, bool NTS>
+__global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(OccLp<WALLS>::waves)))
+    k_fused_lp(const typename SP::T* __restrict__ Pin, int kind, typename SP::T* __restrict__ Pout,
+               Grid g, float tau, float G, float rs, const float* __restrict__ rin,
+               float* __restrict__ rout, int chunk, int order)
+{
+    constexpr int PL = TL::LY * TL::LZ;
+    // window of psi and rho for planes x-1 .. x+2 (slot (x - x0 + 1) mod 4)
+    __shared__ float wpsi[4 * PL], wrho[4 * PL];
+    const int nty = g.ny / TL::TY, ntz = g.nz / TL::TZ;
+    const int ncx = (g.nx + chunk - 1) / chunk;
+    int ty, tz, cx;
+    tile_of_block(blockIdx.x, nty, ntz, ncx, order, ty, tz, cx);
+    const int z0 = tz * TL::TZ;
+    const int x0 = cx * chunk;
+    lp_tile<SP, WALLS, NS, T1, TL, NTS>(Pin, kind, Pout, g, tau, G, rs, rin, rout, ty * TL::TY, z0,
+                                        x0, min(g.nx, x0 + chunk), wpsi, wrho);
 }
 
 }  // namespace fused

@@ -18,6 +18,18 @@ inline bool lowp_node_shift(int precision)
     return precision == LBM_FP16N || precision == LBM_BF16N;
 }
 
+// Whether the fused pair kernel (two nodes per thread, lbm_fused_pair.h)
+// serves this form: 16-bit storage, tau = 1, no walls.
+inline bool lowp_pairs(int precision, const StepArgs& a)
+{
+    const bool b16 = precision == LBM_FP16S || precision == LBM_BF16S || lowp_node_shift(precision);
+    return b16 && !a.walls && a.tau == 1.0;
+}
+// Whether a fused tile shape (one-node or pair id) can run this form and
+// lattice; and the default shape (a pair tile where one fits), or -1.
+bool lowp_shape_ok(int precision, const StepArgs& a, const Grid& g, int shape);
+int lowp_default_shape(int precision, const StepArgs& a, const Grid& g);
+
 // Kernel the handle will use (LBM_KERNEL_*), or -1 if `requested` cannot run.
 int lowp_resolve_kernel(int precision, const StepArgs& a, const Grid& g, int requested);
 // Initial equilibrium from rho0 into P (held as f at its own node); writes

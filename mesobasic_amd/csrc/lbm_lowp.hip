@@ -355,9 +355,25 @@ float shift_of(int precision, double rho_shift)
 
 }  // namespace
 
-int lowp_resolve_kernel(int, const StepArgs& a, const Grid& g, int requested)
+bool lowp_shape_ok(int precision, const StepArgs& a, const Grid& g, int shape)
 {
-    const bool fused_ok = fused::default_shape(g, a.q) >= 0;
+    if (fused::is_pair_shape(shape))
+        return lowp_pairs(precision, a) && fused::pair_supported(g, a.q, shape);
+    return fused::supported(g, a.q, shape);
+}
+
+int lowp_default_shape(int precision, const StepArgs& a, const Grid& g)
+{
+    if (lowp_pairs(precision, a)) {
+        const int s = fused::default_pair_shape(g, a.q);
+        if (s >= 0) return s;
+    }
+    return fused::default_shape(g, a.q);
+}
+
+int lowp_resolve_kernel(int precision, const StepArgs& a, const Grid& g, int requested)
+{
+    const bool fused_ok = lowp_default_shape(precision, a, g) >= 0;
     if (requested == LBM_KERNEL_FUSED) return fused_ok ? LBM_KERNEL_FUSED : -1;
     if (requested == LBM_KERNEL_TWO_PASS) return LBM_KERNEL_TWO_PASS;
     return fused_ok ? LBM_KERNEL_FUSED : LBM_KERNEL_TWO_PASS;
@@ -386,7 +402,7 @@ hipError_t lowp_step(int precision, const StepArgs& a, const Grid& g, double rho
     const float rs = shift_of(precision, rho_shift);
     if (lowp_node_shift(precision) && (!rin || !rout)) return hipErrorInvalidValue;
     if (kernel == LBM_KERNEL_FUSED && rho0 == nullptr) {
-        if (!fused::supported(g, a.q, t.shape) || t.chunk < 1) return hipErrorNotSupported;
+        if (!lowp_shape_ok(precision, a, g, t.shape) || t.chunk < 1) return hipErrorNotSupported;
         switch (precision) {
         case LBM_FP32: return lowp_fused_launch<0, false>(Pin, kind, Pout, g, a, rs, rin, rout, t, s);
         case LBM_FP16S: return lowp_fused_launch<1, false>(Pin, kind, Pout, g, a, rs, rin, rout, t, s);

@@ -7,6 +7,7 @@
 
 #include "../../include/mesobasic_lbm.h"
 #include "lbm_fused_lp.h"
+#include "lbm_fused_pair.h"
 #include "lbm_lowp.h"
 #include "lbm_storage.h"
 
@@ -54,10 +55,42 @@ void by_shape(const void* Pin, int kind, void* Pout, const Grid& g, const StepAr
 #undef LBM_LP_SHAPE
 }
 
+// Pair tiles (16-bit storage, tau = 1, no walls; checked by the caller
+// through lowp_shape_ok).
+template <typename SP, bool NS, int I, bool NTS>
+void launch_pair(const void* Pin, int kind, void* Pout, const Grid& g, const StepArgs& a, float rs,
+                 const float* rin, float* rout, const FusedTuning& t, hipStream_t s)
+{
+    using namespace fused;
+    using T = typename SP::T;
+    using TL = PairTile<kPairTY[I], kPairTZ[I]>;
+    const int ncx = (g.nx + t.chunk - 1) / t.chunk;
+    const unsigned nb = static_cast<unsigned>((g.ny / TL::TY) * (g.nz / TL::TZ) * ncx);
+    k_fused_pair<SP, NS, TL, NTS><<<nb, TL::NT, 0, s>>>(
+        static_cast<const T*>(Pin), kind, static_cast<T*>(Pout), g, (float)a.tau, (float)a.G, rs,
+        rin, rout, t.chunk, t.order);
+}
+
+template <typename SP, bool NS, bool NTS>
+void by_pair_shape(const void* Pin, int kind, void* Pout, const Grid& g, const StepArgs& a,
+                   float rs, const float* rin, float* rout, const FusedTuning& t, hipStream_t s)
+{
+    static_assert(fused::kNumPairShapes == 2, "one case per pair shape");
+    if (t.shape == fused::kNumShapes) launch_pair<SP, NS, 0, NTS>(Pin, kind, Pout, g, a, rs, rin, rout, t, s);
+    else launch_pair<SP, NS, 1, NTS>(Pin, kind, Pout, g, a, rs, rin, rout, t, s);
+}
+
 template <typename SP, bool NS, bool T1>
 void by_flags(const void* Pin, int kind, void* Pout, const Grid& g, const StepArgs& a, float rs,
               const float* rin, float* rout, const FusedTuning& t, hipStream_t s)
 {
+    if constexpr (!std::is_same_v<SP, StF32> && T1) {
+        if (fused::is_pair_shape(t.shape) && !a.walls) {
+            if (t.nts) by_pair_shape<SP, NS, true>(Pin, kind, Pout, g, a, rs, rin, rout, t, s);
+            else by_pair_shape<SP, NS, false>(Pin, kind, Pout, g, a, rs, rin, rout, t, s);
+            return;
+        }
+    }
     if (a.walls) {
         if (t.nts) by_shape<SP, true, NS, T1, true>(Pin, kind, Pout, g, a, rs, rin, rout, t, s);
         else by_shape<SP, true, NS, T1, false>(Pin, kind, Pout, g, a, rs, rin, rout, t, s);

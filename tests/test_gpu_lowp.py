@@ -122,6 +122,62 @@ def test_fused_tile_shapes_chunks_and_orders_agree_bitwise(prec):
             assert np.array_equal(got, ref), (sh, chunk, order)
 
 
+PAIR_SHAPES = (6, 7)   # two nodes per thread: 8x128 and 4x128 tiles
+
+
+@pytest.mark.parametrize("prec", [mb.FP16S, mb.BF16S, mb.FP16N, mb.BF16N])
+def test_pair_tiles_agree_bitwise_with_one_node_tiles(prec):
+    # the pair kernel must give the bits of the one-node kernel everywhere;
+    # noise makes every column carry distinct values, so a wrong half at the
+    # periodic z seam (pairs at z = 0 and nz-2, patched through LDS) shows;
+    # nx = 7 with chunks of 3, 5 and 64 covers partial chunks and the x wrap
+    # of a chunk's prologue
+    shape = (7, 16, 256)
+    rho0 = mb.noise_rho(shape, 5)
+    ref = None
+    for sh, chunk, order in ((3, 64, 3), (6, 64, 3), (7, 3, 0), (6, 3, 2), (7, 2, 1), (6, 5, 1)):
+        lat = mb.Lattice(19, shape, 1.0, -5.5, precision=prec, kernel=mb.KERNEL_FUSED)
+        lat.set_tuning(sh, chunk, order, 1)
+        assert lat.tuning[0] == sh
+        lat.set_rho(rho0); lat.init(); lat.step(6)
+        got = lat.get_f()
+        if ref is None:
+            ref = got
+        else:
+            assert np.array_equal(got, ref), (sh, chunk, order)
+
+
+@pytest.mark.parametrize("prec", [mb.FP16S, mb.FP16N])
+def test_pair_tiles_vs_oracle(prec):
+    # default tiling of a tau = 1 16-bit lattice whose z extent takes pair tiles
+    shape, steps = (16, 32, 256), 30
+    rho0 = mb.noise_rho(shape, 9)
+    o = oracle.Oracle(19, shape, 1.0, -5.0, nthreads=4)
+    o.set_rho(rho0); o.init(); o.step(steps)
+    ref = o.moments()
+    lat = mb.Lattice(19, shape, 1.0, -5.0, precision=prec, rho_shift=0.7, kernel=mb.KERNEL_FUSED)
+    assert lat.tuning[0] in PAIR_SHAPES
+    lat.set_rho(rho0); lat.init(); lat.step(steps)
+    err = rel_l2(lat.fields(mb.FIELDS_CURRENT)["rho"], ref["rho"])
+    print(f"pair tiles prec {prec}: rel L2 rho = {err:.3e}", flush=True)
+    assert err < TOL_NOISE[prec]
+
+
+def test_pair_tiles_refused_where_they_do_not_apply():
+    # tau != 1, walls, FP32 storage or a z extent that is not a multiple of
+    # 128: the library keeps the one-node tiles and rejects a pair shape
+    for kw in (dict(tau=0.8, precision=mb.FP16N, shape=(4, 16, 256)),
+               dict(tau=1.0, precision=mb.FP16N, shape=(4, 16, 256), walls=True),
+               dict(tau=1.0, precision=mb.FP32, shape=(4, 16, 256)),
+               dict(tau=1.0, precision=mb.FP16S, shape=(4, 16, 192))):
+        lat = mb.Lattice(19, kw["shape"], kw["tau"], -5.0, precision=kw["precision"],
+                         walls=kw.get("walls", False), kernel=mb.KERNEL_FUSED)
+        assert lat.tuning[0] not in PAIR_SHAPES
+        with pytest.raises(mb.LbmError):
+            lat.set_tuning(6, 64, 3, 1)
+        lat.close()
+
+
 @pytest.mark.parametrize("prec", [mb.FP16N, mb.BF16N])
 def test_node_shift_fused_matches_two_pass(prec):
     shape = (16, 16, 64)
