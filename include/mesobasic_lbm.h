@@ -150,7 +150,10 @@ int lbm_synchronize(lbm_handle* h);
  * No reference counterpart: the reference has no launch configuration.
  *   shape     tile of the y-z cross-section per workgroup: 0: 1x512, 1: 2x256,
  *             2: 1x256 (whole z rows, nz must equal 512 / 256 / 256),
- *             3: 8x64, 4: 4x64, 5: 8x32; -1 = first that fits
+ *             3: 8x64, 4: 4x64, 5: 8x32; two nodes per thread (16-bit
+ *             storage at tau = 1 without walls, nz a multiple of 128):
+ *             6: 8x128, 7: 4x128; -1 = first that fits (a pair tile where
+ *             one applies)
  *   chunk     x planes marched per workgroup
  *   order     block -> tile order: 0 plain, 1 contiguous runs per XCD,
  *             2 y-tiles of one column per XCD, 3 a y-band x all z-tiles
