@@ -259,6 +259,19 @@ int lbm_batch_set_work_limit(lbm_batch* b, int64_t max_nodes);
  * two-pass kernels (0). */
 int lbm_batch_tuning(const lbm_batch* b, int* d2q9_chunk, int* d2q9_rows_kernel);
 
+/* Sets the launch configuration (and stops the tuner): D2Q9 rows per work
+ * item (>= 1), D2Q9 jobs on the row-marching kernel (1) or the two-pass
+ * kernels (0), and D3Q19 jobs with nx >= 2, ny >= 8 and nz >= 64 on the
+ * fused march kernel (1; the default for LBM_FP16N / LBM_BF16N) or the
+ * two-pass kernels (0; the default for the other forms).  D2Q9 results do
+ * not depend on it; D3Q19 results of the two kernels agree to FP32
+ * rounding, not bit for bit, and the tuner never changes this choice.
+ * lbm_batch_get_launch reports the current configuration.  No reference
+ * counterpart (the reference runs one lattice per process). */
+int lbm_batch_set_launch(lbm_batch* b, int d2q9_chunk, int d2q9_rows_kernel, int d3q19_march);
+int lbm_batch_get_launch(const lbm_batch* b, int* d2q9_chunk, int* d2q9_rows_kernel,
+                         int* d3q19_march);
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,13 +24,19 @@ struct Job {
     int64_t remaining;   // steps still to run (device-updated, 0 = idle)
 };
 
-// Unit of work of one launch: a range of x planes (D2Q9 row kernel) or of
-// nodes (two-pass kernels) of one job.
+// Unit of work of one launch: a range of x planes (D2Q9 row kernel), of
+// nodes (two-pass kernels), or a tile and a range of x planes (D3Q19 march)
+// of one job.
 struct Item {
     int job;
-    int x0, x1;          // row kernel: planes [x0, x1)
+    int x0, x1;          // row kernel and D3Q19 march: planes [x0, x1)
     int64_t c0, c1;      // two-pass kernels: nodes [c0, c1)
+    int y0, z0;          // D3Q19 march: tile origin
 };
+
+// D3Q19 march tile (kMarchTY rows x kMarchTZ columns of a job's y-z cross
+// section); jobs with nx >= 2, ny >= kMarchTY and nz >= kMarchTZ take it.
+constexpr int kMarchTY = 8, kMarchTZ = 64;
 
 // Launch parameters: job and item tables, the ticket counters of the
 // persistent loop (two, alternating by launch parity) and their count.
@@ -47,6 +53,9 @@ hipError_t d2q9_rows_step(int prec, const Tables& t, int parity, int threads, in
                           hipStream_t s);
 hipError_t twopass_psi(int prec, int q, const Tables& t, int parity, int grid, hipStream_t s);
 hipError_t twopass_collide(int prec, int q, const Tables& t, int parity, int grid, hipStream_t s);
+// D3Q19 jobs in one fused pass (psi window in LDS); agrees with the two
+// passes above to rounding.
+hipError_t d3q19_march_step(int prec, const Tables& t, int parity, int grid, hipStream_t s);
 // After a step: remaining -= 1 and first = 0 for every stepped job.
 hipError_t advance_counters(Job* jobs, int njobs, hipStream_t s);
 // Initial equilibrium of one job from rho0 (device FP32): writes the input

@@ -27,6 +27,8 @@
 
 #include "../../include/mesobasic_lbm.h"
 #include "lbm_batch.h"
+#include "lbm_fused_lp.h"
+#include "lbm_internal.h"
 #include "lbm_stencil.h"
 #include "lbm_storage.h"
 
@@ -362,6 +364,56 @@ __global__ void __launch_bounds__(kTP) k_tp_collide(Tables t, int parity)
     }
 }
 
+// ---------------------------------------------------------------------------
+// D3Q19 march: an item is a kMarchTY x kMarchTZ tile of one job's y-z cross
+// section and a chunk of x planes [x0, x1), run by the single-lattice fused
+// march (fused::lp_tile, lbm_fused_lp.h): psi (and rho with the node shift)
+// of planes x-1 .. x+2 for the tile and its one-node ring in LDS, the pulls
+// of plane x+2 in flight during the collision of plane x, every population
+// read once per step.  Sides that are not multiples of the tile run as
+// partial tiles (wrapped pulls, masked stores).  The stored values follow
+// the convention of this engine (d = S (f - w R) relative to the reading
+// node's R), which is the single-lattice one.
+//
+// Its FP32 arithmetic is lp_prep / lp_out, not the expressions of the
+// two-pass kernels above, so the two agree to rounding, not bit for bit; a
+// D3Q19 job therefore always runs the same kernel, whatever the other jobs
+// or the tuner do.  Which one is a property of the batch (lbm_batch.h
+// default_march19: the march for the node-shift forms, where it measured
+// 1.5x faster than the two passes on 64..192-per-side jobs; the two passes
+// for the constant-shift forms, where partial tiles on such sides cost the
+// march more than the second pass costs the two-pass kernels).
+
+template <typename SP, bool NS>
+// Occupancy target 2 wavefronts per SIMD: held to 128 VGPRs the item loop
+// spills 36-108 bytes per thread, and measured slower.
+__global__ void __launch_bounds__(kMarchTY * kMarchTZ) __attribute__((amdgpu_waves_per_eu(2)))
+    k_d3q19_march(Tables t, int parity)
+{
+    using TL = fused::Tile<kMarchTY, kMarchTZ, false>;
+    constexpr int PL = TL::LY * TL::LZ;
+    __shared__ float wpsi[4 * PL], wrho[4 * PL];
+    __shared__ int slot_s;
+    reset_other(t, parity);
+    for (int it = next_item(t, parity, &slot_s); it >= 0; it = next_item(t, parity, &slot_s)) {
+        const Item item = t.items[it];
+        const Job& job = t.jobs[item.job];
+        if (job.remaining <= 0) continue;                 // uniform across the workgroup
+        const int p = parity ^ job.phase;
+        Grid g;
+        g.nx = job.nx;
+        g.ny = job.ny;
+        g.nz = job.nz;
+        g.n = job.n;
+        g.ps = job.ps;
+        fused::lp_tile<SP, false, NS, false, TL, true, true>(
+            static_cast<const typename SP::T*>(job.P[p]), kHoldsPost,
+            static_cast<typename SP::T*>(job.P[p ^ 1]), g, job.tau, job.G, job.rs,
+            NS ? job.R[p] : nullptr, NS ? job.R[p ^ 1] : nullptr, item.y0, item.z0, item.x0,
+            item.x1, wpsi, wrho);
+    }
+}
+
 __global__ void k_advance(Job* jobs, int njobs)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -510,6 +562,16 @@ hipError_t twopass_collide(int prec, int q, const Tables& t, int parity, int gri
         constexpr bool NS = decltype(ns)::value;
         if (q == 9) k_tp_collide<9, SP, NS><<<grid, kTP, 0, s>>>(t, parity);
         else k_tp_collide<19, SP, NS><<<grid, kTP, 0, s>>>(t, parity);
+    });
+    return hipGetLastError();
+}
+
+hipError_t d3q19_march_step(int prec, const Tables& t, int parity, int grid, hipStream_t s)
+{
+    by_prec(prec, [&](auto sp, auto ns) {
+        using SP = decltype(sp);
+        constexpr bool NS = decltype(ns)::value;
+        k_d3q19_march<SP, NS><<<grid, kMarchTY * kMarchTZ, 0, s>>>(t, parity);
     });
     return hipGetLastError();
 }

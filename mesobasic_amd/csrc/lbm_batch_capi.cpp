@@ -31,9 +31,11 @@ namespace {
 
 // Kernel classes of one step: D2Q9 row-marching kernels by row width, and the
 // two-pass kernels (psi then collide) of each stencil.
-enum Cls { kRow64, kRow128, kRow256, kRow512, kRow1024, kTp9Psi, kTp9Col, kTp19Psi, kTp19Col, kNumCls };
+enum Cls { kRow64, kRow128, kRow256, kRow512, kRow1024, kTp9Psi, kTp9Col, kTp19Psi, kTp19Col, kMarch19,
+           kNumCls };
 constexpr int kRowWidth[5] = {64, 128, 256, 512, 1024};
 constexpr int64_t kTpItem = 8192;   // nodes per two-pass work item
+constexpr int kMarchChunk = 32;     // x planes per D3Q19 march item
 
 struct HostJob {
     int64_t id;
@@ -86,8 +88,12 @@ struct lbm_batch {
     // launch configuration
     int chunk = 32;               // D2Q9 rows per row-kernel item
     int rows_kernel = 1;          // D2Q9 jobs on the row kernel (1) or two-pass (0)
+    int march19 = 0;              // D3Q19 jobs on the march kernel (1) or two-pass (0)
     int tune_left = 0;            // ticks left to try candidate configurations
-    std::vector<std::pair<int, int>> candidates;
+    struct Config {
+        int rows_kernel, chunk;
+    };
+    std::vector<Config> candidates;
     std::vector<double> cand_rate;
     int cand_idx = 0;
     // tick graph
@@ -109,6 +115,12 @@ int row_class(int ny)
 bool use_rows(const lbm_batch* b, const HostJob& j)
 {
     return j.d.q == 9 && b->rows_kernel && row_class(j.d.ny) >= 0;
+}
+
+bool use_march(const lbm_batch* b, const HostJob& j)
+{
+    return j.d.q == 19 && b->march19 && j.d.nx >= 2 && j.d.ny >= lbm::batch::kMarchTY &&
+           j.d.nz >= lbm::batch::kMarchTZ;
 }
 
 int ensure_items(lbm_batch* b, int cls, int n)
@@ -202,6 +214,18 @@ int upload_tables(lbm_batch* b)
                 it.x1 = std::min(j.d.nx, x0 + b->chunk);
                 items[cls].push_back(it);
             }
+        } else if (use_march(b, j)) {
+            for (int y0 = 0; y0 < j.d.ny; y0 += lbm::batch::kMarchTY)
+                for (int z0 = 0; z0 < j.d.nz; z0 += lbm::batch::kMarchTZ)
+                    for (int x0 = 0; x0 < j.d.nx; x0 += kMarchChunk) {
+                        Item it{};
+                        it.job = ji;
+                        it.y0 = y0;
+                        it.z0 = z0;
+                        it.x0 = x0;
+                        it.x1 = std::min(j.d.nx, x0 + kMarchChunk);
+                        items[kMarch19].push_back(it);
+                    }
         } else {
             const int cp = j.d.q == 9 ? kTp9Psi : kTp19Psi;
             for (int64_t c0 = 0; c0 < j.n; c0 += kTpItem) {
@@ -265,6 +289,12 @@ int launch_step(lbm_batch* b, int parity)
         BHIP(lbm::batch::twopass_psi(b->d.precision, q, tp, parity, persistent_grid(b, 256), b->stream));
         BHIP(lbm::batch::twopass_collide(b->d.precision, q, tc, parity, persistent_grid(b, 256),
                                          b->stream));
+    }
+    if (b->class_mask & (1u << kMarch19)) {
+        Tables tm{b->d_jobs, b->d_items[kMarch19], b->d_nitems + kMarch19, b->d_tickets + 2 * kMarch19};
+        BHIP(lbm::batch::d3q19_march_step(b->d.precision, tm, parity,
+                                          persistent_grid(b, lbm::batch::kMarchTY * lbm::batch::kMarchTZ),
+                                          b->stream));
     }
     BHIP(lbm::batch::advance_counters(b->d_jobs, b->cap_jobs, b->stream));
     return LBM_OK;
@@ -345,6 +375,9 @@ int lbm_batch_create(const lbm_batch_desc* desc, lbm_batch** out)
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return lbm::set_error(LBM_ENODEV, "lbm_batch_create: no HIP device");
     auto b = std::make_unique<lbm_batch>();
+    // D3Q19 kernel: the march for the node-shift forms, the two passes for
+    // the others (measured faster each way on 64..192-per-side jobs)
+    b->march19 = node_shift(desc->precision) ? 1 : 0;
     b->d = *desc;
     b->d.steps_per_tick += b->d.steps_per_tick & 1;   // even: every tick starts at parity 0
     int dev = 0;
@@ -363,13 +396,15 @@ int lbm_batch_create(const lbm_batch_desc* desc, lbm_batch** out)
     BHIP(lbm::batch::zero_u32(b->d_tickets, 2 * kNumCls, b->stream));
     BHIP(hipStreamSynchronize(b->stream));
     if (b->d.autotune) {
+        // D2Q9 row kernel with row chunks of 16 / 32 / 64, or two-pass (the
+        // D3Q19 kernel is not tuned: march and two-pass differ in rounding)
         for (int rk : {1, 0})
             for (int ch : {16, 32, 64})
-                if (rk || ch == 32) b->candidates.emplace_back(rk, ch);
+                if (rk || ch == 32) b->candidates.push_back({rk, ch});
         b->cand_rate.assign(b->candidates.size(), 0.0);
         b->tune_left = static_cast<int>(b->candidates.size());
-        b->rows_kernel = b->candidates[0].first;
-        b->chunk = b->candidates[0].second;
+        b->rows_kernel = b->candidates[0].rows_kernel;
+        b->chunk = b->candidates[0].chunk;
     }
     *out = b.release();
     return LBM_OK;
@@ -489,15 +524,13 @@ int lbm_batch_tick(lbm_batch* b, int* n_finished)
         b->cand_rate[b->cand_idx] = nodes / std::max(secs, 1e-9);
         b->tune_left--;
         b->cand_idx++;
-        if (b->tune_left > 0) {
-            b->rows_kernel = b->candidates[b->cand_idx].first;
-            b->chunk = b->candidates[b->cand_idx].second;
-        } else {
-            const size_t best = std::max_element(b->cand_rate.begin(), b->cand_rate.end()) -
-                                b->cand_rate.begin();
-            b->rows_kernel = b->candidates[best].first;
-            b->chunk = b->candidates[best].second;
-        }
+        const size_t pick = b->tune_left > 0
+                                ? static_cast<size_t>(b->cand_idx)
+                                : static_cast<size_t>(std::max_element(b->cand_rate.begin(),
+                                                                       b->cand_rate.end()) -
+                                                      b->cand_rate.begin());
+        b->rows_kernel = b->candidates[pick].rows_kernel;
+        b->chunk = b->candidates[pick].chunk;
         b->dirty = true;
         b->graph_valid = false;
     }
@@ -553,6 +586,28 @@ int lbm_batch_set_work_limit(lbm_batch* b, int64_t max_nodes)
     if (!b || max_nodes < 0) return lbm::set_error(LBM_EINVAL, "lbm_batch_set_work_limit: bad argument");
     b->work_limit = max_nodes;
     b->dirty = true;
+    return LBM_OK;
+}
+
+int lbm_batch_set_launch(lbm_batch* b, int d2q9_chunk, int d2q9_rows_kernel, int d3q19_march)
+{
+    if (!b) return lbm::set_error(LBM_EINVAL, "lbm_batch_set_launch: NULL batch");
+    if (d2q9_chunk < 1) return lbm::set_error(LBM_EINVAL, "lbm_batch_set_launch: chunk must be >= 1");
+    b->chunk = d2q9_chunk;
+    b->rows_kernel = d2q9_rows_kernel ? 1 : 0;
+    b->march19 = d3q19_march ? 1 : 0;
+    b->tune_left = 0;
+    b->dirty = true;
+    b->graph_valid = false;
+    return LBM_OK;
+}
+
+int lbm_batch_get_launch(const lbm_batch* b, int* d2q9_chunk, int* d2q9_rows_kernel, int* d3q19_march)
+{
+    if (!b) return lbm::set_error(LBM_EINVAL, "lbm_batch_get_launch: NULL batch");
+    if (d2q9_chunk) *d2q9_chunk = b->chunk;
+    if (d2q9_rows_kernel) *d2q9_rows_kernel = b->rows_kernel;
+    if (d3q19_march) *d3q19_march = b->march19;
     return LBM_OK;
 }
 

@@ -104,6 +104,15 @@ template <> struct Held<StBF16> {
         return (n & 1) ? __uint_as_float(v[n >> 1] & 0xffff0000u) : __uint_as_float(v[n >> 1] << 16);
     }
 };
+template <> struct Held<StF32> {
+    float v[19];
+    __device__ __forceinline__ void pack(const Load<StF32>& l)
+    {
+#pragma unroll
+        for (int k = 0; k < 19; k++) v[k] = l.v[k];
+    }
+    template <int n> __device__ __forceinline__ float get() const { return v[n]; }
+};
 template <typename SP> struct PkFloat {
     float v[19];
     template <int n> __device__ __forceinline__ void load(const typename SP::T* p) { v[n] = SP::d(*p); }
@@ -286,7 +295,13 @@ template <bool WALLS> struct OccLp { static constexpr int waves = WALLS ? 2 : 4;
 
 // One tile of the march (TY x TZ nodes at y0, z0; planes [x0, x1)), with
 // the LDS window wpsi / wrho of 4 * TL::LY * TL::LZ floats each.
-template <typename SP, bool WALLS, bool NS, bool T1, typename TL, bool NTS>
+//
+// PARTIAL: the tile may run past the end of the y or z side (sides that are
+// not multiples of the tile, ny >= TY and nz >= TZ).  Every thread then
+// pulls its node wrapped periodically, so the window past the last valid
+// row or column holds exactly the periodic neighbours that row or column
+// needs, and only nodes inside the lattice collide and store.
+template <typename SP, bool WALLS, bool NS, bool T1, typename TL, bool NTS, bool PARTIAL = false>
 __device__ __forceinline__ void lp_tile(const typename SP::T* __restrict__ Pin, int kind,
                                         typename SP::T* __restrict__ Pout, const Grid& g, float tau,
                                         float G, float rs, const float* __restrict__ rin,
@@ -300,7 +315,9 @@ __device__ __forceinline__ void lp_tile(const typename SP::T* __restrict__ Pin, 
 
     const int tid = threadIdx.x;
     const int ly = tid / TZ, lz = tid - (tid / TZ) * TZ;
-    const int yi = y0 + ly, zi = z0 + lz;
+    const int yi = PARTIAL ? wrap(y0 + ly, g.ny) : y0 + ly;
+    const int zi = PARTIAL ? wrap(z0 + lz, g.nz) : z0 + lz;
+    const bool valid = !PARTIAL || (y0 + ly < g.ny && z0 + lz < g.nz);
     constexpr int c0 = TL::ZF ? 0 : 1;
     const int li = (ly + 1) * LZ + (lz + c0);
     const int dzm = (TL::ZF && lz == 0) ? TZ - 1 : -1;
@@ -349,7 +366,7 @@ __device__ __forceinline__ void lp_tile(const typename SP::T* __restrict__ Pin, 
     auto own_win = [&](int xu, const LD& dv, float R) {
         const float rho = put_win(slot(xu) + li, dv, R);
         if constexpr (NS) {
-            if (xu >= x0 && xu < x1) *elem(rout, static_cast<uint32_t>(xu) * plane + cyz) = rho;
+            if (valid && xu >= x0 && xu < x1) *elem(rout, static_cast<uint32_t>(xu) * plane + cyz) = rho;
         }
     };
 
@@ -374,6 +391,8 @@ __device__ __forceinline__ void lp_tile(const typename SP::T* __restrict__ Pin, 
     const float om = 1.0f / tau, omt = 1.0f - om;
 
     auto collide = [&](int x, const HD& d, float R) {
+        if constexpr (PARTIAL)
+            if (!valid) return;
         const int sl[3] = {slot(x - 1), slot(x), slot(x + 1)};
         const LpNode nd{sl, li, dzm, dzp};
         const LpColl k = lp_prep<SP, NS, T1, LZ>(d, nd, wpsi, wrho, R, tau, G, om, omt);

@@ -103,6 +103,8 @@ def load():
         "lbm_batch_steps_run": (ctypes.c_int64, [P]),
         "lbm_batch_tuning": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
         "lbm_batch_set_work_limit": (ctypes.c_int, [P, ctypes.c_int64]),
+        "lbm_batch_set_launch": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+        "lbm_batch_get_launch": (ctypes.c_int, [P] + [ctypes.POINTER(ctypes.c_int)] * 3),
         "lbm_probe_copy": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, ctypes.c_int, _D]),
         "lbm_probe_tile_map": (ctypes.c_int, [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_int)] * 3),
     }
@@ -366,6 +368,12 @@ class Batch:
 
     @property
     def tuning(self):
-        a, b = ctypes.c_int(), ctypes.c_int()
-        _check(self._lib.lbm_batch_tuning(self._h, ctypes.byref(a), ctypes.byref(b)))
-        return dict(d2q9_chunk=a.value, d2q9_rows_kernel=b.value)
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(self._lib.lbm_batch_get_launch(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return dict(d2q9_chunk=a.value, d2q9_rows_kernel=b.value, d3q19_march=c.value)
+
+    def set_launch(self, d2q9_chunk=32, d2q9_rows_kernel=1, d3q19_march=1):
+        """Fixes the launch configuration (stops the tuner); results do not
+        depend on it."""
+        _check(self._lib.lbm_batch_set_launch(self._h, int(d2q9_chunk), int(d2q9_rows_kernel),
+                                              int(d3q19_march)))

@@ -158,3 +158,50 @@ def test_node_shift_resolves_droplet_jobs(q, shape):
           f"BF16N {err[mb.BF16N]:.2e}", flush=True)
     assert err[mb.FP16N] < 3e-4 and err[mb.BF16N] < 3e-3
     assert err[mb.FP16N] < 0.1 * err[mb.FP16S]
+
+
+# D3Q19 jobs with nx >= 2, ny >= 8 and nz >= 64 take the fused march kernel
+# (8 x 64 tiles); sides that are not multiples of the tile exercise the
+# wrapped partial tiles
+MARCH_SHAPES = [(9, 20, 100), (5, 8, 64), (12, 33, 70), (3, 9, 129), (6, 16, 128)]
+MARCH_TOL = {mb.FP32: 2e-6, mb.FP16S: 1e-4, mb.BF16S: 1e-3, mb.FP16N: 1e-4, mb.BF16N: 5e-4}
+
+
+@pytest.mark.parametrize("prec", [mb.FP32, mb.FP16S, mb.BF16S, mb.FP16N, mb.BF16N])
+def test_d3q19_march_agrees_with_two_pass_and_oracle(prec):
+    # the two D3Q19 kernels differ in FP32 rounding only: both within the
+    # storage tolerance of the FP64 oracle, and of each other
+    jobs = [dict(q=19, shape=sh, tau=t, G=G, steps=st, seed=11 + k)
+            for k, (sh, t, G, st) in enumerate(zip(MARCH_SHAPES, (1.0, 0.7, 1.3, 0.9, 1.0),
+                                                   (-5.0, -5.5, -4.8, -6.0, -5.2), (9, 14, 7, 12, 10)))]
+    runs = []
+    for march in (0, 1):
+        b = Batch(precision=prec, steps_per_tick=4)
+        b.set_launch(32, 1, march)
+        assert b.tuning["d3q19_march"] == march
+        runs.append(_run(b, jobs))
+    for k, j in enumerate(jobs):
+        o = oracle.Oracle(19, j["shape"], j["tau"], j["G"])
+        o.set_rho(mb.noise_rho(j["shape"], j["seed"])); o.init(); o.step(j["steps"])
+        ref = o.moments()["rho"]
+        e_two, e_march = rel_l2(runs[0][k]["rho"], ref), rel_l2(runs[1][k]["rho"], ref)
+        print(f"prec {prec} job {j['shape']}: two-pass {e_two:.2e} march {e_march:.2e}", flush=True)
+        assert e_march < MARCH_TOL[prec] and e_two < MARCH_TOL[prec]
+        assert rel_l2(runs[1][k]["rho"], runs[0][k]["rho"]) < MARCH_TOL[prec]
+
+
+def test_d3q19_march_default_alone_equals_in_the_mix():
+    # node-shift batches run D3Q19 jobs on the march by default; a job's bits
+    # do not depend on the other jobs or on the tuner
+    shape, steps, tau, G = (10, 12, 100), 25, 0.9, -5.0
+    job = dict(q=19, shape=shape, tau=tau, G=G, steps=steps, seed=4)
+    for prec in (mb.FP16N, mb.BF16N):
+        b = Batch(precision=prec, steps_per_tick=6)
+        assert b.tuning["d3q19_march"] == 1
+        alone = _run(b, [job])[0]
+        rng = np.random.default_rng(8)
+        mix = _jobs(rng, 6) + [job] + [dict(q=19, shape=(6, 9, 80), tau=1.1, G=-5.3, steps=17, seed=2)]
+        got = _run(Batch(precision=prec, steps_per_tick=4, autotune=True), mix)[6]
+        for f in ("rho", "u", "v", "w"):
+            assert same_bits(alone[f], got[f]), (prec, f)
+    assert Batch(precision=mb.FP16S).tuning["d3q19_march"] == 0
