@@ -130,7 +130,8 @@ def main():
     if args.copy_calibration:
         nbytes = args.copy_calibration * 2**20
         res = {"copy_bytes": nbytes}
-        for mode, name in ((0, "copy_4x16"), (1, "copy_8x16"), (2, "copy_8x16_nt"), (3, "read_only")):
+        for mode, name in ((4, "copy_1x16"), (0, "copy_4x16"), (1, "copy_8x16"), (2, "copy_8x16_nt"),
+                           (3, "read_only")):
             ms = mb.lbm.probe_copy(nbytes, reps=3, mode=mode)
             moved = nbytes * (1 if mode == 3 else 2)
             res[name] = {"ms": round(ms, 4), "GBs": round(moved / (ms * 1e-3) / 1e9, 1)}

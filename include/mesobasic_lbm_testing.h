@@ -24,8 +24,9 @@ int lbm_probe_tile_map(int nty, int ntz, int ncx, int order, int* ty, int* tz, i
 /* Streaming reference kernels over `bytes` (multiple of 16) on two device
  * buffers it allocates, run `reps` times; *ms = average time per run.
  * mode 0: copy (4 x 16 B per thread), 1: copy (8 x 16 B), 2: copy with
- * non-temporal loads/stores, 3: read only.  The reference points for scaling
- * HBM traffic counters and for the achievable bandwidth. */
+ * non-temporal loads/stores, 3: read only, 4: copy (1 x 16 B per thread).
+ * The reference points for scaling HBM traffic counters and for the
+ * achievable bandwidth. */
 int lbm_probe_copy(int64_t bytes, int reps, int mode, double* ms);
 
 #ifdef __cplusplus
