@@ -261,11 +261,14 @@ int lbm_batch_tuning(const lbm_batch* b, int* d2q9_chunk, int* d2q9_rows_kernel)
 
 /* Sets the launch configuration (and stops the tuner): D2Q9 rows per work
  * item (>= 1), D2Q9 jobs on the row-marching kernel (1) or the two-pass
- * kernels (0), and D3Q19 jobs with nx >= 2, ny >= 8 and nz >= 64 on the
- * fused march kernel (1; the default for LBM_FP16N / LBM_BF16N) or the
- * two-pass kernels (0; the default for the other forms).  D2Q9 results do
- * not depend on it; D3Q19 results of the two kernels agree to FP32
- * rounding, not bit for bit, and the tuner never changes this choice.
+ * kernels (0), and the kernel of D3Q19 jobs: 0 the two-pass kernels, 2 the
+ * fused march kernel for every job with nx >= 2, ny >= 8 and nz >= 64, 1
+ * (default) the march for such jobs in the node-shift forms and the two
+ * passes in the others.  A job's D3Q19 kernel depends only on its shape and
+ * this mode.  D2Q9 results
+ * do not depend on the configuration; D3Q19 results of the two kernels
+ * agree to FP32 rounding, not bit for bit, and the tuner never changes the
+ * D3Q19 mode.
  * lbm_batch_get_launch reports the current configuration.  No reference
  * counterpart (the reference runs one lattice per process). */
 int lbm_batch_set_launch(lbm_batch* b, int d2q9_chunk, int d2q9_rows_kernel, int d3q19_march);

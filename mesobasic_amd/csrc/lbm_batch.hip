@@ -378,11 +378,8 @@ __global__ void __launch_bounds__(kTP) k_tp_collide(Tables t, int parity)
 // Its FP32 arithmetic is lp_prep / lp_out, not the expressions of the
 // two-pass kernels above, so the two agree to rounding, not bit for bit; a
 // D3Q19 job therefore always runs the same kernel, whatever the other jobs
-// or the tuner do.  Which one is a property of the batch (lbm_batch.h
-// default_march19: the march for the node-shift forms, where it measured
-// 1.5x faster than the two passes on 64..192-per-side jobs; the two passes
-// for the constant-shift forms, where partial tiles on such sides cost the
-// march more than the second pass costs the two-pass kernels).
+// or the tuner do: which one follows from the job's own shape and the
+// batch's mode (use_march() in lbm_batch_capi.cpp).
 
 template <typename SP, bool NS>
 // Occupancy target 2 wavefronts per SIMD: held to 128 VGPRs the item loop

@@ -88,7 +88,7 @@ struct lbm_batch {
     // launch configuration
     int chunk = 32;               // D2Q9 rows per row-kernel item
     int rows_kernel = 1;          // D2Q9 jobs on the row kernel (1) or two-pass (0)
-    int march19 = 0;              // D3Q19 jobs on the march kernel (1) or two-pass (0)
+    int march19 = 1;              // D3Q19 kernel mode: 0 two-pass, 1 by shape, 2 march where it fits
     int tune_left = 0;            // ticks left to try candidate configurations
     struct Config {
         int rows_kernel, chunk;
@@ -117,10 +117,21 @@ bool use_rows(const lbm_batch* b, const HostJob& j)
     return j.d.q == 9 && b->rows_kernel && row_class(j.d.ny) >= 0;
 }
 
+// D3Q19 kernel of a job -- a function of its own shape and the batch's mode
+// only, so a job runs the same kernel in any mix.  Mode 1 (default): the
+// march for the node-shift forms, where it measured faster on every job
+// shape tried (FP16, GLUPS, march vs two passes: 35.6 vs 14.7 on 128^3
+// jobs, 20.0 vs 11.1 on 64..192-per-side jobs); the two passes for the
+// other forms, where the march wins on tile-aligned jobs (39.4 vs 29.3)
+// but loses on ragged ones (21.9 vs 24.9) and, run for only part of a mix,
+// slowed the serving stream (an extra, under-filled launch per step).  2:
+// the march wherever it fits; 0: the two passes.
 bool use_march(const lbm_batch* b, const HostJob& j)
 {
-    return j.d.q == 19 && b->march19 && j.d.nx >= 2 && j.d.ny >= lbm::batch::kMarchTY &&
-           j.d.nz >= lbm::batch::kMarchTZ;
+    const bool fits = j.d.q == 19 && j.d.nx >= 2 && j.d.ny >= lbm::batch::kMarchTY &&
+                      j.d.nz >= lbm::batch::kMarchTZ;
+    if (!fits || b->march19 == 0) return false;
+    return b->march19 == 2 || node_shift(b->d.precision);
 }
 
 int ensure_items(lbm_batch* b, int cls, int n)
@@ -375,9 +386,6 @@ int lbm_batch_create(const lbm_batch_desc* desc, lbm_batch** out)
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return lbm::set_error(LBM_ENODEV, "lbm_batch_create: no HIP device");
     auto b = std::make_unique<lbm_batch>();
-    // D3Q19 kernel: the march for the node-shift forms, the two passes for
-    // the others (measured faster each way on 64..192-per-side jobs)
-    b->march19 = node_shift(desc->precision) ? 1 : 0;
     b->d = *desc;
     b->d.steps_per_tick += b->d.steps_per_tick & 1;   // even: every tick starts at parity 0
     int dev = 0;
@@ -595,7 +603,9 @@ int lbm_batch_set_launch(lbm_batch* b, int d2q9_chunk, int d2q9_rows_kernel, int
     if (d2q9_chunk < 1) return lbm::set_error(LBM_EINVAL, "lbm_batch_set_launch: chunk must be >= 1");
     b->chunk = d2q9_chunk;
     b->rows_kernel = d2q9_rows_kernel ? 1 : 0;
-    b->march19 = d3q19_march ? 1 : 0;
+    if (d3q19_march < 0 || d3q19_march > 2)
+        return lbm::set_error(LBM_EINVAL, "lbm_batch_set_launch: d3q19_march must be 0, 1 or 2");
+    b->march19 = d3q19_march;
     b->tune_left = 0;
     b->dirty = true;
     b->graph_valid = false;

@@ -373,7 +373,9 @@ class Batch:
         return dict(d2q9_chunk=a.value, d2q9_rows_kernel=b.value, d3q19_march=c.value)
 
     def set_launch(self, d2q9_chunk=32, d2q9_rows_kernel=1, d3q19_march=1):
-        """Fixes the launch configuration (stops the tuner); results do not
-        depend on it."""
+        """Fixes the launch configuration (stops the tuner).  d3q19_march: 0
+        two-pass, 1 the fused march for the node-shift forms (default), 2 the
+        march wherever it fits; D3Q19 results of the two kernels agree to
+        FP32 rounding."""
         _check(self._lib.lbm_batch_set_launch(self._h, int(d2q9_chunk), int(d2q9_rows_kernel),
                                               int(d3q19_march)))
