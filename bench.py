@@ -42,6 +42,7 @@ WORKLOADS = {
     "d3q19_fp16n_512_noise": (19, (512, 512, 512), mb.FP16N, 1.0, -5.0, ("noise", 1)),
     "d3q19_bf16n_512_noise": (19, (512, 512, 512), mb.BF16N, 1.0, -5.0, ("noise", 1)),
     "d2q9_fp64_256_noise": (9, (256, 256), mb.FP64, 1.0, -5.0, ("noise", 1)),
+    "d2q9_fp64_4096_noise": (9, (4096, 4096), mb.FP64, 1.0, -5.0, ("noise", 1)),
 }
 BYTES_PER_POP = {mb.FP64: 8, mb.FP32: 4, mb.FP16S: 2, mb.BF16S: 2, mb.FP16N: 2, mb.BF16N: 2}
 DTYPE = {mb.FP64: "f64", mb.FP32: "f32", mb.FP16S: "f32 compute / f16 storage",

@@ -46,6 +46,8 @@ enum lbm_precision {
     LBM_BF16N = 5      /* FP32 compute, BF16 node-shifted storage */
 };
 
+/* AUTO: the fused kernel where a tile shape fits (D2Q9: lattices of at
+ * least 2^18 nodes, where it measured faster), else the two passes. */
 enum lbm_kernel {
     LBM_KERNEL_AUTO = 0,
     LBM_KERNEL_TWO_PASS = 1, /* density pass, then force+collide+stream pass */
@@ -152,12 +154,15 @@ int lbm_synchronize(lbm_handle* h);
  *             2: 1x256 (whole z rows, nz must equal 512 / 256 / 256),
  *             3: 8x64, 4: 4x64, 5: 8x32; two nodes per thread (16-bit
  *             storage at tau = 1 without walls, nz a multiple of 128):
- *             6: 8x128, 7: 4x128; -1 = first that fits (a pair tile where
- *             one applies)
- *   chunk     x planes marched per workgroup
+ *             6: 8x128, 7: 4x128; D2Q9 FP64 (any ny): a segment of 8: 256,
+ *             9: 128, 10: 64 nodes of a y row; -1 = first that fits (a
+ *             pair tile where one applies; for D2Q9 the narrowest segment
+ *             that covers a row of <= 128 nodes, else 256).  Every fused
+ *             shape needs nx >= 2.
+ *   chunk     x planes (D2Q9: x rows) marched per workgroup
  *   order     block -> tile order: 0 plain, 1 contiguous runs per XCD,
  *             2 y-tiles of one column per XCD, 3 a y-band x all z-tiles
- *             per XCD
+ *             per XCD (D2Q9 segments: one fixed order, the value is kept)
  *   nt_store  1 = stores carry the non-temporal hint */
 typedef struct lbm_tuning {
     int shape;

@@ -209,8 +209,12 @@ int lbm_create(const lbm_desc* desc, lbm_handle** out)
             delete h;
             return fail(LBM_EINVAL, "fused FP64 kernel does not support this lattice");
         }
-        h->kernel = desc->kernel == LBM_KERNEL_AUTO ? (fused_ok ? LBM_KERNEL_FUSED : LBM_KERNEL_TWO_PASS)
-                                                    : desc->kernel;
+        // D2Q9: the automatic choice keeps small lattices on the two passes
+        const bool fused_auto =
+            fused_ok && (desc->q == 19 || h->g.n >= lbm::fused::kRowAutoMinNodes);
+        h->kernel = desc->kernel == LBM_KERNEL_AUTO
+                        ? (fused_auto ? LBM_KERNEL_FUSED : LBM_KERNEL_TWO_PASS)
+                        : desc->kernel;
     } else {
         h->kernel = lbm::lowp_resolve_kernel(desc->precision, h->a, h->g, desc->kernel);
         if (h->kernel < 0) {
@@ -219,12 +223,14 @@ int lbm_create(const lbm_desc* desc, lbm_handle** out)
         }
     }
     h->tune.shape = desc->precision == LBM_FP64
-                        ? lbm::fused::default_shape(h->g, desc->q)
+                        ? lbm::fp64_default_shape(h->g, desc->q)
                         : lbm::lowp_default_shape(desc->precision, h->a, h->g);
     // x planes per workgroup: 128 for the FP64 and the pair tiles (fewer
     // prologue planes; measured 1.5-2.3% faster than 64 at 256^3 and 512^3
     // on MI355X), 64 for the others (flat between 32 and 256)
     h->tune.chunk = (desc->precision == LBM_FP64 || lbm::fused::is_pair_shape(h->tune.shape)) ? 128 : 64;
+    if (lbm::fused::is_row_shape(h->tune.shape))
+        h->tune.chunk = lbm::fused::default_row_chunk(h->g, h->tune.shape);
     const int64_t pop_bytes = h->g.ps * desc->q * h->elem;
     auto cleanup = [&](int code) {
         lbm_destroy(h);
@@ -468,9 +474,9 @@ int lbm_set_tuning(lbm_handle* h, const lbm_tuning* t)
         return fail(LBM_EINVAL, "lbm_set_tuning: handle does not use the fused kernel");
     const bool fp64 = h->d.precision == LBM_FP64;
     const int shape = t->shape >= 0 ? t->shape
-                      : fp64    ? lbm::fused::default_shape(h->g, h->d.q)
+                      : fp64    ? lbm::fp64_default_shape(h->g, h->d.q)
                                 : lbm::lowp_default_shape(h->d.precision, h->a, h->g);
-    if (fp64 ? !lbm::fused::supported(h->g, h->d.q, shape)
+    if (fp64 ? !lbm::fp64_shape_ok(h->g, h->d.q, shape)
              : !lbm::lowp_shape_ok(h->d.precision, h->a, h->g, shape))
         return fail(LBM_EINVAL, "tile shape %d does not fit the %d x %d cross-section", shape, h->d.ny,
                     h->d.nz);

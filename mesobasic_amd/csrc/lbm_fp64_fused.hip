@@ -1,6 +1,7 @@
-// lbm_fp64_fused.hip -- FP64 instantiation of the single-pass D3Q19 kernel
-// (lbm_fused.h).  Compiled with -ffp-contract=off like lbm_fp64.hip, so its
-// results are bit-identical to the two-pass kernels and to the reference.
+// lbm_fp64_fused.hip -- FP64 single-pass kernels: the D3Q19 instantiation of
+// lbm_fused.h and the D2Q9 row march below.  Compiled with
+// -ffp-contract=off like lbm_fp64.hip, so their results are bit-identical to
+// the two-pass kernels and to the reference.
 #pragma clang fp contract(off)
 
 #include "lbm_fused.h"
@@ -8,9 +9,223 @@
 
 namespace lbm {
 
-bool fp64_fused_supported(const StepArgs& a, const Grid& g) { return fused::default_shape(g, a.q) >= 0; }
+int fp64_default_shape(const Grid& g, int q)
+{
+    return q == 9 ? fused::default_row_shape(g, q) : fused::default_shape(g, q);
+}
+
+bool fp64_shape_ok(const Grid& g, int q, int shape)
+{
+    return q == 9 ? fused::row_supported(g, q, shape) : fused::supported(g, q, shape);
+}
+
+bool fp64_fused_supported(const StepArgs& a, const Grid& g) { return fp64_default_shape(g, a.q) >= 0; }
+
+namespace fused {
+
+// ---- D2Q9 row march --------------------------------------------------------
+//
+// The D2Q9 step of LBfluid2D::updateFluid_SC (LBfluid2D.cpp:227-312) after its
+// first step, with the scmp3D force (scmp3D.cpp:141-179), in one pass.  A
+// workgroup owns TW consecutive nodes of a y row (node index x*ny + y, y
+// contiguous) and marches along x; LDS holds psi of rows x-1 .. x+2 for the
+// segment plus one node on each side.  Lanes past the row end (ny not a
+// multiple of TW, or ny < TW) take the periodically wrapped node, so their
+// psi is the right-hand neighbour the last real node needs; they store
+// nothing.
+
+// f of node (x, y): read in place, or pulled from x - e_n with the y-wall
+// substitution of LBfluid2D.cpp:374-376 (row 1) and :382-384 (row ny-2).
+template <bool WALLS>
+__device__ __forceinline__ void pull9(const double* __restrict__ P, const Grid& g, int kind, int x,
+                                      int y, double (&f)[9])
+{
+    using S = St<9>;
+    const uint32_t c = static_cast<uint32_t>(x) * static_cast<uint32_t>(g.ny) + static_cast<uint32_t>(y);
+    const NbrOff o(g, x, y, 0, kind == kHoldsF ? 0u : ~0u);
+    static_for<9>([&](auto I) {
+        constexpr int n = decltype(I)::value;
+        f[n] = *elem(P + n * g.ps, c + o.template to<-S::ex[n], -S::ey[n], 0>());
+    });
+    if (kind == kHoldsF) return;
+    if constexpr (WALLS) {
+        if (y == 1) {
+            static_for<S::nwall>([&](auto I) {
+                constexpr int m = decltype(I)::value;
+                f[S::wall_up[m]] = *elem(P + S::wall_dn[m] * g.ps, c);
+            });
+        }
+        if (y == g.ny - 2) {
+            static_for<S::nwall>([&](auto I) {
+                constexpr int m = decltype(I)::value;
+                f[S::wall_dn[m]] = *elem(P + S::wall_up[m] * g.ps, c);
+            });
+        }
+    }
+}
+
+// psi(rho) of pulled values, rho summed in direction order (scmp3D.cpp:187-190)
+__device__ __forceinline__ double psi9(const double (&f)[9])
+{
+    double r = 0.0;
+    static_for<9>([&](auto I) { r = r + f[decltype(I)::value]; });
+    return Psi<double>::of(r);
+}
+
+// Moments, force, BGK collision of one node and the store of its
+// post-collision values; the operation order of k_collide<9, *, 0> in
+// lbm_fp64.hip (LBfluid2D.cpp:227-312 with usq = u*u + v*v).  pm/p0/pp are
+// the psi rows x-1, x, x+1 and li the node's slot in them.
+template <bool TAU1, bool NTS>
+__device__ __forceinline__ void collide9(double* __restrict__ Pout, const Grid& g, uint32_t c,
+                                         const double (&f)[9], const double* pm, const double* p0,
+                                         const double* pp, int li, double tau, double G)
+{
+    using S = St<9>;
+    double rho = 0.0, sx = 0.0, sy = 0.0;
+    static_for<9>([&](auto I) {
+        constexpr int n = decltype(I)::value;
+        rho = rho + f[n];
+        acc_e<S::ex[n]>(sx, f[n]);
+        acc_e<S::ey[n]>(sy, f[n]);
+    });
+    const double u = sx / rho, v = sy / rho;
+    const double pc = p0[li];
+    double gx = 0.0, gy = 0.0;
+    static_for<9>([&](auto I) {
+        constexpr int n = decltype(I)::value;
+        if constexpr (n > 0) {
+            const double* pl = S::ex[n] < 0 ? pm : (S::ex[n] > 0 ? pp : p0);
+            const double t = S::w[n] * pl[li + S::ey[n]];
+            acc_e<S::ex[n]>(gx, t);
+            acc_e<S::ey[n]>(gy, t);
+        }
+    });
+    const double mgp = (-G) * pc;
+    const double Fx = mgp * gx, Fy = mgp * gy;
+    const double ueq = u + (tau * Fx) / rho;
+    const double veq = v + (tau * Fy) / rho;
+    const double usq15 = 1.5 * (ueq * ueq + veq * veq);
+    static_for<9>([&](auto I) {
+        constexpr int n = decltype(I)::value;
+        const double ev = edot<S::ex[n], S::ey[n], 0>(ueq, veq, 0.0);
+        const double feq = (((1.0 + 3.0 * ev) + (4.5 * ev) * ev) - usq15) * (rho * S::w[n]);
+        double out;
+        if constexpr (TAU1) out = f[n] - (f[n] - feq);   // x / 1.0 == x exactly
+        else out = f[n] - (f[n] - feq) / tau;
+        if constexpr (NTS) StF64::put_nt(elem(Pout + n * g.ps, c), out);
+        else *elem(Pout + n * g.ps, c) = out;
+    });
+}
+
+// Block b -> segment b % nty of row chunk b / nty (nty = ceil(ny / TW)).
+// The rows x0-1 .. x0+1 are pulled in a prologue; march step x pulls row
+// x+2 (in flight during the collision of row x), collides row x, writes
+// psi(x+2) and waits at one barrier.  Needs nx >= 2 (x+2 wraps by one
+// subtraction).  The barrier is skipped only where `more` is false, which
+// is uniform across the workgroup (x and x1 are).
+template <bool WALLS, bool TAU1, int TW, bool NTS>
+__global__ void __launch_bounds__(TW)
+    k_rows9(const double* __restrict__ Pin, int kind, double* __restrict__ Pout, Grid g, double tau,
+            double G, int chunk)
+{
+    constexpr int PL = TW + 2;
+    __shared__ double psi_lds[4 * PL];
+
+    const int nty = (g.ny + TW - 1) / TW;
+    const int ty = blockIdx.x % nty, cx = blockIdx.x / nty;
+    const int y0 = ty * TW;
+    const int x0 = cx * chunk;
+    const int x1 = min(g.nx, x0 + chunk);
+    const int tid = threadIdx.x;
+    const bool valid = y0 + tid < g.ny;
+    const int yi = (y0 + tid) % g.ny;
+    const int li = tid + 1;
+    // ring nodes: y0-1 (slot 0) on lane 0, y0+TW (slot TW+1) on the last lane
+    const bool ring = tid == 0 || tid == TW - 1;
+    const int ry = tid == 0 ? (y0 == 0 ? g.ny - 1 : y0 - 1) : (y0 + TW) % g.ny;
+    const int lr = tid == 0 ? 0 : TW + 1;
+
+    auto slot = [&](int x) { return ((x - x0 + 1) & 3) * PL; };
+    auto xw = [&](int x) { return x >= g.nx ? x - g.nx : (x < 0 ? x + g.nx : x); };
+    auto ring_psi = [&](int xp, int s) {
+        if (ring) {
+            double fr[9];
+            pull9<WALLS>(Pin, g, kind, xp, ry, fr);
+            psi_lds[s + lr] = psi9(fr);
+        }
+    };
+
+    double fa[9], fb[9], fc[9];
+    ring_psi(xw(x0 - 1), slot(x0 - 1));
+    pull9<WALLS>(Pin, g, kind, xw(x0 - 1), yi, fc);
+    psi_lds[slot(x0 - 1) + li] = psi9(fc);
+    ring_psi(x0, slot(x0));
+    pull9<WALLS>(Pin, g, kind, x0, yi, fa);
+    psi_lds[slot(x0) + li] = psi9(fa);
+    ring_psi(xw(x0 + 1), slot(x0 + 1));
+    pull9<WALLS>(Pin, g, kind, xw(x0 + 1), yi, fb);
+    psi_lds[slot(x0 + 1) + li] = psi9(fb);
+    __syncthreads();
+
+    // Row x+2 is needed while x+2 <= x1 (psi of row x1 enters the force of
+    // row x1-1); the last step of a chunk pulls nothing, so a one-row chunk
+    // is a single round of loads.
+    auto step = [&](int x, double (&fcur)[9], double (&fnn)[9]) {
+        const bool more = x + 2 <= x1;
+        const int x2 = xw(x + 2);
+        double fr[9];
+        if (more) {
+            pull9<WALLS>(Pin, g, kind, x2, yi, fnn);
+            if (ring) pull9<WALLS>(Pin, g, kind, x2, ry, fr);
+        }
+        if (valid) {
+            const uint32_t c = static_cast<uint32_t>(x) * static_cast<uint32_t>(g.ny) +
+                               static_cast<uint32_t>(yi);
+            collide9<TAU1, NTS>(Pout, g, c, fcur, psi_lds + slot(x - 1), psi_lds + slot(x),
+                                psi_lds + slot(x + 1), li, tau, G);
+        }
+        if (more) {
+            if (ring) psi_lds[slot(x + 2) + lr] = psi9(fr);
+            psi_lds[slot(x + 2) + li] = psi9(fnn);
+            __syncthreads();
+        }
+    };
+    int x = x0;
+    for (; x + 2 < x1; x += 3) {
+        step(x, fa, fc);
+        step(x + 1, fb, fa);
+        step(x + 2, fc, fb);
+    }
+    if (x < x1) step(x, fa, fc);
+    if (x + 1 < x1) step(x + 1, fb, fa);
+}
+
+}  // namespace fused
 
 namespace {
+
+template <bool W, bool T1, int TW, bool NTS>
+void launch_rows(const double* Pin, int kind, double* Pout, const Grid& g, const StepArgs& a,
+                 const FusedTuning& t, hipStream_t s)
+{
+    const int nty = (g.ny + TW - 1) / TW;
+    const int ncx = (g.nx + t.chunk - 1) / t.chunk;
+    const unsigned nb = static_cast<unsigned>(static_cast<int64_t>(nty) * ncx);
+    fused::k_rows9<W, T1, TW, NTS><<<nb, TW, 0, s>>>(Pin, kind, Pout, g, a.tau, a.G, t.chunk);
+}
+
+template <bool W, bool T1, bool NTS>
+void launch_row_shape(const double* Pin, int kind, double* Pout, const Grid& g, const StepArgs& a,
+                      const FusedTuning& t, hipStream_t s)
+{
+    using namespace fused;
+    switch (t.shape - kFirstRowShape) {
+    case 0: launch_rows<W, T1, kRowTW[0], NTS>(Pin, kind, Pout, g, a, t, s); break;
+    case 1: launch_rows<W, T1, kRowTW[1], NTS>(Pin, kind, Pout, g, a, t, s); break;
+    default: launch_rows<W, T1, kRowTW[2], NTS>(Pin, kind, Pout, g, a, t, s); break;
+    }
+}
 
 template <bool W, bool T1, typename TL, bool NTS>
 void launch(const double* Pin, int kind, double* Pout, const Grid& g, const StepArgs& a,
@@ -27,6 +242,10 @@ void launch_shape(const double* Pin, int kind, double* Pout, const Grid& g, cons
                   const FusedTuning& t, hipStream_t s)
 {
     using namespace fused;
+    if (a.q == 9) {
+        launch_row_shape<W, T1, NTS>(Pin, kind, Pout, g, a, t, s);
+        return;
+    }
     switch (t.shape) {
     case 0: launch<W, T1, Tile<kShapeTY[0], kShapeTZ[0], kShapeZF[0]>, NTS>(Pin, kind, Pout, g, a, t, s); break;
     case 1: launch<W, T1, Tile<kShapeTY[1], kShapeTZ[1], kShapeZF[1]>, NTS>(Pin, kind, Pout, g, a, t, s); break;
@@ -50,7 +269,7 @@ void launch_nts(const double* Pin, int kind, double* Pout, const Grid& g, const 
 hipError_t fp64_fused_step(const StepArgs& a, const Grid& g, const double* Pin, int kind,
                            double* Pout, const FusedTuning& t, hipStream_t s)
 {
-    if (!fused::supported(g, a.q, t.shape) || t.chunk < 1) return hipErrorNotSupported;
+    if (!fp64_shape_ok(g, a.q, t.shape) || t.chunk < 1) return hipErrorNotSupported;
     const bool tau1 = (a.tau == 1.0);   // x / 1.0 == x: skip 19 divides, same bits
     if (a.walls) {
         if (tau1) launch_nts<true, true>(Pin, kind, Pout, g, a, t, s);

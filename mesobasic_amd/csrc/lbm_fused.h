@@ -67,8 +67,9 @@ inline bool supported(const Grid& g, int q, int shape)
     // element offsets within a population plane are 32-bit byte offsets
     // (elem() below): planes of at most 2^29 nodes, for 8-byte values
     if (g.ps >= (int64_t(1) << 29)) return false;
+    // the march wraps x + 2 by one subtraction of nx: nx >= 2
     const int ty = kShapeTY[shape], tz = kShapeTZ[shape];
-    if (g.ny % ty != 0 || g.nx < 1) return false;
+    if (g.ny % ty != 0 || g.nx < 2) return false;
     return kShapeZF[shape] ? g.nz == tz : g.nz % tz == 0;
 }
 
@@ -97,7 +98,7 @@ inline bool pair_supported(const Grid& g, int q, int shape)
     if (q != 19 || !is_pair_shape(shape)) return false;
     if (g.ps >= (int64_t(1) << 29)) return false;
     const int i = shape - kNumShapes;
-    return g.nx >= 1 && g.ny % kPairTY[i] == 0 && g.nz % kPairTZ[i] == 0;
+    return g.nx >= 2 && g.ny % kPairTY[i] == 0 && g.nz % kPairTZ[i] == 0;
 }
 
 // Default pair tile for a lattice, in order of measured speed, or -1.
@@ -107,6 +108,54 @@ inline int default_pair_shape(const Grid& g, int q)
         if (pair_supported(g, q, kNumShapes + i)) return kNumShapes + i;
     return -1;
 }
+
+// D2Q9 row-segment shapes (tuning ids kNumShapes + kNumPairShapes + i): a
+// workgroup owns TW consecutive nodes of a y row (y is the contiguous axis
+// in 2-D) and marches along x.  Any ny: a segment that runs past the row
+// end wraps periodically and its extra lanes store nothing.
+constexpr int kNumRowShapes = 3;
+constexpr int kRowTW[kNumRowShapes] = {256, 128, 64};
+constexpr int kFirstRowShape = kNumShapes + kNumPairShapes;
+
+inline bool is_row_shape(int shape)
+{
+    return shape >= kFirstRowShape && shape < kFirstRowShape + kNumRowShapes;
+}
+
+inline bool row_supported(const Grid& g, int q, int shape)
+{
+    return q == 9 && is_row_shape(shape) && g.nz == 1 && g.nx >= 2 && g.ny >= 1 &&
+           g.ps < (int64_t(1) << 29);
+}
+
+// Default segment: the narrowest that covers a short row, else 256 nodes.
+inline int default_row_shape(const Grid& g, int q)
+{
+    if (q != 9) return -1;
+    const int i = g.ny <= 64 ? 2 : (g.ny <= 128 ? 1 : 0);
+    return row_supported(g, q, kFirstRowShape + i) ? kFirstRowShape + i : -1;
+}
+
+// Default x rows per workgroup for a row segment: nx/256 in [1, 32] (about
+// 256 row chunks), raised to 32 where that still gives 2048 workgroups.  A
+// chunk re-pulls two boundary rows, so long chunks save traffic once the
+// launch is wide enough.  Measured on MI355X (square lattices, 256-node
+// segments) within 3% of the best chunk from 512^2 to 8192^2; fixed
+// workgroup-count targets were not (8192^2 lost 13% at chunk 64).
+inline int default_row_chunk(const Grid& g, int shape)
+{
+    if (!is_row_shape(shape)) return 64;
+    const int tw = kRowTW[shape - kFirstRowShape];
+    const int64_t nty = (g.ny + tw - 1) / tw;
+    int chunk = g.nx / 256 < 1 ? 1 : (g.nx / 256 > 32 ? 32 : g.nx / 256);
+    if (nty * ((g.nx + 31) / 32) >= 2048) chunk = 32;
+    return chunk;
+}
+
+// Smallest D2Q9 lattice the automatic kernel choice runs fused: below it
+// the march's dependent load rounds cost more than the second pass
+// (MI355X, FP64: 256^2 two-pass 8.8 vs fused 7.3 GLUPS; 512^2 18.2 vs 19.5).
+constexpr int64_t kRowAutoMinNodes = int64_t(1) << 18;
 
 // Ring slot r -> (row, col) in LDS coordinates.  Rows 0 and LY-1 are the y
 // ring; without ZF, columns 0 and LZ-1 are the z ring.

@@ -56,6 +56,10 @@ hipError_t fp64_collide(const StepArgs& a, const Grid& g, const double* Pin, int
 hipError_t fp64_fused_step(const StepArgs& a, const Grid& g, const double* Pin, int kind,
                            double* Pout, const FusedTuning& t, hipStream_t s);
 bool fp64_fused_supported(const StepArgs& a, const Grid& g);
+// Default fused tile shape of an FP64 lattice (-1: none fits) and whether a
+// shape fits: D3Q19 tiles (lbm_fused.h) or D2Q9 row segments.
+int fp64_default_shape(const Grid& g, int q);
+bool fp64_shape_ok(const Grid& g, int q, int shape);
 // Shan-Chen force from a psi field (FP64, reference arithmetic).
 hipError_t fp64_force(const StepArgs& a, const Grid& g, const double* psi, double* fx, double* fy,
                       double* fz, hipStream_t s);

@@ -153,6 +153,56 @@ def test_fused_every_tile_shape_and_chunk_bitwise(shape):
     assert tried >= 4
 
 
+ROW_SHAPES = (8, 9, 10)     # D2Q9 row segments of 256, 128 and 64 nodes
+
+
+@pytest.mark.parametrize("shape,order", [((40, 300), 3), ((9, 70), 2), ((6, 5), 3),
+                                         ((2, 129), 3), ((31, 256), 2)])
+def test_fused_rows9_every_segment_and_chunk_bitwise(shape, order):
+    # D2Q9 row march vs the two-pass kernels: segments longer than the row
+    # (wrapped lanes), ragged last segments (300 = 256 + 44), nx = 2, walls
+    # on rows 1 and ny-2, tau != 1; order 2 runs the LBfluid2D first step
+    # two-pass and the rest fused
+    rho0 = mb.noise_rho(shape, 23)
+    walls = shape[1] >= 3
+    ref = mb.Lattice(9, shape, 0.8, -5.0, order=order, kernel=mb.KERNEL_TWO_PASS, walls=walls)
+    ref.set_rho(rho0); ref.init(); ref.step(11)
+    want = ref.get_f()
+    ref.close()
+    for tile in ROW_SHAPES:
+        for chunk, nts in ((1, 0), (2, 1), (3, 1), (7, 0), (64, 1)):
+            print(f"shape {shape} tile {tile} chunk {chunk} nts {nts}", flush=True)
+            lat = mb.Lattice(9, shape, 0.8, -5.0, order=order, kernel=mb.KERNEL_FUSED, walls=walls)
+            lat.set_tuning(tile, chunk, 3, nts)
+            assert lat.tuning == (tile, chunk, 3, nts)
+            lat.set_rho(rho0); lat.init(); lat.step(11)
+            assert same_bits(lat.get_f(), want), (tile, chunk)
+            lat.close()
+
+
+def test_fused_rows9_tau1_long_run_matches_oracle():
+    # tau = 1 instantiation over many steps, default launch configuration
+    shape = (48, 80)
+    rho0 = mb.noise_rho(shape, 5)
+    o = oracle.Oracle(9, shape, 1.0, -5.0, order=2)
+    o.set_rho(rho0); o.init(); o.step(150)
+    lat = mb.Lattice(9, shape, 1.0, -5.0, order=2, kernel=mb.KERNEL_FUSED)
+    assert lat.kernel == mb.KERNEL_FUSED
+    lat.set_rho(rho0); lat.init(); lat.step(150)
+    assert same_bits(lat.get_f(), o.fields()["f"])
+    lat.close()
+
+
+@pytest.mark.parametrize("q,shape", [(9, (1, 64)), (19, (1, 8, 64))])
+def test_fused_refuses_a_single_x_plane(q, shape):
+    # the march wraps x+2 with one subtraction of nx, so nx = 1 stays two-pass
+    with pytest.raises(mb.LbmError):
+        mb.Lattice(q, shape, 1.0, -5.0, kernel=mb.KERNEL_FUSED)
+    lat = mb.Lattice(q, shape, 1.0, -5.0)
+    assert lat.kernel == mb.KERNEL_TWO_PASS
+    lat.close()
+
+
 def test_set_f_roundtrip_and_restart():
     shape = (8, 6, 5)
     rho0 = mb.noise_rho(shape, 4)
