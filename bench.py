@@ -139,8 +139,9 @@ def main():
         print(json.dumps(res), flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the lattice would shard along x; see DESIGN.md).", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path (the lattice "
+              "would shard into x slabs with a one-plane exchange per face and step).",
+              file=sys.stderr)
         sys.exit(2)
 
     q, shape, prec, tau, G, init = WORKLOADS[args.workload]
