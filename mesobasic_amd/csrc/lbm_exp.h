@@ -24,9 +24,11 @@
 #if defined(__HIP__)
 #include <hip/hip_runtime.h>
 #define LBM_EXP_FN __device__ __forceinline__
+#define LBM_EXP_MEMBER __device__ __forceinline__
 #define LBM_EXP_TAB __device__ __constant__ static const
 #else  // host build of the same code, used to test the algorithm on a CPU
 #define LBM_EXP_FN static inline
+#define LBM_EXP_MEMBER inline
 #define LBM_EXP_TAB static const
 #endif
 
@@ -36,7 +38,9 @@ LBM_EXP_TAB uint64_t kExpTab[256] = {
 #include "exp_table.inc"
 };
 
-LBM_EXP_FN double exp_libm(double x)
+// exp_libm with the table read through tab(i) (e.g. a copy in LDS, which a
+// kernel fills once per workgroup); same bits as exp_libm.
+template <typename Tab> LBM_EXP_FN double exp_libm_tab(double x, const Tab& tab)
 {
     constexpr double kInvLn2N = 0x1.71547652b82fep0 * 128.0;
     constexpr double kShift = 0x1.8p52;
@@ -65,8 +69,8 @@ LBM_EXP_FN double exp_libm(double x)
     const double r = __builtin_fma(kd, kNegLn2loN, __builtin_fma(kd, kNegLn2hiN, x));
     const uint32_t idx = 2u * static_cast<uint32_t>(ki & 127u);
     const uint64_t top = ki << 45;
-    const double tail = __builtin_bit_cast(double, kExpTab[idx]);
-    uint64_t sbits = kExpTab[idx + 1] + top;
+    const double tail = __builtin_bit_cast(double, tab(idx));
+    uint64_t sbits = tab(idx + 1) + top;
     const double r2 = r * r;
     const double tmp = __builtin_fma(r2 * r2, __builtin_fma(r, kC5, kC4),
                                      __builtin_fma(r2, __builtin_fma(r, kC3, kC2), tail + r));
@@ -91,5 +95,11 @@ LBM_EXP_FN double exp_libm(double x)
     const double scale = __builtin_bit_cast(double, sbits);
     return __builtin_fma(scale, tmp, scale);
 }
+
+struct ExpTabConst {
+    LBM_EXP_MEMBER uint64_t operator()(uint32_t i) const { return kExpTab[i]; }
+};
+
+LBM_EXP_FN double exp_libm(double x) { return exp_libm_tab(x, ExpTabConst{}); }
 
 }  // namespace lbm

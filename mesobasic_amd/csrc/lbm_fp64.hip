@@ -150,8 +150,9 @@ __global__ void __launch_bounds__(kBlock) k_psi_pops(const double* __restrict__ 
 // Force + collision (+ the pull stream that produced f).  MODE 0: moments of
 // f; MODE 1: rho = rho0, u = 0 (LBfluid2D's first step uses the stored
 // initial arrays, LBfluid2D.cpp:238-240 after setFtoFeq); MODE 2: initial
-// equilibrium (LBfluid3D::setFtoFeq, LBfluid3D.cpp:180-200).
-template <int Q, bool WALLS, int MODE>
+// equilibrium (LBfluid3D::setFtoFeq, LBfluid3D.cpp:180-200).  TAU1: tau is
+// exactly 1.0, so (f - feq)/tau == f - feq bit for bit and the divides go.
+template <int Q, bool WALLS, int MODE, bool TAU1 = false>
 __global__ void __launch_bounds__(kBlock)
     k_collide(const double* __restrict__ Pin, int kind, double* __restrict__ Pout,
               const double* __restrict__ psi, const double* __restrict__ rho0, Grid g,
@@ -213,6 +214,7 @@ __global__ void __launch_bounds__(kBlock)
         const double feq = (((1.0 + 3.0 * ev) + (4.5 * ev) * ev) - usq15) * (rho * S::w[n]);
         double out;
         if constexpr (MODE == 2) out = feq;
+        else if constexpr (TAU1) out = f[n] - (f[n] - feq);
         else out = f[n] - (f[n] - feq) / tau;
         Pout[n * g.ps + c] = out;
     });
@@ -359,8 +361,13 @@ hipError_t fp64_collide(const StepArgs& a, const Grid& g, const double* Pin, int
                         hipStream_t s)
 {
     const unsigned nb = blocks_for(g.n);
-    if (mode == 0)
+    const bool t1 = a.tau == 1.0;
+    if (mode == 0 && t1)
+        LBM_DISPATCH_QW(a, (k_collide<Q, W, 0, true><<<nb, kBlock, 0, s>>>(Pin, kind, Pout, psi, rho0, g, a.tau, a.G)));
+    else if (mode == 0)
         LBM_DISPATCH_QW(a, (k_collide<Q, W, 0><<<nb, kBlock, 0, s>>>(Pin, kind, Pout, psi, rho0, g, a.tau, a.G)));
+    else if (mode == 1 && t1)
+        LBM_DISPATCH_QW(a, (k_collide<Q, W, 1, true><<<nb, kBlock, 0, s>>>(Pin, kind, Pout, psi, rho0, g, a.tau, a.G)));
     else if (mode == 1)
         LBM_DISPATCH_QW(a, (k_collide<Q, W, 1><<<nb, kBlock, 0, s>>>(Pin, kind, Pout, psi, rho0, g, a.tau, a.G)));
     else

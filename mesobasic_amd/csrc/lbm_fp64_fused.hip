@@ -64,12 +64,13 @@ __device__ __forceinline__ void pull9(const double* __restrict__ P, const Grid& 
     }
 }
 
-// psi(rho) of pulled values, rho summed in direction order (scmp3D.cpp:187-190)
-__device__ __forceinline__ double psi9(const double (&f)[9])
+// psi(rho) of pulled values, rho summed in direction order (scmp3D.cpp:187-190);
+// tab: the exp table in LDS
+__device__ __forceinline__ double psi9(const double (&f)[9], const uint64_t* tab)
 {
     double r = 0.0;
     static_for<9>([&](auto I) { r = r + f[decltype(I)::value]; });
-    return Psi<double>::of(r);
+    return Psi<double>::of(r, tab);
 }
 
 // Moments, force, BGK collision of one node and the store of its
@@ -131,6 +132,7 @@ __global__ void __launch_bounds__(TW)
 {
     constexpr int PL = TW + 2;
     __shared__ double psi_lds[4 * PL];
+    __shared__ uint64_t exp_tab[256];
 
     const int nty = (g.ny + TW - 1) / TW;
     const int ty = blockIdx.x % nty, cx = blockIdx.x / nty;
@@ -148,24 +150,30 @@ __global__ void __launch_bounds__(TW)
 
     auto slot = [&](int x) { return ((x - x0 + 1) & 3) * PL; };
     auto xw = [&](int x) { return x >= g.nx ? x - g.nx : (x < 0 ? x + g.nx : x); };
-    auto ring_psi = [&](int xp, int s) {
-        if (ring) {
-            double fr[9];
-            pull9<WALLS>(Pin, g, kind, xp, ry, fr);
-            psi_lds[s + lr] = psi9(fr);
-        }
-    };
 
+    // prologue: the three own rows' pulls are issued before the (divergent)
+    // ring block, so all loads are in flight together; a ring block between
+    // them kept the compiler from hoisting the later pulls and serialised
+    // six load -> exp round trips, which dominates short chunks
     double fa[9], fb[9], fc[9];
-    ring_psi(xw(x0 - 1), slot(x0 - 1));
-    pull9<WALLS>(Pin, g, kind, xw(x0 - 1), yi, fc);
-    psi_lds[slot(x0 - 1) + li] = psi9(fc);
-    ring_psi(x0, slot(x0));
+    const int xm = xw(x0 - 1), xp = xw(x0 + 1);
+    fill_exp_tab(exp_tab, tid, TW);
+    pull9<WALLS>(Pin, g, kind, xm, yi, fc);
     pull9<WALLS>(Pin, g, kind, x0, yi, fa);
-    psi_lds[slot(x0) + li] = psi9(fa);
-    ring_psi(xw(x0 + 1), slot(x0 + 1));
-    pull9<WALLS>(Pin, g, kind, xw(x0 + 1), yi, fb);
-    psi_lds[slot(x0 + 1) + li] = psi9(fb);
+    pull9<WALLS>(Pin, g, kind, xp, yi, fb);
+    __syncthreads();   // exp table in LDS
+    if (ring) {
+        double r0[9], r1[9], r2[9];
+        pull9<WALLS>(Pin, g, kind, xm, ry, r0);
+        pull9<WALLS>(Pin, g, kind, x0, ry, r1);
+        pull9<WALLS>(Pin, g, kind, xp, ry, r2);
+        psi_lds[slot(x0 - 1) + lr] = psi9(r0, exp_tab);
+        psi_lds[slot(x0) + lr] = psi9(r1, exp_tab);
+        psi_lds[slot(x0 + 1) + lr] = psi9(r2, exp_tab);
+    }
+    psi_lds[slot(x0 - 1) + li] = psi9(fc, exp_tab);
+    psi_lds[slot(x0) + li] = psi9(fa, exp_tab);
+    psi_lds[slot(x0 + 1) + li] = psi9(fb, exp_tab);
     __syncthreads();
 
     // Row x+2 is needed while x+2 <= x1 (psi of row x1 enters the force of
@@ -186,8 +194,8 @@ __global__ void __launch_bounds__(TW)
                                 psi_lds + slot(x + 1), li, tau, G);
         }
         if (more) {
-            if (ring) psi_lds[slot(x + 2) + lr] = psi9(fr);
-            psi_lds[slot(x + 2) + li] = psi9(fnn);
+            if (ring) psi_lds[slot(x + 2) + lr] = psi9(fr, exp_tab);
+            psi_lds[slot(x + 2) + li] = psi9(fnn, exp_tab);
             __syncthreads();
         }
     };

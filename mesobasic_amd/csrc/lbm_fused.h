@@ -252,13 +252,30 @@ __device__ __forceinline__ void pull(const typename SP::T* __restrict__ P, const
     }
 }
 
+// The exp table read from a copy in LDS (filled once per workgroup by
+// fill_exp_tab): an LDS read instead of a global gather per psi.
+struct ExpTabLds {
+    const uint64_t* t;
+    __device__ __forceinline__ uint64_t operator()(uint32_t i) const { return t[i]; }
+};
+
+__device__ __forceinline__ void fill_exp_tab(uint64_t* lds, int tid, int nthreads)
+{
+    for (int i = tid; i < 256; i += nthreads) lds[i] = kExpTab[i];
+}
+
 template <typename CT> struct Psi;
 template <> struct Psi<double> {
     // scmp3D.cpp:187-190 with the libm-faithful exp
     static __device__ __forceinline__ double of(double rho) { return 1.0 - exp_libm(-rho); }
+    static __device__ __forceinline__ double of(double rho, const uint64_t* tab)
+    {
+        return 1.0 - exp_libm_tab(-rho, ExpTabLds{tab});
+    }
 };
 template <> struct Psi<float> {
     static __device__ __forceinline__ float of(float rho) { return 1.0f - __expf(-rho); }
+    static __device__ __forceinline__ float of(float rho, const uint64_t*) { return of(rho); }
 };
 
 // rho of pulled values: sum f (reference order), or rs + sum d.
@@ -418,7 +435,13 @@ __global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(Occ
             Grid g, CT tau, CT G, CT rho_shift, int chunk, int order)
 {
     constexpr int TY = TL::TY, TZ = TL::TZ, LZ = TL::LZ, PL = TL::LY * TL::LZ;
+    constexpr bool kTab = std::is_same_v<CT, double>;
     __shared__ CT psi_lds[4 * PL];
+    __shared__ uint64_t exp_tab[kTab ? 256 : 1];
+    if constexpr (kTab) {
+        fill_exp_tab(exp_tab, threadIdx.x, TL::NT);
+        __syncthreads();
+    }
 
     const int nty = g.ny / TY, ntz = g.nz / TZ;
     const int ncx = (g.nx + chunk - 1) / chunk;
@@ -456,7 +479,7 @@ __global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(Occ
     // LDS slot of plane x: (x - x0 + 1) mod 4
     auto slot = [&](int x) { return ((x - x0 + 1) & 3) * PL; };
     auto xw = [&](int x) { return x >= g.nx ? x - g.nx : (x < 0 ? x + g.nx : x); };
-    auto psi_of = [&](const CT (&v)[19]) { return Psi<CT>::of(rho_of<EXACT>(v, rho_shift)); };
+    auto psi_of = [&](const CT (&v)[19]) { return Psi<CT>::of(rho_of<EXACT>(v, rho_shift), exp_tab); };
     auto ring_psi = [&](int xp, int s) {
 #pragma unroll
         for (int k = 0; k < TL::RPT; k++) {
