@@ -132,6 +132,10 @@ __global__ void __launch_bounds__(T) k_d2q9_rows(Tables t, int parity)
         const int nx = job.nx, ny = job.ny;
         const int64_t ps = job.ps;
         const float tau = job.tau, G = job.G, itau = 1.0f / tau;
+        // constant shift: one register for the whole item (the row values
+        // below then all name it; re-reading job.rs per row, which the
+        // stores may alias, cost 5 VGPRs and a wave of occupancy)
+        const float rs = job.rs;
         const bool first = job.first != 0;
         const float* rho0 = job.rho0;
         const bool active = y < ny;
@@ -154,7 +158,8 @@ __global__ void __launch_bounds__(T) k_d2q9_rows(Tables t, int parity)
         };
         auto pullR = [&](int xr, float (&d)[9]) {
             pull_row9<SP>(Pin, ps, nx, ny, xr, y, d);
-            return ref<NS>(job, p, static_cast<int64_t>(xr) * ny + y);
+            if constexpr (NS) return job.R[p][static_cast<int64_t>(xr) * ny + y];
+            else return rs;
         };
         float fa[9], fb[9], fc[9];
         float Ra = 0.f, Rb = 0.f, Rc = 0.f;
@@ -180,10 +185,9 @@ __global__ void __launch_bounds__(T) k_d2q9_rows(Tables t, int parity)
                     acc_e<S::ex[n]>(jx, fcur[n]);
                     acc_e<S::ey[n]>(jy, fcur[n]);
                 });
-                const int64_t c = static_cast<int64_t>(x) * ny + y;
                 float rho, ux, uy;
                 if (first && rho0) {
-                    rho = rho0[c];
+                    rho = rho0[static_cast<int64_t>(x) * ny + y];
                     ux = 0.f; uy = 0.f;
                 } else {
                     rho = pulled_rho<9, SP, NS>(fcur, Rcur);
@@ -205,6 +209,7 @@ __global__ void __launch_bounds__(T) k_d2q9_rows(Tables t, int parity)
                 const float k = (-G * p0[y]) * tau / rho;
                 const float ue = ux + k * gx, ve = uy + k * gy;
                 const float usq15 = 1.5f * (ue * ue + ve * ve);
+                const int64_t c = static_cast<int64_t>(x) * ny + y;
                 static_for<9>([&](auto I) {
                     constexpr int n = decltype(I)::value;
                     const float ev = edot<S::ex[n], S::ey[n], 0>(ue, ve, 0.f);
