@@ -230,7 +230,9 @@ int64_t lbm_batch_job_bytes(const lbm_batch* b, const lbm_job_desc* job);
 /* Admits a job with initial density rho0 (host, nx*ny*nz doubles, zero
  * velocity).  Returns LBM_ENOMEM, with nothing allocated, if the job does
  * not fit the remaining budget: the caller keeps it queued and retries
- * after jobs finish.  The job starts with the next tick. */
+ * after jobs finish.  The job starts with the next tick.  A job holds fewer
+ * than 2^30 nodes (LBM_EINVAL otherwise; larger lattices belong on a
+ * single-lattice handle). */
 int lbm_batch_submit(lbm_batch* b, const lbm_job_desc* job, const double* rho0, int64_t* job_id);
 
 /* Runs one tick (steps_per_tick steps of every unfinished job; a job stops

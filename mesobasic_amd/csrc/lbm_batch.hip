@@ -267,6 +267,46 @@ __device__ __forceinline__ void pull_node(const typename SP::T* __restrict__ P, 
     });
 }
 
+// The two-pass kernels address in 32-bit node indices: a job holds fewer
+// than 2^30 nodes (lbm_batch_submit), so byte offsets of the 2- and 4-byte
+// values fit 32 bits and every gather takes the SGPR-base + 32-bit VGPR
+// offset form (fused::elem) off the job's wave-uniform buffer pointers.
+// 64-bit node arithmetic (two 64-bit divides per node, a 64-bit
+// multiply-add per gather) held 2 VGPRs per address.
+template <int Q>
+__device__ __forceinline__ void decompose32(uint32_t c, const Job& j, int& x, int& y, int& z)
+{
+    const uint32_t ny = static_cast<uint32_t>(j.ny);
+    uint32_t t = c;
+    z = 0;
+    if constexpr (Q == 19) {
+        const uint32_t nz = static_cast<uint32_t>(j.nz);
+        t = c / nz;
+        z = static_cast<int>(c - t * nz);
+    }
+    x = static_cast<int>(t / ny);
+    y = static_cast<int>(t - static_cast<uint32_t>(x) * ny);
+}
+
+__device__ __forceinline__ fused::NbrOff neighbour_offsets(const Job& j, int x, int y, int z)
+{
+    Grid g;
+    g.nx = j.nx; g.ny = j.ny; g.nz = j.nz; g.n = j.n; g.ps = j.ps;
+    return fused::NbrOff(g, x, y, z);
+}
+
+// Pulled stored values of node c: population n from node c - e_n.
+template <int Q, typename SP>
+__device__ __forceinline__ void pull_node32(const typename SP::T* __restrict__ P, const Job& j,
+                                            uint32_t c, const fused::NbrOff& o, float (&d)[Q])
+{
+    using S = St<Q>;
+    static_for<Q>([&](auto I) {
+        constexpr int n = decltype(I)::value;
+        d[n] = SP::d(*fused::elem(P + n * j.ps, c + o.template to<-S::ex[n], -S::ey[n], -S::ez[n]>()));
+    });
+}
+
 constexpr int kTP = 256;
 
 template <int Q, typename SP, bool NS>
@@ -280,18 +320,21 @@ __global__ void __launch_bounds__(kTP) k_tp_psi(Tables t, int parity)
         if (job.remaining <= 0) continue;
         const int p = parity ^ job.phase;
         const auto* Pin = static_cast<const typename SP::T*>(job.P[p]);
-        for (int64_t c = item.c0 + threadIdx.x; c < item.c1; c += kTP) {
+        const uint32_t c1 = static_cast<uint32_t>(item.c1);
+        for (uint32_t c = static_cast<uint32_t>(item.c0) + threadIdx.x; c < c1; c += kTP) {
             const bool from_rho0 = job.first && job.rho0;
             float rp = 0.f;
             if (!from_rho0 || NS) {
                 int x, y, z;
-                decompose<Q>(c, job, x, y, z);
+                decompose32<Q>(c, job, x, y, z);
                 float d[Q];
-                pull_node<Q, SP>(Pin, job, x, y, z, d);
-                rp = pulled_rho<Q, SP, NS>(d, ref<NS>(job, p, c));
+                pull_node32<Q, SP>(Pin, job, c, neighbour_offsets(job, x, y, z), d);
+                float R = job.rs;
+                if constexpr (NS) R = *fused::elem(job.R[p], c);
+                rp = pulled_rho<Q, SP, NS>(d, R);
             }
-            job.psi[c] = psi_of(from_rho0 ? job.rho0[c] : rp);
-            if constexpr (NS) job.R[p ^ 1][c] = rp;
+            *fused::elem(job.psi, c) = psi_of(from_rho0 ? *fused::elem(job.rho0, c) : rp);
+            if constexpr (NS) *fused::elem(job.R[p ^ 1], c) = rp;
         }
     }
 }
@@ -311,12 +354,16 @@ __global__ void __launch_bounds__(kTP) k_tp_collide(Tables t, int parity)
         auto* Pout = static_cast<typename SP::T*>(job.P[p ^ 1]);
         const float tau = job.tau, G = job.G, itau = 1.0f / tau;
         const float* psi = job.psi;
-        for (int64_t c = item.c0 + threadIdx.x; c < item.c1; c += kTP) {
+        const float rs = job.rs;
+        const uint32_t c1 = static_cast<uint32_t>(item.c1);
+        for (uint32_t c = static_cast<uint32_t>(item.c0) + threadIdx.x; c < c1; c += kTP) {
             int x, y, z;
-            decompose<Q>(c, job, x, y, z);
+            decompose32<Q>(c, job, x, y, z);
+            const fused::NbrOff o = neighbour_offsets(job, x, y, z);
             float d[Q];
-            pull_node<Q, SP>(Pin, job, x, y, z, d);
-            const float Rr = ref<NS>(job, p, c);
+            pull_node32<Q, SP>(Pin, job, c, o, d);
+            float Rr = rs;
+            if constexpr (NS) Rr = *fused::elem(job.R[p], c);
             float jx = 0.f, jy = 0.f, jz = 0.f;
             static_for<Q>([&](auto I) {
                 constexpr int n = decltype(I)::value;
@@ -326,7 +373,7 @@ __global__ void __launch_bounds__(kTP) k_tp_collide(Tables t, int parity)
             });
             float rho, ux, uy, uz;
             if (job.first && job.rho0) {
-                rho = job.rho0[c];
+                rho = *fused::elem(job.rho0, c);
                 ux = uy = uz = 0.f;
             } else {
                 rho = pulled_rho<Q, SP, NS>(d, Rr);
@@ -334,25 +381,22 @@ __global__ void __launch_bounds__(kTP) k_tp_collide(Tables t, int parity)
                 const float js = 1.0f / kScale<SP, NS>;
                 ux = (jx * js) * ir; uy = (jy * js) * ir; uz = (jz * js) * ir;
             }
-            const int xs[3] = {x == 0 ? job.nx - 1 : x - 1, x, x == job.nx - 1 ? 0 : x + 1};
-            const int ys[3] = {y == 0 ? job.ny - 1 : y - 1, y, y == job.ny - 1 ? 0 : y + 1};
-            const int zs[3] = {z == 0 ? job.nz - 1 : z - 1, z, z == job.nz - 1 ? 0 : z + 1};
+            // node c + e_n
             auto nb = [&](auto N) {
                 constexpr int n = decltype(N)::value;
-                return (static_cast<int64_t>(xs[1 + S::ex[n]]) * job.ny + ys[1 + S::ey[n]]) * job.nz +
-                       zs[1 + S::ez[n]];
+                return c + o.template to<S::ex[n], S::ey[n], S::ez[n]>();
             };
             float gx = 0.f, gy = 0.f, gz = 0.f;
             static_for<Q>([&](auto I) {
                 constexpr int n = decltype(I)::value;
                 if constexpr (n > 0) {
-                    const float tt = static_cast<float>(S::w[n]) * psi[nb(I)];
+                    const float tt = static_cast<float>(S::w[n]) * *fused::elem(psi, nb(I));
                     acc_e<S::ex[n]>(gx, tt);
                     acc_e<S::ey[n]>(gy, tt);
                     acc_e<S::ez[n]>(gz, tt);
                 }
             });
-            const float k = (-G * psi[c]) * tau / rho;
+            const float k = (-G * *fused::elem(psi, c)) * tau / rho;
             const float ue = ux + k * gx, ve = uy + k * gy, we = (S::D == 3) ? uz + k * gz : 0.f;
             const float usq15 = 1.5f * (ue * ue + ve * ve + we * we);
             static_for<Q>([&](auto I) {
@@ -360,10 +404,10 @@ __global__ void __launch_bounds__(kTP) k_tp_collide(Tables t, int parity)
                 const float ev = edot<S::ex[n], S::ey[n], S::ez[n]>(ue, ve, we);
                 const float poly = 1.0f + 3.0f * ev + 4.5f * ev * ev - usq15;
                 float Rw = 0.f;
-                if constexpr (NS) Rw = job.R[p ^ 1][nb(I)];
+                if constexpr (NS) Rw = *fused::elem(job.R[p ^ 1], nb(I));
                 const float v = post_value<SP, NS>(d[n], static_cast<float>(S::w[n]), rho, poly, Rr, Rw,
                                                    itau);
-                SP::put_nt(Pout + n * job.ps + c, SP::of_d(v));
+                SP::put_nt(fused::elem(Pout + n * job.ps, c), SP::of_d(v));
             });
         }
     }

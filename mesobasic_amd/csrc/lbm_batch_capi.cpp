@@ -119,13 +119,12 @@ bool use_rows(const lbm_batch* b, const HostJob& j)
 
 // D3Q19 kernel of a job -- a function of its own shape and the batch's mode
 // only, so a job runs the same kernel in any mix.  Mode 1 (default): the
-// march for the node-shift forms, where it measured faster on every job
-// shape tried (FP16, GLUPS, march vs two passes: 35.6 vs 14.7 on 128^3
-// jobs, 20.0 vs 11.1 on 64..192-per-side jobs); the two passes for the
-// other forms, where the march wins on tile-aligned jobs (39.4 vs 29.3)
-// but loses on ragged ones (21.9 vs 24.9) and, run for only part of a mix,
-// slowed the serving stream (an extra, under-filled launch per step).  2:
-// the march wherever it fits; 0: the two passes.
+// march for the node-shift forms (FP16, GLUPS, march vs two passes: 35.7 vs
+// 27.4 on 128^3 jobs, 20.0 vs 22.5 on 64..192-per-side jobs); the two
+// passes for the other forms, where the march wins on tile-aligned jobs
+// (39.4 vs 34.4) but loses on ragged ones (21.9 vs 26.8) and, run for only
+// part of a mix, slowed the serving stream (an extra, under-filled launch
+// per step).  2: the march wherever it fits; 0: the two passes.
 bool use_march(const lbm_batch* b, const HostJob& j)
 {
     const bool fits = j.d.q == 19 && j.d.nx >= 2 && j.d.ny >= lbm::batch::kMarchTY &&
@@ -449,7 +448,11 @@ int lbm_batch_submit(lbm_batch* b, const lbm_job_desc* job, const double* rho0, 
     if (!(job->tau > 0.5) || !std::isfinite(job->G) || job->steps < 0)
         return lbm::set_error(LBM_EINVAL, "job needs tau > 0.5, finite G, steps >= 0");
     const int64_t n = int64_t(job->nx) * job->ny * nz;
-    if (n >= (int64_t(1) << 31)) return lbm::set_error(LBM_EINVAL, "job too large");
+    // the batch kernels address a job's arrays with 32-bit byte offsets of
+    // 4-byte values
+    if (n >= (int64_t(1) << 30))
+        return lbm::set_error(LBM_EINVAL, "job too large (%lld nodes; a batch job holds < 2^30)",
+                              static_cast<long long>(n));
     const int64_t bytes = job_bytes(b, *job);
     if (b->in_use + bytes > b->d.hbm_budget)
         return lbm::set_error(LBM_ENOMEM, "job needs %lld B, %lld B of the %lld B budget left",
