@@ -7,8 +7,10 @@ with the populations already resident in HBM.  Default workload: D3Q19
 256^3 periodic, FP64, tau = 1, G = -5.5, droplet R = 48 (the drop-in FP64
 configuration).  Prints ONE JSON line with MLUPS, the HBM roofline of the
 step kernel (algorithmic bytes 2*Q*b per lattice update over the kernel's
-measured time) and a CPU baseline: the oracle (a restatement of the
-reference's CPU code) timed on this host's cores on a bounded sample.
+measured time) and a CPU baseline timed on this host's cores on a bounded
+sample: the reference's own MPI CPU path (its lattice-Boltzmann sources
+compiled in place into oracle/_ref/ref_harness), or the oracle (a
+restatement of the reference's CPU code) where that harness is not built.
 """
 from __future__ import annotations
 
@@ -58,10 +60,62 @@ def initial_rho(shape, init):
     return mb.noise_rho(shape, init[1])
 
 
+def cpu_baseline_reference(q, shape, tau, G, init):
+    """Times the reference's own MPI CPU path on this host: the harness
+    oracle/_ref/ref_harness (the reference's LBfluid3D/LBfluid2D/scmp3D
+    translation units compiled in place, x split into slabs over MPI ranks as
+    LBApp does) under mpiexec, one rank per core (up to 16).  Two runs of
+    k1 and k2 steps on the same lattice, each ending in the same field dump;
+    the rate is n (k2 - k1) / (t2 - t1), so start-up, initialisation and
+    the dump cancel.  None when the harness or mpiexec is missing or fails."""
+    import shutil
+    import subprocess
+    import tempfile
+    harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    mpiexec = shutil.which("mpiexec") or "/opt/conda/bin/mpiexec"
+    if not (os.path.exists(harness) and os.path.exists(mpiexec)):
+        return None
+    ranks = min(len(os.sched_getaffinity(0)), 16)
+    n = int(np.prod(shape))
+    full = tuple(shape) + ((1,) if len(shape) == 2 else ())
+    k1 = 2
+    k2 = k1 + max(20, min(20000, int(6e8 / n)))   # about 10 s of steps at ~80 MLUPS
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        rho0 = os.path.join(d, "rho0.bin")
+        initial_rho(shape, init).astype(np.float64).ravel().tofile(rho0)
+        times = []
+        for k in (k1, k2):
+            cmd = [mpiexec, "-np", str(ranks), harness, str(q), *map(str, full), repr(tau), repr(G),
+                   "0", rho0, os.path.join(d, f"out{k}"), str(k)]
+            t0 = time.perf_counter()
+            try:
+                r = subprocess.run(cmd, capture_output=True, timeout=300)
+            except (OSError, subprocess.TimeoutExpired):
+                return None
+            times.append(time.perf_counter() - t0)
+            if r.returncode != 0:
+                return None
+            for f in os.listdir(d):
+                if f.startswith("out"):
+                    os.remove(os.path.join(d, f))
+    dt = times[1] - times[0]
+    if dt <= 0:
+        return None
+    return {"value": round(n * (k2 - k1) / dt / 1e6, 3), "unit": "MLUPS", "cores": ranks,
+            "kind": "reference",
+            "sample": f"reference CPU path (its LB sources compiled in place, -O3, MPI x slabs), "
+                      f"mpiexec -np {ranks}, {'x'.join(map(str, shape))} lattice, {k2} minus {k1} "
+                      f"steps ({dt:.1f} s)"}
+
+
 def cpu_baseline(q, shape, tau, G, init, budget_s=15.0):
-    """Times the oracle (CPU restatement of the reference step, x-slab
-    threads) on a bounded sample: the same lattice, as many steps as fit
-    in about budget_s seconds after one warm-up step."""
+    """The reference's CPU path where its harness is built
+    (cpu_baseline_reference), else the oracle (CPU restatement of the
+    reference step, x-slab threads) on a bounded sample: the same lattice,
+    as many steps as fit in about budget_s seconds after one warm-up step."""
+    ref = cpu_baseline_reference(q, shape, tau, G, init)
+    if ref is not None:
+        return ref
     import oracle
     cores = min(os.cpu_count() or 1, 16)
     o = oracle.Oracle(q, shape, tau, G, nthreads=cores)
