@@ -47,7 +47,8 @@ enum lbm_precision {
 };
 
 /* AUTO: the fused kernel where a tile shape fits (D2Q9: lattices of at
- * least 2^18 nodes, where it measured faster), else the two passes. */
+ * least 2^18 nodes, or with ny = 64, 128 or 256, where it measured
+ * faster), else the two passes. */
 enum lbm_kernel {
     LBM_KERNEL_AUTO = 0,
     LBM_KERNEL_TWO_PASS = 1, /* density pass, then force+collide+stream pass */

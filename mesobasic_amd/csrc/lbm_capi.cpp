@@ -209,9 +209,11 @@ int lbm_create(const lbm_desc* desc, lbm_handle** out)
             delete h;
             return fail(LBM_EINVAL, "fused FP64 kernel does not support this lattice");
         }
-        // D2Q9: the automatic choice keeps small lattices on the two passes
+        // D2Q9: the automatic choice keeps small lattices on the two passes,
+        // unless their rows are full segments
         const bool fused_auto =
-            fused_ok && (desc->q == 19 || h->g.n >= lbm::fused::kRowAutoMinNodes);
+            fused_ok && (desc->q == 19 || h->g.n >= lbm::fused::kRowAutoMinNodes ||
+                         lbm::fused::row_full(h->g, lbm::fp64_default_shape(h->g, desc->q)));
         h->kernel = desc->kernel == LBM_KERNEL_AUTO
                         ? (fused_auto ? LBM_KERNEL_FUSED : LBM_KERNEL_TWO_PASS)
                         : desc->kernel;

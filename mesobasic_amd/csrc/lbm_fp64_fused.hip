@@ -76,11 +76,13 @@ __device__ __forceinline__ double psi9(const double (&f)[9], const uint64_t* tab
 // Moments, force, BGK collision of one node and the store of its
 // post-collision values; the operation order of k_collide<9, *, 0> in
 // lbm_fp64.hip (LBfluid2D.cpp:227-312 with usq = u*u + v*v).  pm/p0/pp are
-// the psi rows x-1, x, x+1 and li the node's slot in them.
+// the psi rows x-1, x, x+1, li the node's slot in them and dym/dyp the slot
+// offsets of its y-1 / y+1 neighbours.
 template <bool TAU1, bool NTS>
 __device__ __forceinline__ void collide9(double* __restrict__ Pout, const Grid& g, uint32_t c,
                                          const double (&f)[9], const double* pm, const double* p0,
-                                         const double* pp, int li, double tau, double G)
+                                         const double* pp, int li, int dym, int dyp, double tau,
+                                         double G)
 {
     using S = St<9>;
     double rho = 0.0, sx = 0.0, sy = 0.0;
@@ -97,7 +99,7 @@ __device__ __forceinline__ void collide9(double* __restrict__ Pout, const Grid& 
         constexpr int n = decltype(I)::value;
         if constexpr (n > 0) {
             const double* pl = S::ex[n] < 0 ? pm : (S::ex[n] > 0 ? pp : p0);
-            const double t = S::w[n] * pl[li + S::ey[n]];
+            const double t = S::w[n] * pl[li + (S::ey[n] < 0 ? dym : (S::ey[n] > 0 ? dyp : 0))];
             acc_e<S::ex[n]>(gx, t);
             acc_e<S::ey[n]>(gy, t);
         }
@@ -125,7 +127,12 @@ __device__ __forceinline__ void collide9(double* __restrict__ Pout, const Grid& 
 // psi(x+2) and waits at one barrier.  Needs nx >= 2 (x+2 wraps by one
 // subtraction).  The barrier is skipped only where `more` is false, which
 // is uniform across the workgroup (x and x1 are).
-template <bool WALLS, bool TAU1, int TW, bool NTS>
+//
+// ZF (full rows, ny == TW): the segment is the whole periodic row, so there
+// is no ring; the first and last lanes read their wrapped y neighbour's psi
+// from the other end of the row's LDS slots, and the prologue is a single
+// round of loads.
+template <bool WALLS, bool TAU1, int TW, bool NTS, bool ZF>
 __global__ void __launch_bounds__(TW)
     k_rows9(const double* __restrict__ Pin, int kind, double* __restrict__ Pout, Grid g, double tau,
             double G, int chunk)
@@ -144,7 +151,10 @@ __global__ void __launch_bounds__(TW)
     const int yi = (y0 + tid) % g.ny;
     const int li = tid + 1;
     // ring nodes: y0-1 (slot 0) on lane 0, y0+TW (slot TW+1) on the last lane
-    const bool ring = tid == 0 || tid == TW - 1;
+    const bool ring = !ZF && (tid == 0 || tid == TW - 1);
+    // LDS slot offsets of the y-1 / y+1 neighbours (wrapped within a full row)
+    const int dym = (ZF && tid == 0) ? TW - 1 : -1;
+    const int dyp = (ZF && tid == TW - 1) ? 1 - TW : 1;
     const int ry = tid == 0 ? (y0 == 0 ? g.ny - 1 : y0 - 1) : (y0 + TW) % g.ny;
     const int lr = tid == 0 ? 0 : TW + 1;
 
@@ -191,7 +201,7 @@ __global__ void __launch_bounds__(TW)
             const uint32_t c = static_cast<uint32_t>(x) * static_cast<uint32_t>(g.ny) +
                                static_cast<uint32_t>(yi);
             collide9<TAU1, NTS>(Pout, g, c, fcur, psi_lds + slot(x - 1), psi_lds + slot(x),
-                                psi_lds + slot(x + 1), li, tau, G);
+                                psi_lds + slot(x + 1), li, dym, dyp, tau, G);
         }
         if (more) {
             if (ring) psi_lds[slot(x + 2) + lr] = psi9(fr, exp_tab);
@@ -220,7 +230,10 @@ void launch_rows(const double* Pin, int kind, double* Pout, const Grid& g, const
     const int nty = (g.ny + TW - 1) / TW;
     const int ncx = (g.nx + t.chunk - 1) / t.chunk;
     const unsigned nb = static_cast<unsigned>(static_cast<int64_t>(nty) * ncx);
-    fused::k_rows9<W, T1, TW, NTS><<<nb, TW, 0, s>>>(Pin, kind, Pout, g, a.tau, a.G, t.chunk);
+    if (g.ny == TW)
+        fused::k_rows9<W, T1, TW, NTS, true><<<nb, TW, 0, s>>>(Pin, kind, Pout, g, a.tau, a.G, t.chunk);
+    else
+        fused::k_rows9<W, T1, TW, NTS, false><<<nb, TW, 0, s>>>(Pin, kind, Pout, g, a.tau, a.G, t.chunk);
 }
 
 template <bool W, bool T1, bool NTS>

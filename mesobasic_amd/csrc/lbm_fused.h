@@ -152,10 +152,19 @@ inline int default_row_chunk(const Grid& g, int shape)
     return chunk;
 }
 
-// Smallest D2Q9 lattice the automatic kernel choice runs fused: below it
-// the march's dependent load rounds cost more than the second pass
-// (MI355X, FP64: 256^2 two-pass 8.8 vs fused 7.3 GLUPS; 512^2 18.2 vs 19.5).
+// Smallest D2Q9 lattice the automatic kernel choice runs fused, unless its
+// rows are full segments (row_full): below it the march's dependent load
+// rounds, ring included, cost more than the second pass (MI355X, FP64:
+// 256^2 two-pass 8.8 vs fused 7.3 GLUPS; 512^2 18.2 vs 19.5).
 constexpr int64_t kRowAutoMinNodes = int64_t(1) << 18;
+
+// Rows that are exactly one segment (ny == segment width): the kernel's
+// full-row form, with no ring.  Measured faster than the two passes at any
+// size (128^2: 4.3 vs 6.8 us per step; 256^2: 7.0-7.8 vs 8.1).
+inline bool row_full(const Grid& g, int shape)
+{
+    return is_row_shape(shape) && g.ny == kRowTW[shape - kFirstRowShape];
+}
 
 // Ring slot r -> (row, col) in LDS coordinates.  Rows 0 and LY-1 are the y
 // ring; without ZF, columns 0 and LZ-1 are the z ring.

@@ -157,11 +157,13 @@ ROW_SHAPES = (8, 9, 10)     # D2Q9 row segments of 256, 128 and 64 nodes
 
 
 @pytest.mark.parametrize("shape,order", [((40, 300), 3), ((9, 70), 2), ((6, 5), 3),
-                                         ((2, 129), 3), ((31, 256), 2)])
+                                         ((2, 129), 3), ((31, 256), 2), ((12, 64), 3),
+                                         ((7, 128), 2)])
 def test_fused_rows9_every_segment_and_chunk_bitwise(shape, order):
     # D2Q9 row march vs the two-pass kernels: segments longer than the row
-    # (wrapped lanes), ragged last segments (300 = 256 + 44), nx = 2, walls
-    # on rows 1 and ny-2, tau != 1; order 2 runs the LBfluid2D first step
+    # (wrapped lanes), ragged last segments (300 = 256 + 44), full rows
+    # (ny = 64 / 128 / 256 equal to a segment: no ring), nx = 2, walls on
+    # rows 1 and ny-2, tau != 1; order 2 runs the LBfluid2D first step
     # two-pass and the rest fused
     rho0 = mb.noise_rho(shape, 23)
     walls = shape[1] >= 3
