@@ -89,7 +89,8 @@ def cpu_baseline_reference(q, shape, tau, G, init):
                    "0", rho0, os.path.join(d, f"out{k}"), str(k)]
             t0 = time.perf_counter()
             try:
-                r = subprocess.run(cmd, capture_output=True, timeout=300)
+                # each run takes about 3-12 s; a stuck launcher must not hold the bench
+                r = subprocess.run(cmd, capture_output=True, timeout=120)
             except (OSError, subprocess.TimeoutExpired):
                 return None
             times.append(time.perf_counter() - t0)
