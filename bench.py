@@ -261,6 +261,10 @@ def main():
             forms[f"speedup_{k}_vs_fp32"] = round(forms[k]["mlups"] / forms["fp32"]["mlups"], 3)
             forms[f"speedup_{k}_vs_fp64"] = round(forms[k]["mlups"] / forms["fp64"]["mlups"], 3)
         out["precision_forms_512"] = forms
+        # D2Q9 FP64 drop-in path: the 256^2 spinodal lattice (latency-bound)
+        # and a 4096^2 one (HBM-bound), automatic kernel choice
+        out["d2q9_fp64"] = {"256x256": time_workload("d2q9_fp64_256_noise", steps=2000, warmup=10),
+                            "4096x4096": time_workload("d2q9_fp64_4096_noise", steps=30)}
     if args.workload == "d3q19_fp64_256_droplet":
         from mesobasic_amd import serve
         if not args.no_batched:
