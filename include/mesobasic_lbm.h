@@ -46,9 +46,11 @@ enum lbm_precision {
     LBM_BF16N = 5      /* FP32 compute, BF16 node-shifted storage */
 };
 
-/* AUTO: the fused kernel where a tile shape fits (D2Q9: lattices of at
- * least 2^18 nodes, or with ny = 64, 128 or 256, where it measured
- * faster), else the two passes. */
+/* AUTO: the fused kernel where a tile shape divides the lattice (D2Q9:
+ * lattices of at least 2^18 nodes, or with ny = 64, 128 or 256, where it
+ * measured faster), else the two passes.  FUSED on an FP64 D3Q19 lattice
+ * that no tile divides (ny >= 4, nz >= 32) runs partial tiles: same bits,
+ * measured slower than the two passes at 100^3 - 300^3. */
 enum lbm_kernel {
     LBM_KERNEL_AUTO = 0,
     LBM_KERNEL_TWO_PASS = 1, /* density pass, then force+collide+stream pass */
@@ -153,7 +155,8 @@ int lbm_synchronize(lbm_handle* h);
  * No reference counterpart: the reference has no launch configuration.
  *   shape     tile of the y-z cross-section per workgroup: 0: 1x512, 1: 2x256,
  *             2: 1x256 (whole z rows, nz must equal 512 / 256 / 256),
- *             3: 8x64, 4: 4x64, 5: 8x32; two nodes per thread (16-bit
+ *             3: 8x64, 4: 4x64, 5: 8x32 (FP64: also partial, any ny >= TY,
+ *             nz >= TZ); two nodes per thread (16-bit
  *             storage at tau = 1 without walls, nz a multiple of 128):
  *             6: 8x128, 7: 4x128; D2Q9 FP64 (any ny): a segment of 8: 256,
  *             9: 128, 10: 64 nodes of a y row; -1 = first that fits (a

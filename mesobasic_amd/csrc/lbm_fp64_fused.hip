@@ -9,14 +9,22 @@
 
 namespace lbm {
 
+// D3Q19: a tile that divides the lattice, else a partial tile (8x64, then
+// 8x32, then 4x64) where the lattice is at least one tile wide.
 int fp64_default_shape(const Grid& g, int q)
 {
-    return q == 9 ? fused::default_row_shape(g, q) : fused::default_shape(g, q);
+    if (q == 9) return fused::default_row_shape(g, q);
+    const int s = fused::default_shape(g, q);
+    if (s >= 0) return s;
+    for (int p : {3, 5, 4})
+        if (fused::partial_supported(g, q, p)) return p;
+    return -1;
 }
 
 bool fp64_shape_ok(const Grid& g, int q, int shape)
 {
-    return q == 9 ? fused::row_supported(g, q, shape) : fused::supported(g, q, shape);
+    if (q == 9) return fused::row_supported(g, q, shape);
+    return fused::supported(g, q, shape) || fused::partial_supported(g, q, shape);
 }
 
 bool fp64_fused_supported(const StepArgs& a, const Grid& g) { return fp64_default_shape(g, a.q) >= 0; }
@@ -253,7 +261,16 @@ void launch(const double* Pin, int kind, double* Pout, const Grid& g, const Step
             const FusedTuning& t, hipStream_t s)
 {
     const int ncx = (g.nx + t.chunk - 1) / t.chunk;
-    const unsigned nb = static_cast<unsigned>((g.ny / TL::TY) * (g.nz / TL::TZ) * ncx);
+    const bool partial = !TL::ZF && (g.ny % TL::TY != 0 || g.nz % TL::TZ != 0);
+    const int nty = (g.ny + TL::TY - 1) / TL::TY, ntz = (g.nz + TL::TZ - 1) / TL::TZ;
+    const unsigned nb = static_cast<unsigned>(static_cast<int64_t>(nty) * ntz * ncx);
+    if constexpr (!TL::ZF) {
+        if (partial) {
+            fused::k_fused<StF64, double, W, true, T1, TL, NTS, true><<<nb, TL::NT, 0, s>>>(
+                Pin, kind, Pout, g, a.tau, a.G, 0.0, t.chunk, t.order);
+            return;
+        }
+    }
     fused::k_fused<StF64, double, W, true, T1, TL, NTS><<<nb, TL::NT, 0, s>>>(
         Pin, kind, Pout, g, a.tau, a.G, 0.0, t.chunk, t.order);
 }

@@ -85,6 +85,16 @@ inline int default_shape(const Grid& g, int q)
     return -1;
 }
 
+// FP64 partial tiles: the z-tiled shapes (3, 4, 5) on lattices whose ny or
+// nz is not a multiple of the tile; the wrap of the last tile takes one
+// subtraction, so ny >= TY and nz >= TZ.
+inline bool partial_supported(const Grid& g, int q, int shape)
+{
+    if (q != 19 || shape < 0 || shape >= kNumShapes || kShapeZF[shape]) return false;
+    if (g.ps >= (int64_t(1) << 29)) return false;
+    return g.nx >= 2 && g.ny >= kShapeTY[shape] && g.nz >= kShapeTZ[shape];
+}
+
 // Pair tile shapes (tuning ids kNumShapes + i): two z-adjacent nodes per
 // thread, TY x TZ nodes per tile, 16-bit storage only (lbm_fused_pair.h).
 constexpr int kNumPairShapes = 2;
@@ -107,6 +117,31 @@ inline int default_pair_shape(const Grid& g, int q)
     for (int i = 0; i < kNumPairShapes; i++)
         if (pair_supported(g, q, kNumShapes + i)) return kNumShapes + i;
     return -1;
+}
+
+// Default x planes per workgroup for a D3Q19 tile or pair shape: at most
+// max_chunk, with the x chunks balanced and numerous enough that the launch
+// has at least 256 workgroups (one per CU).  At 256^3 and 512^3 this is
+// max_chunk, the measured best; a fixed 128 would launch 32 workgroups for
+// 128^3 and 26 for 100^3 (fused 2.0 GLUPS there against 11.6 two-pass).
+inline int default_chunk(const Grid& g, int shape, int max_chunk)
+{
+    int ty, tz;
+    if (is_pair_shape(shape)) {
+        ty = kPairTY[shape - kNumShapes];
+        tz = kPairTZ[shape - kNumShapes];
+    } else if (shape >= 0 && shape < kNumShapes) {
+        ty = kShapeTY[shape];
+        tz = kShapeTZ[shape];
+    } else {
+        return max_chunk;
+    }
+    const int64_t tiles = static_cast<int64_t>((g.ny + ty - 1) / ty) * ((g.nz + tz - 1) / tz);
+    int64_t ncx = (g.nx + max_chunk - 1) / max_chunk;
+    const int64_t fill = (256 + tiles - 1) / tiles;
+    if (fill > ncx) ncx = fill;
+    if (ncx > g.nx) ncx = g.nx;
+    return static_cast<int>((g.nx + ncx - 1) / ncx);
 }
 
 // D2Q9 row-segment shapes (tuning ids kNumShapes + kNumPairShapes + i): a
@@ -438,11 +473,18 @@ __host__ __device__ inline void tile_of_block(int b, int nty, int ntz, int ncx, 
 template <typename CT> struct Occ { static constexpr int waves = 2; };
 template <> struct Occ<float> { static constexpr int waves = 4; };
 
-template <typename SP, typename CT, bool WALLS, bool EXACT, bool TAU1, typename TL, bool NTS>
+// PARTIAL: ny or nz is not a multiple of the tile (needs ny >= TY, nz >= TZ).
+// The last tile of a column runs past the lattice; its threads take the
+// periodically wrapped node, so LDS slot s of every tile row or column holds
+// the psi of wrapped coordinate origin + s - 1, which is exactly the
+// neighbour the last real node needs.  Wrapped threads store nothing.
+template <typename SP, typename CT, bool WALLS, bool EXACT, bool TAU1, typename TL, bool NTS,
+          bool PARTIAL = false>
 __global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(Occ<CT>::waves)))
     k_fused(const typename SP::T* __restrict__ Pin, int kind, typename SP::T* __restrict__ Pout,
             Grid g, CT tau, CT G, CT rho_shift, int chunk, int order)
 {
+    static_assert(!(PARTIAL && TL::ZF), "full-row tiles cover nz exactly");
     constexpr int TY = TL::TY, TZ = TL::TZ, LZ = TL::LZ, PL = TL::LY * TL::LZ;
     constexpr bool kTab = std::is_same_v<CT, double>;
     __shared__ CT psi_lds[4 * PL];
@@ -452,7 +494,8 @@ __global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(Occ
         __syncthreads();
     }
 
-    const int nty = g.ny / TY, ntz = g.nz / TZ;
+    const int nty = PARTIAL ? (g.ny + TY - 1) / TY : g.ny / TY;
+    const int ntz = PARTIAL ? (g.nz + TZ - 1) / TZ : g.nz / TZ;
     const int ncx = (g.nx + chunk - 1) / chunk;
     int ty, tz, cx;
     tile_of_block(blockIdx.x, nty, ntz, ncx, order, ty, tz, cx);
@@ -462,7 +505,9 @@ __global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(Occ
 
     const int tid = threadIdx.x;
     const int ly = tid / TZ, lz = tid - (tid / TZ) * TZ;  // tile node of this thread
-    const int yi = y0 + ly, zi = z0 + lz;
+    const int yi = PARTIAL ? wrap(y0 + ly, g.ny) : y0 + ly;
+    const int zi = PARTIAL ? wrap(z0 + lz, g.nz) : z0 + lz;
+    const bool valid = !PARTIAL || (y0 + ly < g.ny && z0 + lz < g.nz);
     constexpr int c0 = TL::ZF ? 0 : 1;                    // LDS column of z0
     const int li = (ly + 1) * LZ + (lz + c0);             // its LDS slot
     // LDS offsets of the z-1 / z+1 neighbours (periodic within a full row)
@@ -524,9 +569,10 @@ __global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(Occ
         const uint32_t c = (static_cast<uint32_t>(x) * static_cast<uint32_t>(g.ny) +
                             static_cast<uint32_t>(yi)) * static_cast<uint32_t>(g.nz) +
                            static_cast<uint32_t>(zi);
-        collide_store<SP, CT, EXACT, TAU1, LZ, NTS>(Pout, g, c, fcur, psi_lds + slot(x - 1),
-                                               psi_lds + slot(x), psi_lds + slot(x + 1), li, dzm,
-                                               dzp, tau, G, rho_shift);
+        if (valid)
+            collide_store<SP, CT, EXACT, TAU1, LZ, NTS>(Pout, g, c, fcur, psi_lds + slot(x - 1),
+                                                   psi_lds + slot(x), psi_lds + slot(x + 1), li,
+                                                   dzm, dzp, tau, G, rho_shift);
 #pragma unroll
         for (int k = 0; k < TL::RPT; k++)
             if (has_ring[k]) psi_lds[slot(x + 2) + lr[k]] = psi_of(fr[k]);

@@ -72,8 +72,8 @@ CASES = [
 ]
 
 
-# lattices the fused kernel takes (ny % 16 == 0, nz % 32 == 0); nx = 70 spans
-# two full 32-plane chunks and a partial one
+# lattices the fused kernel takes: tile multiples (nx = 70 spans two full
+# 32-plane chunks and a partial one), then partial tiles
 FUSED_CASES = [
     (19, (6, 16, 32), 1.0, -5.5, 3, False, 30),
     (19, (5, 32, 32), 0.7, -5.0, 3, True, 30),
@@ -81,6 +81,9 @@ FUSED_CASES = [
     (19, (3, 16, 64), 1.2, -5.5, 2, True, 20),
     (19, (5, 6, 256), 1.0, -5.5, 3, False, 15),
     (19, (3, 2, 512), 0.9, -5.0, 3, True, 12),
+    # partial tiles (ny, nz not multiples of the tile)
+    (19, (7, 12, 90), 1.0, -5.5, 3, False, 20),
+    (19, (6, 21, 70), 0.8, -5.0, 2, True, 15),
 ]
 
 
@@ -148,6 +151,33 @@ def test_fused_every_tile_shape_and_chunk_bitwise(shape):
             assert lat.tuning == (tile, chunk, order, nts)
             lat.set_rho(rho0); lat.init(); lat.step(9)
             assert same_bits(lat.get_f(), want), (tile, chunk)
+            lat.close()
+            tried += 1
+    assert tried >= 4
+
+
+@pytest.mark.parametrize("shape", [(9, 20, 100), (5, 13, 70), (6, 9, 65), (4, 8, 64)])
+def test_fused_partial_tiles_bitwise(shape):
+    # lattices whose ny or nz is not a multiple of the tile: the last tile of
+    # a column wraps periodically and stores only its real nodes; (4, 8, 64)
+    # is a control that every tile divides
+    rho0 = mb.noise_rho(shape, 19)
+    ref = mb.Lattice(19, shape, 0.8, -5.5, kernel=mb.KERNEL_TWO_PASS, walls=True)
+    ref.set_rho(rho0); ref.init(); ref.step(9)
+    want = ref.get_f()
+    ref.close()
+    tried = 0
+    for tile in (3, 4, 5):
+        for chunk, order, nts in ((1, 0, 0), (2, 3, 1), (5, 1, 1), (64, 3, 0)):
+            lat = mb.Lattice(19, shape, 0.8, -5.5, kernel=mb.KERNEL_FUSED, walls=True)
+            try:
+                lat.set_tuning(tile, chunk, order, nts)
+            except mb.LbmError:
+                lat.close()
+                break           # lattice narrower than this tile
+            print(f"shape {shape} tile {tile} chunk {chunk} order {order} nts {nts}", flush=True)
+            lat.set_rho(rho0); lat.init(); lat.step(9)
+            assert same_bits(lat.get_f(), want), (tile, chunk, order)
             lat.close()
             tried += 1
     assert tried >= 4
