@@ -209,12 +209,13 @@ int lbm_create(const lbm_desc* desc, lbm_handle** out)
             delete h;
             return fail(LBM_EINVAL, "fused FP64 kernel does not support this lattice");
         }
-        // automatic choice: D3Q19 only where a tile divides the lattice
-        // (partial tiles measured slower than the two passes: 100^3 7.5 vs
-        // 11.7 GLUPS, 200^3 9.7 vs 11.1); D2Q9 small lattices stay on the
-        // two passes unless their rows are full segments
+        // automatic choice: D3Q19 where a tile divides the lattice and
+        // fused::auto_fused19 holds (partial tiles measured slower than the
+        // two passes: 100^3 7.5 vs 11.7 GLUPS, 200^3 9.7 vs 11.1); D2Q9
+        // small lattices stay on the two passes unless their rows are full
+        // segments
         const bool fused_auto =
-            fused_ok && (desc->q == 19 ? lbm::fused::default_shape(h->g, desc->q) >= 0
+            fused_ok && (desc->q == 19 ? lbm::fused::auto_fused19(h->g, lbm::fused::default_shape(h->g, desc->q), true)
                                        : (h->g.n >= lbm::fused::kRowAutoMinNodes ||
                                           lbm::fused::row_full(h->g, lbm::fp64_default_shape(h->g, desc->q))));
         h->kernel = desc->kernel == LBM_KERNEL_AUTO
@@ -234,9 +235,12 @@ int lbm_create(const lbm_desc* desc, lbm_handle** out)
     // (fewer prologue planes; measured 1.5-2.3% faster than 64 at 256^3 and
     // 512^3 on MI355X), 64 for the others (flat between 32 and 256), fewer
     // where the lattice would not fill the CUs (fused::default_chunk)
-    const int max_chunk =
-        (desc->precision == LBM_FP64 || lbm::fused::is_pair_shape(h->tune.shape)) ? 128 : 64;
-    h->tune.chunk = lbm::fused::default_chunk(h->g, h->tune.shape, max_chunk);
+    const bool wide = desc->precision == LBM_FP64 || lbm::fused::is_pair_shape(h->tune.shape);
+    const int max_chunk = wide ? 128 : 64;
+    // resident waves per CU: 2 per SIMD for FP64, the pair tiles and the
+    // one-node 16-bit tiles with walls, 4 for the other FP32-compute tiles
+    const int waves_per_cu = (wide || desc->walls) ? 8 : 16;
+    h->tune.chunk = lbm::fused::default_chunk(h->g, h->tune.shape, max_chunk, waves_per_cu);
     if (lbm::fused::is_row_shape(h->tune.shape))
         h->tune.chunk = lbm::fused::default_row_chunk(h->g, h->tune.shape);
     const int64_t pop_bytes = h->g.ps * desc->q * h->elem;

@@ -119,29 +119,56 @@ inline int default_pair_shape(const Grid& g, int q)
     return -1;
 }
 
-// Default x planes per workgroup for a D3Q19 tile or pair shape: at most
-// max_chunk, with the x chunks balanced and numerous enough that the launch
-// has at least 256 workgroups (one per CU).  At 256^3 and 512^3 this is
-// max_chunk, the measured best; a fixed 128 would launch 32 workgroups for
-// 128^3 and 26 for 100^3 (fused 2.0 GLUPS there against 11.6 two-pass).
-inline int default_chunk(const Grid& g, int shape, int max_chunk)
+// Default x planes per workgroup for a D3Q19 tile or pair shape.  Chunk
+// counts ncx from ceil(nx / max_chunk) up to nx are scored by a time model,
+// rounds * (planes per chunk + 3 prologue planes), where a round is one
+// set of workgroups resident at once (256 CUs x waves_per_cu / waves per
+// workgroup); chunks are balanced.  It gives max_chunk at 256^3 and 512^3
+// (the measured best) and short chunks where the tiles alone would leave
+// CUs idle: a fixed 128 launched 32 workgroups at 128^3 (FP64 3.5 GLUPS
+// against 14.9 with chunk 16), and a plain "at least 256 workgroups" rule
+// put 192^3 into a second round of 32.
+inline int default_chunk(const Grid& g, int shape, int max_chunk, int waves_per_cu)
 {
-    int ty, tz;
+    int ty, tz, threads;
     if (is_pair_shape(shape)) {
         ty = kPairTY[shape - kNumShapes];
         tz = kPairTZ[shape - kNumShapes];
+        threads = ty * tz / 2;
     } else if (shape >= 0 && shape < kNumShapes) {
         ty = kShapeTY[shape];
         tz = kShapeTZ[shape];
+        threads = ty * tz;
     } else {
         return max_chunk;
     }
     const int64_t tiles = static_cast<int64_t>((g.ny + ty - 1) / ty) * ((g.nz + tz - 1) / tz);
-    int64_t ncx = (g.nx + max_chunk - 1) / max_chunk;
-    const int64_t fill = (256 + tiles - 1) / tiles;
-    if (fill > ncx) ncx = fill;
-    if (ncx > g.nx) ncx = g.nx;
-    return static_cast<int>((g.nx + ncx - 1) / ncx);
+    const int64_t wg_waves = (threads + 63) / 64;
+    const int64_t slots = 256 * (waves_per_cu / wg_waves > 0 ? waves_per_cu / wg_waves : 1);
+    int64_t best_ncx = (g.nx + max_chunk - 1) / max_chunk, best_cost = -1;
+    for (int64_t ncx = best_ncx; ncx <= g.nx; ncx++) {
+        const int64_t chunk = (g.nx + ncx - 1) / ncx;
+        const int64_t rounds = (tiles * ncx + slots - 1) / slots;
+        const int64_t cost = rounds * (chunk + 3);
+        if (best_cost < 0 || cost < best_cost) {
+            best_cost = cost;
+            best_ncx = ncx;
+        }
+    }
+    return static_cast<int>((g.nx + best_ncx - 1) / best_ncx);
+}
+
+// Automatic kernel choice for D3Q19: the fused kernel from 2^21 nodes; in
+// the FP32-compute forms not on the one-node 8x32 tile.  Measured on MI355X
+// with the default chunks (GLUPS, fused vs two-pass): FP64 64^3 10.9 vs
+// 12.0, 96^3 11.2 vs 12.7, 128^3 14.9 vs 10.4, 160^3 (8x32) 13.6 vs 11.0,
+// 192^3 15.4 vs 11.3; FP16 96^3 22.3 vs 32.9, 128^3 (pair) 41.0 vs 40.6,
+// 160^3 (8x32) 30.3 vs 36.5; FP32 96^3 15.6 vs 23.8, 128^3 26.5 vs 24.0.
+constexpr int64_t kAutoMinNodes19 = int64_t(1) << 21;
+
+inline bool auto_fused19(const Grid& g, int shape, bool fp64)
+{
+    return shape >= 0 && g.n >= kAutoMinNodes19 && (fp64 || shape != 5);
 }
 
 // D2Q9 row-segment shapes (tuning ids kNumShapes + kNumPairShapes + i): a

@@ -373,10 +373,11 @@ int lowp_default_shape(int precision, const StepArgs& a, const Grid& g)
 
 int lowp_resolve_kernel(int precision, const StepArgs& a, const Grid& g, int requested)
 {
-    const bool fused_ok = lowp_default_shape(precision, a, g) >= 0;
+    const int shape = lowp_default_shape(precision, a, g);
+    const bool fused_ok = shape >= 0;
     if (requested == LBM_KERNEL_FUSED) return fused_ok ? LBM_KERNEL_FUSED : -1;
     if (requested == LBM_KERNEL_TWO_PASS) return LBM_KERNEL_TWO_PASS;
-    return fused_ok ? LBM_KERNEL_FUSED : LBM_KERNEL_TWO_PASS;
+    return fused::auto_fused19(g, shape, false) ? LBM_KERNEL_FUSED : LBM_KERNEL_TWO_PASS;
 }
 
 hipError_t lowp_init(int precision, const StepArgs& a, const Grid& g, double rho_shift,
