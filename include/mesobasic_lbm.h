@@ -46,12 +46,12 @@ enum lbm_precision {
     LBM_BF16N = 5      /* FP32 compute, BF16 node-shifted storage */
 };
 
-/* AUTO: the fused kernel where a tile shape divides the lattice and it
- * measured faster (D3Q19: at least 2^21 nodes, and in the FP32/16-bit forms
- * not the 8x32 tile; D2Q9: at least 2^18 nodes, or ny = 64, 128 or 256),
- * else the two passes.  FUSED on an FP64 D3Q19 lattice
- * that no tile divides (ny >= 4, nz >= 32) runs partial tiles: same bits,
- * measured slower than the two passes at 100^3 - 300^3. */
+/* AUTO: the fused kernel where it measured faster (D3Q19: at least 2^21
+ * nodes, and in the FP32/16-bit forms a tile that divides the lattice other
+ * than 8x32; FP64 also with partial tiles where no tile divides it;
+ * D2Q9: at least 2^18 nodes, or ny = 64, 128 or 256), else the two passes.
+ * FUSED asks for it wherever a shape fits (FP64 D3Q19 partial tiles:
+ * ny >= 4, nz >= 32; same bits). */
 enum lbm_kernel {
     LBM_KERNEL_AUTO = 0,
     LBM_KERNEL_TWO_PASS = 1, /* density pass, then force+collide+stream pass */

@@ -209,13 +209,13 @@ int lbm_create(const lbm_desc* desc, lbm_handle** out)
             delete h;
             return fail(LBM_EINVAL, "fused FP64 kernel does not support this lattice");
         }
-        // automatic choice: D3Q19 where a tile divides the lattice and
-        // fused::auto_fused19 holds (partial tiles measured slower than the
-        // two passes: 100^3 7.5 vs 11.7 GLUPS, 200^3 9.7 vs 11.1); D2Q9
-        // small lattices stay on the two passes unless their rows are full
-        // segments
+        // automatic choice: D3Q19 where fused::auto_fused19 holds, partial
+        // tiles included (FP64, GLUPS, fused vs two-pass: 200^3 13.2 vs 11.1,
+        // 250^3 13.7 vs 10.9, 300^3 12.8 vs 11.0; 100^3 10.3 vs 11.8 is
+        // below the size threshold); D2Q9 small lattices stay on the two
+        // passes unless their rows are full segments
         const bool fused_auto =
-            fused_ok && (desc->q == 19 ? lbm::fused::auto_fused19(h->g, lbm::fused::default_shape(h->g, desc->q), true)
+            fused_ok && (desc->q == 19 ? lbm::fused::auto_fused19(h->g, lbm::fp64_default_shape(h->g, desc->q), true)
                                        : (h->g.n >= lbm::fused::kRowAutoMinNodes ||
                                           lbm::fused::row_full(h->g, lbm::fp64_default_shape(h->g, desc->q))));
         h->kernel = desc->kernel == LBM_KERNEL_AUTO
