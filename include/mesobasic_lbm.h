@@ -50,8 +50,8 @@ enum lbm_precision {
  * nodes, and in the FP32/16-bit forms a tile that divides the lattice other
  * than 8x32; FP64 also with partial tiles where no tile divides it;
  * D2Q9: at least 2^18 nodes, or ny = 64, 128 or 256), else the two passes.
- * FUSED asks for it wherever a shape fits (FP64 D3Q19 partial tiles:
- * ny >= 4, nz >= 32; same bits). */
+ * FUSED asks for it wherever a shape fits (D3Q19 partial tiles: ny >= 4,
+ * nz >= 32; FP64 results unchanged to the bit). */
 enum lbm_kernel {
     LBM_KERNEL_AUTO = 0,
     LBM_KERNEL_TWO_PASS = 1, /* density pass, then force+collide+stream pass */
@@ -156,7 +156,7 @@ int lbm_synchronize(lbm_handle* h);
  * No reference counterpart: the reference has no launch configuration.
  *   shape     tile of the y-z cross-section per workgroup: 0: 1x512, 1: 2x256,
  *             2: 1x256 (whole z rows, nz must equal 512 / 256 / 256),
- *             3: 8x64, 4: 4x64, 5: 8x32 (FP64: also partial, any ny >= TY,
+ *             3: 8x64, 4: 4x64, 5: 8x32 (also partial: any ny >= TY,
  *             nz >= TZ); two nodes per thread (16-bit
  *             storage at tau = 1 without walls, nz a multiple of 128):
  *             6: 8x128, 7: 4x128; D2Q9 FP64 (any ny): a segment of 8: 256,

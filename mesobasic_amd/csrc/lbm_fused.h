@@ -85,8 +85,8 @@ inline int default_shape(const Grid& g, int q)
     return -1;
 }
 
-// FP64 partial tiles: the z-tiled shapes (3, 4, 5) on lattices whose ny or
-// nz is not a multiple of the tile; the wrap of the last tile takes one
+// Partial tiles: the z-tiled one-node shapes (3, 4, 5) on lattices whose ny
+// or nz is not a multiple of the tile; the wrap of the last tile takes one
 // subtraction, so ny >= TY and nz >= TZ.
 inline bool partial_supported(const Grid& g, int q, int shape)
 {

@@ -433,24 +433,29 @@ __device__ __forceinline__ void lp_tile(const typename SP::T* __restrict__ Pin, 
     if (x + 1 < x1) step(x + 1, fb, Rb, fa, Ra);
 }
 
-// The one-node march kernel: one tile per workgroup.
-template <typename SP, bool WALLS, bool NS, bool T1, typename TL, bool NTS>
+// The one-node march kernel: one tile per workgroup.  PARTIAL: ny or nz is
+// not a multiple of the tile (ny >= TY, nz >= TZ); the last tiles run as
+// lp_tile's partial tiles.
+template <typename SP, bool WALLS, bool NS, bool T1, typename TL, bool NTS, bool PARTIAL = false>
 __global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(OccLp<WALLS>::waves)))
     k_fused_lp(const typename SP::T* __restrict__ Pin, int kind, typename SP::T* __restrict__ Pout,
                Grid g, float tau, float G, float rs, const float* __restrict__ rin,
                float* __restrict__ rout, int chunk, int order)
 {
+    static_assert(!(PARTIAL && TL::ZF), "full-row tiles cover nz exactly");
     constexpr int PL = TL::LY * TL::LZ;
     // window of psi and rho for planes x-1 .. x+2 (slot (x - x0 + 1) mod 4)
     __shared__ float wpsi[4 * PL], wrho[4 * PL];
-    const int nty = g.ny / TL::TY, ntz = g.nz / TL::TZ;
+    const int nty = PARTIAL ? (g.ny + TL::TY - 1) / TL::TY : g.ny / TL::TY;
+    const int ntz = PARTIAL ? (g.nz + TL::TZ - 1) / TL::TZ : g.nz / TL::TZ;
     const int ncx = (g.nx + chunk - 1) / chunk;
     int ty, tz, cx;
     tile_of_block(blockIdx.x, nty, ntz, ncx, order, ty, tz, cx);
     const int z0 = tz * TL::TZ;
     const int x0 = cx * chunk;
-    lp_tile<SP, WALLS, NS, T1, TL, NTS>(Pin, kind, Pout, g, tau, G, rs, rin, rout, ty * TL::TY, z0,
-                                        x0, min(g.nx, x0 + chunk), wpsi, wrho);
+    lp_tile<SP, WALLS, NS, T1, TL, NTS, PARTIAL>(Pin, kind, Pout, g, tau, G, rs, rin, rout,
+                                                 ty * TL::TY, z0, x0, min(g.nx, x0 + chunk), wpsi,
+                                                 wrho);
 }
 
 }  // namespace fused

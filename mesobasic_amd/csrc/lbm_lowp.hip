@@ -359,25 +359,36 @@ bool lowp_shape_ok(int precision, const StepArgs& a, const Grid& g, int shape)
 {
     if (fused::is_pair_shape(shape))
         return lowp_pairs(precision, a) && fused::pair_supported(g, a.q, shape);
-    return fused::supported(g, a.q, shape);
+    return fused::supported(g, a.q, shape) || fused::partial_supported(g, a.q, shape);
 }
 
+// A pair tile where one applies, else a one-node tile that divides the
+// lattice, else a partial one-node tile (8x64, 4x64, 8x32).
 int lowp_default_shape(int precision, const StepArgs& a, const Grid& g)
 {
     if (lowp_pairs(precision, a)) {
         const int s = fused::default_pair_shape(g, a.q);
         if (s >= 0) return s;
     }
-    return fused::default_shape(g, a.q);
+    const int s = fused::default_shape(g, a.q);
+    if (s >= 0) return s;
+    for (int p : {3, 4, 5})
+        if (fused::partial_supported(g, a.q, p)) return p;
+    return -1;
 }
 
 int lowp_resolve_kernel(int precision, const StepArgs& a, const Grid& g, int requested)
 {
-    const int shape = lowp_default_shape(precision, a, g);
-    const bool fused_ok = shape >= 0;
+    const bool fused_ok = lowp_default_shape(precision, a, g) >= 0;
     if (requested == LBM_KERNEL_FUSED) return fused_ok ? LBM_KERNEL_FUSED : -1;
     if (requested == LBM_KERNEL_TWO_PASS) return LBM_KERNEL_TWO_PASS;
-    return fused::auto_fused19(g, shape, false) ? LBM_KERNEL_FUSED : LBM_KERNEL_TWO_PASS;
+    // automatic choice: a tile that divides the lattice; the partial tiles
+    // of these forms measured slower than the two passes (GLUPS, fused vs
+    // two-pass: FP16 200^3 26.5 vs 35.0, 300^3 25.6 vs 37.4; FP32 200^3
+    // 20.0 vs 21.7; FP16 node shift 200^3 24.6 vs 28.6)
+    int exact = lowp_pairs(precision, a) ? fused::default_pair_shape(g, a.q) : -1;
+    if (exact < 0) exact = fused::default_shape(g, a.q);
+    return fused::auto_fused19(g, exact, false) ? LBM_KERNEL_FUSED : LBM_KERNEL_TWO_PASS;
 }
 
 hipError_t lowp_init(int precision, const StepArgs& a, const Grid& g, double rho_shift,

@@ -20,18 +20,30 @@ void launch(const void* Pin, int kind, void* Pout, const Grid& g, const StepArgs
 {
     using T = typename SP::T;
     const int ncx = (g.nx + t.chunk - 1) / t.chunk;
-    const unsigned nb = static_cast<unsigned>((g.ny / TL::TY) * (g.nz / TL::TZ) * ncx);
-    if constexpr (!NS && !T1) {
-        // constant shift (FP32 storage: shift 0, f itself): the shared march
-        // kernel in its FP32-arithmetic form, converting at load
-        fused::k_fused<SP, float, W, false, false, TL, NTS><<<nb, TL::NT, 0, s>>>(
-            static_cast<const T*>(Pin), kind, static_cast<T*>(Pout), g, (float)a.tau, (float)a.G,
-            rs, t.chunk, t.order);
-    } else {
-        fused::k_fused_lp<SP, W, NS, T1, TL, NTS><<<nb, TL::NT, 0, s>>>(
-            static_cast<const T*>(Pin), kind, static_cast<T*>(Pout), g, (float)a.tau, (float)a.G,
-            rs, rin, rout, t.chunk, t.order);
+    const int nty = (g.ny + TL::TY - 1) / TL::TY, ntz = (g.nz + TL::TZ - 1) / TL::TZ;
+    const unsigned nb = static_cast<unsigned>(static_cast<int64_t>(nty) * ntz * ncx);
+    const bool partial = !TL::ZF && (g.ny % TL::TY != 0 || g.nz % TL::TZ != 0);
+    auto go = [&](auto part) {
+        constexpr bool P = decltype(part)::value;
+        if constexpr (!NS && !T1) {
+            // constant shift (FP32 storage: shift 0, f itself): the shared
+            // march kernel in its FP32-arithmetic form, converting at load
+            fused::k_fused<SP, float, W, false, false, TL, NTS, P><<<nb, TL::NT, 0, s>>>(
+                static_cast<const T*>(Pin), kind, static_cast<T*>(Pout), g, (float)a.tau,
+                (float)a.G, rs, t.chunk, t.order);
+        } else {
+            fused::k_fused_lp<SP, W, NS, T1, TL, NTS, P><<<nb, TL::NT, 0, s>>>(
+                static_cast<const T*>(Pin), kind, static_cast<T*>(Pout), g, (float)a.tau,
+                (float)a.G, rs, rin, rout, t.chunk, t.order);
+        }
+    };
+    if constexpr (!TL::ZF) {
+        if (partial) {
+            go(std::true_type{});
+            return;
+        }
     }
+    go(std::false_type{});
 }
 
 template <typename SP, bool W, bool NS, bool T1, bool NTS>

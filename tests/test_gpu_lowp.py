@@ -220,3 +220,35 @@ def test_node_shift_set_get_f_and_mass(prec):
     # measured on MI355X: -3.1e-5 (FP16) and 5.3e-5 (BF16) on this small,
     # high-curvature droplet; the constant-shift forms drift by ~1e-3
     assert abs(m1 - m0) / m0 < (1e-4 if prec == mb.FP16N else 3e-4)
+
+
+# partial one-node tiles (ny, nz not multiples of the tile) against the two
+# passes: FP32 arithmetic in both, plus one storage rounding per value
+TOL_PARTIAL = {mb.FP32: 2e-5, mb.FP16S: 1e-3, mb.BF16S: 1e-2, mb.FP16N: 3e-4, mb.BF16N: 3e-3}
+
+
+@pytest.mark.parametrize("prec", LOWP)
+def test_partial_tiles_agree_and_match_two_pass(prec):
+    for shape, tau, walls in (((9, 20, 100), 1.0, False), ((6, 13, 70), 0.8, True)):
+        rho0 = mb.droplet_rho(shape, 5.0, rl=0.9, rv=0.5) if walls else mb.noise_rho(shape, 5)
+        ref = mb.Lattice(19, shape, tau, -5.0, walls=walls, precision=prec,
+                         kernel=mb.KERNEL_TWO_PASS)
+        ref.set_rho(rho0); ref.init(); ref.step(30)
+        want = ref.fields(mb.FIELDS_CURRENT)["rho"]
+        ref.close()
+        f_first = None
+        for tile in (3, 4, 5):
+            lat = mb.Lattice(19, shape, tau, -5.0, walls=walls, precision=prec,
+                             kernel=mb.KERNEL_FUSED)
+            lat.set_tuning(tile, 7, 3, 1)
+            lat.set_rho(rho0); lat.init(); lat.step(30)
+            f = lat.get_f()
+            got = lat.fields(mb.FIELDS_CURRENT)["rho"]
+            lat.close()
+            if f_first is None:
+                f_first = f
+            else:
+                assert np.array_equal(f, f_first), (shape, tile)
+            err = rel_l2(got, want)
+            print(f"prec {prec} shape {shape} tile {tile}: vs two-pass {err:.2e}", flush=True)
+            assert np.isfinite(got).all() and err < TOL_PARTIAL[prec]
