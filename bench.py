@@ -255,7 +255,9 @@ def main():
     }
     lat.close()
     if not args.no_precision_forms and args.workload == "d3q19_fp64_256_droplet":
-        forms = {k: time_workload(f"d3q19_{k}_512_noise")
+        # 30 timed steps after 5 warm-up ones: single launches at 512^3 vary
+        # by up to 25%, so ten steps leave the form ratios noisy
+        forms = {k: time_workload(f"d3q19_{k}_512_noise", steps=30, warmup=5)
                  for k in ("fp64", "fp32", "fp16", "bf16", "fp16n", "bf16n")}
         for k in ("fp16", "fp16n"):
             forms[f"speedup_{k}_vs_fp32"] = round(forms[k]["mlups"] / forms["fp32"]["mlups"], 3)

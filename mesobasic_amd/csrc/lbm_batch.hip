@@ -161,14 +161,16 @@ __global__ void __launch_bounds__(T) k_d2q9_rows(Tables t, int parity)
             if constexpr (NS) return job.R[p][static_cast<int64_t>(xr) * ny + y];
             else return rs;
         };
+        // prologue: the three rows' pulls are all issued before the first
+        // window entry is made, so they are one round of loads, not three
         float fa[9], fb[9], fc[9];
         float Ra = 0.f, Rb = 0.f, Rc = 0.f;
         if (active) {
             Rc = pullR(xw(x0 - 1), fc);
-            put(x0 - 1, fc, Rc);
             Ra = pullR(x0, fa);
-            put(x0, fa, Ra);
             Rb = pullR(xw(x0 + 1), fb);
+            put(x0 - 1, fc, Rc);
+            put(x0, fa, Ra);
             put(x0 + 1, fb, Rb);
         }
         __syncthreads();
