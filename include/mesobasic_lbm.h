@@ -267,12 +267,14 @@ int64_t lbm_batch_steps_run(const lbm_batch* b);
 int lbm_batch_set_work_limit(lbm_batch* b, int64_t max_nodes);
 
 /* Launch configuration the batch uses (after autotuning): D2Q9 rows per
- * work item and whether D2Q9 jobs use the row-marching kernel (1) or the
+ * work item (at most; a row-width class whose items would not fill two
+ * rounds of resident workgroups gets shorter items, down to 2 rows) and
+ * whether D2Q9 jobs use the row-marching kernel (1) or the
  * two-pass kernels (0). */
 int lbm_batch_tuning(const lbm_batch* b, int* d2q9_chunk, int* d2q9_rows_kernel);
 
 /* Sets the launch configuration (and stops the tuner): D2Q9 rows per work
- * item (>= 1), D2Q9 jobs on the row-marching kernel (1) or the two-pass
+ * item (>= 1; at most, as above), D2Q9 jobs on the row-marching kernel (1) or the two-pass
  * kernels (0), and the kernel of D3Q19 jobs: 0 the two-pass kernels, 2 the
  * fused march kernel for every job with nx >= 2, ny >= 8 and nz >= 64, 1
  * (default) the march for such jobs in the node-shift forms and the two

@@ -36,6 +36,14 @@ enum Cls { kRow64, kRow128, kRow256, kRow512, kRow1024, kTp9Psi, kTp9Col, kTp19P
 constexpr int kRowWidth[5] = {64, 128, 256, 512, 1024};
 constexpr int64_t kTpItem = 8192;   // nodes per two-pass work item
 constexpr int kMarchChunk = 32;     // x planes per D3Q19 march item
+// D2Q9 row items: rounds of resident workgroups a row class's items should
+// fill before its chunk is shortened, and the shortest chunk.  Serving
+// stream (200 requests, D2Q9 128..1024 per side): p50 4.66 -> 4.50 s, p99
+// 16.3 -> 15.9 s at 2 rounds; 512 jobs of 64..512 per side unchanged (66
+// GLUPS); at 4 or 8 rounds the tuner measured the row kernel slower than
+// the two passes on those jobs (61 GLUPS).
+constexpr int kRowRounds = 2;
+constexpr int kMinRowChunk = 2;
 
 struct HostJob {
     int64_t id;
@@ -190,6 +198,26 @@ Job device_view(const HostJob& j, int prec, double rs)
     return v;
 }
 
+int persistent_grid(const lbm_batch* b, int threads)
+{
+    // enough workgroups to fill every CU at the kernel's occupancy
+    const int per_cu = std::max(1, 2048 / threads);
+    return b->cus * std::min(per_cu, 8);
+}
+
+// x rows per item of row class k, which holds `rows` rows in all: the tuned
+// chunk, shortened (to kMinRowChunk at least) until the items fill
+// kRowRounds rounds of the class's resident workgroups.  A class of few
+// narrow jobs otherwise runs a few long items -- each a serial chain of
+// row-load round trips -- on an almost idle GPU, and the classes run one
+// after another.  Results do not depend on the chunk.
+int row_chunk(const lbm_batch* b, int k, int64_t rows)
+{
+    const int64_t per_round = static_cast<int64_t>(kRowRounds) * persistent_grid(b, kRowWidth[k]);
+    const int64_t c = rows / std::max<int64_t>(1, per_round);
+    return static_cast<int>(std::clamp<int64_t>(c, std::min(kMinRowChunk, b->chunk), b->chunk));
+}
+
 // Rebuilds and uploads the job and item tables (between ticks only).
 int upload_tables(lbm_batch* b)
 {
@@ -212,16 +240,23 @@ int upload_tables(lbm_batch* b)
     int rc = ensure_jobs(b, static_cast<int>(b->order.size()) + 1);
     if (rc) return rc;
     std::vector<Item> items[kNumCls];
+    int chunk[5];
+    {
+        int64_t rows[5] = {};
+        for (const HostJob* j : b->order)
+            if (use_rows(b, *j)) rows[row_class(j->d.ny)] += j->d.nx;
+        for (int k = 0; k < 5; k++) chunk[k] = row_chunk(b, k, rows[k]);
+    }
     for (int ji = 0; ji < static_cast<int>(b->order.size()); ji++) {
         const HostJob& j = *b->order[ji];
         b->h_jobs[ji] = device_view(j, b->d.precision, b->d.rho_shift);
         if (use_rows(b, j)) {
             const int cls = row_class(j.d.ny);
-            for (int x0 = 0; x0 < j.d.nx; x0 += b->chunk) {
+            for (int x0 = 0; x0 < j.d.nx; x0 += chunk[cls]) {
                 Item it{};
                 it.job = ji;
                 it.x0 = x0;
-                it.x1 = std::min(j.d.nx, x0 + b->chunk);
+                it.x1 = std::min(j.d.nx, x0 + chunk[cls]);
                 items[cls].push_back(it);
             }
         } else if (use_march(b, j)) {
@@ -272,13 +307,6 @@ int upload_tables(lbm_batch* b)
     b->class_mask = mask;
     b->dirty = false;
     return LBM_OK;
-}
-
-int persistent_grid(const lbm_batch* b, int threads)
-{
-    // enough workgroups to fill every CU at the kernel's occupancy
-    const int per_cu = std::max(1, 2048 / threads);
-    return b->cus * std::min(per_cu, 8);
 }
 
 // One step of every job: row kernels, two-pass kernels, counters.  Only the
