@@ -96,6 +96,32 @@ def test_admission_by_budget():
     assert b.submit(9, (32, 32), 1.0, -5.0, 4, mb.noise_rho((32, 32), 2)) is not None
 
 
+@pytest.mark.parametrize("prec", [mb.FP16S, mb.FP16N])
+def test_row_items_of_every_length_give_the_same_bits(prec):
+    # D2Q9 row items hold at most the launch's chunk of x rows, fewer where
+    # a row-width class holds too few rows in all to fill two rounds of
+    # workgroups (the small mixes of the other tests get 2).  Here the class of rows of
+    # up to 128 nodes holds ~77,000 rows, so caps 1, 5 and 64 give items of
+    # 1, 5 and ~18 rows, with ragged last items (random nx); the node-shift
+    # form also writes each row's reference density from exactly one item.
+    rng = np.random.default_rng(9)
+    jobs = [dict(q=9, shape=(int(rng.integers(400, 700)), 100), tau=float(rng.uniform(0.8, 1.3)),
+                 G=float(rng.uniform(-5.5, -4.5)), steps=7, seed=int(rng.integers(1 << 30)))
+            for _ in range(140)]
+    jobs += [dict(q=9, shape=(300, 50), tau=1.0, G=-5.0, steps=7, seed=3),
+             dict(q=9, shape=(90, 300), tau=0.9, G=-5.2, steps=7, seed=4)]
+    runs = []
+    for cap in (1, 5, 64):
+        b = Batch(precision=prec, steps_per_tick=8)
+        b.set_launch(cap, 1, 1)
+        assert b.tuning["d2q9_rows_kernel"] == 1
+        runs.append(_run(b, jobs))
+    for k in range(len(jobs)):
+        for r in runs[1:]:
+            assert same_bits(runs[0][k]["rho"], r[k]["rho"]), (k, jobs[k])
+            assert same_bits(runs[0][k]["u"], r[k]["u"]), (k, jobs[k])
+
+
 def test_large_rows_take_the_two_pass_path():
     # rows wider than the largest row-kernel workgroup (1024) go to the
     # two-pass kernels; both paths must agree with the oracle
