@@ -34,7 +34,12 @@ namespace {
 enum Cls { kRow64, kRow128, kRow256, kRow512, kRow1024, kTp9Psi, kTp9Col, kTp19Psi, kTp19Col, kMarch19,
            kNumCls };
 constexpr int kRowWidth[5] = {64, 128, 256, 512, 1024};
-constexpr int64_t kTpItem = 8192;   // nodes per two-pass work item
+// Nodes per two-pass work item.  Serving stream (200 requests, 77% of the
+// work in ragged D3Q19 jobs on the two passes), GLUPS / p99 s by item size:
+// 1024 30.5 / 16.6, 2048 33.2 / 15.1, 4096 32.7 / 15.4, 8192 31.5 / 16.0,
+// 16384 29.0 / 17.4, 32768 24.4 / 20.6: short items balance the last
+// round of each persistent launch, very short ones pay per-item overhead.
+constexpr int64_t kTpItem = 2048;
 constexpr int kMarchChunk = 32;     // x planes per D3Q19 march item
 // D2Q9 row items: rounds of resident workgroups a row class's items should
 // fill before its chunk is shortened, and the shortest chunk.  Serving
