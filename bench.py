@@ -30,9 +30,10 @@ import mesobasic_amd as mb  # noqa: E402
 METRIC = "D3Q19 Shan-Chen MLUPS & % HBM roofline; batched-serve p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
 # nodes advanced per step by the serving loop (least remaining work first);
-# measured on MI355X: 48 M keeps 96% of the all-jobs throughput and cuts p50
-# latency 2.6x
-SERVE_WORK_LIMIT = 48_000_000
+# measured on MI355X (200-request stream, p50 / p99 s, GLUPS): all jobs
+# 11.2 / 15.2 / 33.9, 96 M 7.8 / 14.9 / 33.8, 48 M 4.3 / 15.0 / 33.6,
+# 32 M 3.3 / 15.0 / 32.6, 24 M 2.8 / 15.8 / 31.8
+SERVE_WORK_LIMIT = 32_000_000
 
 WORKLOADS = {
     # name: q, shape, precision, tau, G, init
