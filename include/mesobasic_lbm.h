@@ -30,7 +30,8 @@ enum lbm_status {
     LBM_ENOMEM = -2,   /* device allocation failed or over the HBM budget */
     LBM_EHIP = -3,     /* HIP runtime error */
     LBM_ESTATE = -4,   /* call out of order (e.g. step before init) */
-    LBM_ENODEV = -5    /* no usable gfx950 device */
+    LBM_ENODEV = -5,   /* no usable gfx950 device */
+    LBM_EJOB = -6      /* a batch job failed (its mass total went non-finite) */
 };
 
 enum lbm_precision {
@@ -242,15 +243,45 @@ int lbm_batch_submit(lbm_batch* b, const lbm_job_desc* job, const double* rho0, 
 
 /* Runs one tick (steps_per_tick steps of every unfinished job; a job stops
  * exactly at its step count), waits for it, and reports how many jobs are
- * finished and not yet collected. */
+ * finished and not yet collected.  The tick ends with an FP64 mass total of
+ * every job it advanced (the last launch of the tick graph): a job whose
+ * total is not finite -- a NaN or Inf anywhere in its populations, from a
+ * poisoned initial density or a diverging run -- is retired there as
+ * failed, takes no further steps and no further share of the launches, and
+ * leaves the other jobs' results unchanged. */
 int lbm_batch_tick(lbm_batch* b, int* n_finished);
 
-/* Ids of finished, uncollected jobs (up to max). */
+/* Ids of finished, uncollected jobs (up to max), failed ones included. */
 int lbm_batch_finished(lbm_batch* b, int64_t* ids, int max, int* n);
 
 /* Density and velocity (moments of the final populations, FP64) of a
- * finished job into host arrays (any may be NULL); frees the job. */
+ * finished job into host arrays (any may be NULL); frees the job.  For a
+ * failed job the arrays are filled the same way (with its non-finite
+ * values), the job is freed, and the call returns LBM_EJOB with the step of
+ * the failure in lbm_last_error(). */
 int lbm_batch_collect(lbm_batch* b, int64_t job_id, double* rho, double* u, double* v, double* w);
+
+enum lbm_job_state {
+    LBM_JOB_RUNNING = 0,   /* admitted, steps left */
+    LBM_JOB_DONE = 1,      /* ran all its steps; waiting to be collected */
+    LBM_JOB_FAILED = 2     /* retired: its mass total went non-finite */
+};
+
+typedef struct lbm_job_status {
+    int state;              /* enum lbm_job_state */
+    int64_t steps_done;     /* steps run (for a failed job: at the tick that saw the failure) */
+    int64_t steps_finite;   /* steps after which the last mass check was finite */
+    double mass;            /* FP64 total mass at the last check (NaN before the first tick) */
+} lbm_job_status;
+
+/* State of an admitted, uncollected job. */
+int lbm_batch_job_status(const lbm_batch* b, int64_t job_id, lbm_job_status* st);
+
+/* Cancels an admitted job, running or finished: frees its device memory at
+ * once (ticks are synchronous, so none of its work is in flight) and
+ * removes it from the work tables before the next tick.  The other jobs'
+ * results do not change. */
+int lbm_batch_cancel(lbm_batch* b, int64_t job_id);
 
 /* Jobs admitted and not collected; device bytes they hold; steps run. */
 int lbm_batch_active(const lbm_batch* b);

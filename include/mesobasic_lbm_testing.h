@@ -29,6 +29,17 @@ int lbm_probe_tile_map(int nty, int ntz, int ncx, int order, int* ty, int* tz, i
  * achievable bandwidth. */
 int lbm_probe_copy(int64_t bytes, int reps, int mode, double* ms);
 
+/* Device allocations the library has made so far: every hipMalloc of the
+ * lattice handles and the batch engine goes through one wrapper that counts
+ * it (the probes above allocate on their own and are not counted). */
+int64_t lbm_testing_alloc_count(void);
+
+/* Injected allocation failure: the next `after` device allocations succeed
+ * and the one after them fails as hipErrorOutOfMemory would (once, then the
+ * hook disarms); a negative value disarms it.  Lets a test force the
+ * allocation-failure paths of lbm_create and lbm_batch_submit. */
+void lbm_testing_fail_alloc(int64_t after);
+
 #ifdef __cplusplus
 }
 #endif

@@ -22,6 +22,8 @@ struct Job {
     float tau, G, rs;    // relaxation time, coupling, storage shift
     int first;           // 1 until the first step has run (device-updated)
     int64_t remaining;   // steps still to run (device-updated, 0 = idle)
+    int64_t done;        // steps run (device-updated); the current buffer is P[(done ^ phase) & 1]
+    int q;               // 9 or 19
 };
 
 // Unit of work of one launch: a range of x planes (D2Q9 row kernel), of
@@ -56,8 +58,17 @@ hipError_t twopass_collide(int prec, int q, const Tables& t, int parity, int gri
 // D3Q19 jobs in one fused pass (psi window in LDS); agrees with the two
 // passes above to rounding.
 hipError_t d3q19_march_step(int prec, const Tables& t, int parity, int grid, hipStream_t s);
-// After a step: remaining -= 1 and first = 0 for every stepped job.
-hipError_t advance_counters(Job* jobs, int njobs, hipStream_t s);
+// After a step: remaining -= 1, done += 1 and first = 0 for every stepped
+// job; also clears *mass_ticket, the counter of the next job_mass launch.
+hipError_t advance_counters(Job* jobs, int njobs, unsigned* mass_ticket, hipStream_t s);
+// FP64 mass of each item's node range of its job's current populations
+// (sum of the pulled f = stored/S + R over the range, as the conservation
+// sums of the single-lattice forms) into part[item]; a persistent launch
+// over t's items with the ticket counter *ticket (cleared by the
+// advance_counters launch before it).  A non-finite value anywhere in the
+// range makes the partial non-finite.
+hipError_t job_mass(int prec, const Tables& t, unsigned* ticket, double* part, int grid,
+                    hipStream_t s);
 // Initial equilibrium of one job from rho0 (device FP32): writes the input
 // buffer of the job's next launch so that its first pull yields feq.
 hipError_t init_job(int prec, int q, const Job& j, int buf, hipStream_t s);

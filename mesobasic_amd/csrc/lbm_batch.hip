@@ -23,6 +23,7 @@
 // The row-marching and two-pass kernels evaluate the same expressions in the
 // same order (shared helpers below; the file is compiled without FP
 // contraction), so a job's bits do not depend on which of them runs.
+#include <algorithm>
 #include <type_traits>
 
 #include "../../include/mesobasic_lbm.h"
@@ -462,12 +463,55 @@ __global__ void __launch_bounds__(kMarchTY * kMarchTZ) __attribute__((amdgpu_wav
     }
 }
 
-__global__ void k_advance(Job* jobs, int njobs)
+__global__ void k_advance(Job* jobs, int njobs, unsigned* mass_ticket)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *mass_ticket = 0u;
     if (i < njobs && jobs[i].remaining > 0) {
         jobs[i].remaining -= 1;
+        jobs[i].done += 1;
         jobs[i].first = 0;
+    }
+}
+
+// Per-item FP64 mass of a job's current populations.  Every stored value is
+// pulled by exactly one node (periodic wrap is a bijection), so the sum over
+// all slots of d/S plus R per node is the total mass of the pulled f; the
+// slots are read plane by plane, coalesced.  Tree reduction in the block:
+// deterministic for a given item.
+constexpr int kMassBlock = 256;
+
+template <typename SP, bool NS>
+__global__ void __launch_bounds__(kMassBlock) k_job_mass(Tables t, unsigned* ticket, double* part)
+{
+    __shared__ int slot_s;
+    __shared__ double red[kMassBlock];
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) slot_s = static_cast<int>(atomicAdd(ticket, 1u));
+        __syncthreads();
+        const int it = slot_s;
+        if (it >= *t.nitems) break;
+        const Item item = t.items[it];
+        const Job& job = t.jobs[item.job];
+        const int b = static_cast<int>((job.done ^ job.phase) & 1);
+        const auto* P = static_cast<const typename SP::T*>(job.P[b]);
+        const float* R = NS ? job.R[b] : nullptr;
+        const int q = job.q;
+        const double rs = job.rs;
+        double m = 0.0;
+        for (int64_t c = item.c0 + threadIdx.x; c < item.c1; c += kMassBlock) {
+            double s = 0.0;
+            for (int n = 0; n < q; n++) s += static_cast<double>(SP::d(P[n * job.ps + c]));
+            m += s * (1.0 / kScale<SP, NS>) + (NS ? static_cast<double>(R[c]) : rs);
+        }
+        red[threadIdx.x] = m;
+        __syncthreads();
+        for (int k = kMassBlock / 2; k > 0; k >>= 1) {
+            if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) part[it] = red[0];
     }
 }
 
@@ -624,9 +668,20 @@ hipError_t d3q19_march_step(int prec, const Tables& t, int parity, int grid, hip
     return hipGetLastError();
 }
 
-hipError_t advance_counters(Job* jobs, int njobs, hipStream_t s)
+hipError_t advance_counters(Job* jobs, int njobs, unsigned* mass_ticket, hipStream_t s)
 {
-    if (njobs > 0) k_advance<<<(njobs + 255) / 256, 256, 0, s>>>(jobs, njobs);
+    k_advance<<<(std::max(njobs, 1) + 255) / 256, 256, 0, s>>>(jobs, njobs, mass_ticket);
+    return hipGetLastError();
+}
+
+hipError_t job_mass(int prec, const Tables& t, unsigned* ticket, double* part, int grid,
+                    hipStream_t s)
+{
+    by_prec(prec, [&](auto sp, auto ns) {
+        using SP = decltype(sp);
+        constexpr bool NS = decltype(ns)::value;
+        k_job_mass<SP, NS><<<grid, kMassBlock, 0, s>>>(t, ticket, part);
+    });
     return hipGetLastError();
 }
 

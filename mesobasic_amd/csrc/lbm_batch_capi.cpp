@@ -29,10 +29,11 @@ namespace {
                                   __FILE__, __LINE__);                                       \
     } while (0)
 
-// Kernel classes of one step: D2Q9 row-marching kernels by row width, and the
-// two-pass kernels (psi then collide) of each stencil.
+// Kernel classes of one step: D2Q9 row-marching kernels by row width, the
+// two-pass kernels (psi then collide) of each stencil and the D3Q19 march;
+// kMass is the per-tick mass check that follows the steps of a tick.
 enum Cls { kRow64, kRow128, kRow256, kRow512, kRow1024, kTp9Psi, kTp9Col, kTp19Psi, kTp19Col, kMarch19,
-           kNumCls };
+           kMass, kNumCls };
 constexpr int kRowWidth[5] = {64, 128, 256, 512, 1024};
 // Nodes per two-pass work item.  Serving stream (200 requests, 77% of the
 // work in ragged D3Q19 jobs on the two passes), GLUPS / p99 s by item size:
@@ -41,6 +42,7 @@ constexpr int kRowWidth[5] = {64, 128, 256, 512, 1024};
 // round of each persistent launch, very short ones pay per-item overhead.
 constexpr int64_t kTpItem = 2048;
 constexpr int kMarchChunk = 32;     // x planes per D3Q19 march item
+constexpr int64_t kMassItem = 8192; // nodes per mass-check item
 // D2Q9 row items: rounds of resident workgroups a row class's items should
 // fill before its chunk is shortened, and the shortest chunk.  Serving
 // stream (200 requests, D2Q9 128..1024 per side): p50 4.66 -> 4.50 s, p99
@@ -62,6 +64,11 @@ struct HostJob {
     int64_t remaining;
     int first = 1;
     bool finished = false;
+    bool failed = false;        // mass went non-finite; retired at the tick that saw it
+    int64_t done = 0;           // steps run
+    int64_t steps_finite = 0;   // steps after which the last mass check was finite
+    double mass = 0.0;          // FP64 total mass at the last check
+    bool mass_valid = false;
 };
 
 int64_t plane_stride(int64_t n, int elem)
@@ -114,6 +121,12 @@ struct lbm_batch {
     bool graph_valid = false;
     // pinned staging for per-tick counters
     Job* h_jobs = nullptr;
+    // per-tick mass check: one FP64 partial per kMass item, the job (table
+    // index) of each item, and the pinned copy the host adds up
+    double* d_mass = nullptr;
+    double* h_mass = nullptr;
+    int cap_mass = 0;
+    std::vector<int> mass_job;
 };
 
 namespace {
@@ -153,7 +166,7 @@ int ensure_items(lbm_batch* b, int cls, int n)
     if (b->d_items[cls]) (void)hipFree(b->d_items[cls]);
     b->d_items[cls] = nullptr;
     b->cap_items[cls] = 0;
-    BHIP(hipMalloc(&b->d_items[cls], sizeof(Item) * cap));
+    BHIP(lbm::dev_malloc(&b->d_items[cls], sizeof(Item) * cap));
     b->cap_items[cls] = cap;
     b->graph_valid = false;   // kernel arguments hold the table pointers
     return LBM_OK;
@@ -168,7 +181,7 @@ int ensure_jobs(lbm_batch* b, int n)
     b->d_jobs = nullptr;
     b->h_jobs = nullptr;
     b->cap_jobs = 0;
-    BHIP(hipMalloc(&b->d_jobs, sizeof(Job) * cap));
+    BHIP(lbm::dev_malloc(&b->d_jobs, sizeof(Job) * cap));
     BHIP(hipHostMalloc(&b->h_jobs, sizeof(Job) * cap, hipHostMallocDefault));
     static_assert(sizeof(Job) % 4 == 0, "job table is cleared as 32-bit words");
     // unused table entries are cleared: the per-step counter kernel walks the
@@ -177,6 +190,22 @@ int ensure_jobs(lbm_batch* b, int n)
                               static_cast<int>(sizeof(Job) * cap / 4), b->stream));
     b->cap_jobs = cap;
     b->graph_valid = false;
+    return LBM_OK;
+}
+
+int ensure_mass(lbm_batch* b, int n)
+{
+    if (b->cap_mass >= n) return LBM_OK;
+    const int cap = std::max(n, 2 * b->cap_mass + 256);
+    if (b->d_mass) (void)hipFree(b->d_mass);
+    if (b->h_mass) (void)hipHostFree(b->h_mass);
+    b->d_mass = nullptr;
+    b->h_mass = nullptr;
+    b->cap_mass = 0;
+    BHIP(lbm::dev_malloc(&b->d_mass, sizeof(double) * cap));
+    BHIP(hipHostMalloc(&b->h_mass, sizeof(double) * cap, hipHostMallocDefault));
+    b->cap_mass = cap;
+    b->graph_valid = false;   // the mass kernel's argument holds the pointer
     return LBM_OK;
 }
 
@@ -200,6 +229,8 @@ Job device_view(const HostJob& j, int prec, double rs)
     v.rs = static_cast<float>((prec == LBM_FP32 || node_shift(prec)) ? 0.0 : rs);
     v.first = j.first;
     v.remaining = j.remaining;
+    v.done = j.done;
+    v.q = j.d.q;
     return v;
 }
 
@@ -245,6 +276,7 @@ int upload_tables(lbm_batch* b)
     int rc = ensure_jobs(b, static_cast<int>(b->order.size()) + 1);
     if (rc) return rc;
     std::vector<Item> items[kNumCls];
+    b->mass_job.clear();
     int chunk[5];
     {
         int64_t rows[5] = {};
@@ -255,6 +287,14 @@ int upload_tables(lbm_batch* b)
     for (int ji = 0; ji < static_cast<int>(b->order.size()); ji++) {
         const HostJob& j = *b->order[ji];
         b->h_jobs[ji] = device_view(j, b->d.precision, b->d.rho_shift);
+        for (int64_t c0 = 0; c0 < j.n; c0 += kMassItem) {
+            Item it{};
+            it.job = ji;
+            it.c0 = c0;
+            it.c1 = std::min(j.n, c0 + kMassItem);
+            items[kMass].push_back(it);
+            b->mass_job.push_back(ji);
+        }
         if (use_rows(b, j)) {
             const int cls = row_class(j.d.ny);
             for (int x0 = 0; x0 < j.d.nx; x0 += chunk[cls]) {
@@ -288,12 +328,16 @@ int upload_tables(lbm_batch* b)
             }
         }
     }
-    // largest jobs first, so that the long items start early
+    rc = ensure_mass(b, static_cast<int>(items[kMass].size()) + 1);
+    if (rc) return rc;
+    // largest jobs first, so that the long items start early (the mass items
+    // stay in job order: the host adds each job's partials in that order)
     unsigned mask = 0;
     for (int c = 0; c < kNumCls; c++) {
-        std::stable_sort(items[c].begin(), items[c].end(), [&](const Item& a, const Item& z) {
-            return b->order[a.job]->n > b->order[z.job]->n;
-        });
+        if (c != kMass)
+            std::stable_sort(items[c].begin(), items[c].end(), [&](const Item& a, const Item& z) {
+                return b->order[a.job]->n > b->order[z.job]->n;
+            });
         rc = ensure_items(b, c, static_cast<int>(items[c].size()) + 1);
         if (rc) return rc;
         b->nitems[c] = static_cast<int>(items[c].size());
@@ -339,7 +383,18 @@ int launch_step(lbm_batch* b, int parity)
                                           persistent_grid(b, lbm::batch::kMarchTY * lbm::batch::kMarchTZ),
                                           b->stream));
     }
-    BHIP(lbm::batch::advance_counters(b->d_jobs, b->cap_jobs, b->stream));
+    BHIP(lbm::batch::advance_counters(b->d_jobs, b->cap_jobs, b->d_tickets + 2 * kMass, b->stream));
+    return LBM_OK;
+}
+
+// The tick's mass check, after its last step (the last advance_counters
+// launch has cleared its ticket counter).
+int launch_mass(lbm_batch* b)
+{
+    if (!(b->class_mask & (1u << kMass))) return LBM_OK;
+    Tables tm{b->d_jobs, b->d_items[kMass], b->d_nitems + kMass, nullptr};
+    BHIP(lbm::batch::job_mass(b->d.precision, tm, b->d_tickets + 2 * kMass, b->d_mass,
+                              persistent_grid(b, 256), b->stream));
     return LBM_OK;
 }
 
@@ -356,6 +411,7 @@ int prepare_tick(lbm_batch* b)
             BHIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
             int rc = LBM_OK;
             for (int s = 0; s < K && rc == LBM_OK; s++) rc = launch_step(b, s & 1);
+            if (rc == LBM_OK) rc = launch_mass(b);
             hipError_t e = hipStreamEndCapture(b->stream, &g);
             if (rc) return rc;
             if (e != hipSuccess) return lbm::set_error(LBM_EHIP, "graph capture: %s", hipGetErrorString(e));
@@ -378,6 +434,7 @@ int run_tick(lbm_batch* b)
             int rc = launch_step(b, s & 1);
             if (rc) return rc;
         }
+        return launch_mass(b);
     }
     return LBM_OK;
 }
@@ -431,8 +488,8 @@ int lbm_batch_create(const lbm_batch_desc* desc, lbm_batch** out)
         b->d.hbm_budget = static_cast<int64_t>(0.9 * static_cast<double>(free_b));
     }
     BHIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-    BHIP(hipMalloc(&b->d_nitems, sizeof(int) * kNumCls));
-    BHIP(hipMalloc(&b->d_tickets, sizeof(unsigned) * 2 * kNumCls));
+    BHIP(lbm::dev_malloc(&b->d_nitems, sizeof(int) * kNumCls));
+    BHIP(lbm::dev_malloc(&b->d_tickets, sizeof(unsigned) * 2 * kNumCls));
     BHIP(lbm::batch::zero_u32(b->d_tickets, 2 * kNumCls, b->stream));
     BHIP(hipStreamSynchronize(b->stream));
     if (b->d.autotune) {
@@ -462,6 +519,8 @@ void lbm_batch_destroy(lbm_batch* b)
         if (p) (void)hipFree(p);
     if (b->d_nitems) (void)hipFree(b->d_nitems);
     if (b->d_tickets) (void)hipFree(b->d_tickets);
+    if (b->d_mass) (void)hipFree(b->d_mass);
+    if (b->h_mass) (void)hipHostFree(b->h_mass);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -506,13 +565,13 @@ int lbm_batch_submit(lbm_batch* b, const lbm_job_desc* job, const double* rho0, 
         return code;
     };
     for (int k = 0; k < 2; k++)
-        if (hipMalloc(&j->P[k], pop) != hipSuccess)
+        if (lbm::dev_malloc(&j->P[k], pop) != hipSuccess)
             return undo(lbm::set_error(LBM_ENOMEM, "hipMalloc of job populations failed"));
-    if (hipMalloc(&j->psi, n * 4) != hipSuccess || hipMalloc(&j->rho0, n * 4) != hipSuccess)
+    if (lbm::dev_malloc(&j->psi, n * 4) != hipSuccess || lbm::dev_malloc(&j->rho0, n * 4) != hipSuccess)
         return undo(lbm::set_error(LBM_ENOMEM, "hipMalloc of job fields failed"));
     if (node_shift(b->d.precision))
         for (int k = 0; k < 2; k++)
-            if (hipMalloc(&j->R[k], n * 4) != hipSuccess)
+            if (lbm::dev_malloc(&j->R[k], n * 4) != hipSuccess)
                 return undo(lbm::set_error(LBM_ENOMEM, "hipMalloc of job reference density failed"));
     std::vector<float> r32(static_cast<size_t>(n));
     for (int64_t c = 0; c < n; c++) r32[c] = static_cast<float>(rho0[c]);
@@ -550,14 +609,31 @@ int lbm_batch_tick(lbm_batch* b, int* n_finished)
         if (rc) return rc;
         BHIP(hipMemcpyAsync(b->h_jobs, b->d_jobs, sizeof(Job) * b->order.size(),
                             hipMemcpyDeviceToHost, b->stream));
+        if (b->nitems[kMass] > 0)
+            BHIP(hipMemcpyAsync(b->h_mass, b->d_mass, sizeof(double) * b->nitems[kMass],
+                                hipMemcpyDeviceToHost, b->stream));
     }
     BHIP(hipStreamSynchronize(b->stream));
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     b->steps_run += b->d.steps_per_tick;
+    std::vector<double> mass(b->order.size(), 0.0);
+    for (int k = 0; k < b->nitems[kMass]; k++) mass[b->mass_job[k]] += b->h_mass[k];
     for (int ji = 0; ji < static_cast<int>(b->order.size()); ji++) {
         HostJob* j = b->order[ji];
         j->remaining = b->h_jobs[ji].remaining;
         j->first = b->h_jobs[ji].first;
+        j->done = b->h_jobs[ji].done;
+        j->mass = mass[ji];
+        j->mass_valid = true;
+        if (!std::isfinite(mass[ji])) {
+            // a non-finite value in the job's populations: retire it now, so
+            // it stops taking steps, and report it as failed
+            j->failed = true;
+            if (!j->finished) b->dirty = true;
+            j->finished = true;
+        } else {
+            j->steps_finite = j->done;
+        }
         if (j->remaining == 0 && !j->finished) {
             j->finished = true;
             b->dirty = true;   // finished jobs leave the work tables
@@ -603,9 +679,9 @@ int lbm_batch_collect(lbm_batch* b, int64_t job_id, double* rho, double* u, doub
     if (!j->finished) return lbm::set_error(LBM_ESTATE, "job %lld has not finished", (long long)job_id);
     const int64_t n = j->n;
     double* d = nullptr;
-    BHIP(hipMalloc(&d, 4 * n * 8));
+    BHIP(lbm::dev_malloc(&d, 4 * n * 8));
     Job view = device_view(*j, b->d.precision, b->d.rho_shift);
-    const int buf = static_cast<int>(j->d.steps & 1);   // the buffer the last step wrote
+    const int buf = static_cast<int>(j->done & 1);   // the buffer the last step wrote
     hipError_t e = lbm::batch::job_moments(b->d.precision, j->d.q, view, buf, d, d + n, d + 2 * n,
                                            d + 3 * n, b->stream);
     double* outs[4] = {rho, u, v, w};
@@ -614,8 +690,41 @@ int lbm_batch_collect(lbm_batch* b, int64_t job_id, double* rho, double* u, doub
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
     (void)hipFree(d);
     if (e != hipSuccess) return lbm::set_error(LBM_EHIP, "collect: %s", hipGetErrorString(e));
+    const bool failed = j->failed;
+    const long long id = static_cast<long long>(j->id), at = static_cast<long long>(j->done),
+                    ok = static_cast<long long>(j->steps_finite);
     free_job(b, j);
     b->jobs.erase(it);
+    if (failed)
+        return lbm::set_error(LBM_EJOB,
+                              "job %lld failed: non-finite mass after step %lld (finite after step %lld)",
+                              id, at, ok);
+    return LBM_OK;
+}
+
+int lbm_batch_job_status(const lbm_batch* b, int64_t job_id, lbm_job_status* st)
+{
+    if (!b || !st) return lbm::set_error(LBM_EINVAL, "lbm_batch_job_status: NULL argument");
+    auto it = b->jobs.find(job_id);
+    if (it == b->jobs.end()) return lbm::set_error(LBM_EINVAL, "unknown job %lld", (long long)job_id);
+    const HostJob& j = *it->second;
+    st->state = j.failed ? LBM_JOB_FAILED : (j.finished ? LBM_JOB_DONE : LBM_JOB_RUNNING);
+    st->steps_done = j.done;
+    st->steps_finite = j.steps_finite;
+    st->mass = j.mass_valid ? j.mass : std::nan("");
+    return LBM_OK;
+}
+
+int lbm_batch_cancel(lbm_batch* b, int64_t job_id)
+{
+    if (!b) return lbm::set_error(LBM_EINVAL, "lbm_batch_cancel: NULL batch");
+    auto it = b->jobs.find(job_id);
+    if (it == b->jobs.end()) return lbm::set_error(LBM_EINVAL, "unknown job %lld", (long long)job_id);
+    // ticks are synchronous, so no launch of this batch is in flight here
+    free_job(b, it->second.get());
+    b->jobs.erase(it);
+    b->order.clear();
+    b->dirty = true;   // the job leaves the work tables before the next tick
     return LBM_OK;
 }
 

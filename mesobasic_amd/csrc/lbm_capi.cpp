@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -14,11 +15,14 @@
 #include <vector>
 
 #include "lbm_fused.h"
+#include "lbm_handle.h"
 #include "lbm_internal.h"
 #include "lbm_lowp.h"
 
 namespace {
 thread_local std::string g_err;
+std::atomic<int64_t> g_alloc_count{0};   // device allocations made (test hook)
+std::atomic<int64_t> g_fail_after{-1};   // allocations let through before one fails; -1: none
 }  // namespace
 
 namespace lbm {
@@ -32,6 +36,18 @@ int set_error(int code, const char* fmt, ...)
     va_end(ap);
     g_err = buf;
     return code;
+}
+
+hipError_t dev_malloc(void** p, size_t bytes)
+{
+    *p = nullptr;
+    // injected failure (lbm_testing_fail_alloc): counts down, fails once at 0
+    int64_t k = g_fail_after.load();
+    while (k >= 0 && !g_fail_after.compare_exchange_weak(k, k - 1)) {
+    }
+    if (k == 0) return hipErrorOutOfMemory;
+    g_alloc_count.fetch_add(1);
+    return hipMalloc(p, bytes);
 }
 }  // namespace lbm
 
@@ -60,32 +76,6 @@ int64_t plane_stride(int64_t n, int elem_bytes)
 
 }  // namespace
 
-struct lbm_handle {
-    lbm_desc d{};
-    lbm::Grid g{};
-    lbm::StepArgs a{};
-    int elem = 8;             // bytes per stored population
-    void* buf[2] = {nullptr, nullptr};
-    int kind[2] = {lbm::kHoldsF, lbm::kHoldsF};
-    int cur = 0;
-    double* psi = nullptr;    // FP64 two-pass density field
-    float* rref[2] = {nullptr, nullptr};  // node-shift reference density of buf[0] / buf[1]
-    double* rho0 = nullptr;   // initial density (reference r array at N=0)
-    double* tmp = nullptr;    // staging for fields / f / partials
-    int64_t tmp_bytes = 0;
-    hipStream_t own = nullptr;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    int64_t steps = 0;
-    bool initialised = false;
-    bool first_step_stored = false;  // LBfluid2D order: step 1 uses rho0, u=0
-    bool prev_valid = false;         // buf[1-cur] holds f(N-1)
-    bool ref_is_rho0 = false;        // reference arrays still hold rho0, u = 0
-    int kernel = LBM_KERNEL_TWO_PASS;
-    lbm::FusedTuning tune{-1, 64, 3, 1};  // fused kernel launch configuration
-    int64_t bytes = 0;
-};
-
 namespace {
 
 int ensure_tmp(lbm_handle* h, int64_t bytes)
@@ -94,12 +84,10 @@ int ensure_tmp(lbm_handle* h, int64_t bytes)
     if (h->tmp) hipFree(h->tmp);
     h->tmp = nullptr;
     h->tmp_bytes = 0;
-    HIP_OK(hipMalloc(&h->tmp, bytes));
+    HIP_OK(lbm::dev_malloc(&h->tmp, bytes));
     h->tmp_bytes = bytes;
     return LBM_OK;
 }
-
-bool is_lowp(const lbm_handle* h) { return h->d.precision != LBM_FP64; }
 
 int run_step(lbm_handle* h)
 {
@@ -197,6 +185,10 @@ int lbm_create(const lbm_desc* desc, lbm_handle** out)
         return fail(LBM_ENODEV, "lbm_create: no HIP device");
     auto* h = new lbm_handle();
     h->d = *desc;
+    if (hipGetDevice(&h->device) != hipSuccess) {
+        delete h;
+        return fail(LBM_ENODEV, "lbm_create: no current HIP device");
+    }
     h->elem = (desc->precision == LBM_FP64) ? 8 : (desc->precision == LBM_FP32 ? 4 : 2);
     h->g.nx = desc->nx; h->g.ny = desc->ny; h->g.nz = desc->nz;
     h->g.n = static_cast<int64_t>(desc->nx) * desc->ny * desc->nz;
@@ -249,7 +241,7 @@ int lbm_create(const lbm_desc* desc, lbm_handle** out)
         return code;
     };
     for (int b = 0; b < 2; b++) {
-        hipError_t e = hipMalloc(&h->buf[b], pop_bytes);
+        hipError_t e = lbm::dev_malloc(&h->buf[b], pop_bytes);
         if (e != hipSuccess)
             return cleanup(fail(LBM_ENOMEM, "hipMalloc(%lld B) for populations: %s",
                                 (long long)pop_bytes, hipGetErrorString(e)));
@@ -259,16 +251,16 @@ int lbm_create(const lbm_desc* desc, lbm_handle** out)
         // psi field: the two-pass kernels use it every step, and every
         // handle runs them for initialisation and for the LBfluid2D-order
         // first step, so it is always allocated.
-        if (hipMalloc(&h->psi, h->g.n * 8) != hipSuccess)
+        if (lbm::dev_malloc(&h->psi, h->g.n * 8) != hipSuccess)
             return cleanup(fail(LBM_ENOMEM, "hipMalloc for psi failed"));
         h->bytes += h->g.n * 8;
     }
-    if (hipMalloc(&h->rho0, h->g.n * 8) != hipSuccess)
+    if (lbm::dev_malloc(&h->rho0, h->g.n * 8) != hipSuccess)
         return cleanup(fail(LBM_ENOMEM, "hipMalloc for rho0 failed"));
     h->bytes += h->g.n * 8;
     if (lbm::lowp_node_shift(desc->precision)) {
         for (int b = 0; b < 2; b++)
-            if (hipMalloc(&h->rref[b], h->g.n * 4) != hipSuccess)
+            if (lbm::dev_malloc(&h->rref[b], h->g.n * 4) != hipSuccess)
                 return cleanup(fail(LBM_ENOMEM, "hipMalloc for the node-shift density failed"));
         h->bytes += 2 * h->g.n * 4;
     }
@@ -531,6 +523,10 @@ int lbm_synchronize(lbm_handle* h)
 int64_t lbm_device_bytes(const lbm_handle* h) { return h ? h->bytes : 0; }
 
 int lbm_kernel_in_use(const lbm_handle* h) { return h ? h->kernel : -1; }
+
+int64_t lbm_testing_alloc_count(void) { return g_alloc_count.load(); }
+
+void lbm_testing_fail_alloc(int64_t after) { g_fail_after.store(after < 0 ? -1 : after); }
 
 int lbm_fill_noise(uint32_t seed, int64_t n, double mean, double amp, double* out)
 {

@@ -291,9 +291,8 @@ __global__ void __launch_bounds__(kBlock)
     for (int n = 0; n < Q; n++) P[n * g.ps + c] = aos[c * Q + n];
 }
 
-// Fixed-shape block reduction: deterministic for a given lattice.
-constexpr int kConsBlock = 256;
-constexpr int kConsPerThread = 16;
+// Fixed-shape block reduction (kConsBlock x kConsPerThread nodes per
+// block, lbm_internal.h): deterministic for a given lattice.
 
 template <int Q, bool WALLS>
 __global__ void __launch_bounds__(kConsBlock)
@@ -405,11 +404,7 @@ hipError_t fp64_set_f(const StepArgs& a, const Grid& g, const double* aos, doubl
     return hipGetLastError();
 }
 
-int fp64_conservation_blocks(const Grid& g)
-{
-    const int64_t per = static_cast<int64_t>(kConsBlock) * kConsPerThread;
-    return static_cast<int>((g.n + per - 1) / per);
-}
+int fp64_conservation_blocks(const Grid& g) { return static_cast<int>(conservation_blocks(g.n)); }
 
 hipError_t fp64_conservation(const StepArgs& a, const Grid& g, const double* P, int kind,
                              double* partials, int* nblocks, hipStream_t s)

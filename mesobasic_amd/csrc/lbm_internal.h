@@ -10,6 +10,25 @@ namespace lbm {
 // Sets the calling thread's lbm_last_error() message; returns code.
 int set_error(int code, const char* fmt, ...);
 
+// Every device allocation the library makes goes through dev_malloc, so a
+// test can count them and inject a failure (lbm_testing_fail_alloc).
+hipError_t dev_malloc(void** p, size_t bytes);
+template <typename T> inline hipError_t dev_malloc(T** p, size_t bytes)
+{
+    return dev_malloc(reinterpret_cast<void**>(p), bytes);
+}
+
+// Conservation sums (FP64 and low-precision forms alike): blocks of
+// kConsBlock threads, each summing kConsPerThread nodes; one partial
+// (mass, jx, jy, jz) per block, added on the host in block order.
+constexpr int kConsBlock = 256;
+constexpr int kConsPerThread = 16;
+inline int64_t conservation_blocks(int64_t n)
+{
+    const int64_t per = static_cast<int64_t>(kConsBlock) * kConsPerThread;
+    return (n + per - 1) / per;
+}
+
 // Lattice geometry on the device.  Node index c = (x*ny + y)*nz + z
 // (nz = 1 in 2-D), the interior order of the reference arrays without the
 // ghost shell; population n of node c lives at n*ps + c (structure of

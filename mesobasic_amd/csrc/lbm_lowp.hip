@@ -280,9 +280,8 @@ __global__ void __launch_bounds__(kBlock)
     });
 }
 
-constexpr int kConsBlock = 256;
-constexpr int kConsPerThread = 16;
-
+// Fixed-shape block reduction (kConsBlock x kConsPerThread nodes per
+// block, lbm_internal.h), the same blocks as the FP64 one.
 template <int Q, bool WALLS, typename SP, bool NS>
 __global__ void __launch_bounds__(kConsBlock)
     k_conservation(const typename SP::T* __restrict__ P, int kind, Grid g, Ref<NS> rin,
@@ -477,8 +476,7 @@ hipError_t lowp_conservation(int precision, const StepArgs& a, const Grid& g, do
                              const void* P, int kind, const float* rin, double* partials,
                              int* nblocks, hipStream_t s)
 {
-    const int64_t per = static_cast<int64_t>(kConsBlock) * kConsPerThread;
-    const int nb = static_cast<int>((g.n + per - 1) / per);
+    const int nb = static_cast<int>(conservation_blocks(g.n));
     *nblocks = nb;
     const float rs = shift_of(precision, rho_shift);
     LOWP_DISPATCH(precision, a, ({

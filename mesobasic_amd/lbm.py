@@ -17,6 +17,8 @@ LIB_PATH = os.path.join(HERE, "libmesobasic_lbm.so")
 FP64, FP32, FP16S, BF16S, FP16N, BF16N = 0, 1, 2, 3, 4, 5
 KERNEL_AUTO, KERNEL_TWO_PASS, KERNEL_FUSED = 0, 1, 2
 FIELDS_REFERENCE, FIELDS_CURRENT = 0, 1
+JOB_RUNNING, JOB_DONE, JOB_FAILED = 0, 1, 2
+ENOMEM, EJOB = -2, -6
 PRECISIONS = {"fp64": FP64, "fp32": FP32, "fp16": FP16S, "bf16": BF16S, "fp16n": FP16N,
               "bf16n": BF16N}
 
@@ -45,6 +47,11 @@ class BatchDesc(ctypes.Structure):
 class JobDesc(ctypes.Structure):
     _fields_ = [("q", ctypes.c_int), ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
                 ("tau", ctypes.c_double), ("G", ctypes.c_double), ("steps", ctypes.c_int64)]
+
+
+class JobStatus(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int), ("steps_done", ctypes.c_int64),
+                ("steps_finite", ctypes.c_int64), ("mass", ctypes.c_double)]
 
 
 class Tuning(ctypes.Structure):
@@ -105,6 +112,10 @@ def load():
         "lbm_batch_set_work_limit": (ctypes.c_int, [P, ctypes.c_int64]),
         "lbm_batch_set_launch": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
         "lbm_batch_get_launch": (ctypes.c_int, [P] + [ctypes.POINTER(ctypes.c_int)] * 3),
+        "lbm_batch_job_status": (ctypes.c_int, [P, ctypes.c_int64, ctypes.POINTER(JobStatus)]),
+        "lbm_batch_cancel": (ctypes.c_int, [P, ctypes.c_int64]),
+        "lbm_testing_alloc_count": (ctypes.c_int64, []),
+        "lbm_testing_fail_alloc": (None, [ctypes.c_int64]),
         "lbm_probe_copy": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, ctypes.c_int, _D]),
         "lbm_probe_tile_map": (ctypes.c_int, [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_int)] * 3),
     }
@@ -162,6 +173,17 @@ def probe_copy(nbytes, reps=5, mode=0):
     ms = ctypes.c_double()
     _check(lib.lbm_probe_copy(int(nbytes), int(reps), int(mode), ctypes.byref(ms)))
     return ms.value
+
+
+def alloc_count():
+    """Device allocations the library has made (test hook)."""
+    return load().lbm_testing_alloc_count()
+
+
+def fail_alloc(after):
+    """Injects one device-allocation failure after `after` successful ones;
+    a negative value disarms it (test hook)."""
+    load().lbm_testing_fail_alloc(int(after))
 
 
 def probe_tile_map(nty, ntz, ncx, order=2):
@@ -330,11 +352,23 @@ class Batch:
         r = np.ascontiguousarray(rho0, dtype=np.float64).reshape(-1)
         jid = ctypes.c_int64()
         rc = self._lib.lbm_batch_submit(self._h, ctypes.byref(d), _dptr(r), ctypes.byref(jid))
-        if rc == -2:
+        if rc == ENOMEM:
             return None
         _check(rc)
         self.shapes[jid.value] = tuple(int(s) for s in shape)
         return jid.value
+
+    def status(self, job_id):
+        """dict(state, steps_done, steps_finite, mass) of an uncollected job."""
+        st = JobStatus()
+        _check(self._lib.lbm_batch_job_status(self._h, int(job_id), ctypes.byref(st)))
+        return dict(state=st.state, steps_done=st.steps_done, steps_finite=st.steps_finite,
+                    mass=st.mass)
+
+    def cancel(self, job_id):
+        """Drops an admitted job (running or finished) and frees its memory."""
+        _check(self._lib.lbm_batch_cancel(self._h, int(job_id)))
+        self.shapes.pop(job_id, None)
 
     def tick(self):
         n = ctypes.c_int()
@@ -348,11 +382,23 @@ class Batch:
         return [ids[i] for i in range(n.value)]
 
     def collect(self, job_id):
-        shape = self.shapes.pop(job_id)
+        """Fields of a finished job (frees it).  A job the engine retired as
+        failed comes back with failed=True, its step counts and its
+        (non-finite) fields; every other error raises."""
+        shape = self.shapes[job_id]
+        st = self.status(job_id)
         n = int(np.prod(shape))
         rho, u, v, w = (np.empty(n) for _ in range(4))
-        _check(self._lib.lbm_batch_collect(self._h, int(job_id), _dptr(rho), _dptr(u), _dptr(v), _dptr(w)))
-        return dict(rho=rho.reshape(shape), u=u.reshape(shape), v=v.reshape(shape), w=w.reshape(shape))
+        rc = self._lib.lbm_batch_collect(self._h, int(job_id), _dptr(rho), _dptr(u), _dptr(v), _dptr(w))
+        if rc not in (0, EJOB):
+            _check(rc)
+        self.shapes.pop(job_id)
+        out = dict(rho=rho.reshape(shape), u=u.reshape(shape), v=v.reshape(shape), w=w.reshape(shape),
+                   failed=rc == EJOB, steps_done=st["steps_done"],
+                   steps_finite=st["steps_finite"], mass=st["mass"])
+        if rc == EJOB:
+            out["error"] = self._lib.lbm_last_error().decode()
+        return out
 
     @property
     def active(self):

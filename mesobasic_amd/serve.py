@@ -20,7 +20,16 @@ are not starved; then one tick (steps_per_tick steps of every active job,
 one graph replay) runs, and finished jobs are read back and freed, which
 makes room for the next admissions.  Latency = completion - arrival, on
 the host's monotonic clock; aggregate MLUPS = sum(nodes * steps) / wall
-time from the first arrival to the last completion.
+time from the first arrival to the last completion.  Storage is FP16 with
+the node shift by default: droplet requests (densities 2.0 / 0.15) stay
+within 1e-3 of the FP64 path in it, where the constant shift does not.
+
+Failure handling: the engine retires a job whose FP64 mass total goes
+non-finite at the end of the tick that sees it (outcome "failed", its work
+counted up to its last finite check); a request past its optional deadline
+is cancelled; a request larger than the whole budget, or whose allocation
+keeps failing with nothing else resident, is rejected.  None of these
+change the other requests' results.
 
 --batched runs the batched configuration instead: `jobs` D2Q9 lattices
 with NX and NY uniform in {64..512}, G uniform in [-6.0, -4.5], tau uniform
@@ -35,7 +44,7 @@ import time
 
 import numpy as np
 
-from .lbm import FP16S, PRECISIONS, Batch, droplet_rho, noise_rho
+from .lbm import FP16N, FP16S, PRECISIONS, Batch, droplet_rho, noise_rho
 
 
 def make_requests(n, rate=200.0, seed=7):
@@ -62,56 +71,141 @@ def initial_rho(req):
     return noise_rho(req["shape"], req["seed"])
 
 
-def run_stream(reqs, precision=FP16S, budget_bytes=24 << 30, max_active=0, steps_per_tick=20,
-               work_limit=0, keep_results=False, clock=time.perf_counter):
-    """Serves `reqs` in real time; returns a summary dict (and the per-request
-    fields when keep_results).  work_limit > 0: each tick advances the
-    admitted jobs with the least remaining work, up to that many nodes."""
-    b = Batch(precision=precision, hbm_budget=budget_bytes, steps_per_tick=steps_per_tick,
-              use_graph=True, autotune=False, work_limit=work_limit)
+def run_stream(reqs, precision=FP16N, budget_bytes=24 << 30, max_active=0, steps_per_tick=20,
+               work_limit=0, keep_results=False, clock=time.perf_counter, engine=None,
+               alloc_retries=3, keep_ids=()):
+    """Serves `reqs` in real time; returns a summary dict (and, when
+    keep_results, the per-request fields of every request, else of the ids in
+    keep_ids, as a second value).
+
+    work_limit > 0: each tick advances the admitted jobs with the least
+    remaining work, up to that many nodes.  A request may carry a
+    "deadline" (seconds after its arrival): a request still queued or
+    running past it is cancelled and its memory freed.  Outcomes: "ok";
+    "failed" (the engine retired the job because its FP64 mass total went
+    non-finite; its work counts up to the last finite check); "cancelled";
+    "rejected" (larger than the whole budget, or its allocation kept failing
+    with nothing else resident).  `engine` replaces the GPU batch (an object
+    with the Batch methods; tests pass a host-only stand-in)."""
+    b = engine if engine is not None else Batch(
+        precision=precision, hbm_budget=budget_bytes, steps_per_tick=steps_per_tick, use_graph=True,
+        autotune=False, work_limit=work_limit)
     pending = list(reqs)            # not yet admitted, arrival order
     live = {}                       # job id -> request
-    done = {}
+    outcome, latency, fail_info = {}, {}, {}
     results = {}
+    keep_ids = set(keep_ids)
     work = 0
     ticks = 0
     max_live = 0
     peak_bytes = 0
+    retries = 0
     t0 = clock()
+
+    def finish(r, how, now):
+        outcome[r["id"]] = how
+        latency[r["id"]] = now - r["arrival"]
+
     while pending or live:
         now = clock() - t0
+        # deadlines: queued requests past theirs are dropped, running ones cancelled
+        for jid, r in list(live.items()):
+            if r.get("deadline") is not None and now - r["arrival"] > r["deadline"]:
+                b.cancel(jid)
+                del live[jid]
+                finish(r, "cancelled", now)
+        while pending and pending[0].get("deadline") is not None and \
+                now - pending[0]["arrival"] > pending[0]["deadline"]:
+            finish(pending.pop(0), "cancelled", now)
         # admissions, in arrival order, no overtaking
         while pending and pending[0]["arrival"] <= now and (max_active <= 0 or len(live) < max_active):
             r = pending[0]
+            if b.job_bytes(r["q"], r["shape"], r["tau"], r["G"], r["steps"]) > budget_bytes:
+                finish(pending.pop(0), "rejected", now)
+                continue
             jid = b.submit(r["q"], r["shape"], r["tau"], r["G"], r["steps"], initial_rho(r))
             if jid is None:
-                if not live:
-                    raise RuntimeError(f"request {r['id']} does not fit the HBM budget on its own")
-                break
+                if live:
+                    break           # waits for memory to free up
+                # nothing else holds memory: an allocation failure, retried a
+                # few times before the request is turned away
+                retries += 1
+                if retries > alloc_retries:
+                    finish(pending.pop(0), "rejected", now)
+                    retries = 0
+                continue
+            retries = 0
             live[jid] = r
             pending.pop(0)
         max_live = max(max_live, len(live))
         peak_bytes = max(peak_bytes, b.bytes_in_use)
         if not live:
-            time.sleep(max(0.0, pending[0]["arrival"] - (clock() - t0)))
+            if pending:
+                time.sleep(max(0.0, pending[0]["arrival"] - (clock() - t0)))
             continue
         b.tick()
         ticks += 1
         for jid in b.finished():
             r = live.pop(jid)
             out = b.collect(jid)
-            done[r["id"]] = clock() - t0 - r["arrival"]
-            work += int(np.prod(r["shape"])) * r["steps"]
-            if keep_results:
+            now = clock() - t0
+            nodes = int(np.prod(r["shape"]))
+            if out["failed"]:
+                finish(r, "failed", now)
+                fail_info[r["id"]] = dict(steps_done=out["steps_done"], steps_finite=out["steps_finite"])
+                work += nodes * out["steps_finite"]
+            else:
+                finish(r, "ok", now)
+                work += nodes * r["steps"]
+            if keep_results or r["id"] in keep_ids:
                 results[r["id"]] = out
-    wall = clock() - t0 - reqs[0]["arrival"]
-    lat = np.array([done[r["id"]] for r in reqs])
+    end = clock() - t0
+    wall = end - reqs[0]["arrival"]
+    ok = [r["id"] for r in reqs if outcome[r["id"]] == "ok"]
+    lat = np.array([latency[i] for i in ok]) if ok else np.array([np.nan])
+    counts = {k: sum(1 for v in outcome.values() if v == k) for k in ("ok", "failed", "cancelled", "rejected")}
     summary = dict(requests=len(reqs), p50_s=float(np.percentile(lat, 50)),
                    p99_s=float(np.percentile(lat, 99)), mean_s=float(lat.mean()),
                    aggregate_mlups=work / wall / 1e6, wall_s=wall, ticks=ticks,
-                   max_active=max_live, peak_hbm_bytes=int(peak_bytes))
-    b.close()
-    return (summary, results) if keep_results else summary
+                   max_active=max_live, peak_hbm_bytes=int(peak_bytes), outcomes=counts,
+                   failed=fail_info)
+    if engine is None:
+        b.close()
+    return (summary, results) if (keep_results or keep_ids) else summary
+
+
+def check_against_fp64(reqs, results, ids):
+    """Relative L2 of the served density against the FP64 single-lattice
+    path (the bit-exact drop-in) on the same inputs, for the requests `ids`
+    (served ok, fields kept).  Returns {id: rel_l2}."""
+    from .lbm import FIELDS_CURRENT, FP64, Lattice
+    by_id = {r["id"]: r for r in reqs}
+    out = {}
+    for i in ids:
+        r = by_id[i]
+        lat = Lattice(r["q"], r["shape"], r["tau"], r["G"], precision=FP64)
+        lat.set_rho(initial_rho(r))
+        lat.init()
+        lat.step(r["steps"])
+        ref = lat.fields(FIELDS_CURRENT)["rho"]
+        lat.close()
+        got = results[i]["rho"]
+        out[i] = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    return out
+
+
+def sample_ids(reqs, k, seed=5, max_work=2e10):
+    """k requests drawn (seeded) from `reqs`, half of each init type where
+    possible, each of at most max_work lattice updates (so the FP64 check
+    stays short)."""
+    rng = np.random.default_rng(seed)
+    picks = []
+    for init in ("droplet", "noise"):
+        pool = [r["id"] for r in reqs
+                if r["init"] == init and np.prod(r["shape"]) * r["steps"] <= max_work]
+        if pool:
+            picks += [int(v) for v in rng.choice(pool, size=min(len(pool), (k + 1) // 2), replace=False)]
+    return sorted(picks[:k])
 
 
 def make_batched_jobs(n=512, seed=11):
@@ -126,7 +220,9 @@ def make_batched_jobs(n=512, seed=11):
 
 def run_batched(jobs, steps=2000, precision=FP16S, steps_per_tick=20, autotune=True,
                 clock=time.perf_counter):
-    """All jobs submitted, then ticked to completion; returns a summary."""
+    """All jobs submitted, then ticked to completion; returns a summary.
+    Jobs the engine retires as failed (non-finite mass) count their work up
+    to their last finite mass check only."""
     b = Batch(precision=precision, steps_per_tick=steps_per_tick, use_graph=True, autotune=autotune)
     ids = []
     for j in jobs:
@@ -142,11 +238,25 @@ def run_batched(jobs, steps=2000, precision=FP16S, steps_per_tick=20, autotune=T
             break
     wall = clock() - t0
     index = {jid: k for k, jid in enumerate(ids)}
-    nonfinite = sorted(index[jid] for jid in b.finished()
-                       if not np.isfinite(b.collect(jid)["rho"]).all())
+    work = 0
+    failed = {}
+    nonfinite_ok = []
+    for jid in b.finished():
+        k = index[jid]
+        out = b.collect(jid)
+        n = int(np.prod(jobs[k]["shape"]))
+        if out["failed"]:
+            failed[k] = out["steps_done"]
+            work += n * out["steps_finite"]
+        else:
+            work += n * steps
+            if not np.isfinite(out["rho"]).all():
+                nonfinite_ok.append(k)
     summary = dict(jobs=len(jobs), steps=steps, nodes=nodes, wall_s=wall, ticks=n_ticks,
-                   aggregate_mlups=nodes * steps / wall / 1e6, tuning=b.tuning,
-                   finite_results=len(jobs) - len(nonfinite), nonfinite_jobs=nonfinite)
+                   aggregate_mlups=work / wall / 1e6, tuning=b.tuning,
+                   completed=len(jobs) - len(failed), failed_jobs=sorted(failed),
+                   failed_at_step={str(k): v for k, v in sorted(failed.items())},
+                   nonfinite_completed=nonfinite_ok)
     b.close()
     return summary
 
@@ -164,9 +274,10 @@ def main():
     ap.add_argument("--steps-per-tick", type=int, default=20)
     ap.add_argument("--work-limit", type=int, default=0,
                     help="nodes advanced per step, least remaining work first (0: all)")
-    ap.add_argument("--precision", default="fp16", choices=["fp32", "fp16", "bf16", "fp16n", "bf16n"])
+    ap.add_argument("--precision", default=None, choices=["fp32", "fp16", "bf16", "fp16n", "bf16n"],
+                    help="storage form (default: fp16n for the stream, fp16 for --batched)")
     args = ap.parse_args()
-    prec = PRECISIONS[args.precision]
+    prec = PRECISIONS[args.precision or ("fp16" if args.batched else "fp16n")]
     if args.batched:
         s = run_batched(make_batched_jobs(args.jobs), args.steps, prec, args.steps_per_tick)
     else:
