@@ -320,6 +320,20 @@ int lbm_batch_set_launch(lbm_batch* b, int d2q9_chunk, int d2q9_rows_kernel, int
 int lbm_batch_get_launch(const lbm_batch* b, int* d2q9_chunk, int* d2q9_rows_kernel,
                          int* d3q19_march);
 
+/* ---- allocation-free entry point ------------------------------------------
+ * lbm_conservation_noalloc: the totals of lbm_conservation, equal to them bit
+ * for bit, computed without any device allocation.  No counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code
+ * running between HIP graph captures, tight serving loops, harnesses that
+ * forbid hipMalloc).  Its block partial sums live in a fixed device array of
+ * 2 MiB + 32 B per GPU, held until the process ends, so it takes lattices of
+ * at most 2^28 nodes (LBM_EINVAL beyond).  The device current at the call
+ * must be the handle's (LBM_ESTATE otherwise).  Every call on a GPU uses that
+ * GPU's one array: calls that overlapped would race on it, so the library
+ * serialises the calls of a process with a lock held until the totals are on
+ * the host.  Synchronous. */
+int lbm_conservation_noalloc(lbm_handle* h, double out[4]);
+
 #ifdef __cplusplus
 }
 #endif

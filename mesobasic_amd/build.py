@@ -48,6 +48,7 @@ SOURCES = [
     ("lbm_batch.hip", ["-ffp-contract=off"]),
     ("lbm_capi.cpp", ["-ffp-contract=off"]),
     ("lbm_batch_capi.cpp", ["-ffp-contract=off"]),
+    ("lbm_noalloc.cpp", ["-ffp-contract=off"]),
 ]
 
 
