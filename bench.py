@@ -7,10 +7,14 @@ with the populations already resident in HBM.  Default workload: D3Q19
 256^3 periodic, FP64, tau = 1, G = -5.5, droplet R = 48 (the drop-in FP64
 configuration).  Prints ONE JSON line with MLUPS, the HBM roofline of the
 step kernel (algorithmic bytes 2*Q*b per lattice update over the kernel's
-measured time) and a CPU baseline timed on this host's cores on a bounded
-sample: the reference's own MPI CPU path (its lattice-Boltzmann sources
-compiled in place into oracle/_ref/ref_harness), or the oracle (a
-restatement of the reference's CPU code) where that harness is not built.
+measured time), the same roofline block for the D3Q19 512^3 FP64 step, and a
+CPU baseline timed on this host's cores on a bounded sample: the reference's
+own MPI CPU path, its lattice-Boltzmann sources compiled in place into
+oracle/_ref/ref_harness.  That harness is built in the container that holds
+the reference tree (`python -c "import __graft_entry__ as g; g.build()"`)
+and travels with the working tree; where it is missing the bench says so
+on stderr and in the line, and times the oracle (a restatement of the
+reference's CPU code) instead.
 """
 from __future__ import annotations
 
@@ -61,14 +65,16 @@ def initial_rho(shape, init):
     return mb.noise_rho(shape, init[1])
 
 
-def cpu_baseline_reference(q, shape, tau, G, init):
+def cpu_baseline_reference(q, shape, tau, G, init, ranks=None, seconds=10.0):
     """Times the reference's own MPI CPU path on this host: the harness
     oracle/_ref/ref_harness (the reference's LBfluid3D/LBfluid2D/scmp3D
     translation units compiled in place, x split into slabs over MPI ranks as
     LBApp does) under mpiexec, one rank per core (up to 16).  Two runs of
     k1 and k2 steps on the same lattice, each ending in the same field dump;
     the rate is n (k2 - k1) / (t2 - t1), so start-up, initialisation and
-    the dump cancel.  None when the harness or mpiexec is missing or fails."""
+    the dump cancel.  ranks: MPI ranks (default one per core, up to 16);
+    seconds: about how long the timed difference runs.  None when the harness
+    or mpiexec is missing or fails."""
     import shutil
     import subprocess
     import tempfile
@@ -76,11 +82,13 @@ def cpu_baseline_reference(q, shape, tau, G, init):
     mpiexec = shutil.which("mpiexec") or "/opt/conda/bin/mpiexec"
     if not (os.path.exists(harness) and os.path.exists(mpiexec)):
         return None
-    ranks = min(len(os.sched_getaffinity(0)), 16)
+    if ranks is None:
+        ranks = min(len(os.sched_getaffinity(0)), 16)
     n = int(np.prod(shape))
     full = tuple(shape) + ((1,) if len(shape) == 2 else ())
     k1 = 2
-    k2 = k1 + max(20, min(20000, int(6e8 / n)))   # about 10 s of steps at ~80 MLUPS
+    rate = 5e6 * ranks   # lattice updates per second, roughly, at this rank count
+    k2 = k1 + max(20, min(20000, int(seconds * rate / n)))
     with tempfile.TemporaryDirectory(dir="/tmp") as d:
         rho0 = os.path.join(d, "rho0.bin")
         initial_rho(shape, init).astype(np.float64).ravel().tofile(rho0)
@@ -118,6 +126,9 @@ def cpu_baseline(q, shape, tau, G, init, budget_s=15.0):
     ref = cpu_baseline_reference(q, shape, tau, G, init)
     if ref is not None:
         return ref
+    print("# ERROR: the reference CPU harness oracle/_ref/ref_harness is missing or failed to run; "
+          "build it where the reference tree is (__graft_entry__.build()). Timing the oracle port "
+          "instead.", file=sys.stderr, flush=True)
     import oracle
     cores = min(os.cpu_count() or 1, 16)
     o = oracle.Oracle(q, shape, tau, G, nthreads=cores)
@@ -134,6 +145,7 @@ def cpu_baseline(q, shape, tau, G, init, budget_s=15.0):
     n = int(np.prod(shape))
     o.close()
     return {"value": round(n * k / dt / 1e6, 3), "unit": "MLUPS", "cores": cores, "kind": "port",
+            "reference_harness": "missing or failed (see stderr)",
             "sample": f"oracle (CPU restatement of the reference step, AoS FP64, {cores} x-slab "
                       f"threads), {'x'.join(map(str, shape))} lattice, {k} timed steps after 1 warm-up"}
 
@@ -156,11 +168,42 @@ def time_workload(name, steps=10, warmup=2):
 
 
 def load_traffic(kernel_name, workload):
-    path = os.path.join(ROOT, "profiles", f"traffic_{kernel_name}_{workload}.json")
+    """HBM bytes per launch of the step kernel on `workload`, from the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes summarised in
+    profiles/traffic_<kernel>_<workload>.json (counters cannot be read inside
+    this process), and that file's name; (None, None) where no such summary
+    exists."""
+    rel = os.path.join("profiles", f"traffic_{kernel_name}_{workload}.json")
+    path = os.path.join(ROOT, rel)
     if os.path.exists(path):
         with open(path) as fh:
-            return json.load(fh).get("bytes_per_launch")
-    return None
+            return json.load(fh).get("bytes_per_launch"), rel
+    return None, None
+
+
+def roofline_block(q, prec, n, ms_per_step, kname, workload):
+    """The roofline object of a step kernel: algorithmic bytes (2 Q b per
+    lattice update) over the measured time per step."""
+    bpl = 2 * q * BYTES_PER_POP[prec]
+    achieved = bpl * n / (ms_per_step * 1e-3) / 1e9
+    traffic, src = load_traffic(kname, workload)
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "traffic_source": (f"{src}: rocprofv3 FETCH_SIZE (x2.0) + WRITE_SIZE per launch of this "
+                               f"kernel on {workload}" if src else None),
+            "algorithmic_bytes_per_lu": bpl, "lattice_updates_per_launch": n,
+            "kernel": f"step ({kname}); HIP-event time per step over the timed region"}
+
+
+def copy_calibration(nbytes=4 << 30):
+    """Attainable HBM bandwidth on this box: the library's fastest streaming
+    copy kernel and the HIP runtime's device-to-device copy, both over
+    nbytes, read + write bytes per second."""
+    ms_k = mb.lbm.probe_copy(nbytes, reps=5, mode=4)
+    ms_m = mb.lbm.probe_memcpy(nbytes, reps=5)
+    return {"bytes": nbytes,
+            "copy_kernel_GBs": round(2 * nbytes / (ms_k * 1e-3) / 1e9, 1),
+            "hipMemcpyDtoD_GBs": round(2 * nbytes / (ms_m * 1e-3) / 1e9, 1)}
 
 
 def main():
@@ -225,10 +268,8 @@ def main():
     lat.synchronize()
     ms_per_step = ms / args.steps
     mlups = n / (ms_per_step * 1e-3) / 1e6
-    bpl = 2 * q * BYTES_PER_POP[prec]
     cons = lat.conservation()
     kname = KERNEL_NAMES.get(lat.kernel, "unknown")
-    achieved = bpl * n / (ms_per_step * 1e-3) / 1e9
     out = {
         "metric": METRIC,
         "value": round(mlups, 2),
@@ -247,11 +288,7 @@ def main():
                    "boundary": "periodic", "kernel": kname,
                    "tuning": list(lat.tuning) if lat.kernel == mb.KERNEL_FUSED else None,
                    "wall_s_timed_region": round(wall, 4)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": load_traffic(kname, args.workload),
-                     "algorithmic_bytes_per_lu": bpl,
-                     "kernel": f"step ({kname}); HIP-event time per step over the timed region"},
+        "roofline": roofline_block(q, prec, n, ms_per_step, kname, args.workload),
         "mass_after": cons[0],
     }
     lat.close()
@@ -264,6 +301,13 @@ def main():
             forms[f"speedup_{k}_vs_fp32"] = round(forms[k]["mlups"] / forms["fp32"]["mlups"], 3)
             forms[f"speedup_{k}_vs_fp64"] = round(forms[k]["mlups"] / forms["fp64"]["mlups"], 3)
         out["precision_forms_512"] = forms
+        # the north-star step (D3Q19 512^3 FP64) with its own roofline block
+        f64 = forms["fp64"]
+        out["roofline_d3q19_fp64_512"] = dict(
+            roofline_block(19, mb.FP64, 512 ** 3, f64["ms_per_step"], f64["kernel"],
+                           "d3q19_fp64_512_noise"),
+            mlups=f64["mlups"], workload="d3q19_fp64_512_noise")
+        out["copy_calibration"] = copy_calibration()
         # D2Q9 FP64 drop-in path: the 256^2 spinodal lattice (latency-bound)
         # and a 4096^2 one (HBM-bound), automatic kernel choice
         out["d2q9_fp64"] = {"256x256": time_workload("d2q9_fp64_256_noise", steps=2000, warmup=10),
@@ -279,17 +323,41 @@ def main():
             out["batched_d2q9_512"] = r
             print(f"# batched: {r['aggregate_mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
         if not args.no_serving:
-            r = serve.run_stream(serve.make_requests(args.serve_requests, 200.0, 7),
-                                 work_limit=SERVE_WORK_LIMIT)
+            reqs = serve.make_requests(args.serve_requests, 200.0, 7)
+            sample = serve.sample_ids(reqs, 6)
+            r, kept = serve.run_stream(reqs, work_limit=SERVE_WORK_LIMIT, keep_ids=sample)
             r["config"] = (f"{args.serve_requests} requests, Poisson 200/s (seed 7), 70% D2Q9 "
                            "128..1024 per side, 30% D3Q19 64..192 per side, steps U[500, 5000], "
-                           "G U[-6, -4.5], droplet or noisy init, FP16 storage, HBM budget 24 GiB, "
-                           f"shortest remaining work first within {SERVE_WORK_LIMIT} nodes per step")
+                           "G U[-6, -4.5], droplet or noisy init, FP16 node-shifted storage, HBM "
+                           f"budget 24 GiB, shortest remaining work first within {SERVE_WORK_LIMIT} "
+                           "nodes per step")
+            # accuracy of a seeded sample of served requests against the FP64
+            # drop-in path (bit-identical to the reference), after the stream
+            ok = [i for i in sample if i in kept and not kept[i]["failed"]]
+            err = serve.check_against_fp64(reqs, kept, ok, fp32_floor=True)
+            by = {"droplet": [], "noise": []}
+            for i, (e, _) in err.items():
+                by[reqs[i]["init"]].append(e)
+            r["accuracy_vs_fp64"] = {
+                "rel_l2_rho": {str(i): float(f"{e:.3e}") for i, (e, _) in err.items()},
+                "fp32_storage_rel_l2_rho": {str(i): float(f"{f:.3e}") for i, (_, f) in err.items()},
+                "max_rel_l2_droplet": max(by["droplet"]) if by["droplet"] else None,
+                "max_rel_l2_noise": max(by["noise"]) if by["noise"] else None,
+                "sample": "requests drawn by serve.sample_ids(reqs, 6): half droplet, half noise, "
+                          "each at most 2e10 lattice updates; fp32_storage_* is a single-lattice "
+                          "FP32 run of the same request (the FP32 rounding floor)"}
             out["serving"] = r
             print(f"# serving: p50 {r['p50_s']:.2f} s p99 {r['p99_s']:.2f} s "
                   f"{r['aggregate_mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(q, shape, tau, G, init)
+        if args.workload == "d3q19_fp64_256_droplet":
+            # the reference's own D2Q9 case: 256^2 spinodal, one MPI rank
+            qd, shd, _, taud, Gd, initd = WORKLOADS["d2q9_fp64_256_noise"]
+            d2 = cpu_baseline_reference(qd, shd, taud, Gd, initd, ranks=1, seconds=8.0)
+            if d2 is not None:
+                d2["gpu_mlups"] = out.get("d2q9_fp64", {}).get("256x256", {}).get("mlups")
+            out["cpu_baseline_d2q9_256_1rank"] = d2
     print(json.dumps(out), flush=True)
 
 

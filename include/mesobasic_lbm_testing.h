@@ -29,6 +29,12 @@ int lbm_probe_tile_map(int nty, int ntz, int ncx, int order, int* ty, int* tz, i
  * achievable bandwidth. */
 int lbm_probe_copy(int64_t bytes, int reps, int mode, double* ms);
 
+/* The HIP runtime's device-to-device copy (hipMemcpyAsync) of `bytes`
+ * between two buffers it allocates, `reps` times after one untimed run;
+ * *ms = average time per copy.  An attainable-bandwidth figure that does not
+ * depend on this library's kernels. */
+int lbm_probe_memcpy(int64_t bytes, int reps, double* ms);
+
 /* Device allocations the library has made so far: every hipMalloc of the
  * lattice handles and the batch engine goes through one wrapper that counts
  * it (the probes above allocate on their own and are not counted). */

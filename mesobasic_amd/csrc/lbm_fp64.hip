@@ -534,3 +534,37 @@ extern "C" int lbm_probe_copy(int64_t bytes, int reps, int mode, double* ms)
     (void)hipFree(b);
     return rc;
 }
+
+// Device-to-device hipMemcpyAsync of `bytes` between two buffers it
+// allocates, `reps` times after one untimed run; *ms = average time per
+// copy (HIP events).  The runtime's own copy path: an attainable-bandwidth
+// figure independent of the probe kernels above.
+extern "C" int lbm_probe_memcpy(int64_t bytes, int reps, double* ms)
+{
+    if (bytes <= 0 || reps < 1) return -1;
+    void* a = nullptr;
+    void* b = nullptr;
+    if (hipMalloc(&a, bytes) != hipSuccess) return -2;
+    if (hipMalloc(&b, bytes) != hipSuccess) { (void)hipFree(a); return -2; }
+    int rc = 0;
+    if (hipMemset(a, 1, bytes) != hipSuccess) rc = -3;
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    if (!rc) {
+        for (int r = -1; r < reps && !rc; r++) {
+            if (r == 0) (void)hipEventRecord(e0, nullptr);
+            if (hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, nullptr) != hipSuccess) rc = -3;
+        }
+        (void)hipEventRecord(e1, nullptr);
+        if (hipEventSynchronize(e1) != hipSuccess) rc = -3;
+        float t = 0.f;
+        (void)hipEventElapsedTime(&t, e0, e1);
+        if (ms) *ms = t / reps;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return rc;
+}

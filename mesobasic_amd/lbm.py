@@ -117,6 +117,7 @@ def load():
         "lbm_testing_alloc_count": (ctypes.c_int64, []),
         "lbm_testing_fail_alloc": (None, [ctypes.c_int64]),
         "lbm_probe_copy": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, ctypes.c_int, _D]),
+        "lbm_probe_memcpy": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, _D]),
         "lbm_probe_tile_map": (ctypes.c_int, [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_int)] * 3),
     }
     for name, (res, args) in sig.items():
@@ -184,6 +185,14 @@ def fail_alloc(after):
     """Injects one device-allocation failure after `after` successful ones;
     a negative value disarms it (test hook)."""
     load().lbm_testing_fail_alloc(int(after))
+
+
+def probe_memcpy(nbytes, reps=5):
+    """hipMemcpyAsync device-to-device of nbytes; returns ms per copy."""
+    lib = load()
+    ms = ctypes.c_double()
+    _check(lib.lbm_probe_memcpy(int(nbytes), int(reps), ctypes.byref(ms)))
+    return ms.value
 
 
 def probe_tile_map(nty, ntz, ncx, order=2):

@@ -174,23 +174,34 @@ def run_stream(reqs, precision=FP16N, budget_bytes=24 << 30, max_active=0, steps
     return (summary, results) if (keep_results or keep_ids) else summary
 
 
-def check_against_fp64(reqs, results, ids):
+def check_against_fp64(reqs, results, ids, fp32_floor=False):
     """Relative L2 of the served density against the FP64 single-lattice
     path (the bit-exact drop-in) on the same inputs, for the requests `ids`
-    (served ok, fields kept).  Returns {id: rel_l2}."""
-    from .lbm import FIELDS_CURRENT, FP64, Lattice
+    (served ok, fields kept).  Returns {id: rel_l2}; with fp32_floor,
+    {id: (rel_l2, rel_l2 of an FP32-storage run)} -- the error that FP32
+    rounding alone leaves, which on spinodal (noise) requests grows with the
+    instability, not with the storage format."""
+    from .lbm import FIELDS_CURRENT, FP32, FP64, Lattice
     by_id = {r["id"]: r for r in reqs}
-    out = {}
-    for i in ids:
-        r = by_id[i]
-        lat = Lattice(r["q"], r["shape"], r["tau"], r["G"], precision=FP64)
+
+    def run(r, prec):
+        lat = Lattice(r["q"], r["shape"], r["tau"], r["G"], precision=prec)
         lat.set_rho(initial_rho(r))
         lat.init()
         lat.step(r["steps"])
-        ref = lat.fields(FIELDS_CURRENT)["rho"]
+        rho = lat.fields(FIELDS_CURRENT)["rho"]
         lat.close()
-        got = results[i]["rho"]
-        out[i] = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+        return rho
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    out = {}
+    for i in ids:
+        r = by_id[i]
+        ref = run(r, FP64)
+        e = rel(results[i]["rho"], ref)
+        out[i] = (e, rel(run(r, FP32), ref)) if fp32_floor else e
     return out
 
 

@@ -45,3 +45,18 @@ def test_stable_job_stays_finite_in_fp64(k):
     blow, r = _run(job, 2000)
     assert blow is None
     assert 0.0 < r.min() and r.max() < 5.0
+
+
+def test_serving_droplet_corner_blows_up_in_fp64():
+    # the serving stream's D3Q19 droplets (2.0 / 0.15, width 2) at strong
+    # coupling: the batch engine retires request 26 (G = -5.95) at its first
+    # mass check; the FP64 oracle on the same inputs is non-finite by step 10
+    reqs = serve.make_requests(200, 200.0, 7)
+    r = reqs[26]
+    assert r["q"] == 19 and r["init"] == "droplet" and r["G"] < -5.9
+    o = oracle.Oracle(19, r["shape"], r["tau"], r["G"], nthreads=8)
+    o.set_rho(serve.initial_rho(r))
+    o.init()
+    o.step(10)
+    assert not np.isfinite(o.fields()["rho"]).all()
+    o.close()
