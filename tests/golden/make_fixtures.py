@@ -3,7 +3,7 @@ lattice-Boltzmann classes (compiled in place by oracle/Makefile.ref into
 oracle/_ref/ref_harness) on seeded inputs.
 
 Only runnable where the reference tree and MPICH are present:
-    make -C oracle ref && python tests/golden/make_fixtures.py
+    make -C oracle ref && python tests/golden/make_fixtures.py [name ...]
 
 Each case is one .npz: inputs (rho0, parameters) and outputs at the listed
 steps -- the reference's r,u,v,w member arrays and, where stored, f
@@ -34,6 +34,10 @@ CASES = [
     ("d2q9_13x11", 9, (13, 11), 1.0, -5.0, False, ("noise", 14), [0, 1, 2, 10, 1000], [1, 1000], 3),
     ("d2q9_walls_12x10_tau08", 9, (12, 10), 0.8, -5.0, True, ("noise", 15), [1, 10, 300], [300], 2),
     ("d2q9_64x64", 9, (64, 64), 1.0, -5.0, False, ("noise", 16), [1000], [], 4),
+    # large enough for the fused D3Q19 tiles (8 x 64, and partial 8 x 32 /
+    # 4 x 64): z walls at tau 0.7, and periodic at tau 1.3
+    ("d3q19_walls_4x16x64_tau07", 19, (4, 16, 64), 0.7, -5.5, True, ("noise", 17), [1, 30], [30], 2),
+    ("d3q19_5x8x64_tau13", 19, (5, 8, 64), 1.3, -5.0, False, ("noise", 18), [1, 2, 40], [], 2),
 ]
 
 
@@ -72,8 +76,10 @@ def run_case(name, q, shape, tau, G, walls, init, steps, steps_f, ranks):
 def main():
     if not os.path.exists(HARNESS):
         sys.exit("build the harness first: make -C oracle ref")
+    only = set(sys.argv[1:])
     for c in CASES:
-        run_case(*c)
+        if not only or c[0] in only:
+            run_case(*c)
 
 
 if __name__ == "__main__":

@@ -15,15 +15,33 @@ pytestmark = pytest.mark.gpu
 KERNELS = [mb.KERNEL_TWO_PASS, mb.KERNEL_FUSED]
 
 
+def fused_fits(q, shape):
+    """Whether any fused FP64 shape takes the lattice: D2Q9 row segments
+    need two x rows; D3Q19 tiles need two x planes and a y-z cross-section
+    of at least one tile (8 x 32 or 4 x 64, partial or whole; whole z rows
+    of 256 or 512)."""
+    if q == 9:
+        return shape[0] >= 2
+    nx, ny, nz = shape
+    return nx >= 2 and ((ny >= 8 and nz >= 32) or (ny >= 4 and nz >= 64) or nz in (256, 512))
+
+
+def kernels_for(q, shape):
+    return KERNELS if fused_fits(q, shape) else [mb.KERNEL_TWO_PASS]
+
+
 def _lattice(meta, kernel):
-    try:
-        return mb.Lattice(meta["q"], tuple(meta["shape"]), meta["tau"], meta["G"],
-                          order=meta["order"], walls=meta["walls"], precision=mb.FP64,
-                          kernel=kernel)
-    except mb.LbmError as e:
-        if kernel == mb.KERNEL_FUSED and e.code == -1:
-            pytest.skip("fused kernel does not take this lattice")
-        raise
+    # any refusal of a lattice the kernel should take fails the test
+    return mb.Lattice(meta["q"], tuple(meta["shape"]), meta["tau"], meta["G"],
+                      order=meta["order"], walls=meta["walls"], precision=mb.FP64, kernel=kernel)
+
+
+def _fixture_params():
+    out = []
+    for name in FIXTURES:
+        meta, _ = load_fixture(name)
+        out += [(name, k) for k in kernels_for(meta["q"], meta["shape"])]
+    return out
 
 
 def test_device_exp_matches_libm():
@@ -40,8 +58,7 @@ def test_device_exp_matches_libm():
     assert bad.size == 0, f"{bad.size} mismatches, first x={x[bad[0]]!r}"
 
 
-@pytest.mark.parametrize("kernel", KERNELS)
-@pytest.mark.parametrize("name", FIXTURES)
+@pytest.mark.parametrize("name,kernel", _fixture_params())
 def test_fixture_bitwise(name, kernel):
     meta, data = load_fixture(name)
     print(f"case {name} kernel {kernel}", flush=True)
@@ -87,8 +104,8 @@ FUSED_CASES = [
 ]
 
 
-@pytest.mark.parametrize("kernel", KERNELS)
-@pytest.mark.parametrize("case", CASES + FUSED_CASES)
+@pytest.mark.parametrize("case,kernel", [(c, k) for c in CASES + FUSED_CASES
+                                         for k in kernels_for(c[0], c[1])])
 def test_matches_oracle_bitwise(case, kernel):
     q, shape, tau, G, order, walls, steps = case
     print(f"case {case} kernel {kernel}", flush=True)
