@@ -297,6 +297,13 @@ int64_t lbm_batch_steps_run(const lbm_batch* b);
  * Results do not depend on the schedule. */
 int lbm_batch_set_work_limit(lbm_batch* b, int64_t max_nodes);
 
+/* Bound on waiting under the work limit: a resident, unfinished job that the
+ * last `ticks` ticks did not advance goes ahead of the shorter ones (longest
+ * wait first) until it is advanced, so a large job is not starved by a steady
+ * stream of small ones.  0 (default): no bound.  Results do not depend on
+ * it. */
+int lbm_batch_set_max_wait(lbm_batch* b, int ticks);
+
 /* Launch configuration the batch uses (after autotuning): D2Q9 rows per
  * work item (at most; a row-width class whose items would not fill two
  * rounds of resident workgroups gets shorter items, down to 2 rows) and

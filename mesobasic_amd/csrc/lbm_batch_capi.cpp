@@ -69,6 +69,7 @@ struct HostJob {
     int64_t steps_finite = 0;   // steps after which the last mass check was finite
     double mass = 0.0;          // FP64 total mass at the last check
     bool mass_valid = false;
+    int waited = 0;             // ticks in a row it was resident and not advanced
 };
 
 int64_t plane_stride(int64_t n, int elem)
@@ -95,6 +96,7 @@ struct lbm_batch {
     int64_t in_use = 0;
     int64_t steps_run = 0;
     int64_t work_limit = 0;       // max nodes advanced per step (0: all)
+    int max_wait = 0;             // ticks a job may wait under the work limit (0: no bound)
     // device tables
     Job* d_jobs = nullptr;
     int cap_jobs = 0;
@@ -261,10 +263,16 @@ int upload_tables(lbm_batch* b)
     for (auto& kv : b->jobs)
         if (!kv.second->finished) b->order.push_back(kv.second.get());
     if (b->work_limit > 0 && !b->order.empty()) {
-        // shortest remaining work first, within the node limit; a running
+        // shortest remaining work first, within the node limit; jobs that
+        // have waited max_wait ticks go first, longest wait first.  A running
         // job's remaining work only shrinks, so the set changes only when a
-        // job is submitted or finishes (both mark the tables dirty)
-        std::stable_sort(b->order.begin(), b->order.end(), [](const HostJob* a, const HostJob* z) {
+        // job is submitted, finishes or reaches the wait bound (each marks
+        // the tables dirty)
+        const int mw = b->max_wait;
+        std::stable_sort(b->order.begin(), b->order.end(), [mw](const HostJob* a, const HostJob* z) {
+            const bool sa = mw > 0 && a->waited >= mw, sz = mw > 0 && z->waited >= mw;
+            if (sa != sz) return sa;
+            if (sa) return a->waited > z->waited;
             return a->n * a->remaining < z->n * z->remaining;
         });
         int64_t nodes = 0;
@@ -618,6 +626,12 @@ int lbm_batch_tick(lbm_batch* b, int* n_finished)
     b->steps_run += b->d.steps_per_tick;
     std::vector<double> mass(b->order.size(), 0.0);
     for (int k = 0; k < b->nitems[kMass]; k++) mass[b->mass_job[k]] += b->h_mass[k];
+    // waiting times of the resident jobs this tick did not advance
+    for (auto& kv : b->jobs) kv.second->waited += kv.second->finished ? 0 : 1;
+    for (HostJob* j : b->order) j->waited = 0;
+    if (b->max_wait > 0)
+        for (auto& kv : b->jobs)
+            if (!kv.second->finished && kv.second->waited == b->max_wait) b->dirty = true;
     for (int ji = 0; ji < static_cast<int>(b->order.size()); ji++) {
         HostJob* j = b->order[ji];
         j->remaining = b->h_jobs[ji].remaining;
@@ -738,6 +752,14 @@ int lbm_batch_set_work_limit(lbm_batch* b, int64_t max_nodes)
 {
     if (!b || max_nodes < 0) return lbm::set_error(LBM_EINVAL, "lbm_batch_set_work_limit: bad argument");
     b->work_limit = max_nodes;
+    b->dirty = true;
+    return LBM_OK;
+}
+
+int lbm_batch_set_max_wait(lbm_batch* b, int ticks)
+{
+    if (!b || ticks < 0) return lbm::set_error(LBM_EINVAL, "lbm_batch_set_max_wait: bad argument");
+    b->max_wait = ticks;
     b->dirty = true;
     return LBM_OK;
 }

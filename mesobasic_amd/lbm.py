@@ -110,6 +110,7 @@ def load():
         "lbm_batch_steps_run": (ctypes.c_int64, [P]),
         "lbm_batch_tuning": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
         "lbm_batch_set_work_limit": (ctypes.c_int, [P, ctypes.c_int64]),
+        "lbm_batch_set_max_wait": (ctypes.c_int, [P, ctypes.c_int]),
         "lbm_batch_set_launch": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
         "lbm_batch_get_launch": (ctypes.c_int, [P] + [ctypes.POINTER(ctypes.c_int)] * 3),
         "lbm_batch_job_status": (ctypes.c_int, [P, ctypes.c_int64, ctypes.POINTER(JobStatus)]),
@@ -324,7 +325,7 @@ class Batch:
     submit(), advanced by tick(), and read back and freed by collect()."""
 
     def __init__(self, precision=FP16S, rho_shift=0.7, hbm_budget=0, steps_per_tick=20,
-                 use_graph=True, autotune=False, work_limit=0):
+                 use_graph=True, autotune=False, work_limit=0, max_wait=0):
         lib = load()
         if isinstance(precision, str):
             precision = PRECISIONS[precision]
@@ -336,6 +337,8 @@ class Batch:
         self.shapes = {}
         if work_limit:
             _check(lib.lbm_batch_set_work_limit(h, int(work_limit)))
+        if max_wait:
+            _check(lib.lbm_batch_set_max_wait(h, int(max_wait)))
 
     def close(self):
         if getattr(self, "_h", None):

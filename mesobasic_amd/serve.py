@@ -73,13 +73,14 @@ def initial_rho(req):
 
 def run_stream(reqs, precision=FP16N, budget_bytes=24 << 30, max_active=0, steps_per_tick=20,
                work_limit=0, keep_results=False, clock=time.perf_counter, engine=None,
-               alloc_retries=3, keep_ids=()):
+               alloc_retries=3, keep_ids=(), max_wait=50):
     """Serves `reqs` in real time; returns a summary dict (and, when
     keep_results, the per-request fields of every request, else of the ids in
     keep_ids, as a second value).
 
     work_limit > 0: each tick advances the admitted jobs with the least
-    remaining work, up to that many nodes.  A request may carry a
+    remaining work, up to that many nodes; a job left waiting for max_wait
+    ticks goes first (no starvation of large requests).  A request may carry a
     "deadline" (seconds after its arrival): a request still queued or
     running past it is cancelled and its memory freed.  Outcomes: "ok";
     "failed" (the engine retired the job because its FP64 mass total went
@@ -89,7 +90,7 @@ def run_stream(reqs, precision=FP16N, budget_bytes=24 << 30, max_active=0, steps
     with the Batch methods; tests pass a host-only stand-in)."""
     b = engine if engine is not None else Batch(
         precision=precision, hbm_budget=budget_bytes, steps_per_tick=steps_per_tick, use_graph=True,
-        autotune=False, work_limit=work_limit)
+        autotune=False, work_limit=work_limit, max_wait=max_wait if work_limit else 0)
     pending = list(reqs)            # not yet admitted, arrival order
     live = {}                       # job id -> request
     outcome, latency, fail_info = {}, {}, {}

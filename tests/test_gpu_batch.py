@@ -230,3 +230,34 @@ def test_d3q19_march_default_alone_equals_in_the_mix():
         got = _run(Batch(precision=prec, steps_per_tick=4, autotune=True), mix)[6]
         for f in ("rho", "u", "v", "w"):
             assert same_bits(alone[f], got[f]), (prec, shape, f)
+
+
+@pytest.mark.parametrize("max_wait", [0, 3])
+def test_wait_bound_keeps_a_large_job_from_starving(max_wait):
+    # shortest remaining work first under a node limit: a steady supply of
+    # small jobs (one per tick, below the limit's capacity of 1.5 per tick)
+    # keeps a large one waiting for ever, unless waiting is bounded
+    big = dict(q=9, shape=(64, 64), tau=1.0, G=-5.0, steps=24, seed=3)
+    b = Batch(precision=mb.FP16S, steps_per_tick=4, work_limit=3 * 16 * 16, max_wait=max_wait)
+    big_id = b.submit(9, big["shape"], 1.0, -5.0, big["steps"], mb.noise_rho(big["shape"], 3))
+    got, seed = None, 100
+    for tick in range(120):
+        b.submit(9, (16, 16), 1.0, -5.0, 8, mb.noise_rho((16, 16), seed))
+        seed += 1
+        b.tick()
+        for jid in b.finished():
+            out = b.collect(jid)
+            if jid == big_id:
+                got = (tick, out)
+        if got:
+            break
+    print(f"max_wait {max_wait}: large job done at tick {got[0] if got else None}", flush=True)
+    if max_wait == 0:
+        assert got is None and b.status(big_id)["steps_done"] == 0
+    else:
+        assert got is not None and got[0] < 60
+        alone = Batch(precision=mb.FP16S, steps_per_tick=6)
+        jid = alone.submit(9, big["shape"], 1.0, -5.0, big["steps"], mb.noise_rho(big["shape"], 3))
+        while not alone.finished():
+            alone.tick()
+        assert same_bits(alone.collect(jid)["rho"], got[1]["rho"])
