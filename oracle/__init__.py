@@ -51,6 +51,15 @@ def _load():
     lib.scmp_oracle_mt19937.argtypes = [ctypes.c_uint32, ctypes.c_int64,
                                         ctypes.POINTER(ctypes.c_uint32)]
     lib.scmp_oracle_droplet_rho.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 4 + [_D]
+    lib.mcmp_oracle_create.restype = ctypes.c_void_p
+    lib.mcmp_oracle_create.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double] * 3 + [ctypes.c_int] * 2
+    lib.mcmp_oracle_destroy.argtypes = [ctypes.c_void_p]
+    lib.mcmp_oracle_set_rho.argtypes = [ctypes.c_void_p, _D, _D]
+    lib.mcmp_oracle_init.argtypes = [ctypes.c_void_p]
+    lib.mcmp_oracle_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.mcmp_oracle_get.argtypes = [ctypes.c_void_p, ctypes.c_int] + [_D] * 5
+    lib.mcmp_oracle_get_force.argtypes = [ctypes.c_void_p, ctypes.c_int] + [_D] * 3
+    lib.mcmp_oracle_moments.argtypes = [ctypes.c_void_p, ctypes.c_int] + [_D] * 4
     _lib = lib
     return lib
 
@@ -135,6 +144,66 @@ class Oracle:
     def set_exp(self, fnptr):
         """fnptr: ctypes function pointer double(double), or None for libm."""
         self._lib.scmp_oracle_set_exp(self._h, fnptr)
+
+
+class MCOracle:
+    """Two-component Shan-Chen (mcmp2D / mcmp3D) on the CPU, reference
+    semantics: components A (comp 0) and B (comp 1) with taus tauA, tauB and
+    cross coupling G.  order: 2 = mcmp2D's step (default for q=9), 3 =
+    mcmp3D's (default for q=19)."""
+
+    def __init__(self, q, shape, tauA, tauB, G, order=None, nthreads=1):
+        lib = _load()
+        shape = tuple(int(s) for s in shape)
+        if q == 9:
+            nx, ny, nz = shape[0], shape[1], 1
+            order = 2 if order is None else order
+        else:
+            nx, ny, nz = shape
+            order = 3 if order is None else order
+        self.q, self.shape, self.order = q, shape, order
+        self.n = nx * ny * nz
+        self._h = lib.mcmp_oracle_create(q, nx, ny, nz, float(tauA), float(tauB), float(G),
+                                         int(order), int(nthreads))
+        if not self._h:
+            raise ValueError("oracle: bad lattice description")
+        self._lib = lib
+
+    def close(self):
+        if self._h:
+            self._lib.mcmp_oracle_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def set_rho(self, rhoA: np.ndarray, rhoB: np.ndarray):
+        a = np.ascontiguousarray(rhoA, dtype=np.float64).reshape(-1)
+        b = np.ascontiguousarray(rhoB, dtype=np.float64).reshape(-1)
+        assert a.size == self.n and b.size == self.n
+        self._lib.mcmp_oracle_set_rho(self._h, _ptr(a), _ptr(b))
+
+    def init(self):
+        self._lib.mcmp_oracle_init(self._h)
+
+    def step(self, nsteps: int = 1):
+        self._lib.mcmp_oracle_step(self._h, int(nsteps))
+
+    def fields(self, comp):
+        """Reference r,u,v,w arrays, force arrays and f of one component."""
+        rho, u, v, w, fx, fy, fz = (np.empty(self.n) for _ in range(7))
+        f = np.empty(self.n * self.q)
+        self._lib.mcmp_oracle_get(self._h, int(comp), _ptr(rho), _ptr(u), _ptr(v), _ptr(w), _ptr(f))
+        self._lib.mcmp_oracle_get_force(self._h, int(comp), _ptr(fx), _ptr(fy), _ptr(fz))
+        sh = self.shape
+        return dict(rho=rho.reshape(sh), u=u.reshape(sh), v=v.reshape(sh), w=w.reshape(sh),
+                    fx=fx.reshape(sh), fy=fy.reshape(sh), fz=fz.reshape(sh),
+                    f=f.reshape(sh + (self.q,)))
+
+    def moments(self, comp):
+        rho, u, v, w = (np.empty(self.n) for _ in range(4))
+        self._lib.mcmp_oracle_moments(self._h, int(comp), _ptr(rho), _ptr(u), _ptr(v), _ptr(w))
+        sh = self.shape
+        return dict(rho=rho.reshape(sh), u=u.reshape(sh), v=v.reshape(sh), w=w.reshape(sh))
 
 
 def mt19937(seed: int, n: int) -> np.ndarray:

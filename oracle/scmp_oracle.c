@@ -508,3 +508,153 @@ void scmp_oracle_droplet_rho(int nx, int ny, int nz, double R, double W,
                 out[c++] = 0.5 * (rl + rv) - 0.5 * (rl - rv) * tanh(2.0 * (r - R) / W);
             }
 }
+
+/* ---- two components: mcmp2D / mcmp3D ---------------------------------------
+ * Two single-component states (their own tau, r/u/v/w, force arrays, f) and
+ * one coupling G; the Shan-Chen force acts between the components only and
+ * both collide with the common velocity.  mcmp2D.cpp:10-212,
+ * mcmp3D.cpp:10-233. */
+struct mcmp_oracle {
+    scmp_oracle* c[2];
+    double G;
+    int order;
+};
+
+mcmp_oracle* mcmp_oracle_create(int q, int nx, int ny, int nz, double tauA, double tauB, double G,
+                                int order, int nthreads)
+{
+    mcmp_oracle* m = (mcmp_oracle*)calloc(1, sizeof(*m));
+    if (!m) return NULL;
+    m->c[0] = scmp_oracle_create(q, nx, ny, nz, tauA, 0.0, order, 0, nthreads);
+    m->c[1] = scmp_oracle_create(q, nx, ny, nz, tauB, 0.0, order, 0, nthreads);
+    m->G = G;
+    m->order = order;
+    if (!m->c[0] || !m->c[1]) {
+        mcmp_oracle_destroy(m);
+        return NULL;
+    }
+    return m;
+}
+
+void mcmp_oracle_destroy(mcmp_oracle* m)
+{
+    if (!m) return;
+    scmp_oracle_destroy(m->c[0]);
+    scmp_oracle_destroy(m->c[1]);
+    free(m);
+}
+
+/* mcmp2D::calculateShanChenForces (mcmp2D.cpp:145-201) and its 3-D form
+ * (mcmp3D.cpp:155-225): psi of both components at the 19 (9) neighbours in
+ * stencil order, the cross forces -G psi_A S_B and -G psi_B S_A, and the
+ * common velocity u' = (u_A r_A/tau_A + u_B r_B/tau_B) / (r_A/tau_A +
+ * r_B/tau_B) written into both components' u arrays. */
+static void mcmp_forces(mcmp_oracle* m)
+{
+    scmp_oracle* a = m->c[0];
+    scmp_oracle* b = m->c[1];
+    const scmp_oracle* h = a;
+    const int q = a->q;
+    const double G = m->G;
+    FOR_INTERIOR_BEGIN(h)
+        double rhoA = a->r[ndx];
+        double rhoB = b->r[ndx];
+        double psi_rhoA = psi(a, rhoA);
+        double psi_rhoB = psi(b, rhoB);
+        double GsumxA = 0.0, GsumyA = 0.0, GsumzA = 0.0;
+        double GsumxB = 0.0, GsumyB = 0.0, GsumzB = 0.0;
+        for (int n = 0; n < q; n++) {
+            int64_t nbr = nd(h, i + h->exi[n], j + h->eyi[n], h->dim == 3 ? k + h->ezi[n] : 0);
+            double psi_rhoA_nbr = psi(a, a->r[nbr]);
+            double psi_rhoB_nbr = psi(b, b->r[nbr]);
+            GsumxA += h->wa[n] * h->ex[n] * psi_rhoA_nbr;
+            GsumyA += h->wa[n] * h->ey[n] * psi_rhoA_nbr;
+            GsumzA += h->wa[n] * h->ez[n] * psi_rhoA_nbr;
+            GsumxB += h->wa[n] * h->ex[n] * psi_rhoB_nbr;
+            GsumyB += h->wa[n] * h->ey[n] * psi_rhoB_nbr;
+            GsumzB += h->wa[n] * h->ez[n] * psi_rhoB_nbr;
+        }
+        a->fx[ndx] = -G * psi_rhoA * GsumxB;
+        a->fy[ndx] = -G * psi_rhoA * GsumyB;
+        b->fx[ndx] = -G * psi_rhoB * GsumxA;
+        b->fy[ndx] = -G * psi_rhoB * GsumyA;
+        if (h->dim == 3) {
+            a->fz[ndx] = -G * psi_rhoA * GsumzB;
+            b->fz[ndx] = -G * psi_rhoB * GsumzA;
+        }
+        /* LBfluid2D::getURhoDivTau etc. (LBfluid2D.cpp:179-192): u*r/tau, r/tau */
+        double rdivtauA = a->r[ndx] / a->tau;
+        double rdivtauB = b->r[ndx] / b->tau;
+        double uP = (a->u[ndx] * a->r[ndx] / a->tau + b->u[ndx] * b->r[ndx] / b->tau) /
+                    (rdivtauA + rdivtauB);
+        double vP = (a->v[ndx] * a->r[ndx] / a->tau + b->v[ndx] * b->r[ndx] / b->tau) /
+                    (rdivtauA + rdivtauB);
+        a->u[ndx] = uP; a->v[ndx] = vP;
+        b->u[ndx] = uP; b->v[ndx] = vP;
+        if (h->dim == 3) {
+            double wP = (a->w[ndx] * a->r[ndx] / a->tau + b->w[ndx] * b->r[ndx] / b->tau) /
+                        (rdivtauA + rdivtauB);
+            a->w[ndx] = wP;
+            b->w[ndx] = wP;
+        }
+    FOR_INTERIOR_END
+}
+
+void mcmp_oracle_set_rho(mcmp_oracle* m, const double* rhoA, const double* rhoB)
+{
+    /* mcmp2D.cpp:67-81, mcmp3D.cpp:67-85 (velocities zero) */
+    scmp_oracle_set_rho(m->c[0], rhoA);
+    scmp_oracle_set_rho(m->c[1], rhoB);
+}
+
+void mcmp_oracle_init(mcmp_oracle* m)
+{
+    /* mcmp2D.cpp:83-87 / mcmp3D.cpp:87-91 */
+    ghost_fill(m->c[0], m->c[0]->r, 1);
+    ghost_fill(m->c[1], m->c[1]->r, 1);
+    mcmp_forces(m);
+    set_f_to_feq(m->c[0]);
+    set_f_to_feq(m->c[1]);
+}
+
+void mcmp_oracle_step(mcmp_oracle* m, int nsteps)
+{
+    for (int s = 0; s < nsteps; s++) {
+        if (m->order == 3) {
+            /* mcmp3D::updateLatticeBoltzmann, mcmp3D.cpp:97-123 */
+            for (int c = 0; c < 2; c++) {
+                macros(m->c[c]);
+                ghost_fill(m->c[c], m->c[c]->r, 1);
+            }
+            mcmp_forces(m);
+            collide_stream(m->c[0]);
+            collide_stream(m->c[1]);
+        } else {
+            /* mcmp2D::updateLatticeBoltzmann, mcmp2D.cpp:97-123, with
+             * LBfluid2D::updateFluid_SC per component (LBfluid2D.cpp:227-312) */
+            mcmp_forces(m);
+            for (int c = 0; c < 2; c++) {
+                collide_stream(m->c[c]);
+                macros(m->c[c]);
+                ghost_fill(m->c[c], m->c[c]->r, 1);
+            }
+        }
+    }
+}
+
+void mcmp_oracle_get(const mcmp_oracle* m, int comp, double* rho, double* u, double* v, double* w,
+                     double* f)
+{
+    scmp_oracle_get(m->c[comp & 1], rho, u, v, w, f);
+}
+
+void mcmp_oracle_get_force(const mcmp_oracle* m, int comp, double* fx, double* fy, double* fz)
+{
+    scmp_oracle_get_force(m->c[comp & 1], fx, fy, fz);
+}
+
+void mcmp_oracle_moments(const mcmp_oracle* m, int comp, double* rho, double* u, double* v,
+                         double* w)
+{
+    scmp_oracle_moments(m->c[comp & 1], rho, u, v, w);
+}

@@ -92,6 +92,27 @@ void scmp_oracle_mt19937(uint32_t seed, int64_t n, uint32_t* out);
 void scmp_oracle_droplet_rho(int nx, int ny, int nz, double R, double W,
                              double rl, double rv, double* out);
 
+/* ---- two components (mcmp2D / mcmp3D) -------------------------------------
+ * Components A and B with their own tau and one cross coupling G (the
+ * reference's "LBApp/G"); order 2 is mcmp2D's step (forces and common
+ * velocity, then LBfluid2D::updateFluid_SC of A and of B), order 3 mcmp3D's
+ * (macros of both, forces and common velocity, collide-stream of both).
+ * comp: 0 = A, 1 = B.  The arrays have the reference's meanings: after a
+ * step of order 3 the u arrays hold the common velocity, in order 2 each
+ * component's own velocity. */
+typedef struct mcmp_oracle mcmp_oracle;
+mcmp_oracle* mcmp_oracle_create(int q, int nx, int ny, int nz, double tauA, double tauB, double G,
+                                int order, int nthreads);
+void mcmp_oracle_destroy(mcmp_oracle* m);
+void mcmp_oracle_set_rho(mcmp_oracle* m, const double* rhoA, const double* rhoB);
+void mcmp_oracle_init(mcmp_oracle* m);
+void mcmp_oracle_step(mcmp_oracle* m, int nsteps);
+void mcmp_oracle_get(const mcmp_oracle* m, int comp, double* rho, double* u, double* v, double* w,
+                     double* f);
+void mcmp_oracle_get_force(const mcmp_oracle* m, int comp, double* fx, double* fy, double* fz);
+void mcmp_oracle_moments(const mcmp_oracle* m, int comp, double* rho, double* u, double* v,
+                         double* w);
+
 #ifdef __cplusplus
 }
 #endif

@@ -26,7 +26,10 @@ def load_fixture(name):
 
 # small cases with full fields; "full_*" files hold hashes and norms of the
 # full-size runs instead
-FIXTURES = sorted(f[:-4] for f in os.listdir(GOLDEN) if f.endswith(".npz") and not f.startswith("full_"))
+FIXTURES = sorted(f[:-4] for f in os.listdir(GOLDEN)
+                  if f.endswith(".npz") and not f.startswith(("full_", "mcmp")))
+# two-component fixtures (reference mcmp2D / mcmp3D)
+MC_FIXTURES = sorted(f[:-4] for f in os.listdir(GOLDEN) if f.endswith(".npz") and f.startswith("mcmp"))
 FULL_FIXTURES = sorted(f[:-4] for f in os.listdir(GOLDEN) if f.endswith(".npz") and f.startswith("full_"))
 
 

@@ -8,6 +8,11 @@ Only runnable where the reference tree and MPICH are present:
 Each case is one .npz: inputs (rho0, parameters) and outputs at the listed
 steps -- the reference's r,u,v,w member arrays and, where stored, f
 ([node][q]) -- all float64, interior nodes, reference order.
+
+Two-component cases (MC_CASES, names mcmp*) come from the reference's
+mcmp2D / mcmp3D classes through oracle/_ref/ref_harness_mcmp: inputs rhoA0,
+rhoB0; per step and component c in (A, B): rho_c_s, u_c_s, v_c_s, w_c_s,
+fx_c_s, fy_c_s, fz_c_s (the force arrays) and, where stored, f_c_s.
 """
 import json
 import os
@@ -23,6 +28,7 @@ sys.path.insert(0, ROOT)
 import oracle  # noqa: E402
 
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+HARNESS_MC = os.path.join(ROOT, "oracle", "_ref", "ref_harness_mcmp")
 MPIRUN = "/opt/conda/bin/mpirun"
 
 # name, q, shape, tau, G, walls, init, steps, steps_with_f, ranks
@@ -39,6 +45,59 @@ CASES = [
     ("d3q19_walls_4x16x64_tau07", 19, (4, 16, 64), 0.7, -5.5, True, ("noise", 17), [1, 30], [30], 2),
     ("d3q19_5x8x64_tau13", 19, (5, 8, 64), 1.3, -5.0, False, ("noise", 18), [1, 2, 40], [], 2),
 ]
+
+
+# two components: name, q, shape, tauA, tauB, G, init, steps, steps_with_f,
+# ranks; init ("noise", seed): rhoA and rhoB = 0.5 + 0.1 (x - 0.5) from
+# seeds seed and seed + 1 (the reference's defaults rhoAi = rhoBi = 0.5,
+# rhoAi_noise = rhoBi_noise = 0.1, mcmp2D.cpp:21-24); ("droplet", R): an
+# A-rich disc or ball (1.0 / 0.05, width 2) in B, rhoB = 1.05 - rhoA
+MC_CASES = [
+    ("mcmp2d_13x11", 9, (13, 11), 1.0, 0.8, 3.0, ("noise", 21), [0, 1, 2, 10, 200], [1, 200], 3),
+    ("mcmp2d_48x40_droplet", 9, (48, 40), 1.0, 1.0, 3.5, ("droplet", 10.0), [1, 300], [300], 2),
+    ("mcmp3d_7x6x5", 19, (7, 6, 5), 1.0, 0.7, 3.0, ("noise", 23), [0, 1, 2, 10, 100], [1, 100], 3),
+    ("mcmp3d_4x8x64", 19, (4, 8, 64), 1.0, 1.0, 3.2, ("noise", 25), [1, 30], [], 2),
+]
+
+
+def mc_inputs(shape, init):
+    n = int(np.prod(shape))
+    if init[0] == "noise":
+        return (oracle.noise_rho(shape, init[1], mean=0.5, amp=0.1).reshape(-1),
+                oracle.noise_rho(shape, init[1] + 1, mean=0.5, amp=0.1).reshape(-1))
+    ra = oracle.droplet_rho(shape, init[1], rl=1.0, rv=0.05).reshape(-1)
+    assert ra.size == n
+    return ra, 1.05 - ra
+
+
+def run_mc_case(name, q, shape, tauA, tauB, G, init, steps, steps_f, ranks):
+    rA, rB = mc_inputs(shape, init)
+    nx, ny = shape[0], shape[1]
+    nz = shape[2] if len(shape) == 3 else 1
+    n = nx * ny * nz
+    out = {"rhoA0": rA, "rhoB0": rB}
+    with tempfile.TemporaryDirectory() as td:
+        pre = os.path.join(td, name)
+        rA.astype(np.float64).tofile(pre + "_rA0.bin")
+        rB.astype(np.float64).tofile(pre + "_rB0.bin")
+        cmd = [MPIRUN, "-np", str(ranks), HARNESS_MC, str(q), str(nx), str(ny), str(nz), repr(tauA),
+               repr(tauB), repr(G), pre + "_rA0.bin", pre + "_rB0.bin", pre, ",".join(map(str, steps))]
+        subprocess.run(cmd, check=True)
+        for s in steps:
+            for c in "AB":
+                a = np.fromfile(f"{pre}_{c}_{s}.bin")
+                for k, key in enumerate(("rho", "u", "v", "w", "fx", "fy", "fz")):
+                    if q == 9 and key in ("w", "fz"):
+                        continue
+                    out[f"{key}_{c}_{s}"] = a[k * n:(k + 1) * n]
+                if s in steps_f:
+                    out[f"f_{c}_{s}"] = a[7 * n:]
+    meta = dict(q=q, shape=list(shape), tauA=tauA, tauB=tauB, G=G, init=list(init), steps=steps,
+                steps_f=steps_f, ranks=ranks, order=3 if q == 19 else 2, components=2,
+                source="reference mcmp2D / mcmp3D classes via oracle/_ref/ref_harness_mcmp")
+    out["meta"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    print(name, "ok")
 
 
 def run_case(name, q, shape, tau, G, walls, init, steps, steps_f, ranks):
@@ -80,6 +139,9 @@ def main():
     for c in CASES:
         if not only or c[0] in only:
             run_case(*c)
+    for c in MC_CASES:
+        if not only or c[0] in only:
+            run_mc_case(*c)
 
 
 if __name__ == "__main__":
