@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 1
+#define LBM_ABI_VERSION 2
 
 enum lbm_status {
     LBM_OK = 0,
@@ -71,6 +71,14 @@ enum lbm_kernel {
  *   precision  enum lbm_precision
  *   rho_shift  density whose equilibrium is subtracted before 16-bit storage
  *   kernel     enum lbm_kernel
+ *   components 0 or 1: one component (scmp3D, and its D2Q9 form);
+ *              2: two-component Shan-Chen, "LBApp/type" mcmp2D (q = 9,
+ *              order 2, mcmp2D.cpp:97-123) or mcmp3D (q = 19, order 3,
+ *              mcmp3D.cpp:97-123): G couples the components ("LBApp/G",
+ *              mcmp2D.cpp:20), tau is component A's ("LBApp/tauA"),
+ *              tau_b component B's ("LBApp/tauB", mcmp2D.cpp:26).  FP64,
+ *              two-pass kernels, no walls.
+ *   tau_b      relaxation time of component B (components = 2)
  */
 typedef struct lbm_desc {
     int q;
@@ -82,6 +90,8 @@ typedef struct lbm_desc {
     int precision;
     double rho_shift;
     int kernel;
+    int components;
+    double tau_b;
 } lbm_desc;
 
 typedef struct lbm_handle lbm_handle;
@@ -142,6 +152,39 @@ int lbm_set_f(lbm_handle* h, const double* f);
 /* FP64 totals over the lattice of the current f: out[0] = mass,
  * out[1..3] = momentum (x, y, z; z = 0 in 2-D).  Synchronous. */
 int lbm_conservation(lbm_handle* h, double out[4]);
+
+/* ---- components -------------------------------------------------------------
+ * Two-component handles (lbm_desc.components = 2) hold components A (comp 0)
+ * and B (comp 1); the calls above act on A, these on either.  Replacing the
+ * reference's fA / fB members of mcmp2D / mcmp3D (mcmp2D.hpp:14-16). */
+
+/* Components of the handle (1 or 2). */
+int lbm_components(const lbm_handle* h);
+
+/* Initial density of one component (fA.setRho / fB.setRho with zero
+ * velocity, mcmp2D.cpp:67-81); lbm_set_rho sets component A. */
+int lbm_set_rho_component(lbm_handle* h, int comp, const double* rho0);
+
+/* The reference's r,u,v,w member arrays of one component
+ * (LBM_FIELDS_REFERENCE), or the moments of its current f
+ * (LBM_FIELDS_CURRENT).  Two components: in order 3 (mcmp3D) the u arrays
+ * after a step hold the common velocity of both components
+ * (mcmp3D.cpp:201-212), and r the density of f(N-1); in order 2 (mcmp2D)
+ * each component's own moments of f(N).  Synchronous. */
+int lbm_get_fields_component(lbm_handle* h, int comp, int kind, double* rho, double* u, double* v,
+                             double* w);
+
+/* The fx,fy,fz arrays of one component: with two components the cross force
+ * ((-G) psi_c(x)) sum (w e) psi_other(x+e) (mcmp2D.cpp:179-182). */
+int lbm_get_force_component(lbm_handle* h, int comp, double* fx, double* fy, double* fz);
+
+/* Populations of one component, as lbm_get_f / lbm_set_f (set both
+ * components before the next step). */
+int lbm_get_f_component(lbm_handle* h, int comp, double* f);
+int lbm_set_f_component(lbm_handle* h, int comp, const double* f);
+
+/* FP64 totals of one component's current f, as lbm_conservation. */
+int lbm_conservation_component(lbm_handle* h, int comp, double out[4]);
 
 /* Number of completed steps since lbm_init_equilibrium / lbm_set_f. */
 int64_t lbm_steps_done(const lbm_handle* h);

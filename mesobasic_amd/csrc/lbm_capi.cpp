@@ -89,8 +89,46 @@ int ensure_tmp(lbm_handle* h, int64_t bytes)
     return LBM_OK;
 }
 
+// Components of a handle, and the populations of component comp in buffer b
+// (component c holds planes c*q .. c*q+q-1).
+int comps(const lbm_handle* h) { return h->a.components; }
+double* pops64(lbm_handle* h, int b, int comp)
+{
+    return static_cast<double*>(h->buf[b]) + static_cast<int64_t>(comp) * h->d.q * h->g.ps;
+}
+
+int check_comp(const lbm_handle* h, int comp)
+{
+    if (comp < 0 || comp >= comps(h))
+        return fail(LBM_EINVAL, "component %d out of range (handle has %d)", comp, comps(h));
+    return LBM_OK;
+}
+
+// Two-component step (mcmp2D / mcmp3D): psi of both components, then one
+// pass that forms the cross forces and common velocity and collides both.
+int run_step2(lbm_handle* h)
+{
+    const int src = h->cur, dst = 1 - h->cur;
+    const int mode = h->first_step_stored ? 1 : 0;
+    const int64_t n = h->g.n;
+    for (int c = 0; c < 2; c++) {
+        if (mode == 1) HIP_OK(lbm::fp64_psi_from_rho(h->rho0 + c * n, h->psi + c * n, n, h->stream));
+        else HIP_OK(lbm::fp64_psi_from_pops(h->a, h->g, pops64(h, src, c), h->kind[src], h->psi + c * n,
+                                            h->stream));
+    }
+    HIP_OK(lbm::fp64_collide2(h->a, h->g, static_cast<const double*>(h->buf[src]), h->kind[src],
+                              static_cast<double*>(h->buf[dst]), h->psi, h->rho0, mode, h->stream));
+    h->kind[dst] = lbm::kHoldsPost;
+    h->cur = dst;
+    h->first_step_stored = false;
+    h->prev_valid = true;
+    h->steps++;
+    return LBM_OK;
+}
+
 int run_step(lbm_handle* h)
 {
+    if (comps(h) == 2) return run_step2(h);
     const int src = h->cur, dst = 1 - h->cur;
     if (!is_lowp(h)) {
         auto* Pin = static_cast<const double*>(h->buf[src]);
@@ -120,16 +158,16 @@ int run_step(lbm_handle* h)
     return LBM_OK;
 }
 
-// Moments of buffer b into the four host arrays.
-int moments_to_host(lbm_handle* h, int b, double* rho, double* u, double* v, double* w)
+// Moments of component comp of buffer b into the four host arrays.
+int moments_to_host(lbm_handle* h, int b, double* rho, double* u, double* v, double* w, int comp = 0)
 {
     const int64_t n = h->g.n;
     int rc = ensure_tmp(h, 4 * n * 8);
     if (rc) return rc;
     double* d = h->tmp;
     if (!is_lowp(h))
-        HIP_OK(lbm::fp64_moments(h->a, h->g, static_cast<const double*>(h->buf[b]), h->kind[b],
-                                 d, d + n, d + 2 * n, d + 3 * n, h->stream));
+        HIP_OK(lbm::fp64_moments(h->a, h->g, pops64(h, b, comp), h->kind[b], d, d + n, d + 2 * n,
+                                 d + 3 * n, h->stream));
     else
         HIP_OK(lbm::lowp_moments(h->d.precision, h->a, h->g, h->d.rho_shift, h->buf[b],
                                  h->kind[b], h->rref[b], d, d + n, d + 2 * n, d + 3 * n,
@@ -163,6 +201,16 @@ int check_desc(const lbm_desc* d)
         return fail(LBM_EINVAL, "unknown kernel %d", d->kernel);
     const int64_t n = static_cast<int64_t>(d->nx) * d->ny * d->nz;
     if (n >= (int64_t(1) << 32)) return fail(LBM_EINVAL, "lattice too large (%lld nodes)", (long long)n);
+    if (d->components < 0 || d->components > 2)
+        return fail(LBM_EINVAL, "components must be 0, 1 or 2 (got %d)", d->components);
+    if (d->components == 2) {
+        if (d->precision != LBM_FP64)
+            return fail(LBM_EINVAL, "two components run in FP64 only");
+        if (d->walls) return fail(LBM_EINVAL, "two components: walls are not supported");
+        if (d->kernel == LBM_KERNEL_FUSED)
+            return fail(LBM_EINVAL, "two components run on the two-pass kernels");
+        if (!(d->tau_b > 0.0) || !std::isfinite(d->tau_b)) return fail(LBM_EINVAL, "tau_b must be > 0");
+    }
     return LBM_OK;
 }
 
@@ -195,6 +243,9 @@ int lbm_create(const lbm_desc* desc, lbm_handle** out)
     h->g.ps = plane_stride(h->g.n, h->elem);
     h->a.q = desc->q; h->a.walls = desc->walls ? 1 : 0;
     h->a.tau = desc->tau; h->a.G = desc->G;
+    h->a.components = desc->components == 2 ? 2 : 1;
+    h->a.tau_b = desc->components == 2 ? desc->tau_b : desc->tau;
+    const int nc = h->a.components;
     if (desc->precision == LBM_FP64) {
         const bool fused_ok = lbm::fp64_fused_supported(h->a, h->g);
         if (desc->kernel == LBM_KERNEL_FUSED && !fused_ok) {
@@ -211,7 +262,7 @@ int lbm_create(const lbm_desc* desc, lbm_handle** out)
                                        : (h->g.n >= lbm::fused::kRowAutoMinNodes ||
                                           lbm::fused::row_full(h->g, lbm::fp64_default_shape(h->g, desc->q))));
         h->kernel = desc->kernel == LBM_KERNEL_AUTO
-                        ? (fused_auto ? LBM_KERNEL_FUSED : LBM_KERNEL_TWO_PASS)
+                        ? (fused_auto && nc == 1 ? LBM_KERNEL_FUSED : LBM_KERNEL_TWO_PASS)
                         : desc->kernel;
     } else {
         h->kernel = lbm::lowp_resolve_kernel(desc->precision, h->a, h->g, desc->kernel);
@@ -235,7 +286,7 @@ int lbm_create(const lbm_desc* desc, lbm_handle** out)
     h->tune.chunk = lbm::fused::default_chunk(h->g, h->tune.shape, max_chunk, waves_per_cu);
     if (lbm::fused::is_row_shape(h->tune.shape))
         h->tune.chunk = lbm::fused::default_row_chunk(h->g, h->tune.shape);
-    const int64_t pop_bytes = h->g.ps * desc->q * h->elem;
+    const int64_t pop_bytes = h->g.ps * desc->q * h->elem * nc;
     auto cleanup = [&](int code) {
         lbm_destroy(h);
         return code;
@@ -251,13 +302,13 @@ int lbm_create(const lbm_desc* desc, lbm_handle** out)
         // psi field: the two-pass kernels use it every step, and every
         // handle runs them for initialisation and for the LBfluid2D-order
         // first step, so it is always allocated.
-        if (lbm::dev_malloc(&h->psi, h->g.n * 8) != hipSuccess)
+        if (lbm::dev_malloc(&h->psi, nc * h->g.n * 8) != hipSuccess)
             return cleanup(fail(LBM_ENOMEM, "hipMalloc for psi failed"));
-        h->bytes += h->g.n * 8;
+        h->bytes += nc * h->g.n * 8;
     }
-    if (lbm::dev_malloc(&h->rho0, h->g.n * 8) != hipSuccess)
+    if (lbm::dev_malloc(&h->rho0, nc * h->g.n * 8) != hipSuccess)
         return cleanup(fail(LBM_ENOMEM, "hipMalloc for rho0 failed"));
-    h->bytes += h->g.n * 8;
+    h->bytes += nc * h->g.n * 8;
     if (lbm::lowp_node_shift(desc->precision)) {
         for (int b = 0; b < 2; b++)
             if (lbm::dev_malloc(&h->rref[b], h->g.n * 4) != hipSuccess)
@@ -290,20 +341,31 @@ void lbm_destroy(lbm_handle* h)
     delete h;
 }
 
-int lbm_set_rho(lbm_handle* h, const double* rho0)
+int lbm_set_rho(lbm_handle* h, const double* rho0) { return lbm_set_rho_component(h, 0, rho0); }
+
+int lbm_set_rho_component(lbm_handle* h, int comp, const double* rho0)
 {
     if (!h || !rho0) return fail(LBM_EINVAL, "lbm_set_rho: NULL argument");
-    HIP_OK(hipMemcpyAsync(h->rho0, rho0, h->g.n * 8, hipMemcpyHostToDevice, h->stream));
+    int rc = check_comp(h, comp);
+    if (rc) return rc;
+    HIP_OK(hipMemcpyAsync(h->rho0 + comp * h->g.n, rho0, h->g.n * 8, hipMemcpyHostToDevice, h->stream));
     HIP_OK(hipStreamSynchronize(h->stream));
     h->initialised = false;
     return LBM_OK;
 }
 
+int lbm_components(const lbm_handle* h) { return h ? h->a.components : 0; }
+
 int lbm_init_equilibrium(lbm_handle* h)
 {
     if (!h) return fail(LBM_EINVAL, "lbm_init_equilibrium: NULL handle");
     if (!h->psi) return fail(LBM_ESTATE, "psi field not allocated");
-    if (!is_lowp(h)) {
+    if (comps(h) == 2) {
+        for (int c = 0; c < 2; c++)
+            HIP_OK(lbm::fp64_psi_from_rho(h->rho0 + c * h->g.n, h->psi + c * h->g.n, h->g.n, h->stream));
+        HIP_OK(lbm::fp64_collide2(h->a, h->g, nullptr, lbm::kHoldsF, static_cast<double*>(h->buf[0]),
+                                  h->psi, h->rho0, 2, h->stream));
+    } else if (!is_lowp(h)) {
         double* psi = h->psi;
         HIP_OK(lbm::fp64_psi_from_rho(h->rho0, psi, h->g.n, h->stream));
         HIP_OK(lbm::fp64_collide(h->a, h->g, nullptr, lbm::kHoldsF, static_cast<double*>(h->buf[0]),
@@ -351,51 +413,100 @@ int lbm_step_timed(lbm_handle* h, int nsteps, double* ms)
 
 int lbm_get_fields(lbm_handle* h, int kind, double* rho, double* u, double* v, double* w)
 {
+    return lbm_get_fields_component(h, 0, kind, rho, u, v, w);
+}
+
+int lbm_get_fields_component(lbm_handle* h, int comp, int kind, double* rho, double* u, double* v,
+                             double* w)
+{
     if (!h) return fail(LBM_EINVAL, "lbm_get_fields: NULL handle");
+    int rc = check_comp(h, comp);
+    if (rc) return rc;
     if (!h->initialised) return fail(LBM_ESTATE, "lbm_get_fields before initialisation");
     const int64_t n = h->g.n;
-    if (kind == LBM_FIELDS_CURRENT) return moments_to_host(h, h->cur, rho, u, v, w);
+    if (kind == LBM_FIELDS_CURRENT) return moments_to_host(h, h->cur, rho, u, v, w, comp);
     if (kind != LBM_FIELDS_REFERENCE) return fail(LBM_EINVAL, "unknown fields kind %d", kind);
     if (h->steps == 0 && h->ref_is_rho0) {
-        // r = rho0, u = v = w = 0 (scmp3D.cpp:63-76 before any step)
-        if (rho) HIP_OK(hipMemcpyAsync(rho, h->rho0, n * 8, hipMemcpyDeviceToHost, h->stream));
+        // r = rho0, u = v = w = 0 (scmp3D.cpp:63-76 before any step; with two
+        // components the common velocity initLatticeBoltzmann leaves, which is
+        // 0: mcmp2D.cpp:85, mcmp3D.cpp:89)
+        if (rho)
+            HIP_OK(hipMemcpyAsync(rho, h->rho0 + comp * n, n * 8, hipMemcpyDeviceToHost, h->stream));
         HIP_OK(hipStreamSynchronize(h->stream));
         for (double* p : {u, v, w})
             if (p) std::memset(p, 0, n * 8);
         return LBM_OK;
     }
-    if (h->d.order == 2 || h->steps == 0) return moments_to_host(h, h->cur, rho, u, v, w);
+    if (h->d.order == 2 || h->steps == 0) return moments_to_host(h, h->cur, rho, u, v, w, comp);
     if (h->steps == 1 && h->d.order == 3 && !h->prev_valid)
         return fail(LBM_ESTATE, "previous populations unavailable");
-    // order 3: the arrays hold the moments computed at the start of step N
-    return moments_to_host(h, 1 - h->cur, rho, u, v, w);
+    const int prev = 1 - h->cur;
+    if (comps(h) == 1)
+        // order 3: the arrays hold the moments computed at the start of step N
+        return moments_to_host(h, prev, rho, u, v, w);
+    // mcmp3D: r = this component's density of f(N-1), u = the common
+    // velocity formed from both components' moments of f(N-1)
+    rc = ensure_tmp(h, 8 * n * 8);
+    if (rc) return rc;
+    double* A = h->tmp;
+    double* B = h->tmp + 4 * n;
+    HIP_OK(lbm::fp64_moments(h->a, h->g, pops64(h, prev, 0), h->kind[prev], A, A + n, A + 2 * n, A + 3 * n,
+                             h->stream));
+    HIP_OK(lbm::fp64_moments(h->a, h->g, pops64(h, prev, 1), h->kind[prev], B, B + n, B + 2 * n, B + 3 * n,
+                             h->stream));
+    const double* mine = comp == 0 ? A : B;
+    if (rho) HIP_OK(hipMemcpyAsync(rho, mine, n * 8, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(lbm::fp64_common_velocity(h->a, n, A, A + n, A + 2 * n, A + 3 * n, B, B + n, B + 2 * n, B + 3 * n,
+                                     h->stream));
+    double* outs[3] = {u, v, w};
+    for (int k = 0; k < 3; k++)
+        if (outs[k])
+            HIP_OK(hipMemcpyAsync(outs[k], A + (k + 1) * n, n * 8, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    return LBM_OK;
 }
 
 int lbm_get_force(lbm_handle* h, double* fx, double* fy, double* fz)
 {
+    return lbm_get_force_component(h, 0, fx, fy, fz);
+}
+
+int lbm_get_force_component(lbm_handle* h, int comp, double* fx, double* fy, double* fz)
+{
     if (!h) return fail(LBM_EINVAL, "lbm_get_force: NULL handle");
+    int rc = check_comp(h, comp);
+    if (rc) return rc;
     if (!h->initialised) return fail(LBM_ESTATE, "lbm_get_force before initialisation");
     const int64_t n = h->g.n;
-    int rc = ensure_tmp(h, 8 * n * 8);
+    const int nc = comps(h);
+    // per component: density the last step started from, then psi; then F
+    rc = ensure_tmp(h, (5 * nc + 3) * n * 8);
     if (rc) return rc;
-    double* rho = h->tmp;            // density the last step started from
-    double* psi = h->tmp + 4 * n;
-    double* F = h->tmp + 5 * n;
+    double* F = h->tmp + 5 * nc * n;
     const bool from_rho0 = h->ref_is_rho0 && (h->steps == 0 || (h->steps == 1 && h->d.order == 2));
-    if (from_rho0) {
-        HIP_OK(hipMemcpyAsync(rho, h->rho0, n * 8, hipMemcpyDeviceToDevice, h->stream));
-    } else {
-        const int b = (h->steps == 0) ? h->cur : 1 - h->cur;
-        if (!is_lowp(h))
-            HIP_OK(lbm::fp64_moments(h->a, h->g, static_cast<const double*>(h->buf[b]), h->kind[b],
-                                     rho, rho + n, rho + 2 * n, rho + 3 * n, h->stream));
-        else
-            HIP_OK(lbm::lowp_moments(h->d.precision, h->a, h->g, h->d.rho_shift, h->buf[b],
-                                     h->kind[b], h->rref[b], rho, rho + n, rho + 2 * n, rho + 3 * n,
-                                     h->stream));
+    for (int c = 0; c < nc; c++) {
+        double* rho = h->tmp + 5 * c * n;
+        double* psi = rho + 4 * n;
+        if (from_rho0) {
+            HIP_OK(hipMemcpyAsync(rho, h->rho0 + c * n, n * 8, hipMemcpyDeviceToDevice, h->stream));
+        } else {
+            const int b = (h->steps == 0) ? h->cur : 1 - h->cur;
+            if (!is_lowp(h))
+                HIP_OK(lbm::fp64_moments(h->a, h->g, pops64(h, b, c), h->kind[b], rho, rho + n, rho + 2 * n,
+                                         rho + 3 * n, h->stream));
+            else
+                HIP_OK(lbm::lowp_moments(h->d.precision, h->a, h->g, h->d.rho_shift, h->buf[b], h->kind[b],
+                                         h->rref[b], rho, rho + n, rho + 2 * n, rho + 3 * n, h->stream));
+        }
+        HIP_OK(lbm::fp64_psi_from_rho(rho, psi, n, h->stream));
     }
-    HIP_OK(lbm::fp64_psi_from_rho(rho, psi, n, h->stream));
-    HIP_OK(lbm::fp64_force(h->a, h->g, psi, F, F + n, F + 2 * n, h->stream));
+    if (nc == 1) {
+        HIP_OK(lbm::fp64_force(h->a, h->g, h->tmp + 4 * n, F, F + n, F + 2 * n, h->stream));
+    } else {
+        const double* ps = h->tmp + (5 * comp + 4) * n;
+        const double* po = h->tmp + (5 * (1 - comp) + 4) * n;
+        HIP_OK(lbm::fp64_force2(h->a, h->g, ps, po, F, F + n, F + 2 * n, h->stream));
+    }
     double* outs[3] = {fx, fy, fz};
     for (int k = 0; k < 3; k++)
         if (outs[k])
@@ -404,16 +515,19 @@ int lbm_get_force(lbm_handle* h, double* fx, double* fy, double* fz)
     return LBM_OK;
 }
 
-int lbm_get_f(lbm_handle* h, double* f)
+int lbm_get_f(lbm_handle* h, double* f) { return lbm_get_f_component(h, 0, f); }
+
+int lbm_get_f_component(lbm_handle* h, int comp, double* f)
 {
     if (!h || !f) return fail(LBM_EINVAL, "lbm_get_f: NULL argument");
+    int rc = check_comp(h, comp);
+    if (rc) return rc;
     if (!h->initialised) return fail(LBM_ESTATE, "lbm_get_f before initialisation");
     const int64_t n = h->g.n * h->d.q;
-    int rc = ensure_tmp(h, n * 8);
+    rc = ensure_tmp(h, n * 8);
     if (rc) return rc;
     if (!is_lowp(h))
-        HIP_OK(lbm::fp64_get_f(h->a, h->g, static_cast<const double*>(h->buf[h->cur]),
-                               h->kind[h->cur], h->tmp, h->stream));
+        HIP_OK(lbm::fp64_get_f(h->a, h->g, pops64(h, h->cur, comp), h->kind[h->cur], h->tmp, h->stream));
     else
         HIP_OK(lbm::lowp_get_f(h->d.precision, h->a, h->g, h->d.rho_shift, h->buf[h->cur],
                                h->kind[h->cur], h->rref[h->cur], h->tmp, h->stream));
@@ -422,15 +536,26 @@ int lbm_get_f(lbm_handle* h, double* f)
     return LBM_OK;
 }
 
-int lbm_set_f(lbm_handle* h, const double* f)
+int lbm_set_f(lbm_handle* h, const double* f) { return lbm_set_f_component(h, 0, f); }
+
+int lbm_set_f_component(lbm_handle* h, int comp, const double* f)
 {
     if (!h || !f) return fail(LBM_EINVAL, "lbm_set_f: NULL argument");
-    const int64_t n = h->g.n * h->d.q;
-    int rc = ensure_tmp(h, n * 8);
+    int rc = check_comp(h, comp);
     if (rc) return rc;
+    const int64_t n = h->g.n * h->d.q;
+    rc = ensure_tmp(h, 2 * n * 8);
+    if (rc) return rc;
+    if (comps(h) == 2 && h->initialised && (h->cur != 0 || h->kind[h->cur] != lbm::kHoldsF)) {
+        // the other component goes into buffer 0 as f too: its pulled values
+        // (post-collision values read upstream) through the staging buffer
+        const int other = 1 - comp;
+        HIP_OK(lbm::fp64_get_f(h->a, h->g, pops64(h, h->cur, other), h->kind[h->cur], h->tmp + n, h->stream));
+        HIP_OK(lbm::fp64_set_f(h->a, h->g, h->tmp + n, pops64(h, 0, other), h->stream));
+    }
     HIP_OK(hipMemcpyAsync(h->tmp, f, n * 8, hipMemcpyHostToDevice, h->stream));
     if (!is_lowp(h))
-        HIP_OK(lbm::fp64_set_f(h->a, h->g, h->tmp, static_cast<double*>(h->buf[0]), h->stream));
+        HIP_OK(lbm::fp64_set_f(h->a, h->g, h->tmp, pops64(h, 0, comp), h->stream));
     else
         HIP_OK(lbm::lowp_set_f(h->d.precision, h->a, h->g, h->d.rho_shift, h->tmp, h->buf[0],
                                h->rref[0], h->stream));
@@ -445,17 +570,21 @@ int lbm_set_f(lbm_handle* h, const double* f)
     return LBM_OK;
 }
 
-int lbm_conservation(lbm_handle* h, double out[4])
+int lbm_conservation(lbm_handle* h, double out[4]) { return lbm_conservation_component(h, 0, out); }
+
+int lbm_conservation_component(lbm_handle* h, int comp, double out[4])
 {
     if (!h || !out) return fail(LBM_EINVAL, "lbm_conservation: NULL argument");
+    int rc = check_comp(h, comp);
+    if (rc) return rc;
     if (!h->initialised) return fail(LBM_ESTATE, "lbm_conservation before initialisation");
     int nb = 0;
     const int maxb = lbm::fp64_conservation_blocks(h->g);
-    int rc = ensure_tmp(h, int64_t(maxb) * 4 * 8);
+    rc = ensure_tmp(h, int64_t(maxb) * 4 * 8);
     if (rc) return rc;
     if (!is_lowp(h))
-        HIP_OK(lbm::fp64_conservation(h->a, h->g, static_cast<const double*>(h->buf[h->cur]),
-                                      h->kind[h->cur], h->tmp, &nb, h->stream));
+        HIP_OK(lbm::fp64_conservation(h->a, h->g, pops64(h, h->cur, comp), h->kind[h->cur], h->tmp, &nb,
+                                      h->stream));
     else
         HIP_OK(lbm::lowp_conservation(h->d.precision, h->a, h->g, h->d.rho_shift,
                                       h->buf[h->cur], h->kind[h->cur], h->rref[h->cur], h->tmp,

@@ -27,6 +27,7 @@
 #include "lbm_stencil.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace lbm {
 namespace {
@@ -325,6 +326,154 @@ __global__ void __launch_bounds__(kConsBlock)
     if (threadIdx.x < 4) part[blockIdx.x * 4 + threadIdx.x] = red[threadIdx.x][0];
 }
 
+// ---- two components (mcmp2D / mcmp3D) ---------------------------------------
+//
+// Components A and B share one population buffer: component c holds planes
+// c*Q .. c*Q+Q-1.  One node's step pulls both, forms the cross forces from the
+// other component's psi at the neighbours and the common velocity, and
+// collides each component with it, in the reference's operation order:
+//   forces    mcmp2D.cpp:153-182 / mcmp3D.cpp:163-213
+//             F_A = ((-G)*psi_A(x)) * sum (w*e)*psi_B(x+e), F_B likewise
+//   common u  mcmp2D.cpp:185-196 with LBfluid2D.cpp:179-192
+//             u' = ((u_A*r_A)/tau_A + (u_B*r_B)/tau_B) / (r_A/tau_A + r_B/tau_B)
+//   collide   LBfluid2D.cpp:235-249 / LBfluid3D.cpp:259-276 with u = u'
+// MODE 0: moments of the pulled f (mcmp3D's macros at the start of every
+// step; mcmp2D's arrays after its first step); 1: rho = rho0 and u = 0
+// (mcmp2D's first step: the arrays hold the initial densities and the
+// common velocity initLatticeBoltzmann left, which is 0); 2: the initial
+// equilibrium (mcmp2D.cpp:83-87, mcmp3D.cpp:87-91).
+
+// sum_n (w_n e_n) psi(x + e_n) in stencil order (zero terms skipped: exact)
+template <int Q>
+__device__ __forceinline__ void psi_sums(const double* __restrict__ psi, const Nbr& nb, double& gx,
+                                         double& gy, double& gz)
+{
+    using S = St<Q>;
+    gx = 0.0; gy = 0.0; gz = 0.0;
+#pragma unroll
+    for (int n = 1; n < Q; n++) {
+        const double t = S::w[n] * psi[at(nb, S::ex[n], S::ey[n], S::ez[n])];
+        if (S::ex[n] == 1) gx = gx + t;
+        if (S::ex[n] == -1) gx = gx - t;
+        if (S::ey[n] == 1) gy = gy + t;
+        if (S::ey[n] == -1) gy = gy - t;
+        if (S::ez[n] == 1) gz = gz + t;
+        if (S::ez[n] == -1) gz = gz - t;
+    }
+}
+
+// BGK collision of one component with velocity u' + tau*F/rho; writes the
+// post-collision values (MODE 2: the equilibrium itself).
+template <int Q, int MODE>
+__device__ __forceinline__ void collide_one(const double (&f)[Q], double rho, double up, double vp,
+                                            double wp, double Fx, double Fy, double Fz, double tau,
+                                            double* __restrict__ P, int64_t ps, int64_t c)
+{
+    using S = St<Q>;
+    const double ueq = up + (tau * Fx) / rho;
+    const double veq = vp + (tau * Fy) / rho;
+    double weq, usq;
+    if constexpr (S::D == 3) {
+        weq = wp + (tau * Fz) / rho;
+        usq = (ueq * ueq + veq * veq) + weq * weq;
+    } else {
+        weq = 0.0;
+        usq = ueq * ueq + veq * veq;
+    }
+    const double usq15 = 1.5 * usq;
+    static_for<Q>([&](auto I) {
+        constexpr int n = decltype(I)::value;
+        const double ev = edot<S::ex[n], S::ey[n], S::ez[n]>(ueq, veq, weq);
+        const double feq = (((1.0 + 3.0 * ev) + (4.5 * ev) * ev) - usq15) * (rho * S::w[n]);
+        P[n * ps + c] = (MODE == 2) ? feq : f[n] - (f[n] - feq) / tau;
+    });
+}
+
+template <int Q, int MODE>
+__global__ void __launch_bounds__(kBlock)
+    k_collide2(const double* __restrict__ Pin, int kind, double* __restrict__ Pout,
+               const double* __restrict__ psiA, const double* __restrict__ psiB,
+               const double* __restrict__ rho0A, const double* __restrict__ rho0B, Grid g,
+               double tauA, double tauB, double G)
+{
+    using S = St<Q>;
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (c >= g.n) return;
+    const Coord p = decompose(static_cast<uint32_t>(c), g);
+    const Nbr nb = neighbours(p, g);
+    const int64_t off = static_cast<int64_t>(Q) * g.ps;   // component B's planes
+
+    double fA[Q], fB[Q];
+    double rA, rB, uA, vA, wA, uB, vB, wB;
+    if constexpr (MODE == 0) {
+        load_f<Q, false>(Pin, g, kind, p, nb, c, fA);
+        load_f<Q, false>(Pin + off, g, kind, p, nb, c, fB);
+        double sx, sy, sz;
+        moments<Q>(fA, rA, sx, sy, sz);
+        uA = sx / rA; vA = sy / rA; wA = sz / rA;
+        moments<Q>(fB, rB, sx, sy, sz);
+        uB = sx / rB; vB = sy / rB; wB = sz / rB;
+    } else {
+        if constexpr (MODE == 1) {
+            load_f<Q, false>(Pin, g, kind, p, nb, c, fA);
+            load_f<Q, false>(Pin + off, g, kind, p, nb, c, fB);
+        }
+        rA = rho0A[c]; rB = rho0B[c];
+        uA = vA = wA = uB = vB = wB = 0.0;
+    }
+
+    double gAx, gAy, gAz, gBx, gBy, gBz;
+    psi_sums<Q>(psiA, nb, gAx, gAy, gAz);
+    psi_sums<Q>(psiB, nb, gBx, gBy, gBz);
+    const double mg = -G;
+    const double pa = mg * psiA[c], pb = mg * psiB[c];
+
+    const double rdA = rA / tauA, rdB = rB / tauB;
+    const double den = rdA + rdB;
+    const double up = ((uA * rA) / tauA + (uB * rB) / tauB) / den;
+    const double vp = ((vA * rA) / tauA + (vB * rB) / tauB) / den;
+    const double wp = (S::D == 3) ? ((wA * rA) / tauA + (wB * rB) / tauB) / den : 0.0;
+
+    collide_one<Q, MODE>(fA, rA, up, vp, wp, pa * gBx, pa * gBy, pa * gBz, tauA, Pout, g.ps, c);
+    collide_one<Q, MODE>(fB, rB, up, vp, wp, pb * gAx, pb * gAy, pb * gAz, tauB, Pout + off, g.ps, c);
+}
+
+// Cross force on the component whose psi is psi_self: ((-G)*psi_self(x)) *
+// sum (w*e)*psi_other(x+e) (the fx,fy,fz arrays of mcmp2D / mcmp3D).
+template <int Q>
+__global__ void __launch_bounds__(kBlock)
+    k_force2(const double* __restrict__ psi_self, const double* __restrict__ psi_other, Grid g, double G,
+             double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz)
+{
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (c >= g.n) return;
+    const Coord p = decompose(static_cast<uint32_t>(c), g);
+    const Nbr nb = neighbours(p, g);
+    double gx, gy, gz;
+    psi_sums<Q>(psi_other, nb, gx, gy, gz);
+    const double pc = -G * psi_self[c];
+    fx[c] = pc * gx;
+    fy[c] = pc * gy;
+    fz[c] = (St<Q>::D == 3) ? pc * gz : 0.0;
+}
+
+// Common velocity from both components' moments, written over A's velocity
+// arrays (the u arrays mcmp3D leaves after a step).
+__global__ void __launch_bounds__(kBlock)
+    k_common_velocity(int64_t n, int three_d, const double* __restrict__ rA, double* __restrict__ uA,
+                      double* __restrict__ vA, double* __restrict__ wA, const double* __restrict__ rB,
+                      const double* __restrict__ uB, const double* __restrict__ vB,
+                      const double* __restrict__ wB, double tauA, double tauB)
+{
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const double ra = rA[c], rb = rB[c];
+    const double den = ra / tauA + rb / tauB;
+    uA[c] = ((uA[c] * ra) / tauA + (uB[c] * rb) / tauB) / den;
+    vA[c] = ((vA[c] * ra) / tauA + (vB[c] * rb) / tauB) / den;
+    wA[c] = three_d ? ((wA[c] * ra) / tauA + (wB[c] * rb) / tauB) / den : 0.0;
+}
+
 inline unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
 
 }  // namespace
@@ -401,6 +550,44 @@ hipError_t fp64_set_f(const StepArgs& a, const Grid& g, const double* aos, doubl
 {
     if (a.q == 9) k_set_f<9><<<blocks_for(g.n), kBlock, 0, s>>>(aos, g, P);
     else k_set_f<19><<<blocks_for(g.n), kBlock, 0, s>>>(aos, g, P);
+    return hipGetLastError();
+}
+
+hipError_t fp64_collide2(const StepArgs& a, const Grid& g, const double* Pin, int kind, double* Pout,
+                         const double* psi, const double* rho0, int mode, hipStream_t s)
+{
+    if (a.walls) return hipErrorNotSupported;
+    const unsigned nb = blocks_for(g.n);
+    const double* psiB = psi + g.n;
+    const double* r0B = rho0 + g.n;
+    auto go = [&](auto qtag) {
+        constexpr int Q = decltype(qtag)::value;
+        if (mode == 0)
+            k_collide2<Q, 0><<<nb, kBlock, 0, s>>>(Pin, kind, Pout, psi, psiB, rho0, r0B, g, a.tau, a.tau_b, a.G);
+        else if (mode == 1)
+            k_collide2<Q, 1><<<nb, kBlock, 0, s>>>(Pin, kind, Pout, psi, psiB, rho0, r0B, g, a.tau, a.tau_b, a.G);
+        else
+            k_collide2<Q, 2><<<nb, kBlock, 0, s>>>(Pin, kind, Pout, psi, psiB, rho0, r0B, g, a.tau, a.tau_b, a.G);
+    };
+    if (a.q == 9) go(std::integral_constant<int, 9>{});
+    else go(std::integral_constant<int, 19>{});
+    return hipGetLastError();
+}
+
+hipError_t fp64_force2(const StepArgs& a, const Grid& g, const double* psi_self, const double* psi_other,
+                       double* fx, double* fy, double* fz, hipStream_t s)
+{
+    if (a.q == 9) k_force2<9><<<blocks_for(g.n), kBlock, 0, s>>>(psi_self, psi_other, g, a.G, fx, fy, fz);
+    else k_force2<19><<<blocks_for(g.n), kBlock, 0, s>>>(psi_self, psi_other, g, a.G, fx, fy, fz);
+    return hipGetLastError();
+}
+
+hipError_t fp64_common_velocity(const StepArgs& a, int64_t n, const double* rA, double* uA, double* vA,
+                                double* wA, const double* rB, const double* uB, const double* vB,
+                                const double* wB, hipStream_t s)
+{
+    k_common_velocity<<<blocks_for(n), kBlock, 0, s>>>(n, a.q == 19 ? 1 : 0, rA, uA, vA, wA, rB, uB, vB, wB,
+                                                       a.tau, a.tau_b);
     return hipGetLastError();
 }
 

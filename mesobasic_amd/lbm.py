@@ -35,7 +35,8 @@ class Desc(ctypes.Structure):
     _fields_ = [("q", ctypes.c_int), ("nx", ctypes.c_int), ("ny", ctypes.c_int),
                 ("nz", ctypes.c_int), ("tau", ctypes.c_double), ("G", ctypes.c_double),
                 ("order", ctypes.c_int), ("walls", ctypes.c_int), ("precision", ctypes.c_int),
-                ("rho_shift", ctypes.c_double), ("kernel", ctypes.c_int)]
+                ("rho_shift", ctypes.c_double), ("kernel", ctypes.c_int),
+                ("components", ctypes.c_int), ("tau_b", ctypes.c_double)]
 
 
 class BatchDesc(ctypes.Structure):
@@ -118,6 +119,13 @@ def load():
         "lbm_testing_alloc_count": (ctypes.c_int64, []),
         "lbm_testing_fail_alloc": (None, [ctypes.c_int64]),
         "lbm_probe_copy": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, ctypes.c_int, _D]),
+        "lbm_components": (ctypes.c_int, [P]),
+        "lbm_set_rho_component": (ctypes.c_int, [P, ctypes.c_int, _D]),
+        "lbm_get_fields_component": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, _D, _D, _D, _D]),
+        "lbm_get_force_component": (ctypes.c_int, [P, ctypes.c_int, _D, _D, _D]),
+        "lbm_get_f_component": (ctypes.c_int, [P, ctypes.c_int, _D]),
+        "lbm_set_f_component": (ctypes.c_int, [P, ctypes.c_int, _D]),
+        "lbm_conservation_component": (ctypes.c_int, [P, ctypes.c_int, _D]),
         "lbm_probe_memcpy": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, _D]),
         "lbm_probe_tile_map": (ctypes.c_int, [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_int)] * 3),
     }
@@ -209,10 +217,13 @@ def probe_tile_map(nty, ntz, ncx, order=2):
 
 
 class Lattice:
-    """One Shan-Chen lattice resident on the GPU (a C-ABI handle)."""
+    """One Shan-Chen lattice resident on the GPU (a C-ABI handle).
+    components=2 with tau_b: the two-component model (mcmp2D for q=9,
+    mcmp3D for q=19); tau is component A's, G couples the components, and
+    the comp argument of the accessors picks A (0) or B (1)."""
 
     def __init__(self, q, shape, tau, G, order=None, walls=False, precision=FP64,
-                 rho_shift=0.7, kernel=KERNEL_AUTO):
+                 rho_shift=0.7, kernel=KERNEL_AUTO, components=1, tau_b=None):
         lib = load()
         shape = tuple(int(s) for s in shape)
         if q == 9:
@@ -225,8 +236,10 @@ class Lattice:
             precision = PRECISIONS[precision]
         self.q, self.shape, self.order, self.precision = q, shape, order, precision
         self.n = nx * ny * nz
+        self.components = int(components)
         d = Desc(q, nx, ny, nz, float(tau), float(G), int(order), int(bool(walls)),
-                 int(precision), float(rho_shift), int(kernel))
+                 int(precision), float(rho_shift), int(kernel), int(components),
+                 float(tau if tau_b is None else tau_b))
         h = ctypes.c_void_p()
         _check(lib.lbm_create(ctypes.byref(d), ctypes.byref(h)))
         self._h = h
@@ -272,10 +285,10 @@ class Lattice:
         _check(self._lib.lbm_get_tuning(self._h, ctypes.byref(t)))
         return t.shape, t.chunk, t.order, t.nt_store
 
-    def set_rho(self, rho0):
+    def set_rho(self, rho0, comp=0):
         r = np.ascontiguousarray(rho0, dtype=np.float64).reshape(-1)
         assert r.size == self.n
-        _check(self._lib.lbm_set_rho(self._h, _dptr(r)))
+        _check(self._lib.lbm_set_rho_component(self._h, int(comp), _dptr(r)))
 
     def init(self):
         _check(self._lib.lbm_init_equilibrium(self._h))
@@ -291,31 +304,32 @@ class Lattice:
     def synchronize(self):
         _check(self._lib.lbm_synchronize(self._h))
 
-    def fields(self, kind=FIELDS_REFERENCE):
+    def fields(self, kind=FIELDS_REFERENCE, comp=0):
         rho, u, v, w = (np.empty(self.n) for _ in range(4))
-        _check(self._lib.lbm_get_fields(self._h, kind, _dptr(rho), _dptr(u), _dptr(v), _dptr(w)))
+        _check(self._lib.lbm_get_fields_component(self._h, int(comp), kind, _dptr(rho), _dptr(u), _dptr(v),
+                                                  _dptr(w)))
         sh = self.shape
         return dict(rho=rho.reshape(sh), u=u.reshape(sh), v=v.reshape(sh), w=w.reshape(sh))
 
-    def force(self):
+    def force(self, comp=0):
         fx, fy, fz = (np.empty(self.n) for _ in range(3))
-        _check(self._lib.lbm_get_force(self._h, _dptr(fx), _dptr(fy), _dptr(fz)))
+        _check(self._lib.lbm_get_force_component(self._h, int(comp), _dptr(fx), _dptr(fy), _dptr(fz)))
         sh = self.shape
         return fx.reshape(sh), fy.reshape(sh), fz.reshape(sh)
 
-    def get_f(self):
+    def get_f(self, comp=0):
         f = np.empty(self.n * self.q)
-        _check(self._lib.lbm_get_f(self._h, _dptr(f)))
+        _check(self._lib.lbm_get_f_component(self._h, int(comp), _dptr(f)))
         return f.reshape(self.shape + (self.q,))
 
-    def set_f(self, f):
+    def set_f(self, f, comp=0):
         a = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
         assert a.size == self.n * self.q
-        _check(self._lib.lbm_set_f(self._h, _dptr(a)))
+        _check(self._lib.lbm_set_f_component(self._h, int(comp), _dptr(a)))
 
-    def conservation(self):
+    def conservation(self, comp=0):
         out = np.empty(4)
-        _check(self._lib.lbm_conservation(self._h, _dptr(out)))
+        _check(self._lib.lbm_conservation_component(self._h, int(comp), _dptr(out)))
         return out
 
 

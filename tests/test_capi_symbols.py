@@ -37,7 +37,7 @@ def test_symbol_exported(name):
 
 def test_abi_version_and_errors_without_gpu():
     lib = mesobasic_amd.load()
-    assert lib.lbm_abi_version() == 1
+    assert lib.lbm_abi_version() == 2
     # host-only entry points work without a device
     import numpy as np
     r = mesobasic_amd.noise_rho((3, 4), 1)
