@@ -1,6 +1,7 @@
 #include "ScmpModel.hpp"
 
 #include <ctime>
+#include <random>
 #include <stdexcept>
 #include <string>
 
@@ -135,14 +136,130 @@ void scmp2D::outputLatticeBoltzmann()
     write_vtk_2d("vtkoutput", "rhoA", current_step, p.NX, p.NY, p.iskip, p.jskip, rho, us, vs);
 }
 
-// LBBaseClass::LBFactory (LBBaseClass.cpp:21-45).  The reference's default
-// type is mcmp2D; the multi-component, film and capsule models are not part
-// of this build, so they (like unknown names) yield nullptr.
+McmpModel::McmpModel(const CommonParams& pin, const InputParams& in, int q) : p(pin), q_(q)
+{
+    // mcmp2D.cpp:18-37 / mcmp3D.cpp:18-37
+    G = in("LBApp/G", 3.0);
+    rhoAi = in("LBApp/rhoAi", 0.5);
+    rhoBi = in("LBApp/rhoBi", 0.5);
+    rhoAi_noise = in("LBApp/rhoAi_noise", 0.1);
+    rhoBi_noise = in("LBApp/rhoBi_noise", 0.1);
+    tauA = in("LBApp/tauA", 1.0);
+    tauB = in("LBApp/tauB", 1.0);
+    const std::string stype = in("LBApp/stencil", q == 19 ? "D3Q19" : "D2Q9");
+    if ((q == 19 && stype != "D3Q19") || (q == 9 && stype != "D2Q9"))
+        throw std::runtime_error("LBApp/stencil " + stype + " does not match this model");
+    init_kind = in("LBApp/init", "noise");
+    seed = in.has("LBApp/seed") ? static_cast<unsigned>(in("LBApp/seed", 0.0))
+                                : static_cast<unsigned>(std::time(nullptr) * (p.rank + 1));
+    R = in("LBApp/R", 0.25 * p.NX);
+    W = in("LBApp/W", 2.0);
+    rho_l = in("LBApp/rho_l", 1.0);
+    rho_v = in("LBApp/rho_v", 0.05);
+    if (q == 9 && p.nz != 1) throw std::runtime_error("mcmp2D needs Domain/nz = 1");
+
+    lbm_desc d{};
+    d.q = q;
+    d.nx = p.nx;
+    d.ny = p.ny;
+    d.nz = (q == 19) ? p.nz : 1;
+    d.tau = tauA;
+    d.G = G;
+    d.order = (q == 19) ? 3 : 2;   // mcmp3D.cpp:97-123 / mcmp2D.cpp:97-123
+    d.precision = LBM_FP64;
+    d.kernel = LBM_KERNEL_AUTO;
+    d.components = 2;
+    d.tau_b = tauB;
+    check(lbm_create(&d, &h_), "lbm_create");
+}
+
+McmpModel::~McmpModel() { lbm_destroy(h_); }
+
+void McmpModel::check(int rc, const char* what) const
+{
+    if (rc != LBM_OK) throw std::runtime_error(std::string(what) + ": " + lbm_last_error());
+}
+
+void McmpModel::initLatticeBoltzmann()
+{
+    // mcmp2D.cpp:58-89 / mcmp3D.cpp:58-93: per node in loop order, one draw
+    // for A then one for B; a seeded mt19937 in place of rand()/RAND_MAX
+    const size_t n = static_cast<size_t>(nodes());
+    std::vector<double> rA(n), rB(n);
+    if (init_kind == "noise") {
+        std::mt19937 gen(seed);
+        for (size_t c = 0; c < n; c++) {
+            const double xa = static_cast<double>(gen()) / 4294967295.0;
+            const double xb = static_cast<double>(gen()) / 4294967295.0;
+            rA[c] = rhoAi + rhoAi_noise * (xa - 0.5);
+            rB[c] = rhoBi + rhoBi_noise * (xb - 0.5);
+        }
+    } else if (init_kind == "droplet") {
+        check(lbm_fill_droplet(p.nx, p.ny, q_ == 19 ? p.nz : 1, R, W, rho_l, rho_v, rA.data()),
+              "lbm_fill_droplet");
+        for (size_t c = 0; c < n; c++) rB[c] = (rho_l + rho_v) - rA[c];
+    } else {
+        throw std::runtime_error("LBApp/init must be noise or droplet (got " + init_kind + ")");
+    }
+    check(lbm_set_rho_component(h_, 0, rA.data()), "lbm_set_rho_component");
+    check(lbm_set_rho_component(h_, 1, rB.data()), "lbm_set_rho_component");
+    check(lbm_init_equilibrium(h_), "lbm_init_equilibrium");
+}
+
+void McmpModel::updateLatticeBoltzmann()
+{
+    // mcmp2D.cpp:97-123 / mcmp3D.cpp:97-123 (the Barrier has no counterpart
+    // on one GPU)
+    check(lbm_step(h_, 1), "lbm_step");
+}
+
+void McmpModel::fields(int comp, std::vector<double>& rho, std::vector<double>& u,
+                       std::vector<double>& v, std::vector<double>& w)
+{
+    const size_t n = static_cast<size_t>(nodes());
+    rho.resize(n); u.resize(n); v.resize(n); w.resize(n);
+    check(lbm_get_fields_component(h_, comp, LBM_FIELDS_REFERENCE, rho.data(), u.data(), v.data(),
+                                   w.data()),
+          "lbm_get_fields_component");
+}
+
+void McmpModel::force(int comp, std::vector<double>& fx, std::vector<double>& fy,
+                      std::vector<double>& fz)
+{
+    const size_t n = static_cast<size_t>(nodes());
+    fx.resize(n); fy.resize(n); fz.resize(n);
+    check(lbm_get_force_component(h_, comp, fx.data(), fy.data(), fz.data()), "lbm_get_force_component");
+}
+
+void McmpModel::outputLatticeBoltzmann()
+{
+    // mcmp2D.cpp:131-137 / mcmp3D.cpp:131-138: component A only
+    std::vector<double> rho, u, v, w;
+    fields(0, rho, u, v, w);
+    if (q_ == 19) {
+        write_vtk_3d("vtkoutput", "rhoA", current_step, p.NX, p.NY, p.NZ, p.iskip, p.jskip, p.kskip, rho);
+        return;
+    }
+    std::vector<double> fx, fy, fz;
+    force(0, fx, fy, fz);
+    std::vector<double> us(rho.size()), vs(rho.size());
+    for (size_t c = 0; c < rho.size(); c++) {
+        us[c] = u[c] + 0.5 * fx[c] / rho[c];   // LBfluid2D::getUStar (LBfluid2D.cpp:164-172)
+        vs[c] = v[c] + 0.5 * fy[c] / rho[c];
+    }
+    write_vtk_2d("vtkoutput", "rhoA", current_step, p.NX, p.NY, p.iskip, p.jskip, rho, us, vs);
+}
+
+// LBBaseClass::LBFactory (LBBaseClass.cpp:21-45), with the reference's
+// default type mcmp2D.  The film and capsule models are not part of this
+// build, so they (like unknown names) yield nullptr.
 LBBaseClass* LBBaseClass::LBFactory(const CommonParams& p, const InputParams& in)
 {
     const std::string lb_type = in("LBApp/type", "mcmp2D");
     if (lb_type == "scmp3D") return new scmp3D(p, in);
     if (lb_type == "scmp2D") return new scmp2D(p, in);
+    if (lb_type == "mcmp2D") return new mcmp2D(p, in);
+    if (lb_type == "mcmp3D") return new mcmp3D(p, in);
     return nullptr;
 }
 

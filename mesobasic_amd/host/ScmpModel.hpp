@@ -71,4 +71,55 @@ public:
     void outputLatticeBoltzmann() override;
 };
 
+// Two-component Shan-Chen: mcmp2D (D2Q9, mcmp2D.cpp) and mcmp3D (D3Q19,
+// mcmp3D.cpp).  One handle holds both components (lbm_desc.components = 2).
+//
+// Keys (read as mcmp2D.cpp:20-34 / mcmp3D.cpp:20-34 read them, same
+// defaults): LBApp/G (3.0), rhoAi and rhoBi (0.5), rhoAi_noise and
+// rhoBi_noise (0.1), tauA and tauB (1.0), stencil.  Keys of this build only:
+// LBApp/seed (mt19937 seed; default time(NULL)), LBApp/init (noise |
+// droplet: an A-rich drop of radius R, width W, densities rho_l / rho_v in
+// B, with rhoB = rho_l + rho_v - rhoA).
+class McmpModel : public LBBaseClass {
+public:
+    McmpModel(const CommonParams& p, const InputParams& in, int q);
+    ~McmpModel() override;
+    McmpModel(const McmpModel&) = delete;
+    McmpModel& operator=(const McmpModel&) = delete;
+
+    void setTimeStep(int step) override { current_step = step; }
+    void initLatticeBoltzmann() override;
+    void updateLatticeBoltzmann() override;
+    void outputLatticeBoltzmann() override;
+
+    // Host copies of component comp's r,u,v,w arrays and force arrays.
+    void fields(int comp, std::vector<double>& rho, std::vector<double>& u, std::vector<double>& v,
+                std::vector<double>& w);
+    void force(int comp, std::vector<double>& fx, std::vector<double>& fy, std::vector<double>& fz);
+    lbm_handle* handle() { return h_; }
+    long long nodes() const { return static_cast<long long>(p.nx) * p.ny * p.nz; }
+
+private:
+    const CommonParams& p;
+    int q_;
+    int current_step = 0;
+    double G, rhoAi, rhoBi, rhoAi_noise, rhoBi_noise, tauA, tauB;
+    std::string init_kind;
+    unsigned seed;
+    double R, W, rho_l, rho_v;
+    lbm_handle* h_ = nullptr;
+
+    void check(int rc, const char* what) const;
+};
+
+class mcmp2D : public McmpModel {
+public:
+    mcmp2D(const CommonParams& p, const InputParams& in) : McmpModel(p, in, 9) {}
+};
+
+class mcmp3D : public McmpModel {
+public:
+    mcmp3D(const CommonParams& p, const InputParams& in) : McmpModel(p, in, 19) {}
+};
+
 }  // namespace mesobasic

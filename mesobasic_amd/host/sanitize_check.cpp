@@ -123,11 +123,24 @@ void factory_checks()
 {
     CommonParams cp{};
     cp.NX = cp.NY = cp.NZ = 8;
-    for (const char* t : {"mcmp2D", "mcmp3D", "capsule2D", "nosuchtype"}) {
+    for (const char* t : {"mcmp2DFilm", "mcmp3DFilm", "capsule2D", "nosuchtype"}) {
         InputParams in = InputParams::from_string(std::string("[LBApp]\ntype = ") + t + "\n");
         LBBaseClass* m = LBBaseClass::LBFactory(cp, in);
         check(m == nullptr, std::string("factory refuses ") + t);
         delete m;
+    }
+    // the provided types build a model, or (without a GPU) report why not
+    cp.nz = 1;
+    for (const char* t : {"scmp2D", "mcmp2D"}) {
+        InputParams in = InputParams::from_string(std::string("[LBApp]\ntype = ") + t + "\nseed = 3\n");
+        try {
+            LBBaseClass* m = LBBaseClass::LBFactory(cp, in);
+            check(m != nullptr, std::string("factory builds ") + t);
+            delete m;
+        } catch (const std::exception& e) {
+            check(std::string(e.what()).find("lbm_create") != std::string::npos,
+                  std::string("factory error names lbm_create for ") + t);
+        }
     }
 }
 

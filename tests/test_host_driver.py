@@ -65,9 +65,10 @@ def test_getpot_grammar(tmp_path):
 
 
 def test_unknown_type_is_reported(tmp_path):
-    (tmp_path / "input.dat").write_text("[LBApp]\n app = true\n type = mcmp2D\n")
+    (tmp_path / "input.dat").write_text("[LBApp]\n app = true\n type = capsule2D\n")
     r = subprocess.run([MESO], cwd=tmp_path, capture_output=True, text=True)
     assert r.returncode != 0
+    assert "capsule2D" in r.stderr and "not provided" in r.stderr
 
 
 def _read_vtk(path):
@@ -155,3 +156,55 @@ def test_scmp2d_run_writes_rho_and_velocity(tmp_path):
     vs = fl["v"] + 0.5 * fy / fl["rho"]
     vel = lines[12 + nx * ny + 2:]
     assert vel == [f"{us[i, j]:.3f} {vs[i, j]:.3f} 0.000" for j in range(ny) for i in range(nx)]
+
+
+def _mc_rho0(shape, seed):
+    # the drop-in classes' seeded start: one mt19937, per node a draw for A
+    # then one for B (the reference's rand() order, mcmp2D.cpp:67-74)
+    x = oracle.mt19937(seed, 2 * int(np.prod(shape))).astype(np.float64) / 4294967295.0
+    rA = 0.5 + 0.1 * (x[0::2] - 0.5)
+    rB = 0.5 + 0.1 * (x[1::2] - 0.5)
+    return rA.reshape(shape), rB.reshape(shape)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("q", [9, 19])
+def test_mcmp_run_matches_oracle(tmp_path, q):
+    shape = (18, 14) if q == 9 else (10, 8, 6)
+    nstep = 30
+    dom = "".join(f"  n{a} = {v}\n" for a, v in zip("xyz", shape))
+    (tmp_path / "input.dat").write_text(f"""
+[Domain]
+{dom}[Time]
+  nstep = {nstep}
+[Output]
+  numOutputs = 1
+[LBApp]
+  app = true
+  type = {'mcmp2D' if q == 9 else 'mcmp3D'}
+  G = 3.0
+  tauA = 1.0
+  tauB = 0.8
+  seed = 4
+""")
+    r = subprocess.run([MESO], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rA, rB = _mc_rho0(shape, 4)
+    o = oracle.MCOracle(q, shape, 1.0, 0.8, 3.0)
+    o.set_rho(rA, rB)
+    o.init()
+    o.step(nstep)
+    fl = o.fields(0)
+    lines = open(tmp_path / "vtkoutput" / f"rhoA_{nstep}.vtk").read().splitlines()
+    n = int(np.prod(shape))
+    if q == 19:
+        nx, ny, nz = shape
+        expect = [f"{fl['rho'][i, j, k]:.3f}" for k in range(nz) for j in range(ny) for i in range(nx)]
+        assert lines[12:] == expect
+    else:
+        nx, ny = shape
+        assert lines[12:12 + n] == [f"{fl['rho'][i, j]:.3f}" for j in range(ny) for i in range(nx)]
+        us = fl["u"] + 0.5 * fl["fx"] / fl["rho"]
+        vs = fl["v"] + 0.5 * fl["fy"] / fl["rho"]
+        assert lines[12 + n + 2:] == [f"{us[i, j]:.3f} {vs[i, j]:.3f} 0.000" for j in range(ny)
+                                      for i in range(nx)]
