@@ -77,8 +77,14 @@ enum lbm_kernel {
  *              mcmp3D.cpp:97-123): G couples the components ("LBApp/G",
  *              mcmp2D.cpp:20), tau is component A's ("LBApp/tauA"),
  *              tau_b component B's ("LBApp/tauB", mcmp2D.cpp:26).  FP64,
- *              two-pass kernels, no walls.
+ *              two-pass kernels.  With walls = 1: the film models
+ *              mcmp2DFilm / mcmp3DFilm (walls on the last axis; the wall
+ *              rows carry no force and no common velocity, and fluid nodes
+ *              next to them interact with psi(rho_sol_a / rho_sol_b),
+ *              mcmp2DFilm.cpp:149-219, mcmp3DFilm.cpp:167-253).
  *   tau_b      relaxation time of component B (components = 2)
+ *   rho_sol_a, rho_sol_b   wall densities of A and B ("LBApp/rhoA_sol",
+ *              "rhoB_sol", mcmp2DFilm.cpp:21-22; components = 2, walls = 1)
  */
 typedef struct lbm_desc {
     int q;
@@ -92,6 +98,7 @@ typedef struct lbm_desc {
     int kernel;
     int components;
     double tau_b;
+    double rho_sol_a, rho_sol_b;
 } lbm_desc;
 
 typedef struct lbm_handle lbm_handle;

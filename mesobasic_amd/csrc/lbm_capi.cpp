@@ -206,7 +206,8 @@ int check_desc(const lbm_desc* d)
     if (d->components == 2) {
         if (d->precision != LBM_FP64)
             return fail(LBM_EINVAL, "two components run in FP64 only");
-        if (d->walls) return fail(LBM_EINVAL, "two components: walls are not supported");
+        if (d->walls && !(std::isfinite(d->rho_sol_a) && std::isfinite(d->rho_sol_b)))
+            return fail(LBM_EINVAL, "two components with walls: rho_sol_a, rho_sol_b must be finite");
         if (d->kernel == LBM_KERNEL_FUSED)
             return fail(LBM_EINVAL, "two components run on the two-pass kernels");
         if (!(d->tau_b > 0.0) || !std::isfinite(d->tau_b)) return fail(LBM_EINVAL, "tau_b must be > 0");
@@ -245,6 +246,8 @@ int lbm_create(const lbm_desc* desc, lbm_handle** out)
     h->a.tau = desc->tau; h->a.G = desc->G;
     h->a.components = desc->components == 2 ? 2 : 1;
     h->a.tau_b = desc->components == 2 ? desc->tau_b : desc->tau;
+    h->a.rho_sol_a = desc->rho_sol_a;
+    h->a.rho_sol_b = desc->rho_sol_b;
     const int nc = h->a.components;
     if (desc->precision == LBM_FP64) {
         const bool fused_ok = lbm::fp64_fused_supported(h->a, h->g);
@@ -456,8 +459,8 @@ int lbm_get_fields_component(lbm_handle* h, int comp, int kind, double* rho, dou
                              h->stream));
     const double* mine = comp == 0 ? A : B;
     if (rho) HIP_OK(hipMemcpyAsync(rho, mine, n * 8, hipMemcpyDeviceToHost, h->stream));
-    HIP_OK(lbm::fp64_common_velocity(h->a, n, A, A + n, A + 2 * n, A + 3 * n, B, B + n, B + 2 * n, B + 3 * n,
-                                     h->stream));
+    HIP_OK(lbm::fp64_common_velocity(h->a, h->g, A, A + n, A + 2 * n, A + 3 * n, B, B + n, B + 2 * n,
+                                     B + 3 * n, comp, h->stream));
     double* outs[3] = {u, v, w};
     for (int k = 0; k < 3; k++)
         if (outs[k])
@@ -505,7 +508,7 @@ int lbm_get_force_component(lbm_handle* h, int comp, double* fx, double* fy, dou
     } else {
         const double* ps = h->tmp + (5 * comp + 4) * n;
         const double* po = h->tmp + (5 * (1 - comp) + 4) * n;
-        HIP_OK(lbm::fp64_force2(h->a, h->g, ps, po, F, F + n, F + 2 * n, h->stream));
+        HIP_OK(lbm::fp64_force2(h->a, h->g, ps, po, 1 - comp, F, F + n, F + 2 * n, h->stream));
     }
     double* outs[3] = {fx, fy, fz};
     for (int k = 0; k < 3; k++)

@@ -343,16 +343,27 @@ __global__ void __launch_bounds__(kConsBlock)
 // common velocity initLatticeBoltzmann left, which is 0); 2: the initial
 // equilibrium (mcmp2D.cpp:83-87, mcmp3D.cpp:87-91).
 
-// sum_n (w_n e_n) psi(x + e_n) in stencil order (zero terms skipped: exact)
-template <int Q>
+// sum_n (w_n e_n) psi(x + e_n) in stencil order (zero terms skipped: exact).
+// FILM: neighbours in the wall rows (first and last row of the last axis)
+// contribute psi_sol instead (mcmp2DFilm.cpp:173-181, mcmp3DFilm.cpp:199-208);
+// last is this node's coordinate on that axis, nl its extent.
+template <int Q, bool FILM = false>
 __device__ __forceinline__ void psi_sums(const double* __restrict__ psi, const Nbr& nb, double& gx,
-                                         double& gy, double& gz)
+                                         double& gy, double& gz, int last = 0, int nl = 0,
+                                         double psi_sol = 0.0)
 {
     using S = St<Q>;
     gx = 0.0; gy = 0.0; gz = 0.0;
 #pragma unroll
     for (int n = 1; n < Q; n++) {
-        const double t = S::w[n] * psi[at(nb, S::ex[n], S::ey[n], S::ez[n])];
+        double pn;
+        if constexpr (FILM) {
+            const int ln = last + ((S::D == 3) ? S::ez[n] : S::ey[n]);
+            pn = (ln == 0 || ln == nl - 1) ? psi_sol : psi[at(nb, S::ex[n], S::ey[n], S::ez[n])];
+        } else {
+            pn = psi[at(nb, S::ex[n], S::ey[n], S::ez[n])];
+        }
+        const double t = S::w[n] * pn;
         if (S::ex[n] == 1) gx = gx + t;
         if (S::ex[n] == -1) gx = gx - t;
         if (S::ey[n] == 1) gy = gy + t;
@@ -389,12 +400,19 @@ __device__ __forceinline__ void collide_one(const double (&f)[Q], double rho, do
     });
 }
 
-template <int Q, int MODE>
+// Film parameters (mcmp2DFilm / mcmp3DFilm): the wall densities of A and B
+// ("LBApp/rhoA_sol", "rhoB_sol"); their psi is formed in the kernel, with the
+// same exp as every other psi.
+struct Film {
+    double rhoA, rhoB;
+};
+
+template <int Q, int MODE, bool FILM>
 __global__ void __launch_bounds__(kBlock)
     k_collide2(const double* __restrict__ Pin, int kind, double* __restrict__ Pout,
                const double* __restrict__ psiA, const double* __restrict__ psiB,
                const double* __restrict__ rho0A, const double* __restrict__ rho0B, Grid g,
-               double tauA, double tauB, double G)
+               double tauA, double tauB, double G, Film film)
 {
     using S = St<Q>;
     const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
@@ -402,12 +420,16 @@ __global__ void __launch_bounds__(kBlock)
     const Coord p = decompose(static_cast<uint32_t>(c), g);
     const Nbr nb = neighbours(p, g);
     const int64_t off = static_cast<int64_t>(Q) * g.ps;   // component B's planes
+    // film: the first and last row of the last axis are walls
+    const int last = (S::D == 3) ? p.z : p.y;
+    const int nl = (S::D == 3) ? g.nz : g.ny;
+    const bool wall_row = FILM && (last == 0 || last == nl - 1);
 
     double fA[Q], fB[Q];
     double rA, rB, uA, vA, wA, uB, vB, wB;
     if constexpr (MODE == 0) {
-        load_f<Q, false>(Pin, g, kind, p, nb, c, fA);
-        load_f<Q, false>(Pin + off, g, kind, p, nb, c, fB);
+        load_f<Q, FILM>(Pin, g, kind, p, nb, c, fA);
+        load_f<Q, FILM>(Pin + off, g, kind, p, nb, c, fB);
         double sx, sy, sz;
         moments<Q>(fA, rA, sx, sy, sz);
         uA = sx / rA; vA = sy / rA; wA = sz / rA;
@@ -415,16 +437,26 @@ __global__ void __launch_bounds__(kBlock)
         uB = sx / rB; vB = sy / rB; wB = sz / rB;
     } else {
         if constexpr (MODE == 1) {
-            load_f<Q, false>(Pin, g, kind, p, nb, c, fA);
-            load_f<Q, false>(Pin + off, g, kind, p, nb, c, fB);
+            load_f<Q, FILM>(Pin, g, kind, p, nb, c, fA);
+            load_f<Q, FILM>(Pin + off, g, kind, p, nb, c, fB);
         }
         rA = rho0A[c]; rB = rho0B[c];
         uA = vA = wA = uB = vB = wB = 0.0;
     }
 
+    if (wall_row) {
+        // wall rows: no force and no common velocity -- the film models'
+        // force loop skips them, so their force arrays keep the zeros they
+        // were allocated with and their u arrays the component's own
+        // velocity (mcmp2DFilm.cpp:153, mcmp3DFilm.cpp:176)
+        collide_one<Q, MODE>(fA, rA, uA, vA, wA, 0.0, 0.0, 0.0, tauA, Pout, g.ps, c);
+        collide_one<Q, MODE>(fB, rB, uB, vB, wB, 0.0, 0.0, 0.0, tauB, Pout + off, g.ps, c);
+        return;
+    }
     double gAx, gAy, gAz, gBx, gBy, gBz;
-    psi_sums<Q>(psiA, nb, gAx, gAy, gAz);
-    psi_sums<Q>(psiB, nb, gBx, gBy, gBz);
+    const double psolA = FILM ? psi_of(film.rhoA) : 0.0, psolB = FILM ? psi_of(film.rhoB) : 0.0;
+    psi_sums<Q, FILM>(psiA, nb, gAx, gAy, gAz, last, nl, psolA);
+    psi_sums<Q, FILM>(psiB, nb, gBx, gBy, gBz, last, nl, psolB);
     const double mg = -G;
     const double pa = mg * psiA[c], pb = mg * psiB[c];
 
@@ -440,17 +472,26 @@ __global__ void __launch_bounds__(kBlock)
 
 // Cross force on the component whose psi is psi_self: ((-G)*psi_self(x)) *
 // sum (w*e)*psi_other(x+e) (the fx,fy,fz arrays of mcmp2D / mcmp3D).
-template <int Q>
+template <int Q, bool FILM>
 __global__ void __launch_bounds__(kBlock)
     k_force2(const double* __restrict__ psi_self, const double* __restrict__ psi_other, Grid g, double G,
-             double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz)
+             double rho_sol_other, double* __restrict__ fx, double* __restrict__ fy,
+             double* __restrict__ fz)
 {
     const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     if (c >= g.n) return;
     const Coord p = decompose(static_cast<uint32_t>(c), g);
     const Nbr nb = neighbours(p, g);
+    const int last = (St<Q>::D == 3) ? p.z : p.y;
+    const int nl = (St<Q>::D == 3) ? g.nz : g.ny;
+    if (FILM && (last == 0 || last == nl - 1)) {
+        fx[c] = 0.0;   // the wall rows' force arrays are never set
+        fy[c] = 0.0;
+        fz[c] = 0.0;
+        return;
+    }
     double gx, gy, gz;
-    psi_sums<Q>(psi_other, nb, gx, gy, gz);
+    psi_sums<Q, FILM>(psi_other, nb, gx, gy, gz, last, nl, FILM ? psi_of(rho_sol_other) : 0.0);
     const double pc = -G * psi_self[c];
     fx[c] = pc * gx;
     fy[c] = pc * gy;
@@ -458,15 +499,28 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // Common velocity from both components' moments, written over A's velocity
-// arrays (the u arrays mcmp3D leaves after a step).
+// arrays (the u arrays mcmp3D leaves after a step).  film_nz > 0: the first
+// and last z rows are walls, where the arrays keep the velocity of
+// component `own` (0: A's, already in place; 1: B's).
 __global__ void __launch_bounds__(kBlock)
     k_common_velocity(int64_t n, int three_d, const double* __restrict__ rA, double* __restrict__ uA,
                       double* __restrict__ vA, double* __restrict__ wA, const double* __restrict__ rB,
                       const double* __restrict__ uB, const double* __restrict__ vB,
-                      const double* __restrict__ wB, double tauA, double tauB)
+                      const double* __restrict__ wB, double tauA, double tauB, int film_nz, int own)
 {
     const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     if (c >= n) return;
+    if (film_nz > 0) {
+        const int z = static_cast<int>(c % film_nz);
+        if (z == 0 || z == film_nz - 1) {
+            if (own == 1) {
+                uA[c] = uB[c];
+                vA[c] = vB[c];
+                wA[c] = wB[c];
+            }
+            return;
+        }
+    }
     const double ra = rA[c], rb = rB[c];
     const double den = ra / tauA + rb / tauB;
     uA[c] = ((uA[c] * ra) / tauA + (uB[c] * rb) / tauB) / den;
@@ -556,38 +610,57 @@ hipError_t fp64_set_f(const StepArgs& a, const Grid& g, const double* aos, doubl
 hipError_t fp64_collide2(const StepArgs& a, const Grid& g, const double* Pin, int kind, double* Pout,
                          const double* psi, const double* rho0, int mode, hipStream_t s)
 {
-    if (a.walls) return hipErrorNotSupported;
     const unsigned nb = blocks_for(g.n);
     const double* psiB = psi + g.n;
     const double* r0B = rho0 + g.n;
-    auto go = [&](auto qtag) {
+    const Film film{a.rho_sol_a, a.rho_sol_b};
+    auto go = [&](auto qtag, auto ftag) {
         constexpr int Q = decltype(qtag)::value;
+        constexpr bool F = decltype(ftag)::value;
         if (mode == 0)
-            k_collide2<Q, 0><<<nb, kBlock, 0, s>>>(Pin, kind, Pout, psi, psiB, rho0, r0B, g, a.tau, a.tau_b, a.G);
+            k_collide2<Q, 0, F><<<nb, kBlock, 0, s>>>(Pin, kind, Pout, psi, psiB, rho0, r0B, g, a.tau, a.tau_b,
+                                                       a.G, film);
         else if (mode == 1)
-            k_collide2<Q, 1><<<nb, kBlock, 0, s>>>(Pin, kind, Pout, psi, psiB, rho0, r0B, g, a.tau, a.tau_b, a.G);
+            k_collide2<Q, 1, F><<<nb, kBlock, 0, s>>>(Pin, kind, Pout, psi, psiB, rho0, r0B, g, a.tau, a.tau_b,
+                                                       a.G, film);
         else
-            k_collide2<Q, 2><<<nb, kBlock, 0, s>>>(Pin, kind, Pout, psi, psiB, rho0, r0B, g, a.tau, a.tau_b, a.G);
+            k_collide2<Q, 2, F><<<nb, kBlock, 0, s>>>(Pin, kind, Pout, psi, psiB, rho0, r0B, g, a.tau, a.tau_b,
+                                                       a.G, film);
     };
-    if (a.q == 9) go(std::integral_constant<int, 9>{});
-    else go(std::integral_constant<int, 19>{});
+    using T9 = std::integral_constant<int, 9>;
+    using T19 = std::integral_constant<int, 19>;
+    if (a.q == 9) {
+        if (a.walls) go(T9{}, std::true_type{});
+        else go(T9{}, std::false_type{});
+    } else {
+        if (a.walls) go(T19{}, std::true_type{});
+        else go(T19{}, std::false_type{});
+    }
     return hipGetLastError();
 }
 
 hipError_t fp64_force2(const StepArgs& a, const Grid& g, const double* psi_self, const double* psi_other,
-                       double* fx, double* fy, double* fz, hipStream_t s)
+                       int other, double* fx, double* fy, double* fz, hipStream_t s)
 {
-    if (a.q == 9) k_force2<9><<<blocks_for(g.n), kBlock, 0, s>>>(psi_self, psi_other, g, a.G, fx, fy, fz);
-    else k_force2<19><<<blocks_for(g.n), kBlock, 0, s>>>(psi_self, psi_other, g, a.G, fx, fy, fz);
+    const double ps = other == 0 ? a.rho_sol_a : a.rho_sol_b;
+    const unsigned nb = blocks_for(g.n);
+    if (a.q == 9) {
+        if (a.walls) k_force2<9, true><<<nb, kBlock, 0, s>>>(psi_self, psi_other, g, a.G, ps, fx, fy, fz);
+        else k_force2<9, false><<<nb, kBlock, 0, s>>>(psi_self, psi_other, g, a.G, ps, fx, fy, fz);
+    } else {
+        if (a.walls) k_force2<19, true><<<nb, kBlock, 0, s>>>(psi_self, psi_other, g, a.G, ps, fx, fy, fz);
+        else k_force2<19, false><<<nb, kBlock, 0, s>>>(psi_self, psi_other, g, a.G, ps, fx, fy, fz);
+    }
     return hipGetLastError();
 }
 
-hipError_t fp64_common_velocity(const StepArgs& a, int64_t n, const double* rA, double* uA, double* vA,
+hipError_t fp64_common_velocity(const StepArgs& a, const Grid& g, const double* rA, double* uA, double* vA,
                                 double* wA, const double* rB, const double* uB, const double* vB,
-                                const double* wB, hipStream_t s)
+                                const double* wB, int own, hipStream_t s)
 {
-    k_common_velocity<<<blocks_for(n), kBlock, 0, s>>>(n, a.q == 19 ? 1 : 0, rA, uA, vA, wA, rB, uB, vB, wB,
-                                                       a.tau, a.tau_b);
+    const int film_nz = (a.walls && a.q == 19) ? g.nz : 0;
+    k_common_velocity<<<blocks_for(g.n), kBlock, 0, s>>>(g.n, a.q == 19 ? 1 : 0, rA, uA, vA, wA, rB, uB, vB,
+                                                         wB, a.tau, a.tau_b, film_nz, own);
     return hipGetLastError();
 }
 

@@ -61,6 +61,8 @@ struct StepArgs {
     double tau, G;
     int components = 1;   // 2: two-component Shan-Chen (mcmp2D / mcmp3D)
     double tau_b = 1.0;   // relaxation time of the second component
+    // two components with walls (mcmp2DFilm / mcmp3DFilm): wall densities
+    double rho_sol_a = 0.5, rho_sol_b = 0.5;
 };
 
 // FP64 path (lbm_fp64.hip).  All launches go to `s`.
@@ -96,17 +98,20 @@ hipError_t fp64_conservation(const StepArgs& a, const Grid& g, const double* P, 
                              double* partials, int* nblocks, hipStream_t s);
 int fp64_conservation_blocks(const Grid& g);
 
-// Two components (a.components == 2, FP64, no walls): the buffer holds
-// component A in planes 0..Q-1 and B in planes Q..2Q-1; psi and rho0 hold
-// A's n values, then B's.  Step modes as fp64_collide.
+// Two components (a.components == 2, FP64): the buffer holds component A in
+// planes 0..Q-1 and B in planes Q..2Q-1; psi and rho0 hold A's n values, then
+// B's.  Step modes as fp64_collide.  a.walls: the film models (walls on the
+// last axis, fluid-wall interaction with the wall densities).
 hipError_t fp64_collide2(const StepArgs& a, const Grid& g, const double* Pin, int kind, double* Pout,
                          const double* psi, const double* rho0, int mode, hipStream_t s);
-// Cross force on the component with psi_self from the other's psi.
+// Cross force on the component with psi_self from the other component's psi
+// (other: 0 = A, 1 = B, for its wall density).
 hipError_t fp64_force2(const StepArgs& a, const Grid& g, const double* psi_self, const double* psi_other,
-                       double* fx, double* fy, double* fz, hipStream_t s);
-// Common velocity of the two components' moments, over A's u, v, w.
-hipError_t fp64_common_velocity(const StepArgs& a, int64_t n, const double* rA, double* uA, double* vA,
+                       int other, double* fx, double* fy, double* fz, hipStream_t s);
+// Common velocity of the two components' moments, over A's u, v, w; in the
+// film's wall rows the arrays keep component `own`'s velocity.
+hipError_t fp64_common_velocity(const StepArgs& a, const Grid& g, const double* rA, double* uA, double* vA,
                                 double* wA, const double* rB, const double* uB, const double* vB,
-                                const double* wB, hipStream_t s);
+                                const double* wB, int own, hipStream_t s);
 
 }  // namespace lbm

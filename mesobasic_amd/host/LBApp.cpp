@@ -38,7 +38,8 @@ LBApp::LBApp(const InputParams& in)
     lb_object.reset(LBBaseClass::LBFactory(p, in));
     if (!lb_object)
         throw std::runtime_error("LBApp/type '" + in("LBApp/type", "mcmp2D") +
-                                 "' is not provided by this build (scmp3D, scmp2D, mcmp2D, mcmp3D)");
+                                 "' is not provided by this build (scmp3D, scmp2D, mcmp2D, mcmp3D, mcmp2DFilm, "
+                                 "mcmp3DFilm)");
 }
 
 void LBApp::initSystem() { lb_object->initLatticeBoltzmann(); }

@@ -136,10 +136,14 @@ void scmp2D::outputLatticeBoltzmann()
     write_vtk_2d("vtkoutput", "rhoA", current_step, p.NX, p.NY, p.iskip, p.jskip, rho, us, vs);
 }
 
-McmpModel::McmpModel(const CommonParams& pin, const InputParams& in, int q) : p(pin), q_(q)
+McmpModel::McmpModel(const CommonParams& pin, const InputParams& in, int q, bool film)
+    : p(pin), q_(q), film_(film)
 {
-    // mcmp2D.cpp:18-37 / mcmp3D.cpp:18-37
+    // mcmp2D.cpp:18-37 / mcmp3D.cpp:18-37 (the film forms add the wall
+    // densities, mcmp2DFilm.cpp:21-22, mcmp3DFilm.cpp:21-22)
     G = in("LBApp/G", 3.0);
+    rhoA_sol = in("LBApp/rhoA_sol", 0.5);
+    rhoB_sol = in("LBApp/rhoB_sol", 0.5);
     rhoAi = in("LBApp/rhoAi", 0.5);
     rhoBi = in("LBApp/rhoBi", 0.5);
     rhoAi_noise = in("LBApp/rhoAi_noise", 0.1);
@@ -170,6 +174,9 @@ McmpModel::McmpModel(const CommonParams& pin, const InputParams& in, int q) : p(
     d.kernel = LBM_KERNEL_AUTO;
     d.components = 2;
     d.tau_b = tauB;
+    d.walls = film ? 1 : 0;
+    d.rho_sol_a = rhoA_sol;
+    d.rho_sol_b = rhoB_sol;
     check(lbm_create(&d, &h_), "lbm_create");
 }
 
@@ -187,8 +194,17 @@ void McmpModel::initLatticeBoltzmann()
     const size_t n = static_cast<size_t>(nodes());
     std::vector<double> rA(n), rB(n);
     if (init_kind == "noise") {
+        // mcmp3DFilm fills the fluid rows only (k = 2 .. nz-1); its wall rows
+        // keep density 0 (mcmp3DFilm.cpp:71-84)
+        const bool skip_walls = film_ && q_ == 19;
         std::mt19937 gen(seed);
         for (size_t c = 0; c < n; c++) {
+            const int z = static_cast<int>(c % static_cast<size_t>(q_ == 19 ? p.nz : 1));
+            if (skip_walls && (z == 0 || z == p.nz - 1)) {
+                rA[c] = 0.0;
+                rB[c] = 0.0;
+                continue;
+            }
             const double xa = static_cast<double>(gen()) / 4294967295.0;
             const double xb = static_cast<double>(gen()) / 4294967295.0;
             rA[c] = rhoAi + rhoAi_noise * (xa - 0.5);
@@ -251,8 +267,8 @@ void McmpModel::outputLatticeBoltzmann()
 }
 
 // LBBaseClass::LBFactory (LBBaseClass.cpp:21-45), with the reference's
-// default type mcmp2D.  The film and capsule models are not part of this
-// build, so they (like unknown names) yield nullptr.
+// default type mcmp2D.  The capsule model is not part of this build, so it
+// (like unknown names) yields nullptr.
 LBBaseClass* LBBaseClass::LBFactory(const CommonParams& p, const InputParams& in)
 {
     const std::string lb_type = in("LBApp/type", "mcmp2D");
@@ -260,6 +276,8 @@ LBBaseClass* LBBaseClass::LBFactory(const CommonParams& p, const InputParams& in
     if (lb_type == "scmp2D") return new scmp2D(p, in);
     if (lb_type == "mcmp2D") return new mcmp2D(p, in);
     if (lb_type == "mcmp3D") return new mcmp3D(p, in);
+    if (lb_type == "mcmp2DFilm") return new mcmp2DFilm(p, in);
+    if (lb_type == "mcmp3DFilm") return new mcmp3DFilm(p, in);
     return nullptr;
 }
 

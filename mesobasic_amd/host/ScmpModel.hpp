@@ -80,9 +80,14 @@ public:
 // LBApp/seed (mt19937 seed; default time(NULL)), LBApp/init (noise |
 // droplet: an A-rich drop of radius R, width W, densities rho_l / rho_v in
 // B, with rhoB = rho_l + rho_v - rhoA).
+//
+// film = true: mcmp2DFilm / mcmp3DFilm (walls on the last axis; keys
+// LBApp/rhoA_sol and rhoB_sol, default 0.5, mcmp2DFilm.cpp:21-22).  As in the
+// reference, mcmp3DFilm's start fills the fluid rows only and leaves the wall
+// rows at density 0 (mcmp3DFilm.cpp:71-84).
 class McmpModel : public LBBaseClass {
 public:
-    McmpModel(const CommonParams& p, const InputParams& in, int q);
+    McmpModel(const CommonParams& p, const InputParams& in, int q, bool film = false);
     ~McmpModel() override;
     McmpModel(const McmpModel&) = delete;
     McmpModel& operator=(const McmpModel&) = delete;
@@ -103,7 +108,8 @@ private:
     const CommonParams& p;
     int q_;
     int current_step = 0;
-    double G, rhoAi, rhoBi, rhoAi_noise, rhoBi_noise, tauA, tauB;
+    bool film_;
+    double G, rhoAi, rhoBi, rhoAi_noise, rhoBi_noise, tauA, tauB, rhoA_sol, rhoB_sol;
     std::string init_kind;
     unsigned seed;
     double R, W, rho_l, rho_v;
@@ -120,6 +126,16 @@ public:
 class mcmp3D : public McmpModel {
 public:
     mcmp3D(const CommonParams& p, const InputParams& in) : McmpModel(p, in, 19) {}
+};
+
+class mcmp2DFilm : public McmpModel {
+public:
+    mcmp2DFilm(const CommonParams& p, const InputParams& in) : McmpModel(p, in, 9, true) {}
+};
+
+class mcmp3DFilm : public McmpModel {
+public:
+    mcmp3DFilm(const CommonParams& p, const InputParams& in) : McmpModel(p, in, 19, true) {}
 };
 
 }  // namespace mesobasic

@@ -36,7 +36,8 @@ class Desc(ctypes.Structure):
                 ("nz", ctypes.c_int), ("tau", ctypes.c_double), ("G", ctypes.c_double),
                 ("order", ctypes.c_int), ("walls", ctypes.c_int), ("precision", ctypes.c_int),
                 ("rho_shift", ctypes.c_double), ("kernel", ctypes.c_int),
-                ("components", ctypes.c_int), ("tau_b", ctypes.c_double)]
+                ("components", ctypes.c_int), ("tau_b", ctypes.c_double),
+                ("rho_sol_a", ctypes.c_double), ("rho_sol_b", ctypes.c_double)]
 
 
 class BatchDesc(ctypes.Structure):
@@ -220,10 +221,11 @@ class Lattice:
     """One Shan-Chen lattice resident on the GPU (a C-ABI handle).
     components=2 with tau_b: the two-component model (mcmp2D for q=9,
     mcmp3D for q=19); tau is component A's, G couples the components, and
-    the comp argument of the accessors picks A (0) or B (1)."""
+    the comp argument of the accessors picks A (0) or B (1).  With walls as
+    well: the film models, rho_sol = the wall densities of A and B."""
 
     def __init__(self, q, shape, tau, G, order=None, walls=False, precision=FP64,
-                 rho_shift=0.7, kernel=KERNEL_AUTO, components=1, tau_b=None):
+                 rho_shift=0.7, kernel=KERNEL_AUTO, components=1, tau_b=None, rho_sol=(0.5, 0.5)):
         lib = load()
         shape = tuple(int(s) for s in shape)
         if q == 9:
@@ -239,7 +241,7 @@ class Lattice:
         self.components = int(components)
         d = Desc(q, nx, ny, nz, float(tau), float(G), int(order), int(bool(walls)),
                  int(precision), float(rho_shift), int(kernel), int(components),
-                 float(tau if tau_b is None else tau_b))
+                 float(tau if tau_b is None else tau_b), float(rho_sol[0]), float(rho_sol[1]))
         h = ctypes.c_void_p()
         _check(lib.lbm_create(ctypes.byref(d), ctypes.byref(h)))
         self._h = h

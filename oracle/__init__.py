@@ -54,6 +54,7 @@ def _load():
     lib.mcmp_oracle_create.restype = ctypes.c_void_p
     lib.mcmp_oracle_create.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double] * 3 + [ctypes.c_int] * 2
     lib.mcmp_oracle_destroy.argtypes = [ctypes.c_void_p]
+    lib.mcmp_oracle_set_film.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
     lib.mcmp_oracle_set_rho.argtypes = [ctypes.c_void_p, _D, _D]
     lib.mcmp_oracle_init.argtypes = [ctypes.c_void_p]
     lib.mcmp_oracle_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
@@ -152,7 +153,9 @@ class MCOracle:
     cross coupling G.  order: 2 = mcmp2D's step (default for q=9), 3 =
     mcmp3D's (default for q=19)."""
 
-    def __init__(self, q, shape, tauA, tauB, G, order=None, nthreads=1):
+    def __init__(self, q, shape, tauA, tauB, G, order=None, nthreads=1, film=None):
+        """film: (rhoA_sol, rhoB_sol) for the film models (walls on the last
+        axis), None for mcmp2D / mcmp3D."""
         lib = _load()
         shape = tuple(int(s) for s in shape)
         if q == 9:
@@ -168,6 +171,8 @@ class MCOracle:
         if not self._h:
             raise ValueError("oracle: bad lattice description")
         self._lib = lib
+        if film is not None:
+            lib.mcmp_oracle_set_film(self._h, float(film[0]), float(film[1]))
 
     def close(self):
         if self._h:

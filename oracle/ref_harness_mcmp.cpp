@@ -1,6 +1,8 @@
 // ref_harness_mcmp.cpp -- drives the reference's own two-component
 // Shan-Chen classes, mcmp2D (LBTypes/mcmp2D.cpp) and mcmp3D
-// (LBTypes/mcmp3D.cpp), compiled in place from the reference tree by
+// (LBTypes/mcmp3D.cpp), and their film forms mcmp2DFilm / mcmp3DFilm (walls
+// on the last axis, LBTypes/mcmp2DFilm.cpp, mcmp3DFilm.cpp), compiled in
+// place from the reference tree by
 // oracle/Makefile.ref (no reference source is copied), with injected initial
 // densities, and dumps full-precision binary fields.  TEST INFRASTRUCTURE
 // ONLY.
@@ -13,6 +15,9 @@
 //
 // Usage:
 //   ref_harness_mcmp q nx ny nz tauA tauB G rhoA0.bin rhoB0.bin out_prefix step[,step...]
+//                    [rhoA_sol rhoB_sol]
+// With the two wall densities the film classes run instead (their keys
+// LBApp/rhoA_sol, rhoB_sol).
 // Writes out_prefix_<c>_<step>.bin for c = A, B: rho | u | v | w | fx | fy | fz
 // | f (float64, interior nodes, x slowest and the last axis fastest; f is
 // [node][q]; w and fz are zeros in 2-D).  Run under mpirun -np N to split x
@@ -36,7 +41,9 @@
 
 #define private public
 #include "lattice_boltzmann/LBTypes/mcmp2D.hpp"
+#include "lattice_boltzmann/LBTypes/mcmp2DFilm.hpp"
 #include "lattice_boltzmann/LBTypes/mcmp3D.hpp"
+#include "lattice_boltzmann/LBTypes/mcmp3DFilm.hpp"
 #undef private
 
 namespace {
@@ -106,15 +113,124 @@ void gather_write(const std::string& path, const std::vector<double>& blk, long 
 }
 
 std::string input_file(const std::string& out, int rank, const char* stencil, double tauA, double tauB,
-                       double G)
+                       double G, const double* sol)
 {
     char name[512];
     std::snprintf(name, sizeof(name), "%s_rank%d_input.dat", out.c_str(), rank);
     std::ofstream o(name);
     o << std::setprecision(17);
     o << "[LBApp]\n  G = " << G << "\n  tauA = " << tauA << "\n  tauB = " << tauB
-      << "\n  stencil = " << stencil << "\n[]\n";
+      << "\n  stencil = " << stencil << "\n";
+    if (sol) o << "  rhoA_sol = " << sol[0] << "\n  rhoB_sol = " << sol[1] << "\n";
+    o << "[]\n";
     return name;
+}
+
+struct Run {
+    CommonParams p;
+    int NX, NY, NZ;
+    std::vector<double> rA, rB;
+    std::vector<int> dumps;
+    int last;
+    std::string out;
+};
+
+// initLatticeBoltzmann minus the rand() fill, then updateLatticeBoltzmann
+// per step; dumps both components at the listed steps.
+template <typename M>
+void run3d(M& m, const Run& r)
+{
+    const CommonParams& p = r.p;
+    const int nscal = 7;
+    for (int i = 1; i < p.nx + 1; i++)
+        for (int j = 1; j < p.ny + 1; j++)
+            for (int k = 1; k < p.nz + 1; k++) {
+                const int ndx = i * m.deli + j * m.delj + k * m.delk;
+                const long g = ((long)(i - 1 + p.xOff) * r.NY + (j - 1)) * r.NZ + (k - 1);
+                m.fA.setRho(ndx, r.rA[g]);
+                m.fB.setRho(ndx, r.rB[g]);
+                m.fA.setU(ndx, 0.0); m.fA.setV(ndx, 0.0); m.fA.setW(ndx, 0.0);
+                m.fB.setU(ndx, 0.0); m.fB.setV(ndx, 0.0); m.fB.setW(ndx, 0.0);
+            }
+    m.fA.ghostNodesRho();
+    m.fB.ghostNodesRho();
+    m.calculateShanChenForces();
+    m.fA.setFtoFeq(m.s);
+    m.fB.setFtoFeq(m.s);
+    for (int step = 0; step <= r.last; step++) {
+        if (step > 0) {
+            m.setTimeStep(step);
+            m.updateLatticeBoltzmann();
+        }
+        if (std::find(r.dumps.begin(), r.dumps.end(), step) == r.dumps.end()) continue;
+        const long nloc = (long)p.nx * p.ny * p.nz;
+        for (int comp = 0; comp < 2; comp++) {
+            const LBfluid3D& fl = comp == 0 ? m.fA : m.fB;
+            std::vector<double> blk((nscal + 19) * nloc, 0.0);
+            long c = 0;
+            for (int i = 1; i < p.nx + 1; i++)
+                for (int j = 1; j < p.ny + 1; j++)
+                    for (int k = 1; k < p.nz + 1; k++, c++) {
+                        const long ndx = i * m.deli + j * m.delj + k * m.delk;
+                        blk[0 * nloc + c] = fl.r[ndx];
+                        blk[1 * nloc + c] = fl.u[ndx];
+                        blk[2 * nloc + c] = fl.v[ndx];
+                        blk[3 * nloc + c] = fl.w[ndx];
+                        blk[4 * nloc + c] = fl.fx[ndx];
+                        blk[5 * nloc + c] = fl.fy[ndx];
+                        blk[6 * nloc + c] = fl.fz[ndx];
+                        for (int n = 0; n < 19; n++) blk[7 * nloc + c * 19 + n] = fl.f[ndx * 19 + n];
+                    }
+            gather_write(r.out + (comp == 0 ? "_A_" : "_B_") + std::to_string(step) + ".bin", blk, nloc,
+                         nscal, 19);
+        }
+    }
+}
+
+template <typename M>
+void run2d(M& m, const Run& r)
+{
+    const CommonParams& p = r.p;
+    const int nscal = 7;
+    for (int i = 1; i < p.nx + 1; i++)
+        for (int j = 1; j < p.ny + 1; j++) {
+            const int ndx = i * m.deli + j * m.delj;
+            const long g = (long)(i - 1 + p.xOff) * r.NY + (j - 1);
+            m.fA.setRho(ndx, r.rA[g]);
+            m.fB.setRho(ndx, r.rB[g]);
+            m.fA.setU(ndx, 0.0); m.fA.setV(ndx, 0.0);
+            m.fB.setU(ndx, 0.0); m.fB.setV(ndx, 0.0);
+        }
+    m.fA.ghostNodesRho();
+    m.fB.ghostNodesRho();
+    m.calculateShanChenForces();
+    m.fA.setFtoFeq(m.s);
+    m.fB.setFtoFeq(m.s);
+    for (int step = 0; step <= r.last; step++) {
+        if (step > 0) {
+            m.setTimeStep(step);
+            m.updateLatticeBoltzmann();
+        }
+        if (std::find(r.dumps.begin(), r.dumps.end(), step) == r.dumps.end()) continue;
+        const long nloc = (long)p.nx * p.ny;
+        for (int comp = 0; comp < 2; comp++) {
+            const LBfluid2D& fl = comp == 0 ? m.fA : m.fB;
+            std::vector<double> blk((nscal + 9) * nloc, 0.0);
+            long c = 0;
+            for (int i = 1; i < p.nx + 1; i++)
+                for (int j = 1; j < p.ny + 1; j++, c++) {
+                    const long ndx = i * m.deli + j * m.delj;
+                    blk[0 * nloc + c] = fl.r[ndx];
+                    blk[1 * nloc + c] = fl.u[ndx];
+                    blk[2 * nloc + c] = fl.v[ndx];
+                    blk[4 * nloc + c] = fl.fx[ndx];
+                    blk[5 * nloc + c] = fl.fy[ndx];
+                    for (int n = 0; n < 9; n++) blk[7 * nloc + c * 9 + n] = fl.f[ndx * 9 + n];
+                }
+            gather_write(r.out + (comp == 0 ? "_A_" : "_B_") + std::to_string(step) + ".bin", blk, nloc,
+                         nscal, 9);
+        }
+    }
 }
 
 }  // namespace
@@ -122,122 +238,45 @@ std::string input_file(const std::string& out, int rank, const char* stencil, do
 int main(int argc, char** argv)
 {
     MPI::Init();
-    if (argc != 12) {
+    if (argc != 12 && argc != 14) {
         if (MPI::COMM_WORLD.Get_rank() == 0)
-            std::fprintf(stderr,
-                         "usage: ref_harness_mcmp q nx ny nz tauA tauB G rhoA0.bin rhoB0.bin out steps\n");
+            std::fprintf(stderr, "usage: ref_harness_mcmp q nx ny nz tauA tauB G rhoA0.bin rhoB0.bin out steps "
+                                 "[rhoA_sol rhoB_sol]\n");
         MPI::Finalize();
         return 2;
     }
+    Run r;
     const int q = std::atoi(argv[1]);
-    const int NX = std::atoi(argv[2]), NY = std::atoi(argv[3]), NZ = std::atoi(argv[4]);
+    r.NX = std::atoi(argv[2]); r.NY = std::atoi(argv[3]); r.NZ = std::atoi(argv[4]);
     const double tauA = std::strtod(argv[5], nullptr), tauB = std::strtod(argv[6], nullptr);
     const double G = std::strtod(argv[7], nullptr);
-    const std::string out = argv[10];
-    std::vector<int> dumps;
+    r.out = argv[10];
     {
         std::stringstream ss(argv[11]);
         std::string tok;
-        while (std::getline(ss, tok, ',')) dumps.push_back(std::atoi(tok.c_str()));
+        while (std::getline(ss, tok, ',')) r.dumps.push_back(std::atoi(tok.c_str()));
     }
-    const int last = dumps.empty() ? 0 : *std::max_element(dumps.begin(), dumps.end());
-    CommonParams p = slab_params(NX, NY, NZ);
-    const size_t N = static_cast<size_t>(NX) * NY * NZ;
-    const std::vector<double> rA = read_doubles(argv[8], N), rB = read_doubles(argv[9], N);
-    const int nscal = 7;
-
+    r.last = r.dumps.empty() ? 0 : *std::max_element(r.dumps.begin(), r.dumps.end());
+    double sol[2] = {0.0, 0.0};
+    const bool film = argc == 14;
+    if (film) {
+        sol[0] = std::strtod(argv[12], nullptr);
+        sol[1] = std::strtod(argv[13], nullptr);
+    }
+    r.p = slab_params(r.NX, r.NY, r.NZ);
+    const size_t N = static_cast<size_t>(r.NX) * r.NY * r.NZ;
+    r.rA = read_doubles(argv[8], N);
+    r.rB = read_doubles(argv[9], N);
+    const std::string inp =
+        input_file(r.out, r.p.rank, q == 19 ? "D3Q19" : "D2Q9", tauA, tauB, G, film ? sol : nullptr);
+    GetPot gp(inp.c_str());
+    std::remove(inp.c_str());
     if (q == 19) {
-        const std::string inp = input_file(out, p.rank, "D3Q19", tauA, tauB, G);
-        GetPot gp(inp.c_str());
-        std::remove(inp.c_str());
-        mcmp3D m(p, gp);
-        for (int i = 1; i < p.nx + 1; i++)
-            for (int j = 1; j < p.ny + 1; j++)
-                for (int k = 1; k < p.nz + 1; k++) {
-                    const int ndx = i * m.deli + j * m.delj + k * m.delk;
-                    const long g = ((long)(i - 1 + p.xOff) * NY + (j - 1)) * NZ + (k - 1);
-                    m.fA.setRho(ndx, rA[g]);
-                    m.fB.setRho(ndx, rB[g]);
-                    m.fA.setU(ndx, 0.0); m.fA.setV(ndx, 0.0); m.fA.setW(ndx, 0.0);
-                    m.fB.setU(ndx, 0.0); m.fB.setV(ndx, 0.0); m.fB.setW(ndx, 0.0);
-                }
-        m.fA.ghostNodesRho();
-        m.fB.ghostNodesRho();
-        m.calculateShanChenForces();
-        m.fA.setFtoFeq(m.s);
-        m.fB.setFtoFeq(m.s);
-        for (int step = 0; step <= last; step++) {
-            if (step > 0) {
-                m.setTimeStep(step);
-                m.updateLatticeBoltzmann();
-            }
-            if (std::find(dumps.begin(), dumps.end(), step) == dumps.end()) continue;
-            const long nloc = (long)p.nx * p.ny * p.nz;
-            for (int comp = 0; comp < 2; comp++) {
-                const LBfluid3D& fl = comp == 0 ? m.fA : m.fB;
-                std::vector<double> blk((nscal + 19) * nloc, 0.0);
-                long c = 0;
-                for (int i = 1; i < p.nx + 1; i++)
-                    for (int j = 1; j < p.ny + 1; j++)
-                        for (int k = 1; k < p.nz + 1; k++, c++) {
-                            const long ndx = i * m.deli + j * m.delj + k * m.delk;
-                            blk[0 * nloc + c] = fl.r[ndx];
-                            blk[1 * nloc + c] = fl.u[ndx];
-                            blk[2 * nloc + c] = fl.v[ndx];
-                            blk[3 * nloc + c] = fl.w[ndx];
-                            blk[4 * nloc + c] = fl.fx[ndx];
-                            blk[5 * nloc + c] = fl.fy[ndx];
-                            blk[6 * nloc + c] = fl.fz[ndx];
-                            for (int n = 0; n < 19; n++) blk[7 * nloc + c * 19 + n] = fl.f[ndx * 19 + n];
-                        }
-                gather_write(out + (comp == 0 ? "_A_" : "_B_") + std::to_string(step) + ".bin", blk, nloc,
-                             nscal, 19);
-            }
-        }
+        if (film) { mcmp3DFilm m(r.p, gp); run3d(m, r); }
+        else { mcmp3D m(r.p, gp); run3d(m, r); }
     } else {
-        const std::string inp = input_file(out, p.rank, "D2Q9", tauA, tauB, G);
-        GetPot gp(inp.c_str());
-        std::remove(inp.c_str());
-        mcmp2D m(p, gp);
-        for (int i = 1; i < p.nx + 1; i++)
-            for (int j = 1; j < p.ny + 1; j++) {
-                const int ndx = i * m.deli + j * m.delj;
-                const long g = (long)(i - 1 + p.xOff) * NY + (j - 1);
-                m.fA.setRho(ndx, rA[g]);
-                m.fB.setRho(ndx, rB[g]);
-                m.fA.setU(ndx, 0.0); m.fA.setV(ndx, 0.0);
-                m.fB.setU(ndx, 0.0); m.fB.setV(ndx, 0.0);
-            }
-        m.fA.ghostNodesRho();
-        m.fB.ghostNodesRho();
-        m.calculateShanChenForces();
-        m.fA.setFtoFeq(m.s);
-        m.fB.setFtoFeq(m.s);
-        for (int step = 0; step <= last; step++) {
-            if (step > 0) {
-                m.setTimeStep(step);
-                m.updateLatticeBoltzmann();
-            }
-            if (std::find(dumps.begin(), dumps.end(), step) == dumps.end()) continue;
-            const long nloc = (long)p.nx * p.ny;
-            for (int comp = 0; comp < 2; comp++) {
-                const LBfluid2D& fl = comp == 0 ? m.fA : m.fB;
-                std::vector<double> blk((nscal + 9) * nloc, 0.0);
-                long c = 0;
-                for (int i = 1; i < p.nx + 1; i++)
-                    for (int j = 1; j < p.ny + 1; j++, c++) {
-                        const long ndx = i * m.deli + j * m.delj;
-                        blk[0 * nloc + c] = fl.r[ndx];
-                        blk[1 * nloc + c] = fl.u[ndx];
-                        blk[2 * nloc + c] = fl.v[ndx];
-                        blk[4 * nloc + c] = fl.fx[ndx];
-                        blk[5 * nloc + c] = fl.fy[ndx];
-                        for (int n = 0; n < 9; n++) blk[7 * nloc + c * 9 + n] = fl.f[ndx * 9 + n];
-                    }
-                gather_write(out + (comp == 0 ? "_A_" : "_B_") + std::to_string(step) + ".bin", blk, nloc,
-                             nscal, 9);
-            }
-        }
+        if (film) { mcmp2DFilm m(r.p, gp); run2d(m, r); }
+        else { mcmp2D m(r.p, gp); run2d(m, r); }
     }
     MPI::Finalize();
     return 0;

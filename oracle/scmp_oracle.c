@@ -518,6 +518,8 @@ struct mcmp_oracle {
     scmp_oracle* c[2];
     double G;
     int order;
+    int film;              /* mcmp2DFilm / mcmp3DFilm: walls on the last axis */
+    double rho_sol[2];     /* wall densities of A and B */
 };
 
 mcmp_oracle* mcmp_oracle_create(int q, int nx, int ny, int nz, double tauA, double tauB, double G,
@@ -600,6 +602,92 @@ static void mcmp_forces(mcmp_oracle* m)
     FOR_INTERIOR_END
 }
 
+/* mcmp2DFilm::calculateShanChenForces (mcmp2DFilm.cpp:149-219) and
+ * mcmp3DFilm's (mcmp3DFilm.cpp:167-253): rows 1 and n of the last axis are
+ * walls; the loop skips them (their force and u arrays are left as they
+ * are), and a neighbour in a wall row contributes psi(rho_sol) of the
+ * component instead of psi of its density. */
+static void mcmp_forces_film(mcmp_oracle* m)
+{
+    scmp_oracle* a = m->c[0];
+    scmp_oracle* b = m->c[1];
+    const scmp_oracle* h = a;
+    const int q = a->q;
+    const double G = m->G;
+    const int nl = (h->dim == 3) ? h->nz : h->ny;
+    _Pragma("omp parallel for num_threads(h->nthreads) schedule(static)")
+    for (int i = 1; i < h->nx + 1; i++) {
+        for (int j = (h->dim == 3 ? 1 : 2); j < (h->dim == 3 ? h->ny + 1 : h->ny); j++) {
+            for (int k = (h->dim == 3 ? 2 : 0); k < (h->dim == 3 ? h->nz : 1); k++) {
+                int64_t ndx = nd(h, i, j, k);
+                double rhoA = a->r[ndx];
+                double rhoB = b->r[ndx];
+                double psi_rhoA = psi(a, rhoA);
+                double psi_rhoB = psi(b, rhoB);
+                double GsumxA = 0.0, GsumyA = 0.0, GsumzA = 0.0;
+                double GsumxB = 0.0, GsumyB = 0.0, GsumzB = 0.0;
+                for (int n = 0; n < q; n++) {
+                    int inbr = i + h->exi[n], jnbr = j + h->eyi[n];
+                    int knbr = (h->dim == 3) ? k + h->ezi[n] : 0;
+                    int lnbr = (h->dim == 3) ? knbr : jnbr;
+                    double psi_rhoA_nbr, psi_rhoB_nbr;
+                    if (lnbr == 1 || lnbr == nl) {
+                        psi_rhoA_nbr = psi(a, m->rho_sol[0]);
+                        psi_rhoB_nbr = psi(b, m->rho_sol[1]);
+                    } else {
+                        int64_t nbr = nd(h, inbr, jnbr, knbr);
+                        psi_rhoA_nbr = psi(a, a->r[nbr]);
+                        psi_rhoB_nbr = psi(b, b->r[nbr]);
+                    }
+                    GsumxA += h->wa[n] * h->ex[n] * psi_rhoA_nbr;
+                    GsumyA += h->wa[n] * h->ey[n] * psi_rhoA_nbr;
+                    GsumzA += h->wa[n] * h->ez[n] * psi_rhoA_nbr;
+                    GsumxB += h->wa[n] * h->ex[n] * psi_rhoB_nbr;
+                    GsumyB += h->wa[n] * h->ey[n] * psi_rhoB_nbr;
+                    GsumzB += h->wa[n] * h->ez[n] * psi_rhoB_nbr;
+                }
+                a->fx[ndx] = -G * psi_rhoA * GsumxB;
+                a->fy[ndx] = -G * psi_rhoA * GsumyB;
+                b->fx[ndx] = -G * psi_rhoB * GsumxA;
+                b->fy[ndx] = -G * psi_rhoB * GsumyA;
+                if (h->dim == 3) {
+                    a->fz[ndx] = -G * psi_rhoA * GsumzB;
+                    b->fz[ndx] = -G * psi_rhoB * GsumzA;
+                }
+                double rdivtauA = a->r[ndx] / a->tau;
+                double rdivtauB = b->r[ndx] / b->tau;
+                double uP = (a->u[ndx] * a->r[ndx] / a->tau + b->u[ndx] * b->r[ndx] / b->tau) /
+                            (rdivtauA + rdivtauB);
+                double vP = (a->v[ndx] * a->r[ndx] / a->tau + b->v[ndx] * b->r[ndx] / b->tau) /
+                            (rdivtauA + rdivtauB);
+                a->u[ndx] = uP; a->v[ndx] = vP;
+                b->u[ndx] = uP; b->v[ndx] = vP;
+                if (h->dim == 3) {
+                    double wP = (a->w[ndx] * a->r[ndx] / a->tau + b->w[ndx] * b->r[ndx] / b->tau) /
+                                (rdivtauA + rdivtauB);
+                    a->w[ndx] = wP;
+                    b->w[ndx] = wP;
+                }
+            }
+        }
+    }
+}
+
+static void forces_of(mcmp_oracle* m)
+{
+    if (m->film) mcmp_forces_film(m);
+    else mcmp_forces(m);
+}
+
+void mcmp_oracle_set_film(mcmp_oracle* m, double rhoA_sol, double rhoB_sol)
+{
+    m->film = 1;
+    m->rho_sol[0] = rhoA_sol;
+    m->rho_sol[1] = rhoB_sol;
+    m->c[0]->walls = 1;
+    m->c[1]->walls = 1;
+}
+
 void mcmp_oracle_set_rho(mcmp_oracle* m, const double* rhoA, const double* rhoB)
 {
     /* mcmp2D.cpp:67-81, mcmp3D.cpp:67-85 (velocities zero) */
@@ -612,7 +700,7 @@ void mcmp_oracle_init(mcmp_oracle* m)
     /* mcmp2D.cpp:83-87 / mcmp3D.cpp:87-91 */
     ghost_fill(m->c[0], m->c[0]->r, 1);
     ghost_fill(m->c[1], m->c[1]->r, 1);
-    mcmp_forces(m);
+    forces_of(m);
     set_f_to_feq(m->c[0]);
     set_f_to_feq(m->c[1]);
 }
@@ -621,20 +709,27 @@ void mcmp_oracle_step(mcmp_oracle* m, int nsteps)
 {
     for (int s = 0; s < nsteps; s++) {
         if (m->order == 3) {
-            /* mcmp3D::updateLatticeBoltzmann, mcmp3D.cpp:97-123 */
+            /* mcmp3D::updateLatticeBoltzmann, mcmp3D.cpp:97-123; the film
+             * adds bounce-back on both after streaming, mcmp3DFilm.cpp:132-140 */
             for (int c = 0; c < 2; c++) {
                 macros(m->c[c]);
                 ghost_fill(m->c[c], m->c[c]->r, 1);
             }
-            mcmp_forces(m);
+            forces_of(m);
             collide_stream(m->c[0]);
             collide_stream(m->c[1]);
+            if (m->film) {
+                bounce_back(m->c[0]);
+                bounce_back(m->c[1]);
+            }
         } else {
             /* mcmp2D::updateLatticeBoltzmann, mcmp2D.cpp:97-123, with
-             * LBfluid2D::updateFluid_SC per component (LBfluid2D.cpp:227-312) */
-            mcmp_forces(m);
+             * LBfluid2D::updateFluid_SC per component (LBfluid2D.cpp:227-312);
+             * the film sets yBC = 1 (mcmp2DFilm.cpp:119-123) */
+            forces_of(m);
             for (int c = 0; c < 2; c++) {
                 collide_stream(m->c[c]);
+                if (m->film) bounce_back(m->c[c]);
                 macros(m->c[c]);
                 ghost_fill(m->c[c], m->c[c]->r, 1);
             }

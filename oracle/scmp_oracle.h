@@ -104,6 +104,10 @@ typedef struct mcmp_oracle mcmp_oracle;
 mcmp_oracle* mcmp_oracle_create(int q, int nx, int ny, int nz, double tauA, double tauB, double G,
                                 int order, int nthreads);
 void mcmp_oracle_destroy(mcmp_oracle* m);
+/* The film models mcmp2DFilm / mcmp3DFilm: walls on the last axis (rows 1
+ * and n), fluid-wall interaction through psi(rhoA_sol / rhoB_sol), bounce-back
+ * of both components.  Call before init. */
+void mcmp_oracle_set_film(mcmp_oracle* m, double rhoA_sol, double rhoB_sol);
 void mcmp_oracle_set_rho(mcmp_oracle* m, const double* rhoA, const double* rhoB);
 void mcmp_oracle_init(mcmp_oracle* m);
 void mcmp_oracle_step(mcmp_oracle* m, int nsteps);

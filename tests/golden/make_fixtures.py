@@ -48,7 +48,7 @@ CASES = [
 
 
 # two components: name, q, shape, tauA, tauB, G, init, steps, steps_with_f,
-# ranks; init ("noise", seed): rhoA and rhoB = 0.5 + 0.1 (x - 0.5) from
+# ranks[, film = (rhoA_sol, rhoB_sol)]; init ("noise", seed): rhoA and rhoB = 0.5 + 0.1 (x - 0.5) from
 # seeds seed and seed + 1 (the reference's defaults rhoAi = rhoBi = 0.5,
 # rhoAi_noise = rhoBi_noise = 0.1, mcmp2D.cpp:21-24); ("droplet", R): an
 # A-rich disc or ball (1.0 / 0.05, width 2) in B, rhoB = 1.05 - rhoA
@@ -57,6 +57,11 @@ MC_CASES = [
     ("mcmp2d_48x40_droplet", 9, (48, 40), 1.0, 1.0, 3.5, ("droplet", 10.0), [1, 300], [300], 2),
     ("mcmp3d_7x6x5", 19, (7, 6, 5), 1.0, 0.7, 3.0, ("noise", 23), [0, 1, 2, 10, 100], [1, 100], 3),
     ("mcmp3d_4x8x64", 19, (4, 8, 64), 1.0, 1.0, 3.2, ("noise", 25), [1, 30], [], 2),
+    # film models (walls on the last axis, wall densities rhoA_sol, rhoB_sol)
+    ("mcmpfilm2d_20x12", 9, (20, 12), 1.0, 1.0, 3.0, ("noise", 27), [0, 1, 2, 10, 150], [1, 150], 3,
+     (0.3, 0.7)),
+    ("mcmpfilm3d_6x5x8", 19, (6, 5, 8), 1.0, 0.9, 3.0, ("noise", 29), [0, 1, 2, 30], [1, 30], 2,
+     (0.6, 0.4)),
 ]
 
 
@@ -70,8 +75,14 @@ def mc_inputs(shape, init):
     return ra, 1.05 - ra
 
 
-def run_mc_case(name, q, shape, tauA, tauB, G, init, steps, steps_f, ranks):
+def run_mc_case(name, q, shape, tauA, tauB, G, init, steps, steps_f, ranks, film=None):
     rA, rB = mc_inputs(shape, init)
+    if film is not None and q == 19:
+        # mcmp3DFilm initialises the fluid rows only (mcmp3DFilm.cpp:71-84):
+        # the wall rows keep density 0
+        for r in (rA, rB):
+            r.reshape(shape)[:, :, 0] = 0.0
+            r.reshape(shape)[:, :, -1] = 0.0
     nx, ny = shape[0], shape[1]
     nz = shape[2] if len(shape) == 3 else 1
     n = nx * ny * nz
@@ -82,6 +93,8 @@ def run_mc_case(name, q, shape, tauA, tauB, G, init, steps, steps_f, ranks):
         rB.astype(np.float64).tofile(pre + "_rB0.bin")
         cmd = [MPIRUN, "-np", str(ranks), HARNESS_MC, str(q), str(nx), str(ny), str(nz), repr(tauA),
                repr(tauB), repr(G), pre + "_rA0.bin", pre + "_rB0.bin", pre, ",".join(map(str, steps))]
+        if film is not None:
+            cmd += [repr(film[0]), repr(film[1])]
         subprocess.run(cmd, check=True)
         for s in steps:
             for c in "AB":
@@ -94,7 +107,9 @@ def run_mc_case(name, q, shape, tauA, tauB, G, init, steps, steps_f, ranks):
                     out[f"f_{c}_{s}"] = a[7 * n:]
     meta = dict(q=q, shape=list(shape), tauA=tauA, tauB=tauB, G=G, init=list(init), steps=steps,
                 steps_f=steps_f, ranks=ranks, order=3 if q == 19 else 2, components=2,
-                source="reference mcmp2D / mcmp3D classes via oracle/_ref/ref_harness_mcmp")
+                film=list(film) if film is not None else None,
+                source="reference mcmp2D / mcmp3D (film: mcmp2DFilm / mcmp3DFilm) classes via "
+                       "oracle/_ref/ref_harness_mcmp")
     out["meta"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
     np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
     print(name, "ok")

@@ -1,8 +1,9 @@
-"""Two-component Shan-Chen (mcmp2D / mcmp3D) on the GPU, FP64, through the C
-ABI: bit for bit against fixtures written by the reference's own mcmp2D and
-mcmp3D classes (every dumped array of both components: r, u, v, w, the
-force arrays and f), and against the oracle on further lattices, taus and
-both step orders, with restarts through set_f."""
+"""Two-component Shan-Chen (mcmp2D / mcmp3D, and the film forms mcmp2DFilm /
+mcmp3DFilm) on the GPU, FP64, through the C ABI: bit for bit against
+fixtures written by the reference's own classes (every dumped array of both
+components: r, u, v, w, the force arrays and f), and against the oracle on
+further lattices, taus, wall densities and both step orders, with restarts
+through set_f."""
 import numpy as np
 import pytest
 
@@ -13,8 +14,9 @@ from conftest import MC_FIXTURES, load_fixture, rel_l2, same_bits
 pytestmark = pytest.mark.gpu
 
 
-def _mc_lattice(q, shape, tauA, tauB, G, order=None):
-    return mb.Lattice(q, shape, tauA, G, order=order, precision=mb.FP64, components=2, tau_b=tauB)
+def _mc_lattice(q, shape, tauA, tauB, G, order=None, film=None):
+    return mb.Lattice(q, shape, tauA, G, order=order, precision=mb.FP64, components=2, tau_b=tauB,
+                      walls=film is not None, rho_sol=film if film is not None else (0.5, 0.5))
 
 
 @pytest.mark.parametrize("name", MC_FIXTURES)
@@ -22,7 +24,7 @@ def test_mc_fixture_bitwise(name):
     meta, data = load_fixture(name)
     q, shape = meta["q"], tuple(meta["shape"])
     print(f"mcmp fixture {name}", flush=True)
-    lat = _mc_lattice(q, shape, meta["tauA"], meta["tauB"], meta["G"])
+    lat = _mc_lattice(q, shape, meta["tauA"], meta["tauB"], meta["G"], film=meta.get("film"))
     assert lat.kernel == mb.KERNEL_TWO_PASS
     lat.set_rho(data["rhoA0"], comp=0)
     lat.set_rho(data["rhoB0"], comp=1)
@@ -48,24 +50,28 @@ def test_mc_fixture_bitwise(name):
 
 
 MC_CASES = [
-    # q, shape, tauA, tauB, G, order, steps
+    # q, shape, tauA, tauB, G, order, steps[, film]
     (9, (31, 27), 1.0, 0.7, 3.2, 2, 60),
     (9, (20, 64), 0.8, 1.3, 2.6, 3, 40),
     (19, (9, 8, 7), 0.9, 1.1, 3.0, 3, 30),
     (19, (6, 10, 33), 1.0, 1.0, 3.4, 2, 25),
+    (9, (26, 19), 0.9, 1.2, 3.0, 2, 50, (0.2, 0.8)),
+    (19, (7, 9, 12), 1.0, 0.8, 2.8, 3, 30, (0.7, 0.3)),
+    (19, (5, 6, 9), 1.1, 1.0, 3.0, 2, 25, (0.5, 0.5)),
 ]
 
 
 @pytest.mark.parametrize("case", MC_CASES)
 def test_mc_matches_oracle_bitwise(case):
-    q, shape, tauA, tauB, G, order, steps = case
+    q, shape, tauA, tauB, G, order, steps = case[:7]
+    film = case[7] if len(case) > 7 else None
     print(f"mcmp case {case}", flush=True)
     rA = mb.noise_rho(shape, 31, mean=0.5, amp=0.1)
     rB = mb.noise_rho(shape, 32, mean=0.5, amp=0.1)
-    o = oracle.MCOracle(q, shape, tauA, tauB, G, order=order)
+    o = oracle.MCOracle(q, shape, tauA, tauB, G, order=order, film=film)
     o.set_rho(rA, rB)
     o.init()
-    lat = _mc_lattice(q, shape, tauA, tauB, G, order=order)
+    lat = _mc_lattice(q, shape, tauA, tauB, G, order=order, film=film)
     lat.set_rho(rA, comp=0)
     lat.set_rho(rB, comp=1)
     lat.init()
@@ -117,7 +123,7 @@ def test_mc_restart_and_conservation(q, shape, order):
 
 
 def test_mc_descriptions_refused():
-    for kw in (dict(walls=True), dict(precision=mb.FP16N), dict(kernel=mb.KERNEL_FUSED)):
+    for kw in (dict(precision=mb.FP16N), dict(kernel=mb.KERNEL_FUSED)):
         with pytest.raises(mb.LbmError):
             mb.Lattice(19, (8, 8, 64), 1.0, 3.0, components=2, tau_b=1.0, **kw)
     lat = mb.Lattice(9, (8, 8), 1.0, 3.0, components=2, tau_b=1.0)

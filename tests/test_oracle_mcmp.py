@@ -1,7 +1,8 @@
-"""The CPU oracle's two-component model (mcmp2D / mcmp3D restated in
-oracle/scmp_oracle.c) against fixtures written by the reference's own
-mcmp2D and mcmp3D classes (tests/golden/mcmp*.npz, via
-oracle/_ref/ref_harness_mcmp): bit for bit, every array, every dumped step;
+"""The CPU oracle's two-component model (mcmp2D / mcmp3D and the film forms
+mcmp2DFilm / mcmp3DFilm, restated in oracle/scmp_oracle.c) against fixtures
+written by the reference's own classes (tests/golden/mcmp*.npz, via
+oracle/_ref/ref_harness_mcmp): bit for bit, every array, every dumped step
+(NaN where the reference has NaN: mcmp3DFilm's wall rows);
 plus the invariants the model fixes whatever its rounding: each component's
 mass is conserved, the two cross forces cancel in total (momentum), the
 common velocity is the same in both components' arrays after an mcmp3D
@@ -20,7 +21,7 @@ KEYS = ("rho", "u", "v", "w", "fx", "fy", "fz")
 def test_mc_oracle_matches_reference_fixture_bitwise(name):
     meta, data = load_fixture(name)
     q, shape = meta["q"], tuple(meta["shape"])
-    o = oracle.MCOracle(q, shape, meta["tauA"], meta["tauB"], meta["G"], nthreads=3)
+    o = oracle.MCOracle(q, shape, meta["tauA"], meta["tauB"], meta["G"], nthreads=3, film=meta.get("film"))
     o.set_rho(data["rhoA0"], data["rhoB0"])
     o.init()
     cur = 0
