@@ -193,6 +193,22 @@ int lbm_set_f_component(lbm_handle* h, int comp, const double* f);
 /* FP64 totals of one component's current f, as lbm_conservation. */
 int lbm_conservation_component(lbm_handle* h, int comp, double out[4]);
 
+/* ---- asynchronous snapshot -----------------------------------------------------
+ * The output step of the reference models (scmp3D::outputLatticeBoltzmann,
+ * scmp3D.cpp:185-245; mcmp2D.cpp:255-300) reads r, u and fx of the lattice
+ * and writes a VTK file while the run waits.  lbm_snapshot_begin enqueues the
+ * same fields on the handle's stream (as lbm_get_fields_component with kind,
+ * then lbm_get_force_component, of component comp) into a device buffer and
+ * their copy to pinned host memory on a second stream of the handle, and
+ * returns at once: the following lbm_step calls run while the copy goes on.
+ * lbm_snapshot_wait waits for that copy and hands out the seven arrays of
+ * nx*ny*nz doubles (any pointer may be NULL) and the step count they were
+ * taken at.  One snapshot is held at a time: a second begin first waits for
+ * the previous copy (whose data it then overwrites). */
+int lbm_snapshot_begin(lbm_handle* h, int comp, int kind);
+int lbm_snapshot_wait(lbm_handle* h, int64_t* step, double* rho, double* u, double* v, double* w,
+                      double* fx, double* fy, double* fz);
+
 /* Number of completed steps since lbm_init_equilibrium / lbm_set_f. */
 int64_t lbm_steps_done(const lbm_handle* h);
 

@@ -158,13 +158,11 @@ int run_step(lbm_handle* h)
     return LBM_OK;
 }
 
-// Moments of component comp of buffer b into the four host arrays.
-int moments_to_host(lbm_handle* h, int b, double* rho, double* u, double* v, double* w, int comp = 0)
+// Enqueues the moments (rho, u, v, w) of component comp of buffer b into the
+// device arrays d, d+n, d+2n, d+3n.
+int moments_to_device(lbm_handle* h, int b, int comp, double* d)
 {
     const int64_t n = h->g.n;
-    int rc = ensure_tmp(h, 4 * n * 8);
-    if (rc) return rc;
-    double* d = h->tmp;
     if (!is_lowp(h))
         HIP_OK(lbm::fp64_moments(h->a, h->g, pops64(h, b, comp), h->kind[b], d, d + n, d + 2 * n,
                                  d + 3 * n, h->stream));
@@ -172,12 +170,30 @@ int moments_to_host(lbm_handle* h, int b, double* rho, double* u, double* v, dou
         HIP_OK(lbm::lowp_moments(h->d.precision, h->a, h->g, h->d.rho_shift, h->buf[b],
                                  h->kind[b], h->rref[b], d, d + n, d + 2 * n, d + 3 * n,
                                  h->stream));
-    double* outs[4] = {rho, u, v, w};
-    for (int k = 0; k < 4; k++)
-        if (outs[k])
-            HIP_OK(hipMemcpyAsync(outs[k], d + k * n, n * 8, hipMemcpyDeviceToHost, h->stream));
+    return LBM_OK;
+}
+
+// Copies k consecutive device arrays of n doubles to the non-NULL host
+// arrays and waits.
+int arrays_to_host(lbm_handle* h, const double* d, double* const* outs, int k)
+{
+    const int64_t n = h->g.n;
+    for (int i = 0; i < k; i++)
+        if (outs[i])
+            HIP_OK(hipMemcpyAsync(outs[i], d + i * n, n * 8, hipMemcpyDeviceToHost, h->stream));
     HIP_OK(hipStreamSynchronize(h->stream));
     return LBM_OK;
+}
+
+// Moments of component comp of buffer b into the four host arrays.
+int moments_to_host(lbm_handle* h, int b, double* rho, double* u, double* v, double* w, int comp = 0)
+{
+    int rc = ensure_tmp(h, 4 * h->g.n * 8);
+    if (rc) return rc;
+    rc = moments_to_device(h, b, comp, h->tmp);
+    if (rc) return rc;
+    double* outs[4] = {rho, u, v, w};
+    return arrays_to_host(h, h->tmp, outs, 4);
 }
 
 int check_desc(const lbm_desc* d)
@@ -340,6 +356,14 @@ void lbm_destroy(lbm_handle* h)
     if (h->tmp) hipFree(h->tmp);
     if (h->ev0) hipEventDestroy(h->ev0);
     if (h->ev1) hipEventDestroy(h->ev1);
+    if (h->copy_stream) {
+        hipStreamSynchronize(h->copy_stream);
+        hipStreamDestroy(h->copy_stream);
+    }
+    if (h->ev_snap) hipEventDestroy(h->ev_snap);
+    if (h->ev_copied) hipEventDestroy(h->ev_copied);
+    if (h->snap_dev) hipFree(h->snap_dev);
+    if (h->snap_host) hipHostFree(h->snap_host);
     if (h->own) hipStreamDestroy(h->own);
     delete h;
 }
@@ -419,37 +443,36 @@ int lbm_get_fields(lbm_handle* h, int kind, double* rho, double* u, double* v, d
     return lbm_get_fields_component(h, 0, kind, rho, u, v, w);
 }
 
-int lbm_get_fields_component(lbm_handle* h, int comp, int kind, double* rho, double* u, double* v,
-                             double* w)
+}  // extern "C"
+
+namespace {
+
+// Enqueues component comp's r,u,v,w arrays (kind as lbm_get_fields) into the
+// device arrays out, out+n, out+2n, out+3n.  Uses the staging buffer (8n
+// doubles at most), which must not overlap out.
+int fields_to_device(lbm_handle* h, int comp, int kind, double* out)
 {
-    if (!h) return fail(LBM_EINVAL, "lbm_get_fields: NULL handle");
-    int rc = check_comp(h, comp);
-    if (rc) return rc;
-    if (!h->initialised) return fail(LBM_ESTATE, "lbm_get_fields before initialisation");
     const int64_t n = h->g.n;
-    if (kind == LBM_FIELDS_CURRENT) return moments_to_host(h, h->cur, rho, u, v, w, comp);
+    if (kind == LBM_FIELDS_CURRENT) return moments_to_device(h, h->cur, comp, out);
     if (kind != LBM_FIELDS_REFERENCE) return fail(LBM_EINVAL, "unknown fields kind %d", kind);
     if (h->steps == 0 && h->ref_is_rho0) {
         // r = rho0, u = v = w = 0 (scmp3D.cpp:63-76 before any step; with two
         // components the common velocity initLatticeBoltzmann leaves, which is
         // 0: mcmp2D.cpp:85, mcmp3D.cpp:89)
-        if (rho)
-            HIP_OK(hipMemcpyAsync(rho, h->rho0 + comp * n, n * 8, hipMemcpyDeviceToHost, h->stream));
-        HIP_OK(hipStreamSynchronize(h->stream));
-        for (double* p : {u, v, w})
-            if (p) std::memset(p, 0, n * 8);
+        HIP_OK(hipMemcpyAsync(out, h->rho0 + comp * n, n * 8, hipMemcpyDeviceToDevice, h->stream));
+        HIP_OK(hipMemsetAsync(out + n, 0, 3 * n * 8, h->stream));
         return LBM_OK;
     }
-    if (h->d.order == 2 || h->steps == 0) return moments_to_host(h, h->cur, rho, u, v, w, comp);
+    if (h->d.order == 2 || h->steps == 0) return moments_to_device(h, h->cur, comp, out);
     if (h->steps == 1 && h->d.order == 3 && !h->prev_valid)
         return fail(LBM_ESTATE, "previous populations unavailable");
     const int prev = 1 - h->cur;
     if (comps(h) == 1)
         // order 3: the arrays hold the moments computed at the start of step N
-        return moments_to_host(h, prev, rho, u, v, w);
+        return moments_to_device(h, prev, 0, out);
     // mcmp3D: r = this component's density of f(N-1), u = the common
     // velocity formed from both components' moments of f(N-1)
-    rc = ensure_tmp(h, 8 * n * 8);
+    int rc = ensure_tmp(h, 8 * n * 8);
     if (rc) return rc;
     double* A = h->tmp;
     double* B = h->tmp + 4 * n;
@@ -457,15 +480,119 @@ int lbm_get_fields_component(lbm_handle* h, int comp, int kind, double* rho, dou
                              h->stream));
     HIP_OK(lbm::fp64_moments(h->a, h->g, pops64(h, prev, 1), h->kind[prev], B, B + n, B + 2 * n, B + 3 * n,
                              h->stream));
-    const double* mine = comp == 0 ? A : B;
-    if (rho) HIP_OK(hipMemcpyAsync(rho, mine, n * 8, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipMemcpyAsync(out, comp == 0 ? A : B, n * 8, hipMemcpyDeviceToDevice, h->stream));
     HIP_OK(lbm::fp64_common_velocity(h->a, h->g, A, A + n, A + 2 * n, A + 3 * n, B, B + n, B + 2 * n,
                                      B + 3 * n, comp, h->stream));
-    double* outs[3] = {u, v, w};
-    for (int k = 0; k < 3; k++)
-        if (outs[k])
-            HIP_OK(hipMemcpyAsync(outs[k], A + (k + 1) * n, n * 8, hipMemcpyDeviceToHost, h->stream));
-    HIP_OK(hipStreamSynchronize(h->stream));
+    HIP_OK(hipMemcpyAsync(out + n, A + n, 3 * n * 8, hipMemcpyDeviceToDevice, h->stream));
+    return LBM_OK;
+}
+
+// Enqueues component comp's fx, fy, fz arrays into out, out+n, out+2n
+// (staging buffer: (5*components + 3) n doubles, not overlapping out).
+int force_to_device(lbm_handle* h, int comp, double* out)
+{
+    const int64_t n = h->g.n;
+    const int nc = comps(h);
+    // per component: density the last step started from, then psi; then F
+    int rc = ensure_tmp(h, (5 * nc + 3) * n * 8);
+    if (rc) return rc;
+    double* F = h->tmp + 5 * nc * n;
+    const bool from_rho0 = h->ref_is_rho0 && (h->steps == 0 || (h->steps == 1 && h->d.order == 2));
+    for (int c = 0; c < nc; c++) {
+        double* rho = h->tmp + 5 * c * n;
+        double* psi = rho + 4 * n;
+        if (from_rho0) {
+            HIP_OK(hipMemcpyAsync(rho, h->rho0 + c * n, n * 8, hipMemcpyDeviceToDevice, h->stream));
+        } else {
+            const int b = (h->steps == 0) ? h->cur : 1 - h->cur;
+            rc = moments_to_device(h, b, c, rho);
+            if (rc) return rc;
+        }
+        HIP_OK(lbm::fp64_psi_from_rho(rho, psi, n, h->stream));
+    }
+    if (nc == 1) {
+        HIP_OK(lbm::fp64_force(h->a, h->g, h->tmp + 4 * n, F, F + n, F + 2 * n, h->stream));
+    } else {
+        const double* ps = h->tmp + (5 * comp + 4) * n;
+        const double* po = h->tmp + (5 * (1 - comp) + 4) * n;
+        HIP_OK(lbm::fp64_force2(h->a, h->g, ps, po, 1 - comp, F, F + n, F + 2 * n, h->stream));
+    }
+    HIP_OK(hipMemcpyAsync(out, F, 3 * n * 8, hipMemcpyDeviceToDevice, h->stream));
+    return LBM_OK;
+}
+
+// Device buffer of the snapshot (7n doubles) and its pinned host copy.
+int ensure_snapshot(lbm_handle* h)
+{
+    const int64_t bytes = 7 * h->g.n * 8;
+    if (h->snap_dev) return LBM_OK;
+    HIP_OK(lbm::dev_malloc(&h->snap_dev, bytes));
+    HIP_OK(hipHostMalloc(&h->snap_host, bytes, hipHostMallocDefault));
+    HIP_OK(hipStreamCreateWithFlags(&h->copy_stream, hipStreamNonBlocking));
+    HIP_OK(hipEventCreateWithFlags(&h->ev_snap, hipEventDisableTiming));
+    HIP_OK(hipEventCreateWithFlags(&h->ev_copied, hipEventDisableTiming));
+    return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_get_fields_component(lbm_handle* h, int comp, int kind, double* rho, double* u, double* v,
+                             double* w)
+{
+    if (!h) return fail(LBM_EINVAL, "lbm_get_fields: NULL handle");
+    int rc = check_comp(h, comp);
+    if (rc) return rc;
+    if (!h->initialised) return fail(LBM_ESTATE, "lbm_get_fields before initialisation");
+    // the fields are formed behind the staging area the two-component
+    // reference arrays use (8n doubles), then copied out
+    const int64_t n = h->g.n;
+    rc = ensure_tmp(h, 12 * n * 8);
+    if (rc) return rc;
+    double* out = h->tmp + 8 * n;
+    rc = fields_to_device(h, comp, kind, out);
+    if (rc) return rc;
+    double* outs[4] = {rho, u, v, w};
+    return arrays_to_host(h, out, outs, 4);
+}
+
+int lbm_snapshot_begin(lbm_handle* h, int comp, int kind)
+{
+    if (!h) return fail(LBM_EINVAL, "lbm_snapshot_begin: NULL handle");
+    int rc = check_comp(h, comp);
+    if (rc) return rc;
+    if (!h->initialised) return fail(LBM_ESTATE, "lbm_snapshot_begin before initialisation");
+    rc = ensure_snapshot(h);
+    if (rc) return rc;
+    // the previous snapshot's copy still reads snap_dev and fills snap_host
+    if (h->snap_pending) HIP_OK(hipEventSynchronize(h->ev_copied));
+    h->snap_pending = false;
+    rc = fields_to_device(h, comp, kind, h->snap_dev);
+    if (rc) return rc;
+    rc = force_to_device(h, comp, h->snap_dev + 4 * h->g.n);
+    if (rc) return rc;
+    HIP_OK(hipEventRecord(h->ev_snap, h->stream));
+    HIP_OK(hipStreamWaitEvent(h->copy_stream, h->ev_snap, 0));
+    HIP_OK(hipMemcpyAsync(h->snap_host, h->snap_dev, 7 * h->g.n * 8, hipMemcpyDeviceToHost, h->copy_stream));
+    HIP_OK(hipEventRecord(h->ev_copied, h->copy_stream));
+    h->snap_pending = true;
+    h->snap_step = h->steps;
+    return LBM_OK;
+}
+
+int lbm_snapshot_wait(lbm_handle* h, int64_t* step, double* rho, double* u, double* v, double* w,
+                      double* fx, double* fy, double* fz)
+{
+    if (!h) return fail(LBM_EINVAL, "lbm_snapshot_wait: NULL handle");
+    if (!h->snap_pending) return fail(LBM_ESTATE, "lbm_snapshot_wait without lbm_snapshot_begin");
+    HIP_OK(hipEventSynchronize(h->ev_copied));
+    h->snap_pending = false;
+    const int64_t n = h->g.n;
+    double* outs[7] = {rho, u, v, w, fx, fy, fz};
+    for (int k = 0; k < 7; k++)
+        if (outs[k]) std::memcpy(outs[k], h->snap_host + k * n, n * 8);
+    if (step) *step = h->snap_step;
     return LBM_OK;
 }
 
@@ -481,41 +608,14 @@ int lbm_get_force_component(lbm_handle* h, int comp, double* fx, double* fy, dou
     if (rc) return rc;
     if (!h->initialised) return fail(LBM_ESTATE, "lbm_get_force before initialisation");
     const int64_t n = h->g.n;
-    const int nc = comps(h);
-    // per component: density the last step started from, then psi; then F
-    rc = ensure_tmp(h, (5 * nc + 3) * n * 8);
+    const int base = 5 * comps(h) + 3;   // force_to_device's staging
+    rc = ensure_tmp(h, (base + 3) * n * 8);
     if (rc) return rc;
-    double* F = h->tmp + 5 * nc * n;
-    const bool from_rho0 = h->ref_is_rho0 && (h->steps == 0 || (h->steps == 1 && h->d.order == 2));
-    for (int c = 0; c < nc; c++) {
-        double* rho = h->tmp + 5 * c * n;
-        double* psi = rho + 4 * n;
-        if (from_rho0) {
-            HIP_OK(hipMemcpyAsync(rho, h->rho0 + c * n, n * 8, hipMemcpyDeviceToDevice, h->stream));
-        } else {
-            const int b = (h->steps == 0) ? h->cur : 1 - h->cur;
-            if (!is_lowp(h))
-                HIP_OK(lbm::fp64_moments(h->a, h->g, pops64(h, b, c), h->kind[b], rho, rho + n, rho + 2 * n,
-                                         rho + 3 * n, h->stream));
-            else
-                HIP_OK(lbm::lowp_moments(h->d.precision, h->a, h->g, h->d.rho_shift, h->buf[b], h->kind[b],
-                                         h->rref[b], rho, rho + n, rho + 2 * n, rho + 3 * n, h->stream));
-        }
-        HIP_OK(lbm::fp64_psi_from_rho(rho, psi, n, h->stream));
-    }
-    if (nc == 1) {
-        HIP_OK(lbm::fp64_force(h->a, h->g, h->tmp + 4 * n, F, F + n, F + 2 * n, h->stream));
-    } else {
-        const double* ps = h->tmp + (5 * comp + 4) * n;
-        const double* po = h->tmp + (5 * (1 - comp) + 4) * n;
-        HIP_OK(lbm::fp64_force2(h->a, h->g, ps, po, 1 - comp, F, F + n, F + 2 * n, h->stream));
-    }
+    double* out = h->tmp + base * n;
+    rc = force_to_device(h, comp, out);
+    if (rc) return rc;
     double* outs[3] = {fx, fy, fz};
-    for (int k = 0; k < 3; k++)
-        if (outs[k])
-            HIP_OK(hipMemcpyAsync(outs[k], F + k * n, n * 8, hipMemcpyDeviceToHost, h->stream));
-    HIP_OK(hipStreamSynchronize(h->stream));
-    return LBM_OK;
+    return arrays_to_host(h, out, outs, 3);
 }
 
 int lbm_get_f(lbm_handle* h, double* f) { return lbm_get_f_component(h, 0, f); }

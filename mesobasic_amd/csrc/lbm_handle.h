@@ -34,6 +34,15 @@ struct lbm_handle {
     lbm::FusedTuning tune{-1, 64, 3, 1};  // fused kernel launch configuration
     int64_t bytes = 0;
     int device = 0;           // device current at lbm_create
+    // asynchronous snapshot (lbm_snapshot_begin / lbm_snapshot_wait): fields
+    // formed in snap_dev on the stream, copied to pinned snap_host on
+    // copy_stream while the next steps run
+    double* snap_dev = nullptr;
+    double* snap_host = nullptr;
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t ev_snap = nullptr, ev_copied = nullptr;
+    bool snap_pending = false;
+    int64_t snap_step = 0;
 };
 
 inline bool is_lowp(const lbm_handle* h) { return h->d.precision != LBM_FP64; }

@@ -54,4 +54,6 @@ void LBApp::stepForward(int step)
 
 void LBApp::writeOutput(int) { lb_object->outputLatticeBoltzmann(); }
 
+void LBApp::finishOutput() { lb_object->finishOutput(); }
+
 }  // namespace mesobasic

@@ -16,6 +16,7 @@ public:
     virtual void initSystem() = 0;
     virtual void stepForward(int) = 0;
     virtual void writeOutput(int) = 0;
+    virtual void finishOutput() {}
     virtual ~MesoBase() {}
 };
 
@@ -25,6 +26,7 @@ public:
     void initSystem() override;
     void stepForward(int step) override;
     void writeOutput(int step) override;
+    void finishOutput() override;
     const CommonParams& params() const { return p; }
     LBBaseClass* model() { return lb_object.get(); }
 

@@ -1,0 +1,102 @@
+#include "OutputQueue.hpp"
+
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+#include "VtkWriter.hpp"
+
+namespace mesobasic {
+
+namespace {
+
+struct Frame {
+    std::vector<double> rho, u, v, fx, fy;
+};
+
+void check(int rc, const char* what)
+{
+    if (rc != LBM_OK) throw std::runtime_error(std::string(what) + ": " + lbm_last_error());
+}
+
+void write_frame(const CommonParams& p, int q, int step, const Frame& f)
+{
+    if (q == 19) {
+        write_vtk_3d("vtkoutput", "rhoA", step, p.NX, p.NY, p.NZ, p.iskip, p.jskip, p.kskip, f.rho);
+        return;
+    }
+    std::vector<double> us(f.rho.size()), vs(f.rho.size());
+    for (size_t c = 0; c < f.rho.size(); c++) {
+        us[c] = f.u[c] + 0.5 * f.fx[c] / f.rho[c];   // LBfluid2D::getUStar (LBfluid2D.cpp:164-172)
+        vs[c] = f.v[c] + 0.5 * f.fy[c] / f.rho[c];
+    }
+    write_vtk_2d("vtkoutput", "rhoA", step, p.NX, p.NY, p.iskip, p.jskip, f.rho, us, vs);
+}
+
+}  // namespace
+
+OutputQueue::~OutputQueue()
+{
+    if (writer_.joinable()) writer_.join();
+}
+
+void OutputQueue::join_writer()
+{
+    if (writer_.joinable()) writer_.join();
+    if (writer_error_) {
+        std::exception_ptr e = writer_error_;
+        writer_error_ = nullptr;
+        std::rethrow_exception(e);
+    }
+}
+
+void OutputQueue::collect_and_write(lbm_handle* h)
+{
+    const size_t n = static_cast<size_t>(p_.nx) * p_.ny * p_.nz;
+    auto f = std::make_shared<Frame>();
+    f->rho.resize(n);
+    f->u.resize(n);
+    f->v.resize(n);
+    const bool two_d = (q_ == 9);
+    if (two_d) {
+        f->fx.resize(n);
+        f->fy.resize(n);
+    }
+    int64_t taken = 0;
+    check(lbm_snapshot_wait(h, &taken, f->rho.data(), f->u.data(), f->v.data(), nullptr,
+                            two_d ? f->fx.data() : nullptr, two_d ? f->fy.data() : nullptr, nullptr),
+          "lbm_snapshot_wait");
+    pending_ = false;
+    const int step = pending_step_;
+    join_writer();
+    if (!async_) {
+        write_frame(p_, q_, step, *f);
+        return;
+    }
+    const CommonParams* p = &p_;
+    const int q = q_;
+    writer_ = std::thread([this, p, q, step, f]() {
+        try {
+            write_frame(*p, q, step, *f);
+        } catch (...) {
+            writer_error_ = std::current_exception();
+        }
+    });
+}
+
+void OutputQueue::output(lbm_handle* h, int comp, int step)
+{
+    if (pending_) collect_and_write(h);
+    check(lbm_snapshot_begin(h, comp, LBM_FIELDS_REFERENCE), "lbm_snapshot_begin");
+    pending_ = true;
+    pending_step_ = step;
+    if (!async_) collect_and_write(h);
+}
+
+void OutputQueue::finish(lbm_handle* h)
+{
+    if (pending_) collect_and_write(h);
+    join_writer();
+}
+
+}  // namespace mesobasic

@@ -65,6 +65,7 @@ ScmpModel::ScmpModel(const CommonParams& pin, const InputParams& in, int q, int 
     d.kernel = kernel_of(in("LBApp/kernel", "auto"));
     if (q == 9 && p.nz != 1) throw std::runtime_error("scmp2D needs Domain/nz = 1");
     check(lbm_create(&d, &h_), "lbm_create");
+    out_.reset(new OutputQueue(p, q, in("Output/async", 1) != 0));
 }
 
 ScmpModel::~ScmpModel() { lbm_destroy(h_); }
@@ -113,28 +114,9 @@ void ScmpModel::force(std::vector<double>& fx, std::vector<double>& fy, std::vec
     check(lbm_get_force(h_, fx.data(), fy.data(), fz.data()), "lbm_get_force");
 }
 
-void scmp3D::outputLatticeBoltzmann()
-{
-    // scmp3D.cpp:127-133
-    std::vector<double> rho, u, v, w;
-    fields(rho, u, v, w);
-    write_vtk_3d("vtkoutput", "rhoA", current_step, p.NX, p.NY, p.NZ, p.iskip, p.jskip, p.kskip, rho);
-}
+void ScmpModel::outputLatticeBoltzmann() { out_->output(h_, 0, current_step); }
 
-void scmp2D::outputLatticeBoltzmann()
-{
-    // as mcmp2D::outputLatticeBoltzmann (mcmp2D.cpp:131-137) with the
-    // Shan-Chen velocity u* = u + F/(2 rho) (LBfluid2D.cpp:164-172)
-    std::vector<double> rho, u, v, w, fx, fy, fz;
-    fields(rho, u, v, w);
-    force(fx, fy, fz);
-    std::vector<double> us(rho.size()), vs(rho.size());
-    for (size_t c = 0; c < rho.size(); c++) {
-        us[c] = u[c] + 0.5 * fx[c] / rho[c];
-        vs[c] = v[c] + 0.5 * fy[c] / rho[c];
-    }
-    write_vtk_2d("vtkoutput", "rhoA", current_step, p.NX, p.NY, p.iskip, p.jskip, rho, us, vs);
-}
+void ScmpModel::finishOutput() { out_->finish(h_); }
 
 McmpModel::McmpModel(const CommonParams& pin, const InputParams& in, int q, bool film)
     : p(pin), q_(q), film_(film)
@@ -178,6 +160,7 @@ McmpModel::McmpModel(const CommonParams& pin, const InputParams& in, int q, bool
     d.rho_sol_a = rhoA_sol;
     d.rho_sol_b = rhoB_sol;
     check(lbm_create(&d, &h_), "lbm_create");
+    out_.reset(new OutputQueue(p, q, in("Output/async", 1) != 0));
 }
 
 McmpModel::~McmpModel() { lbm_destroy(h_); }
@@ -247,24 +230,10 @@ void McmpModel::force(int comp, std::vector<double>& fx, std::vector<double>& fy
     check(lbm_get_force_component(h_, comp, fx.data(), fy.data(), fz.data()), "lbm_get_force_component");
 }
 
-void McmpModel::outputLatticeBoltzmann()
-{
-    // mcmp2D.cpp:131-137 / mcmp3D.cpp:131-138: component A only
-    std::vector<double> rho, u, v, w;
-    fields(0, rho, u, v, w);
-    if (q_ == 19) {
-        write_vtk_3d("vtkoutput", "rhoA", current_step, p.NX, p.NY, p.NZ, p.iskip, p.jskip, p.kskip, rho);
-        return;
-    }
-    std::vector<double> fx, fy, fz;
-    force(0, fx, fy, fz);
-    std::vector<double> us(rho.size()), vs(rho.size());
-    for (size_t c = 0; c < rho.size(); c++) {
-        us[c] = u[c] + 0.5 * fx[c] / rho[c];   // LBfluid2D::getUStar (LBfluid2D.cpp:164-172)
-        vs[c] = v[c] + 0.5 * fy[c] / rho[c];
-    }
-    write_vtk_2d("vtkoutput", "rhoA", current_step, p.NX, p.NY, p.iskip, p.jskip, rho, us, vs);
-}
+// mcmp2D.cpp:131-137 / mcmp3D.cpp:131-138: component A only
+void McmpModel::outputLatticeBoltzmann() { out_->output(h_, 0, current_step); }
+
+void McmpModel::finishOutput() { out_->finish(h_); }
 
 // LBBaseClass::LBFactory (LBBaseClass.cpp:21-45), with the reference's
 // default type mcmp2D.  The capsule model is not part of this build, so it

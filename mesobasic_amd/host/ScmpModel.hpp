@@ -15,13 +15,17 @@
 //   LBApp/rho_shift  shift of the 16-bit storage (default rhoAi)
 //   LBApp/kernel     auto | two_pass | fused
 //   LBApp/order      3 (scmp3D step order) | 2 (LBfluid2D step order)
+//   Output/async     1 (default): VTK output overlapped with the next steps
+//                    (OutputQueue); 0: written before the output returns
 #pragma once
 
+#include <memory>
 #include <string>
 #include <vector>
 
 #include "../../include/mesobasic_lbm.h"
 #include "LBBaseClass.hpp"
+#include "OutputQueue.hpp"
 
 namespace mesobasic {
 
@@ -35,6 +39,10 @@ public:
     void setTimeStep(int step) override { current_step = step; }
     void initLatticeBoltzmann() override;
     void updateLatticeBoltzmann() override;
+    // scmp3D.cpp:127-133 (density) for D3Q19; for D2Q9 density and the
+    // Shan-Chen velocity as mcmp2D.cpp:131-137
+    void outputLatticeBoltzmann() override;
+    void finishOutput() override;
 
     // Host copies of the reference's r,u,v,w arrays (interior, x slowest).
     void fields(std::vector<double>& rho, std::vector<double>& u, std::vector<double>& v,
@@ -52,6 +60,7 @@ protected:
     unsigned seed;
     double R, W, rho_l, rho_v;
     lbm_handle* h_ = nullptr;
+    std::unique_ptr<OutputQueue> out_;
 
     void check(int rc, const char* what) const;
 };
@@ -59,7 +68,6 @@ protected:
 class scmp3D : public ScmpModel {
 public:
     scmp3D(const CommonParams& p, const InputParams& in) : ScmpModel(p, in, 19, 3) {}
-    void outputLatticeBoltzmann() override;
 };
 
 // D2Q9 single-component Shan-Chen: the scmp3D force on the D2Q9 stencil in
@@ -68,7 +76,6 @@ public:
 class scmp2D : public ScmpModel {
 public:
     scmp2D(const CommonParams& p, const InputParams& in) : ScmpModel(p, in, 9, 2) {}
-    void outputLatticeBoltzmann() override;
 };
 
 // Two-component Shan-Chen: mcmp2D (D2Q9, mcmp2D.cpp) and mcmp3D (D3Q19,
@@ -96,6 +103,7 @@ public:
     void initLatticeBoltzmann() override;
     void updateLatticeBoltzmann() override;
     void outputLatticeBoltzmann() override;
+    void finishOutput() override;
 
     // Host copies of component comp's r,u,v,w arrays and force arrays.
     void fields(int comp, std::vector<double>& rho, std::vector<double>& u, std::vector<double>& v,
@@ -114,6 +122,7 @@ private:
     unsigned seed;
     double R, W, rho_l, rho_v;
     lbm_handle* h_ = nullptr;
+    std::unique_ptr<OutputQueue> out_;
 
     void check(int rc, const char* what) const;
 };

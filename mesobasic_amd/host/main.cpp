@@ -89,6 +89,7 @@ int main(int argc, char** argv)
             if (step % outInterval == 0)
                 for (auto& a : apps) a->writeOutput(step);
         }
+        for (auto& a : apps) a->finishOutput();
         const double secs =
             std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         long long nodes = 0;

@@ -128,6 +128,8 @@ def load():
         "lbm_set_f_component": (ctypes.c_int, [P, ctypes.c_int, _D]),
         "lbm_conservation_component": (ctypes.c_int, [P, ctypes.c_int, _D]),
         "lbm_probe_memcpy": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, _D]),
+        "lbm_snapshot_begin": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),
+        "lbm_snapshot_wait": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int64)] + [_D] * 7),
         "lbm_probe_tile_map": (ctypes.c_int, [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_int)] * 3),
     }
     for name, (res, args) in sig.items():
@@ -318,6 +320,21 @@ class Lattice:
         _check(self._lib.lbm_get_force_component(self._h, int(comp), _dptr(fx), _dptr(fy), _dptr(fz)))
         sh = self.shape
         return fx.reshape(sh), fy.reshape(sh), fz.reshape(sh)
+
+    def snapshot_begin(self, comp=0, kind=FIELDS_REFERENCE):
+        """Starts an asynchronous copy of fields(kind, comp) and force(comp);
+        steps may be enqueued while it runs."""
+        _check(self._lib.lbm_snapshot_begin(self._h, int(comp), kind))
+
+    def snapshot_wait(self):
+        """Waits for the snapshot begun last; returns (step, fields dict with
+        rho, u, v, w, fx, fy, fz)."""
+        arrs = [np.empty(self.n) for _ in range(7)]
+        step = ctypes.c_int64()
+        _check(self._lib.lbm_snapshot_wait(self._h, ctypes.byref(step), *(_dptr(a) for a in arrs)))
+        sh = self.shape
+        names = ("rho", "u", "v", "w", "fx", "fy", "fz")
+        return step.value, {k: a.reshape(sh) for k, a in zip(names, arrs)}
 
     def get_f(self, comp=0):
         f = np.empty(self.n * self.q)
