@@ -327,6 +327,26 @@ int lbm_batch_finished(lbm_batch* b, int64_t* ids, int max, int* n);
  * the failure in lbm_last_error(). */
 int lbm_batch_collect(lbm_batch* b, int64_t job_id, double* rho, double* u, double* v, double* w);
 
+/* The reference's r,u,v,w and fx,fy,fz arrays of an admitted job after the
+ * steps it has run (status.steps_done), as lbm_get_fields
+ * (LBM_FIELDS_REFERENCE) and lbm_get_force return them for a single lattice
+ * in the job's step order (D3Q19: scmp3D, D2Q9: LBfluid2D): the moments,
+ * in FP64, of the job's stored populations, and the FP64 force of the density
+ * its last step started from.  Host arrays of nx*ny*nz doubles, any may be
+ * NULL; fz is 0 for D2Q9.  The job stays admitted.  Call between ticks.  The
+ * output step of the reference's models (scmp3D.cpp:127-133,
+ * mcmp2D.cpp:131-137) for a job of the batch. */
+int lbm_batch_read(lbm_batch* b, int64_t job_id, double* rho, double* u, double* v, double* w, double* fx,
+                   double* fy, double* fz);
+
+/* Gives a finished (not failed) job nsteps more steps: it runs again from
+ * the next tick, from the populations it finished with, exactly as if it had
+ * been submitted with the larger step count.  With lbm_batch_read this runs a
+ * job to an output cadence (MesoExecute.cpp:107-152: output every
+ * nstep/numOutputs steps).  LBM_ESTATE for a running job, LBM_EJOB for a
+ * failed one. */
+int lbm_batch_extend(lbm_batch* b, int64_t job_id, int64_t nsteps);
+
 enum lbm_job_state {
     LBM_JOB_RUNNING = 0,   /* admitted, steps left */
     LBM_JOB_DONE = 1,      /* ran all its steps; waiting to be collected */

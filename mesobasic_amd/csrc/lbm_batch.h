@@ -16,13 +16,13 @@ struct Job {
     float* R[2];         // node shift: reference density of P[0] / P[1] (nullptr otherwise)
     const float* rho0;   // initial density for the LBfluid2D-order first step (nullptr: none)
     int nx, ny, nz;      // nz = 1 for D2Q9
-    int phase;
+    int phase;           // done & 1 when the table was uploaded (a tick boundary)
     int64_t n;           // nodes
     int64_t ps;          // population plane stride (elements)
     float tau, G, rs;    // relaxation time, coupling, storage shift
     int first;           // 1 until the first step has run (device-updated)
     int64_t remaining;   // steps still to run (device-updated, 0 = idle)
-    int64_t done;        // steps run (device-updated); the current buffer is P[(done ^ phase) & 1]
+    int64_t done;        // steps run (device-updated); the current buffer is P[done & 1]
     int q;               // 9 or 19
 };
 
@@ -76,6 +76,8 @@ hipError_t init_job(int prec, int q, const Job& j, int buf, hipStream_t s);
 hipError_t job_moments(int prec, int q, const Job& j, int buf, double* rho, double* u, double* v,
                        double* w, hipStream_t s);
 hipError_t zero_u32(unsigned* p, int n, hipStream_t s);
+// b[i] = a[i] for n values (FP32 to FP64).
+hipError_t widen(const float* a, double* b, int64_t n, hipStream_t s);
 
 }  // namespace batch
 }  // namespace lbm

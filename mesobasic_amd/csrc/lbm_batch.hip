@@ -494,7 +494,7 @@ __global__ void __launch_bounds__(kMassBlock) k_job_mass(Tables t, unsigned* tic
         if (it >= *t.nitems) break;
         const Item item = t.items[it];
         const Job& job = t.jobs[item.job];
-        const int b = static_cast<int>((job.done ^ job.phase) & 1);
+        const int b = static_cast<int>(job.done & 1);
         const auto* P = static_cast<const typename SP::T*>(job.P[b]);
         const float* R = NS ? job.R[b] : nullptr;
         const int q = job.q;
@@ -596,6 +596,12 @@ __global__ void __launch_bounds__(kTP) k_job_moments(Job j, int buf, double* __r
     u[c] = sx / r;
     v[c] = sy / r;
     w[c] = (S::D == 3) ? sz / r : 0.0;
+}
+
+__global__ void k_widen(const float* __restrict__ a, double* __restrict__ b, int64_t n)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kTP + threadIdx.x;
+    if (i < n) b[i] = static_cast<double>(a[i]);
 }
 
 __global__ void k_zero(unsigned* p, int n)
@@ -710,6 +716,12 @@ hipError_t job_moments(int prec, int q, const Job& j, int buf, double* rho, doub
         if (q == 9) k_job_moments<9, SP, NS><<<blocks(j.n), kTP, 0, s>>>(j, buf, rho, u, v, w);
         else k_job_moments<19, SP, NS><<<blocks(j.n), kTP, 0, s>>>(j, buf, rho, u, v, w);
     });
+    return hipGetLastError();
+}
+
+hipError_t widen(const float* a, double* b, int64_t n, hipStream_t s)
+{
+    k_widen<<<blocks(n), kTP, 0, s>>>(a, b, n);
     return hipGetLastError();
 }
 

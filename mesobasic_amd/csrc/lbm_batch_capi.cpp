@@ -223,7 +223,9 @@ Job device_view(const HostJob& j, int prec, double rs)
     v.nx = j.d.nx;
     v.ny = j.d.ny;
     v.nz = j.d.q == 9 ? 1 : j.d.nz;
-    v.phase = 0;   // jobs join at tick boundaries, where the launch parity is 0
+    // jobs join (and resume after lbm_batch_extend) at tick boundaries, where
+    // the launch parity is 0: that launch must read the current buffer
+    v.phase = static_cast<int>(j.done & 1);
     v.n = j.n;
     v.ps = j.ps;
     v.tau = static_cast<float>(j.d.tau);
@@ -713,6 +715,85 @@ int lbm_batch_collect(lbm_batch* b, int64_t job_id, double* rho, double* u, doub
         return lbm::set_error(LBM_EJOB,
                               "job %lld failed: non-finite mass after step %lld (finite after step %lld)",
                               id, at, ok);
+    return LBM_OK;
+}
+
+int lbm_batch_read(lbm_batch* b, int64_t job_id, double* rho, double* u, double* v, double* w, double* fx,
+                   double* fy, double* fz)
+{
+    if (!b) return lbm::set_error(LBM_EINVAL, "lbm_batch_read: NULL batch");
+    auto it = b->jobs.find(job_id);
+    if (it == b->jobs.end()) return lbm::set_error(LBM_EINVAL, "unknown job %lld", (long long)job_id);
+    const HostJob& j = *it->second;
+    const int64_t n = j.n;
+    const int64_t done = j.done;
+    const bool order3 = (j.d.q == 19);   // D3Q19 jobs: scmp3D order; D2Q9: LBfluid2D order
+    // device arrays: the reference's r,u,v,w; the density the last step
+    // started from (its moments' u,v,w unused); psi; fx,fy,fz
+    double* d = nullptr;
+    BHIP(lbm::dev_malloc(&d, 12 * n * 8));
+    double *R = d, *S = d + 4 * n, *psi = d + 8 * n, *F = d + 9 * n;
+    Job view = device_view(j, b->d.precision, b->d.rho_shift);
+    const int cur = static_cast<int>(done & 1), prev = 1 - cur;
+    hipError_t e = hipSuccess;
+    if (done == 0) {
+        // rho0 and zero velocity before any step (scmp3D.cpp:63-76,
+        // LBfluid2D::setRho / setU)
+        e = lbm::batch::widen(j.rho0, R, n, b->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(R + n, 0, 3 * n * 8, b->stream);
+    } else {
+        e = lbm::batch::job_moments(b->d.precision, j.d.q, view, order3 ? prev : cur, R, R + n, R + 2 * n,
+                                    R + 3 * n, b->stream);
+    }
+    // the density the last step took its force from: rho0 before any step
+    // and, in the LBfluid2D order, for the first step; otherwise the moments
+    // of f(N-1) (in the scmp3D order the r array itself)
+    const bool force_rho0 = (done == 0) || (!order3 && done == 1);
+    if (e == hipSuccess) {
+        if (force_rho0) e = lbm::batch::widen(j.rho0, S, n, b->stream);
+        else if (order3) e = hipMemcpyAsync(S, R, n * 8, hipMemcpyDeviceToDevice, b->stream);
+        else e = lbm::batch::job_moments(b->d.precision, j.d.q, view, prev, S, S + n, S + 2 * n, S + 3 * n,
+                                         b->stream);
+    }
+    if (e == hipSuccess) e = lbm::fp64_psi_from_rho(S, psi, n, b->stream);
+    if (e == hipSuccess) {
+        lbm::StepArgs a{};
+        a.q = j.d.q;
+        a.walls = 0;
+        a.tau = j.d.tau;
+        a.G = j.d.G;
+        lbm::Grid g{j.d.nx, j.d.ny, j.d.q == 9 ? 1 : j.d.nz, n, n};
+        e = lbm::fp64_force(a, g, psi, F, F + n, F + 2 * n, b->stream);
+    }
+    double* outs[7] = {rho, u, v, w, fx, fy, fz};
+    const double* srcs[7] = {R, R + n, R + 2 * n, R + 3 * n, F, F + n, F + 2 * n};
+    for (int k = 0; k < 7 && e == hipSuccess; k++)
+        if (outs[k]) e = hipMemcpyAsync(outs[k], srcs[k], n * 8, hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return lbm::set_error(LBM_EHIP, "read: %s", hipGetErrorString(e));
+    return LBM_OK;
+}
+
+int lbm_batch_extend(lbm_batch* b, int64_t job_id, int64_t nsteps)
+{
+    if (!b) return lbm::set_error(LBM_EINVAL, "lbm_batch_extend: NULL batch");
+    if (nsteps < 1) return lbm::set_error(LBM_EINVAL, "lbm_batch_extend: nsteps %lld < 1", (long long)nsteps);
+    auto it = b->jobs.find(job_id);
+    if (it == b->jobs.end()) return lbm::set_error(LBM_EINVAL, "unknown job %lld", (long long)job_id);
+    HostJob* j = it->second.get();
+    if (j->failed)
+        return lbm::set_error(LBM_EJOB, "job %lld failed after step %lld and cannot continue", (long long)job_id,
+                              (long long)j->done);
+    if (!j->finished)
+        return lbm::set_error(LBM_ESTATE, "job %lld is still running (%lld steps left)", (long long)job_id,
+                              (long long)j->remaining);
+    j->remaining = nsteps;
+    j->d.steps = j->done + nsteps;
+    j->finished = false;
+    j->waited = 0;
+    b->order.clear();
+    b->dirty = true;   // the job rejoins the work tables at the next tick
     return LBM_OK;
 }
 

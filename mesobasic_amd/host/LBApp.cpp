@@ -5,9 +5,10 @@
 
 namespace mesobasic {
 
-LBApp::LBApp(const InputParams& in)
+CommonParams read_common_params(const InputParams& in)
 {
     // LBApp.cpp:18-31
+    CommonParams p{};
     p.NX = in("Domain/nx", 1);
     p.NY = in("Domain/ny", 1);
     p.NZ = in("Domain/nz", 1);
@@ -35,6 +36,11 @@ LBApp::LBApp(const InputParams& in)
     p.xOff = 0;
     if (p.iskip < 1 || p.jskip < 1 || p.kskip < 1)
         throw std::runtime_error("Output/iskip, jskip, kskip must be >= 1");
+    return p;
+}
+
+LBApp::LBApp(const InputParams& in) : p(read_common_params(in))
+{
     lb_object.reset(LBBaseClass::LBFactory(p, in));
     if (!lb_object)
         throw std::runtime_error("LBApp/type '" + in("LBApp/type", "mcmp2D") +

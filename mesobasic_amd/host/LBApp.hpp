@@ -11,6 +11,10 @@
 
 namespace mesobasic {
 
+// Domain/Time/Output keys into CommonParams (LBApp.cpp:18-31), one GPU
+// holding the whole lattice (rank 0 of 1).
+CommonParams read_common_params(const InputParams& in);
+
 class MesoBase {
 public:
     virtual void initSystem() = 0;

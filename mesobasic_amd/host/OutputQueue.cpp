@@ -21,16 +21,7 @@ void check(int rc, const char* what)
 
 void write_frame(const CommonParams& p, int q, int step, const Frame& f)
 {
-    if (q == 19) {
-        write_vtk_3d("vtkoutput", "rhoA", step, p.NX, p.NY, p.NZ, p.iskip, p.jskip, p.kskip, f.rho);
-        return;
-    }
-    std::vector<double> us(f.rho.size()), vs(f.rho.size());
-    for (size_t c = 0; c < f.rho.size(); c++) {
-        us[c] = f.u[c] + 0.5 * f.fx[c] / f.rho[c];   // LBfluid2D::getUStar (LBfluid2D.cpp:164-172)
-        vs[c] = f.v[c] + 0.5 * f.fy[c] / f.rho[c];
-    }
-    write_vtk_2d("vtkoutput", "rhoA", step, p.NX, p.NY, p.iskip, p.jskip, f.rho, us, vs);
+    write_model_vtk("vtkoutput", p, q, step, f.rho, f.u, f.v, f.fx, f.fy);
 }
 
 }  // namespace

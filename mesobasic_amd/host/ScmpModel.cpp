@@ -32,38 +32,56 @@ int kernel_of(const std::string& s)
 
 }  // namespace
 
-ScmpModel::ScmpModel(const CommonParams& pin, const InputParams& in, int q, int default_order)
-    : p(pin), q_(q)
+ScmpKeys ScmpKeys::read(const CommonParams& p, const InputParams& in, int q)
 {
     // scmp3D.cpp:21-31
-    G = in("LBApp/G", -6.0);
-    rhoAi = in("LBApp/rhoAi", 0.5);
-    rhoAi_noise = in("LBApp/rhoAi_noise", 0.1);
-    tauA = in("LBApp/tauA", 1.0);
+    ScmpKeys k;
+    k.G = in("LBApp/G", -6.0);
+    k.rhoAi = in("LBApp/rhoAi", 0.5);
+    k.rhoAi_noise = in("LBApp/rhoAi_noise", 0.1);
+    k.tauA = in("LBApp/tauA", 1.0);
     const std::string stype = in("LBApp/stencil", q == 19 ? "D3Q19" : "D2Q9");
     if ((q == 19 && stype != "D3Q19") || (q == 9 && stype != "D2Q9"))
         throw std::runtime_error("LBApp/stencil " + stype + " does not match this model");
-    init_kind = in("LBApp/init", "noise");
-    seed = in.has("LBApp/seed") ? static_cast<unsigned>(in("LBApp/seed", 0.0))
-                                : static_cast<unsigned>(std::time(nullptr) * (p.rank + 1));
-    R = in("LBApp/R", 0.25 * p.NX);
-    W = in("LBApp/W", 2.0);
-    rho_l = in("LBApp/rho_l", 2.0);
-    rho_v = in("LBApp/rho_v", 0.15);
+    k.init_kind = in("LBApp/init", "noise");
+    if (k.init_kind != "noise" && k.init_kind != "droplet")
+        throw std::runtime_error("LBApp/init must be noise or droplet (got " + k.init_kind + ")");
+    k.seed = in.has("LBApp/seed") ? static_cast<unsigned>(in("LBApp/seed", 0.0))
+                                  : static_cast<unsigned>(std::time(nullptr) * (p.rank + 1));
+    k.R = in("LBApp/R", 0.25 * p.NX);
+    k.W = in("LBApp/W", 2.0);
+    k.rho_l = in("LBApp/rho_l", 2.0);
+    k.rho_v = in("LBApp/rho_v", 0.15);
+    if (q == 9 && p.nz != 1) throw std::runtime_error("scmp2D needs Domain/nz = 1");
+    return k;
+}
 
+std::vector<double> ScmpKeys::initial_rho(const CommonParams& p, int q) const
+{
+    const int nz = (q == 19) ? p.nz : 1;
+    std::vector<double> rho(static_cast<size_t>(p.nx) * p.ny * nz);
+    int rc = (init_kind == "droplet")
+                 ? lbm_fill_droplet(p.nx, p.ny, nz, R, W, rho_l, rho_v, rho.data())
+                 : lbm_fill_noise(seed, static_cast<int64_t>(rho.size()), rhoAi, rhoAi_noise, rho.data());
+    if (rc != LBM_OK) throw std::runtime_error(std::string("initial density: ") + lbm_last_error());
+    return rho;
+}
+
+ScmpModel::ScmpModel(const CommonParams& pin, const InputParams& in, int q, int default_order)
+    : p(pin), q_(q), k_(ScmpKeys::read(pin, in, q))
+{
     lbm_desc d{};
     d.q = q;
     d.nx = p.nx;
     d.ny = p.ny;
     d.nz = (q == 19) ? p.nz : 1;
-    d.tau = tauA;
-    d.G = G;
+    d.tau = k_.tauA;
+    d.G = k_.G;
     d.order = in("LBApp/order", default_order);
     d.walls = in("LBApp/walls", 0);
     d.precision = precision_of(in("LBApp/precision", "fp64"));
-    d.rho_shift = in("LBApp/rho_shift", rhoAi);
+    d.rho_shift = in("LBApp/rho_shift", k_.rhoAi);
     d.kernel = kernel_of(in("LBApp/kernel", "auto"));
-    if (q == 9 && p.nz != 1) throw std::runtime_error("scmp2D needs Domain/nz = 1");
     check(lbm_create(&d, &h_), "lbm_create");
     out_.reset(new OutputQueue(p, q, in("Output/async", 1) != 0));
 }
@@ -78,16 +96,8 @@ void ScmpModel::check(int rc, const char* what) const
 void ScmpModel::initLatticeBoltzmann()
 {
     // scmp3D.cpp:54-82: initial density, zero velocity, ghost rho, force,
-    // f = feq.  The noise uses a seeded mt19937 in place of rand().
-    std::vector<double> rho(static_cast<size_t>(nodes()));
-    if (init_kind == "droplet") {
-        check(lbm_fill_droplet(p.nx, p.ny, q_ == 19 ? p.nz : 1, R, W, rho_l, rho_v, rho.data()),
-              "lbm_fill_droplet");
-    } else if (init_kind == "noise") {
-        check(lbm_fill_noise(seed, nodes(), rhoAi, rhoAi_noise, rho.data()), "lbm_fill_noise");
-    } else {
-        throw std::runtime_error("LBApp/init must be noise or droplet (got " + init_kind + ")");
-    }
+    // f = feq
+    const std::vector<double> rho = k_.initial_rho(p, q_);
     check(lbm_set_rho(h_, rho.data()), "lbm_set_rho");
     check(lbm_init_equilibrium(h_), "lbm_init_equilibrium");
 }

@@ -29,6 +29,20 @@
 
 namespace mesobasic {
 
+// The keys scmp3D reads (scmp3D.cpp:21-31) with the same defaults, the
+// keys of this build listed above, and the initial density they give.
+struct ScmpKeys {
+    double G, rhoAi, rhoAi_noise, tauA;
+    std::string init_kind;
+    unsigned seed;
+    double R, W, rho_l, rho_v;
+
+    static ScmpKeys read(const CommonParams& p, const InputParams& in, int q);
+    // rho0 of the lattice, interior order (scmp3D.cpp:63-76 with a seeded
+    // mt19937 in place of rand(), or the droplet of LBApp/init = droplet)
+    std::vector<double> initial_rho(const CommonParams& p, int q) const;
+};
+
 class ScmpModel : public LBBaseClass {
 public:
     ScmpModel(const CommonParams& p, const InputParams& in, int q, int default_order);
@@ -55,10 +69,7 @@ protected:
     const CommonParams& p;
     int q_;
     int current_step = 0;
-    double G, rhoAi, rhoAi_noise, tauA;
-    std::string init_kind;
-    unsigned seed;
-    double R, W, rho_l, rho_v;
+    ScmpKeys k_;
     lbm_handle* h_ = nullptr;
     std::unique_ptr<OutputQueue> out_;
 

@@ -84,4 +84,21 @@ void write_vtk_2d(const std::string& dir, const std::string& tag, int step, int 
             }
 }
 
+void write_model_vtk(const std::string& dir, const CommonParams& p, int q, int step,
+                     const std::vector<double>& rho, const std::vector<double>& u,
+                     const std::vector<double>& v, const std::vector<double>& fx,
+                     const std::vector<double>& fy)
+{
+    if (q == 19) {
+        write_vtk_3d(dir, "rhoA", step, p.NX, p.NY, p.NZ, p.iskip, p.jskip, p.kskip, rho);
+        return;
+    }
+    std::vector<double> us(rho.size()), vs(rho.size());
+    for (size_t c = 0; c < rho.size(); c++) {
+        us[c] = u[c] + 0.5 * fx[c] / rho[c];
+        vs[c] = v[c] + 0.5 * fy[c] / rho[c];
+    }
+    write_vtk_2d(dir, "rhoA", step, p.NX, p.NY, p.iskip, p.jskip, rho, us, vs);
+}
+
 }  // namespace mesobasic

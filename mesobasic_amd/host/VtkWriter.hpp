@@ -7,6 +7,8 @@
 #include <string>
 #include <vector>
 
+#include "CommonParams.h"
+
 namespace mesobasic {
 
 // rho: interior array (x slowest, z fastest) of NX*NY*NZ values.
@@ -17,5 +19,14 @@ void write_vtk_3d(const std::string& dir, const std::string& tag, int step, int 
 void write_vtk_2d(const std::string& dir, const std::string& tag, int step, int NX, int NY,
                   int iskip, int jskip, const std::vector<double>& rho,
                   const std::vector<double>& ustar, const std::vector<double>& vstar);
+
+// The output file of one step of a model: for D3Q19 (q = 19) the density
+// (scmp3D.cpp:127-133); for D2Q9 the density and the Shan-Chen velocity
+// u + F/(2 rho) (mcmp2D.cpp:131-137, LBfluid2D::getUStar, LBfluid2D.cpp:
+// 164-172).  Written as dir/rhoA_<step>.vtk.
+void write_model_vtk(const std::string& dir, const CommonParams& p, int q, int step,
+                     const std::vector<double>& rho, const std::vector<double>& u,
+                     const std::vector<double>& v, const std::vector<double>& fx,
+                     const std::vector<double>& fy);
 
 }  // namespace mesobasic

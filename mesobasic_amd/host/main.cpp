@@ -3,6 +3,14 @@
 //
 //   meso_amd [input.dat]            run the simulation in the current directory
 //   meso_amd --print-params FILE    print the parameters the run would use
+//   meso_amd --serve [options] DIR...
+//                                   run every DIR/input.dat (scmp3D, scmp2D)
+//                                   together on the batch engine, each
+//                                   writing DIR/vtkoutput at its own cadence;
+//                                   options: --precision fp32|fp16|bf16|fp16n|
+//                                   bf16n (default fp32), --steps-per-tick K,
+//                                   --hbm-gib G, --work-limit NODES,
+//                                   --max-wait TICKS, --rho-shift X, --quiet
 //
 // Output: vtkoutput/rhoA_<step>.vtk at step 0 and every nstep/numOutputs
 // steps, as the reference writes them; a timing summary on stderr.
@@ -18,6 +26,8 @@
 #include <string>
 #include <vector>
 
+#include "../../include/mesobasic_lbm.h"
+#include "BatchServer.hpp"
 #include "InputParams.hpp"
 #include "LBApp.hpp"
 
@@ -48,12 +58,53 @@ int print_params(const char* path)
     return 0;
 }
 
+int serve(int argc, char** argv)
+{
+    ServeOptions opt;
+    std::vector<std::string> dirs;
+    for (int i = 2; i < argc; i++) {
+        const std::string a = argv[i];
+        auto value = [&](const char* name) -> std::string {
+            if (i + 1 >= argc) throw std::runtime_error(std::string(name) + " needs a value");
+            return argv[++i];
+        };
+        if (a == "--precision") {
+            const std::string v = value("--precision");
+            if (v == "fp32") opt.precision = LBM_FP32;
+            else if (v == "fp16") opt.precision = LBM_FP16S;
+            else if (v == "bf16") opt.precision = LBM_BF16S;
+            else if (v == "fp16n") opt.precision = LBM_FP16N;
+            else if (v == "bf16n") opt.precision = LBM_BF16N;
+            else throw std::runtime_error("--precision must be fp32, fp16, bf16, fp16n or bf16n");
+        } else if (a == "--steps-per-tick") {
+            opt.steps_per_tick = std::stoi(value("--steps-per-tick"));
+        } else if (a == "--hbm-gib") {
+            opt.hbm_budget = static_cast<int64_t>(std::stod(value("--hbm-gib")) * (1ll << 30));
+        } else if (a == "--work-limit") {
+            opt.work_limit = std::stoll(value("--work-limit"));
+        } else if (a == "--max-wait") {
+            opt.max_wait = std::stoi(value("--max-wait"));
+        } else if (a == "--rho-shift") {
+            opt.rho_shift = std::stod(value("--rho-shift"));
+        } else if (a == "--quiet") {
+            opt.quiet = true;
+        } else if (a.size() > 1 && a[0] == '-' && a[1] == '-') {
+            throw std::runtime_error("unknown option " + a);
+        } else {
+            dirs.push_back(a);
+        }
+    }
+    if (dirs.empty()) throw std::runtime_error("--serve needs at least one run directory");
+    return serve_runs(dirs, opt) == 0 ? 0 : 1;
+}
+
 }  // namespace
 
 int main(int argc, char** argv)
 {
     try {
         if (argc == 3 && std::strcmp(argv[1], "--print-params") == 0) return print_params(argv[2]);
+        if (argc >= 2 && std::strcmp(argv[1], "--serve") == 0) return serve(argc, argv);
         const char* path = argc > 1 ? argv[1] : "input.dat";
         InputParams in(path);
         prepare_output_dir();

@@ -128,6 +128,8 @@ def load():
         "lbm_set_f_component": (ctypes.c_int, [P, ctypes.c_int, _D]),
         "lbm_conservation_component": (ctypes.c_int, [P, ctypes.c_int, _D]),
         "lbm_probe_memcpy": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, _D]),
+        "lbm_batch_read": (ctypes.c_int, [P, ctypes.c_int64] + [_D] * 7),
+        "lbm_batch_extend": (ctypes.c_int, [P, ctypes.c_int64, ctypes.c_int64]),
         "lbm_snapshot_begin": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),
         "lbm_snapshot_wait": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int64)] + [_D] * 7),
         "lbm_probe_tile_map": (ctypes.c_int, [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_int)] * 3),
@@ -414,6 +416,20 @@ class Batch:
         """Drops an admitted job (running or finished) and frees its memory."""
         _check(self._lib.lbm_batch_cancel(self._h, int(job_id)))
         self.shapes.pop(job_id, None)
+
+    def read(self, job_id):
+        """The reference's r,u,v,w and fx,fy,fz arrays of an admitted job at
+        the steps it has run (the job stays admitted)."""
+        shape = self.shapes[job_id]
+        n = int(np.prod(shape))
+        arrs = [np.empty(n) for _ in range(7)]
+        _check(self._lib.lbm_batch_read(self._h, int(job_id), *(_dptr(a) for a in arrs)))
+        names = ("rho", "u", "v", "w", "fx", "fy", "fz")
+        return {k: a.reshape(shape) for k, a in zip(names, arrs)}
+
+    def extend(self, job_id, nsteps):
+        """Gives a finished job nsteps more steps."""
+        _check(self._lib.lbm_batch_extend(self._h, int(job_id), int(nsteps)))
 
     def tick(self):
         n = ctypes.c_int()
