@@ -61,6 +61,19 @@ def _load():
     lib.mcmp_oracle_get.argtypes = [ctypes.c_void_p, ctypes.c_int] + [_D] * 5
     lib.mcmp_oracle_get_force.argtypes = [ctypes.c_void_p, ctypes.c_int] + [_D] * 3
     lib.mcmp_oracle_moments.argtypes = [ctypes.c_void_p, ctypes.c_int] + [_D] * 4
+    lib.capsule_oracle_create.restype = ctypes.c_void_p
+    lib.capsule_oracle_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int] + \
+        [ctypes.c_double] * 4
+    lib.capsule_oracle_destroy.argtypes = [ctypes.c_void_p]
+    lib.capsule_oracle_set_rho.argtypes = [ctypes.c_void_p, _D]
+    lib.capsule_oracle_set_nodes.argtypes = [ctypes.c_void_p, _D, _D]
+    lib.capsule_oracle_move_nodes.argtypes = [ctypes.c_void_p, _D, _D]
+    lib.capsule_oracle_circle.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 3
+    lib.capsule_oracle_init.argtypes = [ctypes.c_void_p]
+    lib.capsule_oracle_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.capsule_oracle_get.argtypes = [ctypes.c_void_p] + [_D] * 6
+    lib.capsule_oracle_get_nodes.argtypes = [ctypes.c_void_p] + [_D] * 6
+    lib.capsule_oracle_get_ghost_force.argtypes = [ctypes.c_void_p] + [_D] * 2
     _lib = lib
     return lib
 
@@ -209,6 +222,81 @@ class MCOracle:
         self._lib.mcmp_oracle_moments(self._h, int(comp), _ptr(rho), _ptr(u), _ptr(v), _ptr(w))
         sh = self.shape
         return dict(rho=rho.reshape(sh), u=u.reshape(sh), v=v.reshape(sh), w=w.reshape(sh))
+
+
+class CapsuleOracle:
+    """Immersed-boundary capsule (capsule2D) on the CPU, reference semantics:
+    a D2Q9 fluid of relaxation time tau in the LBfluid2D step order with
+    bounce-back walls in y and no Shan-Chen force, and nnodes IB nodes tied
+    to their rest positions with stiffness kstiff.  body: the body force on
+    the rows between the walls (the reference's is (0.000008, 0))."""
+
+    def __init__(self, shape, tau, nnodes, kstiff=1.0, dt=1.0, body=(0.000008, 0.0)):
+        lib = _load()
+        self.shape = (int(shape[0]), int(shape[1]))
+        self.n = self.shape[0] * self.shape[1]
+        self.nnodes = int(nnodes)
+        self._h = lib.capsule_oracle_create(self.shape[0], self.shape[1], float(tau), self.nnodes,
+                                            float(kstiff), float(dt), float(body[0]), float(body[1]))
+        if not self._h:
+            raise ValueError("oracle: bad capsule description")
+        self._lib = lib
+
+    def close(self):
+        if self._h:
+            self._lib.capsule_oracle_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def set_rho(self, rho0: np.ndarray):
+        a = np.ascontiguousarray(rho0, dtype=np.float64).reshape(-1)
+        assert a.size == self.n
+        self._lib.capsule_oracle_set_rho(self._h, _ptr(a))
+
+    def set_nodes(self, x, y):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        assert x.size == self.nnodes and y.size == self.nnodes
+        self._lib.capsule_oracle_set_nodes(self._h, _ptr(x), _ptr(y))
+
+    def move_nodes(self, x, y):
+        """Node positions only; the rest positions stay."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        assert x.size == self.nnodes and y.size == self.nnodes
+        self._lib.capsule_oracle_move_nodes(self._h, _ptr(x), _ptr(y))
+
+    def circle(self, xo, yo, rad):
+        self._lib.capsule_oracle_circle(self._h, float(xo), float(yo), float(rad))
+
+    def init(self):
+        self._lib.capsule_oracle_init(self._h)
+
+    def step(self, nsteps: int = 1):
+        self._lib.capsule_oracle_step(self._h, int(nsteps))
+
+    def fields(self):
+        """The fluid's r, u, v, force arrays and f."""
+        rho, u, v, fx, fy = (np.empty(self.n) for _ in range(5))
+        f = np.empty(self.n * 9)
+        self._lib.capsule_oracle_get(self._h, _ptr(rho), _ptr(u), _ptr(v), _ptr(f), _ptr(fx), _ptr(fy))
+        sh = self.shape
+        return dict(rho=rho.reshape(sh), u=u.reshape(sh), v=v.reshape(sh), fx=fx.reshape(sh),
+                    fy=fy.reshape(sh), f=f.reshape(sh + (9,)))
+
+    def nodes(self):
+        """IB node positions x, y, velocities vx, vy and membrane forces fx, fy."""
+        arrs = [np.empty(self.nnodes) for _ in range(6)]
+        self._lib.capsule_oracle_get_nodes(self._h, *(_ptr(a) for a in arrs))
+        return dict(zip(("x", "y", "vx", "vy", "fx", "fy"), arrs))
+
+    def ghost_force(self):
+        """Force accumulated on the ghost rows j = 0 and j = ny+1: (gfx, gfy),
+        each of shape (2, nx)."""
+        gfx, gfy = np.empty(2 * self.shape[0]), np.empty(2 * self.shape[0])
+        self._lib.capsule_oracle_get_ghost_force(self._h, _ptr(gfx), _ptr(gfy))
+        return gfx.reshape(2, -1), gfy.reshape(2, -1)
 
 
 def mt19937(seed: int, n: int) -> np.ndarray:

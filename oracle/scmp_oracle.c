@@ -753,3 +753,221 @@ void mcmp_oracle_moments(const mcmp_oracle* m, int comp, double* rho, double* u,
 {
     scmp_oracle_moments(m->c[comp & 1], rho, u, v, w);
 }
+
+/* ---- immersed-boundary capsule (capsule2D) ----------------------------------
+ * capsule2D.cpp:84-150 with IBcapsules2D.cpp:130-484: one D2Q9 fluid in the
+ * LBfluid2D step order with bounce-back walls in y (yBC = 1), no Shan-Chen
+ * force; its force arrays hold a body force plus the membrane forces of the
+ * IB nodes spread with the bilinear kernel; the nodes move with the
+ * interpolated Shan-Chen velocity u* = u + F/(2 rho). */
+struct capsule_oracle {
+    scmp_oracle* fl;
+    int nnodes;
+    double kstiff, dt, body_fx, body_fy;
+    double *x, *y, *x0, *y0, *vx, *vy, *nfx, *nfy;
+};
+
+capsule_oracle* capsule_oracle_create(int nx, int ny, double tau, int nnodes, double kstiff,
+                                      double dt, double body_fx, double body_fy)
+{
+    if (nnodes < 1) return NULL;
+    capsule_oracle* c = (capsule_oracle*)calloc(1, sizeof(*c));
+    if (!c) return NULL;
+    c->fl = scmp_oracle_create(9, nx, ny, 1, tau, 0.0, 2, 1, 1);
+    c->nnodes = nnodes;
+    c->kstiff = kstiff;
+    c->dt = dt;
+    c->body_fx = body_fx;
+    c->body_fy = body_fy;
+    double** arrs[8] = {&c->x, &c->y, &c->x0, &c->y0, &c->vx, &c->vy, &c->nfx, &c->nfy};
+    int ok = c->fl != NULL;
+    for (int a = 0; a < 8; a++) {
+        *arrs[a] = (double*)calloc((size_t)nnodes, sizeof(double));
+        ok = ok && *arrs[a];
+    }
+    if (!ok) {
+        capsule_oracle_destroy(c);
+        return NULL;
+    }
+    return c;
+}
+
+void capsule_oracle_destroy(capsule_oracle* c)
+{
+    if (!c) return;
+    scmp_oracle_destroy(c->fl);
+    free(c->x); free(c->y); free(c->x0); free(c->y0);
+    free(c->vx); free(c->vy); free(c->nfx); free(c->nfy);
+    free(c);
+}
+
+void capsule_oracle_set_rho(capsule_oracle* c, const double* rho0)
+{
+    /* capsule2D.cpp:92-107: rho, u = v = 0 and fx = fy = 0 on the interior */
+    scmp_oracle_set_rho(c->fl, rho0);
+}
+
+void capsule_oracle_set_nodes(capsule_oracle* c, const double* x, const double* y)
+{
+    for (int n = 0; n < c->nnodes; n++) {
+        c->x[n] = x[n];
+        c->y[n] = y[n];
+        c->x0[n] = x[n];
+        c->y0[n] = y[n];
+    }
+}
+
+void capsule_oracle_move_nodes(capsule_oracle* c, const double* x, const double* y)
+{
+    for (int n = 0; n < c->nnodes; n++) {
+        c->x[n] = x[n];
+        c->y[n] = y[n];
+    }
+}
+
+void capsule_oracle_circle(capsule_oracle* c, double xo, double yo, double rad)
+{
+    /* IBcapsules2D::initCircularCapsule, IBcapsules2D.cpp:130-141 */
+    double theta = 0.0;
+    const double dTheta = (2.0 * 3.14159265) / (double)c->nnodes;
+    for (int n = 0; n < c->nnodes; n++) {
+        c->x[n] = xo + rad * cos(theta);
+        c->y[n] = yo + rad * sin(theta);
+        c->x0[n] = c->x[n];
+        c->y0[n] = c->y[n];
+        theta += dTheta;
+    }
+}
+
+void capsule_oracle_init(capsule_oracle* c)
+{
+    /* capsule2D.cpp:118: setFtoFeq with the zero forces of the start */
+    set_f_to_feq(c->fl);
+}
+
+/* Footprint origin of an IB node: i0 = int(floor(x)) + 1 and the same in y,
+ * ghost-shell indices (IBcapsules2D.cpp:238-239, 352-353).  Returns 0 when
+ * the reference skips the node, isMyNode(i0) false on one rank
+ * (IBcapsules2D.cpp:418-426: 0 <= i0 <= nx), which covers every non-finite
+ * x.  A node whose y leaves [-1, ny) would index outside the ghost shell in
+ * the reference (no y check there); such a node is skipped here. */
+static int ib_origin(const scmp_oracle* h, double x, double y, int* i0, int* j0)
+{
+    if (!(x >= -1.0 && x < (double)h->nx)) return 0;
+    if (!(y >= -1.0 && y < (double)h->ny)) return 0;
+    *i0 = (int)floor(x) + 1;
+    *j0 = (int)floor(y) + 1;
+    return 1;
+}
+
+static void capsule_step(capsule_oracle* c)
+{
+    scmp_oracle* h = c->fl;
+    /* IBcapsules2D::computeNodeForces, IBcapsules2D.cpp:477-484 */
+    for (int n = 0; n < c->nnodes; n++) {
+        c->nfx[n] = -c->kstiff * (c->x[n] - c->x0[n]);
+        c->nfy[n] = -c->kstiff * (c->y[n] - c->y0[n]);
+    }
+    /* LBfluid2D::zeroForces (interior only, LBfluid2D.cpp:137-147), then the
+     * body force on rows j = 2 .. ny-1 (capsule2D.cpp:128-136) */
+    for (int i = 1; i < h->nx + 1; i++)
+        for (int j = 1; j < h->ny + 1; j++) {
+            int64_t ndx = nd(h, i, j, 0);
+            h->fx[ndx] = 0.0;
+            h->fy[ndx] = 0.0;
+        }
+    for (int i = 1; i < h->nx + 1; i++)
+        for (int j = 2; j < h->ny; j++) {
+            int64_t ndx = nd(h, i, j, 0);
+            h->fx[ndx] = c->body_fx;
+            h->fy[ndx] = c->body_fy;
+        }
+    /* IBcapsules2D::extrapolateForce, IBcapsules2D.cpp:339-386: bilinear
+     * spread onto i0..i0+1 (inside 1..nx) and j0..j0+1 (ghost rows
+     * included: they are never zeroed, so what lands there adds up over the
+     * steps) */
+    for (int n = 0; n < c->nnodes; n++) {
+        int i0, j0;
+        if (!ib_origin(h, c->x[n], c->y[n], &i0, &j0)) continue;
+        for (int i = i0; i <= i0 + 1; i++) {
+            if (i < 1 || i > h->nx) continue;
+            for (int j = j0; j <= j0 + 1; j++) {
+                int64_t ndx = nd(h, i, j, 0);
+                double rx = c->x[n] - (double)(i - 1);
+                double ry = c->y[n] - (double)(j - 1);
+                double del = (1.0 - fabs(rx)) * (1.0 - fabs(ry));
+                h->fx[ndx] += del * c->nfx[n];
+                h->fy[ndx] += del * c->nfy[n];
+            }
+        }
+    }
+    /* LBfluid2D::updateFluid_SC with xBC = 0, yBC = 1 and no ghost rho
+     * (capsule2D.cpp:142-145, LBfluid2D.cpp:227-312) */
+    collide_stream(h);
+    bounce_back(h);
+    macros(h);
+    /* IBcapsules2D::interpolateVelocity, IBcapsules2D.cpp:218-287 (the
+     * Allreduce of one rank is the identity), with getUStar
+     * (LBfluid2D.cpp:164-172); ghost rows hold r = u = v = 0 */
+    for (int n = 0; n < c->nnodes; n++) {
+        c->vx[n] = 0.0;
+        c->vy[n] = 0.0;
+        int i0, j0;
+        if (!ib_origin(h, c->x[n], c->y[n], &i0, &j0)) continue;
+        for (int i = i0; i <= i0 + 1; i++) {
+            if (i < 1 || i > h->nx) continue;
+            for (int j = j0; j <= j0 + 1; j++) {
+                int64_t ndx = nd(h, i, j, 0);
+                double rx = c->x[n] - (double)(i - 1);
+                double ry = c->y[n] - (double)(j - 1);
+                double del = (1.0 - fabs(rx)) * (1.0 - fabs(ry));
+                double us = h->u[ndx] + 0.5 * h->fx[ndx] / h->r[ndx];
+                double vs = h->v[ndx] + 0.5 * h->fy[ndx] / h->r[ndx];
+                c->vx[n] += del * us;
+                c->vy[n] += del * vs;
+            }
+        }
+    }
+    /* IBcapsules2D::updateNodePositions, IBcapsules2D.cpp:463-469 */
+    for (int n = 0; n < c->nnodes; n++) {
+        c->x[n] += c->vx[n] * c->dt;
+        c->y[n] += c->vy[n] * c->dt;
+    }
+}
+
+void capsule_oracle_step(capsule_oracle* c, int nsteps)
+{
+    for (int s = 0; s < nsteps; s++) capsule_step(c);
+}
+
+void capsule_oracle_get(const capsule_oracle* c, double* rho, double* u, double* v, double* f,
+                        double* fx, double* fy)
+{
+    scmp_oracle_get(c->fl, rho, u, v, NULL, f);
+    scmp_oracle_get_force(c->fl, fx, fy, NULL);
+}
+
+void capsule_oracle_get_nodes(const capsule_oracle* c, double* x, double* y, double* vx, double* vy,
+                              double* fx, double* fy)
+{
+    for (int n = 0; n < c->nnodes; n++) {
+        if (x) x[n] = c->x[n];
+        if (y) y[n] = c->y[n];
+        if (vx) vx[n] = c->vx[n];
+        if (vy) vy[n] = c->vy[n];
+        if (fx) fx[n] = c->nfx[n];
+        if (fy) fy[n] = c->nfy[n];
+    }
+}
+
+void capsule_oracle_get_ghost_force(const capsule_oracle* c, double* gfx, double* gfy)
+{
+    /* rows j = 0 and j = ny+1 of the force arrays, i = 1..nx */
+    const scmp_oracle* h = c->fl;
+    for (int s = 0; s < 2; s++)
+        for (int i = 1; i < h->nx + 1; i++) {
+            int64_t ndx = nd(h, i, s == 0 ? 0 : h->ny + 1, 0);
+            gfx[s * h->nx + (i - 1)] = h->fx[ndx];
+            gfy[s * h->nx + (i - 1)] = h->fy[ndx];
+        }
+}

@@ -117,6 +117,36 @@ void mcmp_oracle_get_force(const mcmp_oracle* m, int comp, double* fx, double* f
 void mcmp_oracle_moments(const mcmp_oracle* m, int comp, double* rho, double* u, double* v,
                          double* w);
 
+/* ---- immersed-boundary capsule (capsule2D) ---------------------------------
+ * One D2Q9 fluid of relaxation time tau in the LBfluid2D step order with
+ * bounce-back walls in y and no Shan-Chen force; nnodes IB nodes tied to
+ * their rest positions by springs of stiffness kstiff, spread onto the fluid
+ * and moved with the interpolated velocity (capsule2D.cpp:84-150,
+ * IBcapsules2D.cpp:130-484).  body_fx, body_fy: the body force on the rows
+ * between the walls (capsule2D.cpp:128-136 uses 0.000008, 0). */
+typedef struct capsule_oracle capsule_oracle;
+capsule_oracle* capsule_oracle_create(int nx, int ny, double tau, int nnodes, double kstiff,
+                                      double dt, double body_fx, double body_fy);
+void capsule_oracle_destroy(capsule_oracle* c);
+void capsule_oracle_set_rho(capsule_oracle* c, const double* rho0);
+/* Node positions, which are also the rest positions. */
+void capsule_oracle_set_nodes(capsule_oracle* c, const double* x, const double* y);
+/* Node positions only (rest positions kept): a displaced start. */
+void capsule_oracle_move_nodes(capsule_oracle* c, const double* x, const double* y);
+/* IBcapsules2D::initCircularCapsule(xo, yo, rad). */
+void capsule_oracle_circle(capsule_oracle* c, double xo, double yo, double rad);
+void capsule_oracle_init(capsule_oracle* c);
+void capsule_oracle_step(capsule_oracle* c, int nsteps);
+/* The fluid's r,u,v arrays, f ([node][9]) and force arrays (interior). */
+void capsule_oracle_get(const capsule_oracle* c, double* rho, double* u, double* v, double* f,
+                        double* fx, double* fy);
+/* Node positions, velocities (last interpolation) and membrane forces. */
+void capsule_oracle_get_nodes(const capsule_oracle* c, double* x, double* y, double* vx, double* vy,
+                              double* fx, double* fy);
+/* Force accumulated on the ghost rows below and above the lattice (2*nx
+ * values each: row j = 0, then row j = ny+1). */
+void capsule_oracle_get_ghost_force(const capsule_oracle* c, double* gfx, double* gfy);
+
 #ifdef __cplusplus
 }
 #endif

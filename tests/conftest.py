@@ -27,9 +27,11 @@ def load_fixture(name):
 # small cases with full fields; "full_*" files hold hashes and norms of the
 # full-size runs instead
 FIXTURES = sorted(f[:-4] for f in os.listdir(GOLDEN)
-                  if f.endswith(".npz") and not f.startswith(("full_", "mcmp")))
+                  if f.endswith(".npz") and not f.startswith(("full_", "mcmp", "capsule")))
 # two-component fixtures (reference mcmp2D / mcmp3D)
 MC_FIXTURES = sorted(f[:-4] for f in os.listdir(GOLDEN) if f.endswith(".npz") and f.startswith("mcmp"))
+# immersed-boundary capsule fixtures (reference capsule2D / IBcapsules2D)
+CAPSULE_FIXTURES = sorted(f[:-4] for f in os.listdir(GOLDEN) if f.endswith(".npz") and f.startswith("capsule"))
 FULL_FIXTURES = sorted(f[:-4] for f in os.listdir(GOLDEN) if f.endswith(".npz") and f.startswith("full_"))
 
 

@@ -13,6 +13,13 @@ Two-component cases (MC_CASES, names mcmp*) come from the reference's
 mcmp2D / mcmp3D classes through oracle/_ref/ref_harness_mcmp: inputs rhoA0,
 rhoB0; per step and component c in (A, B): rho_c_s, u_c_s, v_c_s, w_c_s,
 fx_c_s, fy_c_s, fz_c_s (the force arrays) and, where stored, f_c_s.
+
+Capsule cases (CAPSULE_CASES, names capsule*) come from the reference's
+capsule2D / IBcapsules2D classes through oracle/_ref/ref_harness_capsule (one
+rank): input rho0 and the circle (xo, yo, rad); per step s: rho_s, u_s, v_s,
+fx_s, fy_s (the fluid's arrays), the IB node arrays x_s, y_s, vx_s, vy_s,
+nfx_s, nfy_s, the ghost-row forces gfx_s, gfy_s ((2, nx): rows j = 0 and
+j = ny+1) and, where stored, f_s.
 """
 import json
 import os
@@ -29,6 +36,7 @@ import oracle  # noqa: E402
 
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
 HARNESS_MC = os.path.join(ROOT, "oracle", "_ref", "ref_harness_mcmp")
+HARNESS_CAP = os.path.join(ROOT, "oracle", "_ref", "ref_harness_capsule")
 MPIRUN = "/opt/conda/bin/mpirun"
 
 # name, q, shape, tau, G, walls, init, steps, steps_with_f, ranks
@@ -63,6 +71,50 @@ MC_CASES = [
     ("mcmpfilm3d_6x5x8", 19, (6, 5, 8), 1.0, 0.9, 3.0, ("noise", 29), [0, 1, 2, 30], [1, 30], 2,
      (0.6, 0.4)),
 ]
+
+
+# name, shape, tau, nnodes, kstiff, (xo, yo, rad), noise seed, steps, steps_with_f
+CAPSULE_CASES = [
+    ("capsule_48x40", (48, 40), 1.0, 24, 1.0, (20.0, 20.0, 6.0), 31, [0, 1, 2, 50, 300], [1, 300]),
+    ("capsule_40x24_tau08", (40, 24), 0.8, 20, 0.5, (15.3, 11.7, 5.0), 33, [1, 3, 150], [150]),
+    # a capsule reaching into the ghost row below the lower wall: its nodes
+    # there spread force onto the ghost row and read r = 0 back (non-finite
+    # velocities, after which the reference skips them)
+    ("capsule_30x16_wall", (30, 16), 1.2, 16, 2.0, (10.0, 3.0, 3.5), 35, [1, 2, 5, 40], [40]),
+]
+
+
+def run_capsule_case(name, shape, tau, nnodes, kstiff, circle, seed, steps, steps_f):
+    nx, ny = shape
+    n = nx * ny
+    rho0 = oracle.noise_rho(shape, seed, mean=0.5, amp=0.1).reshape(-1)
+    out = {"rho0": rho0}
+    with tempfile.TemporaryDirectory() as td:
+        pre = os.path.join(td, name)
+        rho0.tofile(pre + "_rho0.bin")
+        cmd = [MPIRUN, "-np", "1", HARNESS_CAP, str(nx), str(ny), repr(tau), str(nnodes), repr(kstiff),
+               repr(circle[0]), repr(circle[1]), repr(circle[2]), pre + "_rho0.bin", pre,
+               ",".join(map(str, steps))]
+        subprocess.run(cmd, check=True)
+        for s in steps:
+            a = np.fromfile(f"{pre}_{s}.bin")
+            for k, key in enumerate(("rho", "u", "v", "fx", "fy")):
+                out[f"{key}_{s}"] = a[k * n:(k + 1) * n]
+            if s in steps_f:
+                out[f"f_{s}"] = a[5 * n:14 * n]
+            off = 14 * n
+            for key in ("x", "y", "vx", "vy", "nfx", "nfy"):
+                out[f"{key}_{s}"] = a[off:off + nnodes]
+                off += nnodes
+            out[f"gfx_{s}"] = a[off:off + 2 * nx].reshape(2, nx)
+            out[f"gfy_{s}"] = a[off + 2 * nx:off + 4 * nx].reshape(2, nx)
+            assert off + 4 * nx == a.size
+    meta = dict(shape=list(shape), tau=tau, nnodes=nnodes, kstiff=kstiff, circle=list(circle), seed=seed,
+                steps=steps, steps_f=steps_f, body=[0.000008, 0.0], dt=1.0,
+                source="reference capsule2D / IBcapsules2D classes via oracle/_ref/ref_harness_capsule")
+    out["meta"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    print(name, "ok")
 
 
 def mc_inputs(shape, init):
@@ -157,6 +209,9 @@ def main():
     for c in MC_CASES:
         if not only or c[0] in only:
             run_mc_case(*c)
+    for c in CAPSULE_CASES:
+        if not only or c[0] in only:
+            run_capsule_case(*c)
 
 
 if __name__ == "__main__":
