@@ -193,6 +193,40 @@ int lbm_set_f_component(lbm_handle* h, int comp, const double* f);
 /* FP64 totals of one component's current f, as lbm_conservation. */
 int lbm_conservation_component(lbm_handle* h, int comp, double out[4]);
 
+/* ---- immersed-boundary capsule ----------------------------------------------
+ * The reference's capsule2D model (capsule2D.cpp:84-150 with IBcapsules2D,
+ * IBcapsules2D.cpp:130-484): a D2Q9 fluid in the LBfluid2D step order (a
+ * handle with q = 9, order 2, one component, FP64; walls = 1 gives the
+ * model's bounce-back walls in y) whose force arrays hold, in place of the
+ * Shan-Chen force (G is not used), a body force on the rows between the
+ * walls plus the membrane forces of nnodes IB nodes, F = -kstiff (x - x0),
+ * spread with the bilinear kernel; after each fluid step the nodes move with
+ * the interpolated velocity u* = u + F/(2 rho), x += u* dt.
+ * lbm_get_force returns the force arrays of the last step (zero at the
+ * start).  Nodes with a non-finite x, or x outside [-1, nx), are skipped as
+ * the reference skips them; y must stay in [-1, ny) (beyond it the
+ * reference indexes outside its ghost shell; such nodes are skipped here). */
+typedef struct lbm_capsule_desc {
+    int nnodes;        /* IB nodes ("LBApp/nnodes", IBcapsules2D.cpp:46) */
+    double kstiff;     /* membrane stiffness ("LBApp/kstiff", :48) */
+    double dt;         /* time step of the node move ("Time/dt", :25) */
+    double body_fx;    /* body force (capsule2D.cpp:132-133: 0.000008, 0) */
+    double body_fy;
+} lbm_capsule_desc;
+
+/* Attaches a capsule to a handle before lbm_init_equilibrium; the start is
+ * then f = feq(rho0, 0) with zero forces (capsule2D.cpp:92-118). */
+int lbm_capsule_attach(lbm_handle* h, const lbm_capsule_desc* d);
+/* Node positions (host, nnodes each), which are also the rest positions x0
+ * (IBcapsules2D::initCircularCapsule sets both, IBcapsules2D.cpp:130-141).
+ * Required before the first step. */
+int lbm_capsule_set_nodes(lbm_handle* h, const double* x, const double* y);
+/* Node positions, the velocities of the last interpolation and the membrane
+ * forces of the last step (host arrays of nnodes doubles; any may be NULL). */
+int lbm_capsule_get_nodes(lbm_handle* h, double* x, double* y, double* vx, double* vy, double* fx, double* fy);
+/* IB nodes of the attached capsule (0: none). */
+int lbm_capsule_nodes(const lbm_handle* h);
+
 /* ---- asynchronous snapshot -----------------------------------------------------
  * The output step of the reference models (scmp3D::outputLatticeBoltzmann,
  * scmp3D.cpp:185-245; mcmp2D.cpp:255-300) reads r, u and fx of the lattice

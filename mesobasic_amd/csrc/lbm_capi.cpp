@@ -126,9 +126,29 @@ int run_step2(lbm_handle* h)
     return LBM_OK;
 }
 
+// One capsule2D step: force (body + spread IB forces) and collision, then
+// the IB nodes' velocities and move (capsule2D.cpp:123-150).
+int run_step_capsule(lbm_handle* h)
+{
+    if (!h->capsule_nodes_set) return fail(LBM_ESTATE, "capsule nodes not set (lbm_capsule_set_nodes)");
+    const int src = h->cur, dst = 1 - h->cur;
+    auto* Pin = static_cast<const double*>(h->buf[src]);
+    auto* Pout = static_cast<double*>(h->buf[dst]);
+    const int mode = h->first_step_stored ? 1 : 0;
+    HIP_OK(lbm::fp64_capsule_collide(h->a, h->g, Pin, h->kind[src], Pout, h->rho0, mode, h->cap, h->stream));
+    HIP_OK(lbm::fp64_capsule_finish(h->a, h->g, Pout, lbm::kHoldsPost, h->cap, h->stream));
+    h->kind[dst] = lbm::kHoldsPost;
+    h->cur = dst;
+    h->first_step_stored = false;
+    h->prev_valid = true;
+    h->steps++;
+    return LBM_OK;
+}
+
 int run_step(lbm_handle* h)
 {
     if (comps(h) == 2) return run_step2(h);
+    if (h->has_capsule) return run_step_capsule(h);
     const int src = h->cur, dst = 1 - h->cur;
     if (!is_lowp(h)) {
         auto* Pin = static_cast<const double*>(h->buf[src]);
@@ -364,6 +384,7 @@ void lbm_destroy(lbm_handle* h)
     if (h->ev_copied) hipEventDestroy(h->ev_copied);
     if (h->snap_dev) hipFree(h->snap_dev);
     if (h->snap_host) hipHostFree(h->snap_host);
+    if (h->capsule_mem) hipFree(h->capsule_mem);
     if (h->own) hipStreamDestroy(h->own);
     delete h;
 }
@@ -402,6 +423,13 @@ int lbm_init_equilibrium(lbm_handle* h)
                               reinterpret_cast<float*>(h->psi), h->rref[0], h->stream));
     }
     HIP_OK(hipStreamSynchronize(h->stream));
+    if (h->has_capsule) {
+        // the capsule model's start: zero force arrays (capsule2D.cpp:104-105)
+        HIP_OK(hipMemsetAsync(h->cap.fx, 0, 2 * h->g.n * 8, h->stream));
+        HIP_OK(lbm::fp64_capsule_collide(h->a, h->g, nullptr, lbm::kHoldsF, static_cast<double*>(h->buf[0]),
+                                         h->rho0, 2, h->cap, h->stream));
+        HIP_OK(hipStreamSynchronize(h->stream));
+    }
     h->cur = 0;
     h->kind[0] = lbm::kHoldsF;
     h->steps = 0;
@@ -493,6 +521,14 @@ int force_to_device(lbm_handle* h, int comp, double* out)
 {
     const int64_t n = h->g.n;
     const int nc = comps(h);
+    if (h->has_capsule) {
+        // the capsule's force arrays: body force plus the spread IB forces of
+        // the last step (zero at the start)
+        HIP_OK(hipMemcpyAsync(out, h->cap.fx, n * 8, hipMemcpyDeviceToDevice, h->stream));
+        HIP_OK(hipMemcpyAsync(out + n, h->cap.fy, n * 8, hipMemcpyDeviceToDevice, h->stream));
+        HIP_OK(hipMemsetAsync(out + 2 * n, 0, n * 8, h->stream));
+        return LBM_OK;
+    }
     // per component: density the last step started from, then psi; then F
     int rc = ensure_tmp(h, (5 * nc + 3) * n * 8);
     if (rc) return rc;
@@ -702,6 +738,74 @@ int lbm_conservation_component(lbm_handle* h, int comp, double out[4])
 }
 
 int64_t lbm_steps_done(const lbm_handle* h) { return h ? h->steps : -1; }
+
+int lbm_capsule_attach(lbm_handle* h, const lbm_capsule_desc* d)
+{
+    if (!h || !d) return fail(LBM_EINVAL, "lbm_capsule_attach: NULL argument");
+    if (h->has_capsule) return fail(LBM_ESTATE, "a capsule is already attached");
+    if (h->initialised) return fail(LBM_ESTATE, "lbm_capsule_attach after initialisation");
+    if (h->d.q != 9 || h->d.precision != LBM_FP64 || comps(h) != 1 || h->d.order != 2)
+        return fail(LBM_EINVAL, "the capsule model needs a one-component D2Q9 FP64 lattice in step order 2");
+    if (h->d.kernel == LBM_KERNEL_FUSED) return fail(LBM_EINVAL, "the capsule model runs the two-pass kernels");
+    if (d->nnodes < 1) return fail(LBM_EINVAL, "nnodes must be >= 1 (got %d)", d->nnodes);
+    if (!std::isfinite(d->kstiff) || !std::isfinite(d->dt) || !std::isfinite(d->body_fx) ||
+        !std::isfinite(d->body_fy))
+        return fail(LBM_EINVAL, "capsule parameters must be finite");
+    const int64_t n = h->g.n, nn = d->nnodes, nx = h->d.nx;
+    const int64_t count = 8 * nn + 2 * n + 4 * nx;
+    double* m = nullptr;
+    HIP_OK(lbm::dev_malloc(&m, count * 8));
+    HIP_OK(hipMemsetAsync(m, 0, count * 8, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    lbm::Capsule& c = h->cap;
+    c.x = m; c.y = m + nn; c.x0 = m + 2 * nn; c.y0 = m + 3 * nn;
+    c.vx = m + 4 * nn; c.vy = m + 5 * nn; c.nfx = m + 6 * nn; c.nfy = m + 7 * nn;
+    c.fx = m + 8 * nn; c.fy = c.fx + n;
+    c.gfx = c.fy + n; c.gfy = c.gfx + 2 * nx;
+    c.nnodes = d->nnodes;
+    c.kstiff = d->kstiff;
+    c.dt = d->dt;
+    c.body_fx = d->body_fx;
+    c.body_fy = d->body_fy;
+    h->capsule_mem = m;
+    h->has_capsule = true;
+    h->capsule_nodes_set = false;
+    h->kernel = LBM_KERNEL_TWO_PASS;
+    h->bytes += count * 8;
+    return LBM_OK;
+}
+
+int lbm_capsule_set_nodes(lbm_handle* h, const double* x, const double* y)
+{
+    if (!h || !x || !y) return fail(LBM_EINVAL, "lbm_capsule_set_nodes: NULL argument");
+    if (!h->has_capsule) return fail(LBM_ESTATE, "no capsule attached");
+    const int nn = h->cap.nnodes;
+    for (int k = 0; k < nn; k++)
+        if (!(y[k] >= -1.0 && y[k] < static_cast<double>(h->d.ny)) || !std::isfinite(x[k]))
+            return fail(LBM_EINVAL, "node %d at (%g, %g): x must be finite and -1 <= y < ny", k, x[k], y[k]);
+    HIP_OK(hipStreamSynchronize(h->stream));
+    HIP_OK(hipMemcpy(h->cap.x, x, nn * 8, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(h->cap.y, y, nn * 8, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(h->cap.x0, x, nn * 8, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(h->cap.y0, y, nn * 8, hipMemcpyHostToDevice));
+    h->capsule_nodes_set = true;
+    return LBM_OK;
+}
+
+int lbm_capsule_get_nodes(lbm_handle* h, double* x, double* y, double* vx, double* vy, double* fx, double* fy)
+{
+    if (!h) return fail(LBM_EINVAL, "lbm_capsule_get_nodes: NULL handle");
+    if (!h->has_capsule) return fail(LBM_ESTATE, "no capsule attached");
+    const int nn = h->cap.nnodes;
+    double* outs[6] = {x, y, vx, vy, fx, fy};
+    const double* srcs[6] = {h->cap.x, h->cap.y, h->cap.vx, h->cap.vy, h->cap.nfx, h->cap.nfy};
+    for (int k = 0; k < 6; k++)
+        if (outs[k]) HIP_OK(hipMemcpyAsync(outs[k], srcs[k], nn * 8, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    return LBM_OK;
+}
+
+int lbm_capsule_nodes(const lbm_handle* h) { return (h && h->has_capsule) ? h->cap.nnodes : 0; }
 
 int lbm_set_tuning(lbm_handle* h, const lbm_tuning* t)
 {

@@ -530,6 +530,213 @@ __global__ void __launch_bounds__(kBlock)
 
 inline unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
 
+
+// ---- immersed-boundary capsule (capsule2D) ----------------------------------
+// A D2Q9 fluid in the LBfluid2D step order whose force arrays hold a body
+// force plus the membrane forces of the IB nodes spread with the bilinear
+// kernel (capsule2D.cpp:123-150, IBcapsules2D.cpp:339-386), in place of the
+// Shan-Chen force.  Each fluid node gathers the contributions of the IB nodes
+// whose footprint covers it, in IB node order, which is the order the
+// reference's scatter adds them in; the IB nodes of a block's x range are
+// compacted into LDS in that order first.
+
+// Footprint origin of an IB node, 0-based: the nodes it touches are
+// (ox + a, oy + b), a, b in {0, 1} (IBcapsules2D.cpp:238-239 with i0 - 1).
+// False where the reference skips the node (isMyNode, IBcapsules2D.cpp:
+// 418-426: -1 <= x < nx, which excludes every non-finite x), and for y
+// outside [-1, ny), where the reference would index outside its ghost shell.
+__device__ __forceinline__ bool ib_origin(double x, double y, int nx, int ny, int& ox, int& oy)
+{
+    if (!(x >= -1.0 && x < static_cast<double>(nx))) return false;
+    if (!(y >= -1.0 && y < static_cast<double>(ny))) return false;
+    ox = static_cast<int>(floor(x));
+    oy = static_cast<int>(floor(y));
+    return true;
+}
+
+// Bilinear weight of lattice point (px, py) for a node at (x, y)
+// (IBcapsules2D.cpp:376-378).
+__device__ __forceinline__ double ib_weight(double x, double y, int px, int py)
+{
+    const double rx = x - static_cast<double>(px);
+    const double ry = y - static_cast<double>(py);
+    return (1.0 - fabs(rx)) * (1.0 - fabs(ry));
+}
+
+constexpr int kIbChunk = kBlock;
+
+template <bool WALLS, int MODE>
+__global__ void __launch_bounds__(kBlock)
+    k_collide_ib(const double* __restrict__ Pin, int kind, double* __restrict__ Pout,
+                 const double* __restrict__ rho0, Grid g, double tau, Capsule cap)
+{
+    __shared__ double s_x[kIbChunk], s_y[kIbChunk], s_fx[kIbChunk], s_fy[kIbChunk];
+    __shared__ int s_ox[kIbChunk], s_oy[kIbChunk];
+    __shared__ int s_count;
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    const bool live = c < g.n;
+    const int64_t cl = live ? c : g.n - 1;
+    const Coord p = decompose(static_cast<uint32_t>(cl), g);
+
+    // force: body force on the rows between the walls, then the IB nodes
+    // (capsule2D.cpp:123-139); none at the start (MODE 2: the zero forces of
+    // capsule2D.cpp:104-105)
+    double Fx = 0.0, Fy = 0.0;
+    if constexpr (MODE != 2) {
+        if (p.y >= 1 && p.y <= g.ny - 2) {
+            Fx = cap.body_fx;
+            Fy = cap.body_fy;
+        }
+        // x range of this block's nodes (rows of ny nodes, x slowest)
+        const int64_t c_lo = static_cast<int64_t>(blockIdx.x) * kBlock;
+        const int64_t c_hi = std::min<int64_t>(c_lo + kBlock, g.n) - 1;
+        const int x_lo = static_cast<int>(c_lo / g.ny), x_hi = static_cast<int>(c_hi / g.ny);
+        for (int base = 0; base < cap.nnodes; base += kIbChunk) {
+            __syncthreads();
+            if (threadIdx.x == 0) s_count = 0;
+            __syncthreads();
+            // compact the chunk's nodes that can touch x_lo..x_hi, in order:
+            // one wavefront at a time, ballot + prefix within it
+            const int n = base + static_cast<int>(threadIdx.x);
+            int ox = 0, oy = 0;
+            bool keep = false;
+            double xn = 0.0, yn = 0.0;
+            if (n < cap.nnodes) {
+                xn = cap.x[n];
+                yn = cap.y[n];
+                keep = ib_origin(xn, yn, g.nx, g.ny, ox, oy) && ox + 1 >= x_lo && ox <= x_hi;
+            }
+            const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+            for (int wv = 0; wv < kBlock / 64; wv++) {
+                if (wave == wv) {
+                    const uint64_t m = __ballot(keep);
+                    const int before = __popcll(m & ((1ull << lane) - 1ull));
+                    const int slot = s_count + before;
+                    if (keep) {
+                        s_x[slot] = xn;
+                        s_y[slot] = yn;
+                        // membrane force, IBcapsules2D.cpp:480-483
+                        s_fx[slot] = -cap.kstiff * (xn - cap.x0[n]);
+                        s_fy[slot] = -cap.kstiff * (yn - cap.y0[n]);
+                        s_ox[slot] = ox;
+                        s_oy[slot] = oy;
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    if (lane == 0) s_count += __popcll(m);
+                }
+                __syncthreads();
+            }
+            const int cnt = s_count;
+            for (int k = 0; k < cnt; k++) {
+                const int ax = p.x - s_ox[k], ay = p.y - s_oy[k];
+                if ((ax == 0 || ax == 1) && (ay == 0 || ay == 1)) {
+                    const double del = ib_weight(s_x[k], s_y[k], p.x, p.y);
+                    Fx = Fx + del * s_fx[k];
+                    Fy = Fy + del * s_fy[k];
+                }
+            }
+        }
+    }
+    if (!live) return;
+    if constexpr (MODE != 2) {
+        cap.fx[c] = Fx;
+        cap.fy[c] = Fy;
+    }
+
+    double f[9];
+    double rho, u, v;
+    if constexpr (MODE == 0) {
+        const Nbr nb = neighbours(p, g);
+        load_f<9, WALLS>(Pin, g, kind, p, nb, c, f);
+        double sx, sy, sz;
+        moments<9>(f, rho, sx, sy, sz);
+        u = sx / rho;
+        v = sy / rho;
+    } else {
+        if constexpr (MODE == 1) {
+            const Nbr nb = neighbours(p, g);
+            load_f<9, WALLS>(Pin, g, kind, p, nb, c, f);
+        } else {
+#pragma unroll
+            for (int n = 0; n < 9; n++) f[n] = 0.0;
+        }
+        rho = rho0[c];
+        u = 0.0;
+        v = 0.0;
+    }
+    collide_one<9, MODE>(f, rho, u, v, 0.0, Fx, Fy, 0.0, tau, Pout, g.ps, c);
+}
+
+// After the fluid step (one workgroup): the force that the spread put on the
+// ghost rows below and above the lattice, which LBfluid2D::zeroForces never
+// clears (LBfluid2D.cpp:137-147), then per IB node the interpolated velocity
+// u* = u + F/(2 rho) of the new fluid state (IBcapsules2D.cpp:218-287,
+// LBfluid2D.cpp:164-172; the ghost rows read r = u = 0) and the move
+// x += v dt (IBcapsules2D.cpp:463-469).
+template <bool WALLS>
+__global__ void __launch_bounds__(1024) k_ib_finish(const double* __restrict__ P, int kind, Grid g, Capsule cap)
+{
+    const int nx = g.nx, ny = g.ny;
+    for (int t = threadIdx.x; t < 2 * nx; t += blockDim.x) {
+        const int side = t / nx, px = t - side * nx;
+        const int py = side ? ny : -1;
+        double gx = cap.gfx[t], gy = cap.gfy[t];
+        for (int n = 0; n < cap.nnodes; n++) {
+            int ox, oy;
+            const double xn = cap.x[n], yn = cap.y[n];
+            if (!ib_origin(xn, yn, nx, ny, ox, oy)) continue;
+            const int ax = px - ox, ay = py - oy;
+            if ((ax == 0 || ax == 1) && (ay == 0 || ay == 1)) {
+                const double del = ib_weight(xn, yn, px, py);
+                gx = gx + del * (-cap.kstiff * (xn - cap.x0[n]));
+                gy = gy + del * (-cap.kstiff * (yn - cap.y0[n]));
+            }
+        }
+        cap.gfx[t] = gx;
+        cap.gfy[t] = gy;
+    }
+    for (int n = threadIdx.x; n < cap.nnodes; n += blockDim.x) {
+        cap.nfx[n] = -cap.kstiff * (cap.x[n] - cap.x0[n]);
+        cap.nfy[n] = -cap.kstiff * (cap.y[n] - cap.y0[n]);
+    }
+    __syncthreads();
+    for (int n = threadIdx.x; n < cap.nnodes; n += blockDim.x) {
+        const double xn = cap.x[n], yn = cap.y[n];
+        double vx = 0.0, vy = 0.0;
+        int ox, oy;
+        if (ib_origin(xn, yn, nx, ny, ox, oy)) {
+            for (int px = ox; px <= ox + 1; px++) {
+                if (px < 0 || px > nx - 1) continue;
+                for (int py = oy; py <= oy + 1; py++) {
+                    const double del = ib_weight(xn, yn, px, py);
+                    double us, vs;
+                    if (py >= 0 && py < ny) {
+                        const int64_t c = static_cast<int64_t>(px) * ny + py;
+                        const Coord q{px, py, 0};
+                        const Nbr nb = neighbours(q, g);
+                        double f[9];
+                        load_f<9, WALLS>(P, g, kind, q, nb, c, f);
+                        double r, sx, sy, sz;
+                        moments<9>(f, r, sx, sy, sz);
+                        us = sx / r + (0.5 * cap.fx[c]) / r;
+                        vs = sy / r + (0.5 * cap.fy[c]) / r;
+                    } else {
+                        const int t = (py < 0 ? 0 : nx) + px;
+                        us = 0.0 + (0.5 * cap.gfx[t]) / 0.0;
+                        vs = 0.0 + (0.5 * cap.gfy[t]) / 0.0;
+                    }
+                    vx = vx + del * us;
+                    vy = vy + del * vs;
+                }
+            }
+        }
+        cap.vx[n] = vx;
+        cap.vy[n] = vy;
+        cap.x[n] = xn + vx * cap.dt;
+        cap.y[n] = yn + vy * cap.dt;
+    }
+}
+
 }  // namespace
 
 // ---- launchers -------------------------------------------------------------
@@ -672,6 +879,32 @@ hipError_t fp64_conservation(const StepArgs& a, const Grid& g, const double* P, 
     const int nb = fp64_conservation_blocks(g);
     *nblocks = nb;
     LBM_DISPATCH_QW(a, (k_conservation<Q, W><<<nb, kConsBlock, 0, s>>>(P, kind, g, partials)));
+    return hipGetLastError();
+}
+
+
+hipError_t fp64_capsule_collide(const StepArgs& a, const Grid& g, const double* Pin, int kind, double* Pout,
+                                const double* rho0, int mode, const Capsule& cap, hipStream_t s)
+{
+    if (a.q != 9) return hipErrorInvalidValue;
+    const unsigned nb = static_cast<unsigned>(blocks_for(g.n));
+    auto go = [&](auto walls) {
+        constexpr bool W = decltype(walls)::value;
+        if (mode == 0) k_collide_ib<W, 0><<<nb, kBlock, 0, s>>>(Pin, kind, Pout, rho0, g, a.tau, cap);
+        else if (mode == 1) k_collide_ib<W, 1><<<nb, kBlock, 0, s>>>(Pin, kind, Pout, rho0, g, a.tau, cap);
+        else k_collide_ib<W, 2><<<nb, kBlock, 0, s>>>(Pin, kind, Pout, rho0, g, a.tau, cap);
+    };
+    if (a.walls) go(std::true_type{});
+    else go(std::false_type{});
+    return hipGetLastError();
+}
+
+hipError_t fp64_capsule_finish(const StepArgs& a, const Grid& g, const double* P, int kind, const Capsule& cap,
+                               hipStream_t s)
+{
+    if (a.q != 9) return hipErrorInvalidValue;
+    if (a.walls) k_ib_finish<true><<<1, 1024, 0, s>>>(P, kind, g, cap);
+    else k_ib_finish<false><<<1, 1024, 0, s>>>(P, kind, g, cap);
     return hipGetLastError();
 }
 

@@ -43,6 +43,12 @@ struct lbm_handle {
     hipEvent_t ev_snap = nullptr, ev_copied = nullptr;
     bool snap_pending = false;
     int64_t snap_step = 0;
+    // immersed-boundary capsule (lbm_capsule_attach): device arrays in one
+    // allocation, nodes placed by lbm_capsule_set_nodes
+    bool has_capsule = false;
+    bool capsule_nodes_set = false;
+    double* capsule_mem = nullptr;
+    lbm::Capsule cap{};
 };
 
 inline bool is_lowp(const lbm_handle* h) { return h->d.precision != LBM_FP64; }

@@ -114,4 +114,24 @@ hipError_t fp64_common_velocity(const StepArgs& a, const Grid& g, const double* 
                                 double* wA, const double* rB, const double* uB, const double* vB,
                                 const double* wB, int own, hipStream_t s);
 
+// Immersed-boundary capsule (capsule2D, D2Q9 FP64): device arrays of the
+// IB nodes (positions, rest positions, last velocities and membrane
+// forces), the fluid force arrays of the last step (n each) and the force
+// the spread left on the ghost rows below and above the lattice (2*nx each:
+// row y = -1, then y = ny).
+struct Capsule {
+    double *x, *y, *x0, *y0, *vx, *vy, *nfx, *nfy;
+    double *fx, *fy, *gfx, *gfy;
+    int nnodes;
+    double kstiff, dt, body_fx, body_fy;
+};
+// Force (body + spread IB forces, stored in cap.fx/fy) and collision of the
+// capsule fluid; modes as fp64_collide (mode 2: zero force, the start).
+hipError_t fp64_capsule_collide(const StepArgs& a, const Grid& g, const double* Pin, int kind, double* Pout,
+                                const double* rho0, int mode, const Capsule& cap, hipStream_t s);
+// Ghost-row forces, membrane forces, interpolated velocities and the move of
+// the IB nodes after a step that wrote P.
+hipError_t fp64_capsule_finish(const StepArgs& a, const Grid& g, const double* P, int kind, const Capsule& cap,
+                               hipStream_t s);
+
 }  // namespace lbm

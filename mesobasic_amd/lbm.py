@@ -31,6 +31,11 @@ class LbmError(RuntimeError):
         self.code = code
 
 
+class CapsuleDesc(ctypes.Structure):
+    _fields_ = [("nnodes", ctypes.c_int), ("kstiff", ctypes.c_double), ("dt", ctypes.c_double),
+                ("body_fx", ctypes.c_double), ("body_fy", ctypes.c_double)]
+
+
 class Desc(ctypes.Structure):
     _fields_ = [("q", ctypes.c_int), ("nx", ctypes.c_int), ("ny", ctypes.c_int),
                 ("nz", ctypes.c_int), ("tau", ctypes.c_double), ("G", ctypes.c_double),
@@ -128,6 +133,10 @@ def load():
         "lbm_set_f_component": (ctypes.c_int, [P, ctypes.c_int, _D]),
         "lbm_conservation_component": (ctypes.c_int, [P, ctypes.c_int, _D]),
         "lbm_probe_memcpy": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, _D]),
+        "lbm_capsule_attach": (ctypes.c_int, [P, ctypes.POINTER(CapsuleDesc)]),
+        "lbm_capsule_set_nodes": (ctypes.c_int, [P, _D, _D]),
+        "lbm_capsule_get_nodes": (ctypes.c_int, [P] + [_D] * 6),
+        "lbm_capsule_nodes": (ctypes.c_int, [P]),
         "lbm_batch_read": (ctypes.c_int, [P, ctypes.c_int64] + [_D] * 7),
         "lbm_batch_extend": (ctypes.c_int, [P, ctypes.c_int64, ctypes.c_int64]),
         "lbm_snapshot_begin": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),
@@ -322,6 +331,27 @@ class Lattice:
         _check(self._lib.lbm_get_force_component(self._h, int(comp), _dptr(fx), _dptr(fy), _dptr(fz)))
         sh = self.shape
         return fx.reshape(sh), fy.reshape(sh), fz.reshape(sh)
+
+    def attach_capsule(self, nnodes, kstiff=1.0, dt=1.0, body=(0.000008, 0.0)):
+        """Turns the lattice into the capsule2D model: nnodes IB nodes with
+        membrane stiffness kstiff and a body force (call before init)."""
+        d = CapsuleDesc(int(nnodes), float(kstiff), float(dt), float(body[0]), float(body[1]))
+        _check(self._lib.lbm_capsule_attach(self._h, ctypes.byref(d)))
+
+    def set_capsule_nodes(self, x, y):
+        """Node positions, also their rest positions."""
+        nn = self._lib.lbm_capsule_nodes(self._h)
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        y = np.ascontiguousarray(y, dtype=np.float64).reshape(-1)
+        assert x.size == nn and y.size == nn
+        _check(self._lib.lbm_capsule_set_nodes(self._h, _dptr(x), _dptr(y)))
+
+    def capsule_nodes(self):
+        """dict of node positions x, y, velocities vx, vy and forces fx, fy."""
+        nn = self._lib.lbm_capsule_nodes(self._h)
+        arrs = [np.empty(nn) for _ in range(6)]
+        _check(self._lib.lbm_capsule_get_nodes(self._h, *(_dptr(a) for a in arrs)))
+        return dict(zip(("x", "y", "vx", "vy", "fx", "fy"), arrs))
 
     def snapshot_begin(self, comp=0, kind=FIELDS_REFERENCE):
         """Starts an asynchronous copy of fields(kind, comp) and force(comp);
