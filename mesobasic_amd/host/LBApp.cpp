@@ -45,7 +45,7 @@ LBApp::LBApp(const InputParams& in) : p(read_common_params(in))
     if (!lb_object)
         throw std::runtime_error("LBApp/type '" + in("LBApp/type", "mcmp2D") +
                                  "' is not provided by this build (scmp3D, scmp2D, mcmp2D, mcmp3D, mcmp2DFilm, "
-                                 "mcmp3DFilm)");
+                                 "mcmp3DFilm, capsule2D)");
 }
 
 void LBApp::initSystem() { lb_object->initLatticeBoltzmann(); }

@@ -12,6 +12,7 @@ namespace {
 
 struct Frame {
     std::vector<double> rho, u, v, fx, fy;
+    std::vector<double> cap_x, cap_y;   // IB mesh (capsule models)
 };
 
 void check(int rc, const char* what)
@@ -22,6 +23,7 @@ void check(int rc, const char* what)
 void write_frame(const CommonParams& p, int q, int step, const Frame& f)
 {
     write_model_vtk("vtkoutput", p, q, step, f.rho, f.u, f.v, f.fx, f.fy);
+    if (!f.cap_x.empty()) write_capsule_vtk("vtkoutput", "capsule", step, f.cap_x, f.cap_y);
 }
 
 }  // namespace
@@ -58,6 +60,8 @@ void OutputQueue::collect_and_write(lbm_handle* h)
                             two_d ? f->fx.data() : nullptr, two_d ? f->fy.data() : nullptr, nullptr),
           "lbm_snapshot_wait");
     pending_ = false;
+    f->cap_x.swap(cap_x_);
+    f->cap_y.swap(cap_y_);
     const int step = pending_step_;
     join_writer();
     if (!async_) {
@@ -78,6 +82,14 @@ void OutputQueue::collect_and_write(lbm_handle* h)
 void OutputQueue::output(lbm_handle* h, int comp, int step)
 {
     if (pending_) collect_and_write(h);
+    if (capsule_) {
+        // the mesh is a few values: read now, at the step of the snapshot
+        const int nn = lbm_capsule_nodes(h);
+        cap_x_.resize(static_cast<size_t>(nn));
+        cap_y_.resize(static_cast<size_t>(nn));
+        check(lbm_capsule_get_nodes(h, cap_x_.data(), cap_y_.data(), nullptr, nullptr, nullptr, nullptr),
+              "lbm_capsule_get_nodes");
+    }
     check(lbm_snapshot_begin(h, comp, LBM_FIELDS_REFERENCE), "lbm_snapshot_begin");
     pending_ = true;
     pending_step_ = step;

@@ -26,7 +26,12 @@ class OutputQueue {
 public:
     // q = 19: density only (LBfluid3D::writeVTKFile); q = 9: density and the
     // Shan-Chen velocity u + F/(2 rho) (LBfluid2D::writeVTKFile).
-    OutputQueue(const CommonParams& p, int q, bool async) : p_(p), q_(q), async_(async) {}
+    // capsule: the handle carries an IB capsule, whose mesh is written with
+    // each output as capsule_<step>.vtk (capsule2D.cpp:158-164)
+    OutputQueue(const CommonParams& p, int q, bool async, bool capsule = false)
+        : p_(p), q_(q), async_(async), capsule_(capsule)
+    {
+    }
     ~OutputQueue();
     OutputQueue(const OutputQueue&) = delete;
     OutputQueue& operator=(const OutputQueue&) = delete;
@@ -40,7 +45,9 @@ private:
     const CommonParams& p_;
     int q_;
     bool async_;
+    bool capsule_;
     bool pending_ = false;
+    std::vector<double> cap_x_, cap_y_;   // mesh of the pending output
     int pending_step_ = 0;
     std::thread writer_;
     std::exception_ptr writer_error_;

@@ -1,5 +1,6 @@
 #include "ScmpModel.hpp"
 
+#include <cmath>
 #include <ctime>
 #include <random>
 #include <stdexcept>
@@ -245,9 +246,85 @@ void McmpModel::outputLatticeBoltzmann() { out_->output(h_, 0, current_step); }
 
 void McmpModel::finishOutput() { out_->finish(h_); }
 
+capsule2D::capsule2D(const CommonParams& pin, const InputParams& in) : p(pin)
+{
+    // capsule2D.cpp:25-37, IBcapsules2D.cpp:45-48
+    rhoAi = in("LBApp/rhoAi", 0.5);
+    rhoAi_noise = in("LBApp/rhoAi_noise", 0.1);
+    tauA = in("LBApp/tauA", 1.0);
+    const std::string stype = in("LBApp/stencil", "D2Q9");
+    if (stype != "D2Q9") throw std::runtime_error("LBApp/stencil " + stype + " does not match capsule2D");
+    if (p.nz != 1) throw std::runtime_error("capsule2D needs Domain/nz = 1");
+    nnodes = in("LBApp/nnodes", 20);
+    kstiff = in("LBApp/kstiff", 1.0);
+    cx = in("LBApp/capsule_x", 30.0);
+    cy = in("LBApp/capsule_y", 30.0);
+    cr = in("LBApp/capsule_r", 8.0);
+    seed = in.has("LBApp/seed") ? static_cast<unsigned>(in("LBApp/seed", 0.0))
+                                : static_cast<unsigned>(std::time(nullptr) * (p.rank + 1));
+    lbm_desc d{};
+    d.q = 9;
+    d.nx = p.nx;
+    d.ny = p.ny;
+    d.nz = 1;
+    d.tau = tauA;
+    d.G = 0.0;
+    d.order = 2;   // LBfluid2D::updateFluid_SC (capsule2D.cpp:142-145)
+    d.walls = 1;   // yBC = 1
+    d.precision = LBM_FP64;
+    d.kernel = LBM_KERNEL_TWO_PASS;
+    check(lbm_create(&d, &h_), "lbm_create");
+    lbm_capsule_desc cd{};
+    cd.nnodes = nnodes;
+    cd.kstiff = kstiff;
+    cd.dt = p.dt;
+    cd.body_fx = in("LBApp/body_fx", 0.000008);
+    cd.body_fy = in("LBApp/body_fy", 0.0);
+    check(lbm_capsule_attach(h_, &cd), "lbm_capsule_attach");
+    out_.reset(new OutputQueue(p, 9, in("Output/async", 1) != 0, true));
+}
+
+capsule2D::~capsule2D() { lbm_destroy(h_); }
+
+void capsule2D::check(int rc, const char* what) const
+{
+    if (rc != LBM_OK) throw std::runtime_error(std::string(what) + ": " + lbm_last_error());
+}
+
+void capsule2D::initLatticeBoltzmann()
+{
+    // capsule2D.cpp:84-120: noisy density on every node (a seeded mt19937 in
+    // place of rand()), zero velocity and forces, the circular capsule
+    // (IBcapsules2D::initCircularCapsule, IBcapsules2D.cpp:130-141), f = feq
+    std::vector<double> rho(static_cast<size_t>(p.nx) * p.ny);
+    check(lbm_fill_noise(seed, static_cast<int64_t>(rho.size()), rhoAi, rhoAi_noise, rho.data()),
+          "lbm_fill_noise");
+    std::vector<double> x(static_cast<size_t>(nnodes)), y(static_cast<size_t>(nnodes));
+    double theta = 0.0;
+    const double dTheta = (2.0 * 3.14159265) / double(nnodes);
+    for (int n = 0; n < nnodes; n++) {
+        x[n] = cx + cr * std::cos(theta);
+        y[n] = cy + cr * std::sin(theta);
+        theta += dTheta;
+    }
+    check(lbm_capsule_set_nodes(h_, x.data(), y.data()), "lbm_capsule_set_nodes");
+    check(lbm_set_rho(h_, rho.data()), "lbm_set_rho");
+    check(lbm_init_equilibrium(h_), "lbm_init_equilibrium");
+}
+
+void capsule2D::updateLatticeBoltzmann()
+{
+    // capsule2D.cpp:123-150 (the Barrier has no counterpart on one GPU)
+    check(lbm_step(h_, 1), "lbm_step");
+}
+
+// capsule2D.cpp:158-164: the fluid (density and u*) and the IB mesh
+void capsule2D::outputLatticeBoltzmann() { out_->output(h_, 0, current_step); }
+
+void capsule2D::finishOutput() { out_->finish(h_); }
+
 // LBBaseClass::LBFactory (LBBaseClass.cpp:21-45), with the reference's
-// default type mcmp2D.  The capsule model is not part of this build, so it
-// (like unknown names) yields nullptr.
+// default type mcmp2D; unknown names yield nullptr.
 LBBaseClass* LBBaseClass::LBFactory(const CommonParams& p, const InputParams& in)
 {
     const std::string lb_type = in("LBApp/type", "mcmp2D");
@@ -257,6 +334,7 @@ LBBaseClass* LBBaseClass::LBFactory(const CommonParams& p, const InputParams& in
     if (lb_type == "mcmp3D") return new mcmp3D(p, in);
     if (lb_type == "mcmp2DFilm") return new mcmp2DFilm(p, in);
     if (lb_type == "mcmp3DFilm") return new mcmp3DFilm(p, in);
+    if (lb_type == "capsule2D") return new capsule2D(p, in);
     return nullptr;
 }
 

@@ -138,6 +138,46 @@ private:
     void check(int rc, const char* what) const;
 };
 
+// Immersed-boundary capsule in a channel (capsule2D, capsule2D.cpp:10-166
+// with IBcapsules2D): a D2Q9 fluid in the LBfluid2D step order with
+// bounce-back walls in y, driven by a body force on the rows between the
+// walls, and a circular capsule of nnodes IB nodes tied to their rest
+// positions.  Keys (read as capsule2D.cpp:25-31 and IBcapsules2D.cpp:45-48
+// read them, same defaults): LBApp/rhoAi (0.5), rhoAi_noise (0.1), tauA
+// (1.0), stencil (D2Q9), ncaps (1; one capsule is placed, as in the
+// reference), nnodes (20), h (2.0; the footprint half-width of the cosine
+// kernel, which the bilinear kernel the reference uses ignores), kstiff
+// (1.0); Time/dt (1.0).  Keys of this build only: LBApp/seed, LBApp/capsule_x,
+// capsule_y, capsule_r (the reference's fixed circle 30, 30, 8:
+// capsule2D.cpp:113), body_fx, body_fy (its fixed 0.000008, 0:
+// capsule2D.cpp:132-133).
+class capsule2D : public LBBaseClass {
+public:
+    capsule2D(const CommonParams& p, const InputParams& in);
+    ~capsule2D() override;
+    capsule2D(const capsule2D&) = delete;
+    capsule2D& operator=(const capsule2D&) = delete;
+
+    void setTimeStep(int step) override { current_step = step; }
+    void initLatticeBoltzmann() override;
+    void updateLatticeBoltzmann() override;
+    void outputLatticeBoltzmann() override;
+    void finishOutput() override;
+    lbm_handle* handle() { return h_; }
+
+private:
+    const CommonParams& p;
+    int current_step = 0;
+    double rhoAi, rhoAi_noise, tauA;
+    int nnodes;
+    double kstiff, cx, cy, cr;
+    unsigned seed;
+    lbm_handle* h_ = nullptr;
+    std::unique_ptr<OutputQueue> out_;
+
+    void check(int rc, const char* what) const;
+};
+
 class mcmp2D : public McmpModel {
 public:
     mcmp2D(const CommonParams& p, const InputParams& in) : McmpModel(p, in, 9) {}

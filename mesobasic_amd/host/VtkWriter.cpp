@@ -101,4 +101,27 @@ void write_model_vtk(const std::string& dir, const CommonParams& p, int q, int s
     write_vtk_2d(dir, "rhoA", step, p.NX, p.NY, p.iskip, p.jskip, rho, us, vs);
 }
 
+void write_capsule_vtk(const std::string& dir, const std::string& tag, int step, const std::vector<double>& x,
+                       const std::vector<double>& y)
+{
+    // IBcapsules2D::writeVTKFile, IBcapsules2D.cpp:492-562
+    std::ofstream out = open_vtk(dir, tag, step);
+    const size_t nn = x.size();
+    out << "# vtk DataFile Version 3.1" << std::endl;
+    out << "VTK file containing IBM data" << std::endl;
+    out << "ASCII" << std::endl;
+    out << " " << std::endl;
+    out << "DATASET POLYDATA" << std::endl;
+    out << " " << std::endl;
+    out << "POINTS " << nn << " float" << std::endl;
+    for (size_t n = 0; n < nn; n++)
+        out << std::fixed << std::setprecision(3) << x[n] << "  " << y[n] << "  " << 0.0 << std::endl;
+    out << " " << std::endl;
+    out << "LINES " << nn << " " << 3 * nn << std::endl;
+    for (size_t n = 0; n < nn; n++) out << "2  " << n << "  " << (n + 1 == nn ? 0 : n + 1) << std::endl;
+    out << " " << std::endl;
+    out << "VERTICES " << nn << " " << 2 * nn << std::endl;
+    for (size_t n = 0; n < nn; n++) out << "1 " << n << std::endl;
+}
+
 }  // namespace mesobasic

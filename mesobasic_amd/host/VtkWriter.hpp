@@ -20,6 +20,12 @@ void write_vtk_2d(const std::string& dir, const std::string& tag, int step, int 
                   int iskip, int jskip, const std::vector<double>& rho,
                   const std::vector<double>& ustar, const std::vector<double>& vstar);
 
+// The IB mesh of a capsule (IBcapsules2D::writeVTKFile, IBcapsules2D.cpp:
+// 492-562): node positions with 3 decimals, the closed polyline through them
+// and one vertex per node.  Written as dir/<tag>_<step>.vtk.
+void write_capsule_vtk(const std::string& dir, const std::string& tag, int step, const std::vector<double>& x,
+                       const std::vector<double>& y);
+
 // The output file of one step of a model: for D3Q19 (q = 19) the density
 // (scmp3D.cpp:127-133); for D2Q9 the density and the Shan-Chen velocity
 // u + F/(2 rho) (mcmp2D.cpp:131-137, LBfluid2D::getUStar, LBfluid2D.cpp:

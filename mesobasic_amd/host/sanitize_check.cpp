@@ -123,7 +123,7 @@ void factory_checks()
 {
     CommonParams cp{};
     cp.NX = cp.NY = cp.NZ = 8;
-    for (const char* t : {"capsule2D", "nosuchtype"}) {
+    for (const char* t : {"capsule3D", "nosuchtype"}) {
         InputParams in = InputParams::from_string(std::string("[LBApp]\ntype = ") + t + "\n");
         LBBaseClass* m = LBBaseClass::LBFactory(cp, in);
         check(m == nullptr, std::string("factory refuses ") + t);
@@ -131,7 +131,7 @@ void factory_checks()
     }
     // the provided types build a model, or (without a GPU) report why not
     cp.nz = 1;
-    for (const char* t : {"scmp2D", "mcmp2D", "mcmp2DFilm"}) {
+    for (const char* t : {"scmp2D", "mcmp2D", "mcmp2DFilm", "capsule2D"}) {
         InputParams in = InputParams::from_string(std::string("[LBApp]\ntype = ") + t + "\nseed = 3\n");
         try {
             LBBaseClass* m = LBBaseClass::LBFactory(cp, in);

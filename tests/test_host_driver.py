@@ -65,10 +65,10 @@ def test_getpot_grammar(tmp_path):
 
 
 def test_unknown_type_is_reported(tmp_path):
-    (tmp_path / "input.dat").write_text("[LBApp]\n app = true\n type = capsule2D\n")
+    (tmp_path / "input.dat").write_text("[LBApp]\n app = true\n type = capsule3D\n")
     r = subprocess.run([MESO], cwd=tmp_path, capture_output=True, text=True)
     assert r.returncode != 0
-    assert "capsule2D" in r.stderr and "not provided" in r.stderr
+    assert "capsule3D" in r.stderr and "not provided" in r.stderr
 
 
 def _read_vtk(path):
@@ -218,3 +218,54 @@ def test_mcmp_run_matches_oracle(tmp_path, q, film):
         vs = fl["v"] + 0.5 * fl["fy"] / fl["rho"]
         assert lines[12 + n + 2:] == [f"{us[i, j]:.3f} {vs[i, j]:.3f} 0.000" for j in range(ny)
                                       for i in range(nx)]
+
+
+@pytest.mark.gpu
+def test_capsule2d_run_matches_oracle(tmp_path):
+    nx, ny, nstep, nn = 48, 40, 60, 24
+    (tmp_path / "input.dat").write_text(f"""
+[Domain]
+  nx = {nx}
+  ny = {ny}
+[Time]
+  nstep = {nstep}
+[Output]
+  numOutputs = 2
+[LBApp]
+  app = true
+  type = capsule2D
+  tauA = 0.9
+  nnodes = {nn}
+  kstiff = 0.8
+  capsule_x = 20.0
+  capsule_y = 19.5
+  capsule_r = 6.0
+  seed = 5
+""")
+    r = subprocess.run([MESO], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    files = sorted(os.listdir(tmp_path / "vtkoutput"))
+    assert files == sorted(f"{t}_{s}.vtk" for t in ("rhoA", "capsule") for s in (0, 30, 60))
+    o = oracle.CapsuleOracle((nx, ny), 0.9, nn, 0.8)
+    o.set_rho(oracle.noise_rho((nx, ny), 5, mean=0.5, amp=0.1))
+    o.circle(20.0, 19.5, 6.0)
+    o.init()
+    for s in (0, 30, 60):
+        if s:
+            o.step(30)
+        fl, nd = o.fields(), o.nodes()
+        lines = open(tmp_path / "vtkoutput" / f"rhoA_{s}.vtk").read().splitlines()
+        n = nx * ny
+        assert lines[12:12 + n] == [f"{fl['rho'][i, j]:.3f}" for j in range(ny) for i in range(nx)]
+        us = fl["u"] + 0.5 * fl["fx"] / fl["rho"]
+        vs = fl["v"] + 0.5 * fl["fy"] / fl["rho"]
+        assert lines[12 + n + 2:] == [f"{us[i, j]:.3f} {vs[i, j]:.3f} 0.000" for j in range(ny)
+                                      for i in range(nx)]
+        cap = open(tmp_path / "vtkoutput" / f"capsule_{s}.vtk").read().splitlines()
+        assert cap[:7] == ["# vtk DataFile Version 3.1", "VTK file containing IBM data", "ASCII", " ",
+                           "DATASET POLYDATA", " ", f"POINTS {nn} float"]
+        assert cap[7:7 + nn] == [f"{x:.3f}  {y:.3f}  0.000" for x, y in zip(nd["x"], nd["y"])]
+        assert cap[7 + nn:9 + nn] == [" ", f"LINES {nn} {3 * nn}"]
+        assert cap[9 + nn:9 + 2 * nn] == [f"2  {k}  {(k + 1) % nn}" for k in range(nn)]
+        assert cap[9 + 2 * nn:11 + 2 * nn] == [" ", f"VERTICES {nn} {2 * nn}"]
+        assert cap[11 + 2 * nn:] == [f"1 {k}" for k in range(nn)]
