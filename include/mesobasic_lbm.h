@@ -261,9 +261,14 @@ int lbm_synchronize(lbm_handle* h);
  *             nz >= TZ); two nodes per thread (16-bit
  *             storage at tau = 1 without walls, nz a multiple of 128):
  *             6: 8x128, 7: 4x128; D2Q9 FP64 (any ny): a segment of 8: 256,
- *             9: 128, 10: 64 nodes of a y row; -1 = first that fits (a
- *             pair tile where one applies; for D2Q9 the narrowest segment
- *             that covers a row of <= 128 nodes, else 256).  Every fused
+ *             9: 128, 10: 64 nodes of a y row; D2Q9 FP64 temporal
+ *             blocks (periodic, one component; K steps per launch on T x T
+ *             tiles, T dividing nx and ny, the last step of each lbm_step
+ *             call a single step): 11: 16x16 x 2 steps, 12: 16x16 x 4,
+ *             13: 8x8 x 4; -1 = the defaults (a pair tile where one
+ *             applies; for D2Q9 the narrowest segment that covers a row of
+ *             <= 128 nodes, else 256, and a temporal block for periodic
+ *             lattices of up to 2^20 nodes that one fits).  Every fused
  *             shape needs nx >= 2.
  *   chunk     x planes (D2Q9: x rows) marched per workgroup
  *   order     block -> tile order: 0 plain, 1 contiguous runs per XCD,

@@ -303,6 +303,13 @@ int lbm_create(const lbm_desc* desc, lbm_handle** out)
         h->kernel = desc->kernel == LBM_KERNEL_AUTO
                         ? (fused_auto && nc == 1 ? LBM_KERNEL_FUSED : LBM_KERNEL_TWO_PASS)
                         : desc->kernel;
+        // temporal blocks (K steps per launch) for the small periodic D2Q9
+        // lattices, which are latency-bound one step at a time
+        const int blk = nc == 1 ? lbm::fused::default_block(h->g, desc->q, desc->walls != 0) : -1;
+        if (blk >= 0 && fused_ok && desc->kernel != LBM_KERNEL_TWO_PASS) {
+            h->kernel = LBM_KERNEL_FUSED;
+            h->block = blk;
+        }
     } else {
         h->kernel = lbm::lowp_resolve_kernel(desc->precision, h->a, h->g, desc->kernel);
         if (h->kernel < 0) {
@@ -445,9 +452,27 @@ int lbm_step(lbm_handle* h, int nsteps)
     if (!h) return fail(LBM_EINVAL, "lbm_step: NULL handle");
     if (!h->initialised) return fail(LBM_ESTATE, "lbm_step before lbm_init_equilibrium / lbm_set_f");
     if (nsteps < 0) return fail(LBM_EINVAL, "lbm_step: nsteps < 0");
-    for (int s = 0; s < nsteps; s++) {
+    const int K = lbm::fused::block_steps(h->block);
+    for (int s = 0; s < nsteps;) {
+        // K steps per launch where a temporal block is set, after the first
+        // step (post-collision values in); the last step of the call is a
+        // single one, so the other buffer holds f(N-1) as the reference
+        // arrays and the force getters expect
+        if (h->block >= 0 && !h->first_step_stored && h->kind[h->cur] == lbm::kHoldsPost &&
+            nsteps - s > K) {
+            const int src = h->cur, dst = 1 - h->cur;
+            HIP_OK(lbm::fp64_block_step(h->a, h->g, static_cast<const double*>(h->buf[src]),
+                                        static_cast<double*>(h->buf[dst]), h->block, h->stream));
+            h->kind[dst] = lbm::kHoldsPost;
+            h->cur = dst;
+            h->prev_valid = false;
+            h->steps += K;
+            s += K;
+            continue;
+        }
         int rc = run_step(h);
         if (rc) return rc;
+        s++;
     }
     return LBM_OK;
 }
@@ -771,6 +796,7 @@ int lbm_capsule_attach(lbm_handle* h, const lbm_capsule_desc* d)
     h->has_capsule = true;
     h->capsule_nodes_set = false;
     h->kernel = LBM_KERNEL_TWO_PASS;
+    h->block = -1;
     h->bytes += count * 8;
     return LBM_OK;
 }
@@ -813,6 +839,19 @@ int lbm_set_tuning(lbm_handle* h, const lbm_tuning* t)
     if (h->kernel != LBM_KERNEL_FUSED)
         return fail(LBM_EINVAL, "lbm_set_tuning: handle does not use the fused kernel");
     const bool fp64 = h->d.precision == LBM_FP64;
+    if (lbm::fused::is_block_shape(t->shape)) {
+        // a temporal block: K steps per launch; single steps keep the row shape
+        if (!fp64 || comps(h) != 1 || h->has_capsule ||
+            !lbm::fused::block_supported(h->g, h->d.q, h->d.walls != 0, t->shape))
+            return fail(LBM_EINVAL, "temporal block %d needs a periodic one-component D2Q9 FP64 lattice "
+                                    "whose sides it divides", t->shape);
+        h->block = t->shape;
+        return LBM_OK;
+    }
+    // shape -1: the defaults, a temporal block included where one applies
+    h->block = (t->shape < 0 && fp64 && comps(h) == 1 && !h->has_capsule)
+                   ? lbm::fused::default_block(h->g, h->d.q, h->d.walls != 0)
+                   : -1;
     const int shape = t->shape >= 0 ? t->shape
                       : fp64    ? lbm::fp64_default_shape(h->g, h->d.q)
                                 : lbm::lowp_default_shape(h->d.precision, h->a, h->g);
@@ -832,7 +871,7 @@ int lbm_set_tuning(lbm_handle* h, const lbm_tuning* t)
 int lbm_get_tuning(const lbm_handle* h, lbm_tuning* t)
 {
     if (!h || !t) return fail(LBM_EINVAL, "lbm_get_tuning: NULL argument");
-    t->shape = h->tune.shape;
+    t->shape = h->block >= 0 ? h->block : h->tune.shape;
     t->chunk = h->tune.chunk;
     t->order = h->tune.order;
     t->nt_store = h->tune.nts;

@@ -227,6 +227,113 @@ __global__ void __launch_bounds__(TW)
     if (x + 1 < x1) step(x + 1, fb, fa);
 }
 
+
+// ---- D2Q9 temporal blocks --------------------------------------------------
+//
+// K steps of the D2Q9 step above (after the first step, post-collision
+// values in, no walls) per launch.  A workgroup owns a T x T tile and loads
+// the post-collision values of the tile plus a halo of H = 2K nodes (wrapped
+// periodically) into LDS once: a step needs f pulled from radius 1 and psi
+// of the neighbours, whose densities are pulled from radius 2, so the
+// region whose values are valid shrinks by 2 per step and the tile is what
+// is left after K.  Each step: every node of the region pulls f from LDS,
+// sums rho and writes psi (barrier), then every node one further in forms
+// the force from the psi window and collides into its own LDS slots (the
+// pulls all happened before the barrier, so the update is in place;
+// barrier).  Per node the arithmetic is collide9's, so K steps here give
+// the bits of K single steps.  One thread per node of the region, W = T+2H
+// <= 32.
+template <int T, int K, bool TAU1>
+__global__ void __launch_bounds__(((T + 4 * K) * (T + 4 * K) + 63) / 64 * 64)
+    k_block9(const double* __restrict__ Pin, double* __restrict__ Pout, Grid g, double tau, double G)
+{
+    using S = St<9>;
+    constexpr int H = 2 * K, W = T + 2 * H, W2 = W * W;
+    static_assert(W <= 32, "one thread per region node");
+    __shared__ double fs[9 * W2];
+    __shared__ double pw[W2];
+    __shared__ uint64_t exp_tab[256];
+
+    const int nby = g.ny / T;
+    const int bx = blockIdx.x / nby, by = blockIdx.x - bx * nby;
+    const int tid = threadIdx.x;
+    const bool live = tid < W2;
+    const int i = tid / W, j = tid - (tid / W) * W;
+    // global node of region slot (i, j), wrapped (a region wider than the
+    // lattice holds some nodes twice; every copy computes the same values)
+    int gx = (bx * T - H + i) % g.nx;
+    int gy = (by * T - H + j) % g.ny;
+    gx += gx < 0 ? g.nx : 0;
+    gy += gy < 0 ? g.ny : 0;
+    const uint32_t c = static_cast<uint32_t>(gx) * static_cast<uint32_t>(g.ny) + static_cast<uint32_t>(gy);
+
+    fill_exp_tab(exp_tab, tid, blockDim.x);
+    if (live) {
+        static_for<9>([&](auto I) {
+            constexpr int n = decltype(I)::value;
+            fs[n * W2 + tid] = *elem(Pin + n * g.ps, c);
+        });
+    }
+    __syncthreads();
+
+    const double mg = -G;
+    for (int s = 0; s < K; s++) {
+        const int lo = 2 * s + 1, hi = W - 2 * s - 1;
+        double f[9];
+        double rho = 0.0, sx = 0.0, sy = 0.0;
+        const bool in_a = live && i >= lo && i < hi && j >= lo && j < hi;
+        if (in_a) {
+            static_for<9>([&](auto I) {
+                constexpr int n = decltype(I)::value;
+                f[n] = fs[n * W2 + tid - (S::ex[n] * W + S::ey[n])];
+            });
+            static_for<9>([&](auto I) {
+                constexpr int n = decltype(I)::value;
+                rho = rho + f[n];
+                acc_e<S::ex[n]>(sx, f[n]);
+                acc_e<S::ey[n]>(sy, f[n]);
+            });
+            pw[tid] = Psi<double>::of(rho, exp_tab);
+        }
+        __syncthreads();
+        const bool in_b = live && i > lo && i < hi - 1 && j > lo && j < hi - 1;
+        if (in_b) {
+            const double u = sx / rho, v = sy / rho;
+            const double pc = pw[tid];
+            double gxs = 0.0, gys = 0.0;
+            static_for<9>([&](auto I) {
+                constexpr int n = decltype(I)::value;
+                if constexpr (n > 0) {
+                    const double t = S::w[n] * pw[tid + S::ex[n] * W + S::ey[n]];
+                    acc_e<S::ex[n]>(gxs, t);
+                    acc_e<S::ey[n]>(gys, t);
+                }
+            });
+            const double mgp = mg * pc;
+            const double Fx = mgp * gxs, Fy = mgp * gys;
+            const double ueq = u + (tau * Fx) / rho;
+            const double veq = v + (tau * Fy) / rho;
+            const double usq15 = 1.5 * (ueq * ueq + veq * veq);
+            static_for<9>([&](auto I) {
+                constexpr int n = decltype(I)::value;
+                const double ev = edot<S::ex[n], S::ey[n], 0>(ueq, veq, 0.0);
+                const double feq = (((1.0 + 3.0 * ev) + (4.5 * ev) * ev) - usq15) * (rho * S::w[n]);
+                double out;
+                if constexpr (TAU1) out = f[n] - (f[n] - feq);
+                else out = f[n] - (f[n] - feq) / tau;
+                fs[n * W2 + tid] = out;
+            });
+        }
+        __syncthreads();
+    }
+    if (live && i >= H && i < W - H && j >= H && j < W - H) {
+        static_for<9>([&](auto I) {
+            constexpr int n = decltype(I)::value;
+            *elem(Pout + n * g.ps, c) = fs[n * W2 + tid];
+        });
+    }
+}
+
 }  // namespace fused
 
 namespace {
@@ -316,6 +423,34 @@ hipError_t fp64_fused_step(const StepArgs& a, const Grid& g, const double* Pin, 
         if (tau1) launch_nts<false, true>(Pin, kind, Pout, g, a, t, s);
         else launch_nts<false, false>(Pin, kind, Pout, g, a, t, s);
     }
+    return hipGetLastError();
+}
+
+template <int I, bool T1>
+void launch_block(const double* Pin, double* Pout, const Grid& g, const StepArgs& a, hipStream_t s)
+{
+    using namespace fused;
+    constexpr int T = kBlockT[I], K = kBlockK[I], W = T + 4 * K;
+    const unsigned nb = static_cast<unsigned>((g.nx / T) * (g.ny / T));
+    k_block9<T, K, T1><<<nb, (W * W + 63) / 64 * 64, 0, s>>>(Pin, Pout, g, a.tau, a.G);
+}
+
+template <bool T1>
+void launch_block_shape(const double* Pin, double* Pout, const Grid& g, const StepArgs& a, int i, hipStream_t s)
+{
+    if (i == 0) launch_block<0, T1>(Pin, Pout, g, a, s);
+    else if (i == 1) launch_block<1, T1>(Pin, Pout, g, a, s);
+    else launch_block<2, T1>(Pin, Pout, g, a, s);
+}
+
+hipError_t fp64_block_step(const StepArgs& a, const Grid& g, const double* Pin, double* Pout, int shape,
+                           hipStream_t s)
+{
+    using namespace fused;
+    if (!block_supported(g, a.q, a.walls != 0, shape)) return hipErrorNotSupported;
+    const int i = shape - kFirstBlockShape;
+    if (a.tau == 1.0) launch_block_shape<true>(Pin, Pout, g, a, i, s);
+    else launch_block_shape<false>(Pin, Pout, g, a, i, s);
     return hipGetLastError();
 }
 

@@ -45,6 +45,8 @@ struct lbm_handle {
     int64_t snap_step = 0;
     // immersed-boundary capsule (lbm_capsule_attach): device arrays in one
     // allocation, nodes placed by lbm_capsule_set_nodes
+    // D2Q9 FP64 temporal block in use (a fused::kFirstBlockShape id), -1: none
+    int block = -1;
     bool has_capsule = false;
     bool capsule_nodes_set = false;
     double* capsule_mem = nullptr;

@@ -83,6 +83,11 @@ bool fp64_fused_supported(const StepArgs& a, const Grid& g);
 // shape fits: D3Q19 tiles (lbm_fused.h) or D2Q9 row segments.
 int fp64_default_shape(const Grid& g, int q);
 bool fp64_shape_ok(const Grid& g, int q, int shape);
+// K steps (K = fused::block_steps(shape)) of a periodic D2Q9 lattice in one
+// launch from post-collision values (not the first step); hipErrorNotSupported
+// where the block shape does not fit.
+hipError_t fp64_block_step(const StepArgs& a, const Grid& g, const double* Pin, double* Pout, int shape,
+                           hipStream_t s);
 // Shan-Chen force from a psi field (FP64, reference arithmetic).
 hipError_t fp64_force(const StepArgs& a, const Grid& g, const double* psi, double* fx, double* fy,
                       double* fz, hipStream_t s);

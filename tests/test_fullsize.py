@@ -68,3 +68,23 @@ def test_gpu_full_size_matches_reference_hash(name, kernel):
         assert _sha(fl["w"].ravel()) == meta["sha256_w"]
     assert _sha(lat.get_f().ravel()) == meta["sha256_f"]
     lat.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("block", [8, 11, 12, 13])
+def test_gpu_full_d2q9_temporal_blocks_match_reference_hash(block):
+    """The 10,000-step 256^2 spinodal with K steps per launch (temporal
+    blocks of 16x16 tiles x 2 or 4 steps, 8x8 x 4; the default here) and
+    with the single-step row march (shape 8) gives the reference's bits."""
+    meta, z = _meta("full_d2q9_256_spinodal_10000")
+    shape = tuple(meta["shape"])
+    print(f"full-size d2q9 block {block}", flush=True)
+    lat = mb.Lattice(9, shape, meta["tau"], meta["G"], kernel=mb.KERNEL_FUSED)
+    lat.set_tuning(block)
+    lat.set_rho(_rho0(meta)); lat.init(); lat.step(meta["steps"])
+    fl = lat.fields(mb.FIELDS_REFERENCE)
+    assert _sha(fl["rho"].ravel()) == meta["sha256_rho"]
+    assert _sha(fl["u"].ravel()) == meta["sha256_u"]
+    assert _sha(fl["v"].ravel()) == meta["sha256_v"]
+    assert _sha(lat.get_f().ravel()) == meta["sha256_f"]
+    lat.close()

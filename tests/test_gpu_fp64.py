@@ -229,6 +229,60 @@ def test_fused_rows9_every_segment_and_chunk_bitwise(shape, order):
             lat.close()
 
 
+BLOCK_SHAPES = (11, 12, 13)  # D2Q9 temporal blocks: 16x16 tiles x 2 steps, 16x16 x 4, 8x8 x 4
+
+
+@pytest.mark.parametrize("shape,order,tau", [((32, 48), 2, 1.0), ((64, 16), 3, 0.8), ((16, 16), 2, 1.3),
+                                             ((80, 96), 2, 1.0), ((8, 24), 3, 1.0)])
+def test_temporal_blocks_bitwise(shape, order, tau):
+    # K steps per launch against single steps of the two-pass kernels: every
+    # block shape whose tile divides the lattice (16 x 16 and 8 x 24 are
+    # narrower than the 24- and 32-node regions, which then hold some nodes
+    # twice), step counts that leave remainders, both step orders and tau;
+    # populations, reference arrays and force after each call
+    rho0 = mb.noise_rho(shape, 29)
+    ref = mb.Lattice(9, shape, tau, -5.0, order=order, kernel=mb.KERNEL_TWO_PASS)
+    ref.set_rho(rho0); ref.init()
+    calls = (1, 6, 11, 5, 2)
+    want = []
+    for n in calls:
+        ref.step(n)
+        want.append((ref.get_f(), ref.fields(mb.FIELDS_REFERENCE), ref.force()))
+    ref.close()
+    tried = 0
+    for block in BLOCK_SHAPES:
+        t = 16 if block < 13 else 8
+        if shape[0] % t or shape[1] % t:
+            continue
+        print(f"shape {shape} order {order} tau {tau} block {block}", flush=True)
+        lat = mb.Lattice(9, shape, tau, -5.0, order=order, kernel=mb.KERNEL_FUSED)
+        lat.set_tuning(block)
+        assert lat.tuning[0] == block
+        lat.set_rho(rho0); lat.init()
+        for n, (f, fl, F) in zip(calls, want):
+            lat.step(n)
+            assert same_bits(lat.get_f(), f), (block, n)
+            got = lat.fields(mb.FIELDS_REFERENCE)
+            for k in ("rho", "u", "v"):
+                assert same_bits(got[k], fl[k]), (block, n, k)
+            assert all(same_bits(a, b) for a, b in zip(lat.force(), F)), (block, n)
+        lat.close()
+        tried += 1
+    assert tried >= 1
+
+
+def test_temporal_blocks_refuse_what_they_do_not_fit():
+    lat = mb.Lattice(9, (40, 32), 1.0, -5.0, kernel=mb.KERNEL_FUSED)
+    with pytest.raises(mb.LbmError):
+        lat.set_tuning(11)                 # 40 is not a multiple of 16
+    lat.set_tuning(13)                     # 8 x 8 tiles fit
+    lat.close()
+    walls = mb.Lattice(9, (32, 32), 1.0, -5.0, kernel=mb.KERNEL_FUSED, walls=True)
+    with pytest.raises(mb.LbmError):
+        walls.set_tuning(12)               # periodic lattices only
+    walls.close()
+
+
 def test_fused_rows9_tau1_long_run_matches_oracle():
     # tau = 1 instantiation over many steps, default launch configuration
     shape = (48, 80)
