@@ -432,13 +432,16 @@ int lbm_batch_set_max_wait(lbm_batch* b, int ticks);
 /* Launch configuration the batch uses (after autotuning): D2Q9 rows per
  * work item (at most; a row-width class whose items would not fill two
  * rounds of resident workgroups gets shorter items, down to 2 rows) and
- * whether D2Q9 jobs use the row-marching kernel (1) or the
- * two-pass kernels (0). */
+ * whether D2Q9 jobs use the row-marching kernel (1), the two-pass kernels
+ * (0) or temporal blocks (2; see lbm_batch_set_launch).  The tuner times
+ * each on one tick. */
 int lbm_batch_tuning(const lbm_batch* b, int* d2q9_chunk, int* d2q9_rows_kernel);
 
 /* Sets the launch configuration (and stops the tuner): D2Q9 rows per work
- * item (>= 1; at most, as above), D2Q9 jobs on the row-marching kernel (1) or the two-pass
- * kernels (0), and the kernel of D3Q19 jobs: 0 the two-pass kernels, 2 the
+ * item (>= 1; at most, as above), D2Q9 jobs on the row-marching kernel (1), the two-pass
+ * kernels (0) or temporal blocks (2: 3 steps per launch on 32x32 tiles held in
+ * LDS, single steps for the last few; the constant-shift forms -- the
+ * node-shift forms keep the row kernel), and the kernel of D3Q19 jobs: 0 the two-pass kernels, 2 the
  * fused march kernel for every job with nx >= 2, ny >= 8 and nz >= 64, 1
  * (default) the march for such jobs in the node-shift forms and the two
  * passes in the others.  A job's D3Q19 kernel depends only on its shape and

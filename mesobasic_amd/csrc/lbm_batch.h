@@ -24,6 +24,9 @@ struct Job {
     int64_t remaining;   // steps still to run (device-updated, 0 = idle)
     int64_t done;        // steps run (device-updated); the current buffer is P[done & 1]
     int q;               // 9 or 19
+    int blk;             // 1: D2Q9 temporal blocks (k_d2q9_block; advances by kBlkK steps per
+                         // launch while more are left, then by 1: always an odd count, so
+                         // P[done & 1] stays current, and the last step is a single one)
 };
 
 // Unit of work of one launch: a range of x planes (D2Q9 row kernel), of
@@ -39,6 +42,12 @@ struct Item {
 // D3Q19 march tile (kMarchTY rows x kMarchTZ columns of a job's y-z cross
 // section); jobs with nx >= 2, ny >= kMarchTY and nz >= kMarchTZ take it.
 constexpr int kMarchTY = 8, kMarchTZ = 64;
+
+// D2Q9 temporal blocks: kBlkK steps of a kBlkT x kBlkT tile (and its halo
+// of 2*kBlkK nodes) per item, in LDS.  kBlkK is odd: a launch advances a job
+// by kBlkK steps while more than kBlkK are left, else by 1, and flips its
+// buffer once.
+constexpr int kBlkT = 32, kBlkK = 3;
 
 // Launch parameters: job and item tables, the ticket counters of the
 // persistent loop (two, alternating by launch parity) and their count.
@@ -61,6 +70,12 @@ hipError_t d3q19_march_step(int prec, const Tables& t, int parity, int grid, hip
 // After a step: remaining -= 1, done += 1 and first = 0 for every stepped
 // job; also clears *mass_ticket, the counter of the next job_mass launch.
 hipError_t advance_counters(Job* jobs, int njobs, unsigned* mass_ticket, hipStream_t s);
+// D2Q9 temporal-block jobs (constant-shift forms) in tiles: each launch runs
+// kBlkK steps of every block job with more than kBlkK left, 1 of the others,
+// bit-identical to as many single steps of the row kernel; advance_blocks
+// then updates their counters (and clears *mass_ticket too).
+hipError_t d2q9_block_step(int prec, const Tables& t, int parity, int grid, hipStream_t s);
+hipError_t advance_blocks(Job* jobs, int njobs, unsigned* mass_ticket, hipStream_t s);
 // FP64 mass of each item's node range of its job's current populations
 // (sum of the pulled f = stored/S + R over the range, as the conservation
 // sums of the single-lattice forms) into part[item]; a persistent launch

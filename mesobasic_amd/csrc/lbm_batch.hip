@@ -241,6 +241,180 @@ __global__ void __launch_bounds__(T) k_d2q9_rows(Tables t, int parity)
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// D2Q9 temporal blocks (constant shift): an item is a T x T tile of one job.
+// The stored values of the tile and a halo of H = 2K nodes (wrapped
+// periodically) go to LDS once; K steps run there (the region whose values
+// are valid shrinks by 2 per step: f is pulled from radius 1, psi of the
+// neighbours from radius 2), each step decoding the pulled stored values,
+// forming rho and psi (barrier), then the force, collision and the stored
+// value of the result, written back in place (all pulls of the step are
+// done at the barrier; a second barrier ends the step).  The arithmetic and
+// the rounding to the stored type are those of k_d2q9_rows, so K steps here
+// give the bits of K single steps.  Tiles past the lattice end wrap; their
+// extra nodes repeat nodes of the first tiles and are not stored.
+
+template <typename SP, int T, int K>
+__global__ void __launch_bounds__(1024) k_d2q9_block(Tables t, int parity)
+{
+    using S = St<9>;
+    using ST = typename SP::T;
+    constexpr int H = 2 * K, W = T + 2 * H, W2 = W * W;
+    constexpr int NPT = (W2 + 1023) / 1024;   // region nodes per thread
+    __shared__ __align__(16) unsigned char fs_raw[9 * W2 * sizeof(ST)];
+    __shared__ float pw[W2];
+    __shared__ float r0[W2];
+    __shared__ int slot_s;
+    ST* fs = reinterpret_cast<ST*>(fs_raw);
+    reset_other(t, parity);
+    // The stored values of the next item are loaded into registers while
+    // the current one computes, so a tile's load latency is hidden behind
+    // the previous tile's steps.
+    ST pre[NPT][9];
+    float pre0[NPT];
+    // region coordinates of this thread's slots (fixed for the launch)
+    int si[NPT], sj[NPT];
+    for (int q = 0; q < NPT; q++) {
+        const int l = threadIdx.x + q * 1024;
+        si[q] = l / W;
+        sj[q] = l - si[q] * W;
+    }
+    auto load = [&](int itn) {
+        const Item item = t.items[itn];
+        const Job& job = t.jobs[item.job];
+        if (job.remaining <= 0) return;
+        const auto* Pin = static_cast<const ST*>(job.P[job.done & 1]);
+        const int nx = job.nx, ny = job.ny;
+        const bool first = job.first != 0 && job.rho0 != nullptr;
+        for (int q = 0; q < NPT; q++) {
+            const int l = threadIdx.x + q * 1024;
+            const int i = si[q], j = sj[q];
+            int gx = (item.x0 - H + i) % nx, gy = (item.y0 - H + j) % ny;
+            gx += gx < 0 ? nx : 0;
+            gy += gy < 0 ? ny : 0;
+            if (l < W2) {
+                const int64_t c = static_cast<int64_t>(gx) * ny + gy;
+                static_for<9>([&](auto I) {
+                    constexpr int n = decltype(I)::value;
+                    pre[q][n] = Pin[n * job.ps + c];
+                });
+                if (first) pre0[q] = job.rho0[c];
+            }
+        }
+    };
+    int it = next_item(t, parity, &slot_s);
+    if (it >= 0) load(it);
+    while (it >= 0) {
+        const Item item = t.items[it];
+        const Job& job = t.jobs[item.job];
+        const bool live = job.remaining > 0;            // uniform across the workgroup
+        const int k = job.remaining > K ? K : 1;
+        // K steps while more than K are left, then single steps: the last
+        // step before a stop is a single one, so the other buffer then
+        // holds f(N-1), as lbm_batch_read expects
+        auto* Pout = static_cast<ST*>(job.P[(job.done & 1) ^ 1]);
+        const int nx = job.nx, ny = job.ny;
+        const int64_t ps = job.ps;
+        const float tau = job.tau, G = job.G, itau = 1.0f / tau, rs = job.rs;
+        const bool first = job.first != 0 && job.rho0 != nullptr;
+        if (live) {
+            for (int q = 0; q < NPT; q++) {
+                const int l = threadIdx.x + q * 1024;
+                if (l < W2) {
+                    static_for<9>([&](auto I) {
+                        constexpr int n = decltype(I)::value;
+                        fs[n * W2 + l] = pre[q][n];
+                    });
+                    if (first) r0[l] = pre0[q];
+                }
+            }
+        }
+        // next item (its barrier also publishes the LDS writes above), its
+        // loads in flight during this item's steps
+        const int nit = next_item(t, parity, &slot_s);
+        if (nit >= 0) load(nit);
+        if (live) {
+            float d[NPT][9];
+            float rp[NPT];
+            for (int s = 0; s < k; s++) {
+                const int lo = 2 * s + 1, hi = W - 2 * s - 1;
+                const bool f0 = first && s == 0;
+                for (int q = 0; q < NPT; q++) {
+                    const int l = threadIdx.x + q * 1024;
+                    const int i = si[q], j = sj[q];
+                    if (l < W2 && i >= lo && i < hi && j >= lo && j < hi) {
+                        static_for<9>([&](auto I) {
+                            constexpr int n = decltype(I)::value;
+                            d[q][n] = SP::d(fs[n * W2 + l - (S::ex[n] * W + S::ey[n])]);
+                        });
+                        rp[q] = pulled_rho<9, SP, false>(d[q], rs);
+                        pw[l] = psi_of(f0 ? r0[l] : rp[q]);
+                    }
+                }
+                __syncthreads();
+                for (int q = 0; q < NPT; q++) {
+                    const int l = threadIdx.x + q * 1024;
+                    const int i = si[q], j = sj[q];
+                    if (l < W2 && i > lo && i < hi - 1 && j > lo && j < hi - 1) {
+                        float jx = 0.f, jy = 0.f;
+                        static_for<9>([&](auto I) {
+                            constexpr int n = decltype(I)::value;
+                            acc_e<S::ex[n]>(jx, d[q][n]);
+                            acc_e<S::ey[n]>(jy, d[q][n]);
+                        });
+                        float rho, ux, uy;
+                        if (f0) {
+                            rho = r0[l];
+                            ux = 0.f; uy = 0.f;
+                        } else {
+                            rho = rp[q];
+                            const float ir = 1.0f / rho;
+                            const float js = 1.0f / kScale<SP, false>;
+                            ux = (jx * js) * ir; uy = (jy * js) * ir;
+                        }
+                        float gx = 0.f, gy = 0.f;
+                        static_for<9>([&](auto I) {
+                            constexpr int n = decltype(I)::value;
+                            if constexpr (n > 0) {
+                                const float tt = static_cast<float>(S::w[n]) * pw[l + S::ex[n] * W + S::ey[n]];
+                                acc_e<S::ex[n]>(gx, tt);
+                                acc_e<S::ey[n]>(gy, tt);
+                            }
+                        });
+                        const float kk = (-G * pw[l]) * tau / rho;
+                        const float ue = ux + kk * gx, ve = uy + kk * gy;
+                        const float usq15 = 1.5f * (ue * ue + ve * ve);
+                        static_for<9>([&](auto I) {
+                            constexpr int n = decltype(I)::value;
+                            const float ev = edot<S::ex[n], S::ey[n], 0>(ue, ve, 0.f);
+                            const float poly = 1.0f + 3.0f * ev + 4.5f * ev * ev - usq15;
+                            const float v = post_value<SP, false>(d[q][n], static_cast<float>(S::w[n]), rho,
+                                                                  poly, rs, 0.f, itau);
+                            fs[n * W2 + l] = SP::of_d(v);
+                        });
+                    }
+                }
+                __syncthreads();
+            }
+            for (int q = 0; q < NPT; q++) {
+                const int l = threadIdx.x + q * 1024;
+                const int i = si[q], j = sj[q];
+                const int xu = item.x0 - H + i, yu = item.y0 - H + j;
+                if (l < W2 && i >= H && i < W - H && j >= H && j < W - H && xu < nx && yu < ny) {
+                    const int64_t c = static_cast<int64_t>(xu) * ny + yu;
+                    static_for<9>([&](auto I) {
+                        constexpr int n = decltype(I)::value;
+                        SP::put_nt(Pout + n * ps + c, fs[n * W2 + l]);
+                    });
+                }
+            }
+        }
+        __syncthreads();   // the stores read fs before the next item's values replace it
+        it = nit;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Two-pass kernels for any job (D2Q9 or D3Q19, any size): items are node
 // ranges; pass 1 writes psi (and the node-shift reference), pass 2 collides.
@@ -467,9 +641,22 @@ __global__ void k_advance(Job* jobs, int njobs, unsigned* mass_ticket)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) *mass_ticket = 0u;
-    if (i < njobs && jobs[i].remaining > 0) {
+    if (i < njobs && !jobs[i].blk && jobs[i].remaining > 0) {
         jobs[i].remaining -= 1;
         jobs[i].done += 1;
+        jobs[i].first = 0;
+    }
+}
+
+// After a temporal-block launch: the steps k_d2q9_block ran for each block job.
+__global__ void k_advance_blk(Job* jobs, int njobs, unsigned* mass_ticket)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *mass_ticket = 0u;
+    if (i < njobs && jobs[i].blk && jobs[i].remaining > 0) {
+        const int64_t k = jobs[i].remaining > kBlkK ? kBlkK : 1;
+        jobs[i].remaining -= k;
+        jobs[i].done += k;
         jobs[i].first = 0;
     }
 }
@@ -677,6 +864,23 @@ hipError_t d3q19_march_step(int prec, const Tables& t, int parity, int grid, hip
 hipError_t advance_counters(Job* jobs, int njobs, unsigned* mass_ticket, hipStream_t s)
 {
     k_advance<<<(std::max(njobs, 1) + 255) / 256, 256, 0, s>>>(jobs, njobs, mass_ticket);
+    return hipGetLastError();
+}
+
+hipError_t advance_blocks(Job* jobs, int njobs, unsigned* mass_ticket, hipStream_t s)
+{
+    k_advance_blk<<<(std::max(njobs, 1) + 255) / 256, 256, 0, s>>>(jobs, njobs, mass_ticket);
+    return hipGetLastError();
+}
+
+hipError_t d2q9_block_step(int prec, const Tables& t, int parity, int grid, hipStream_t s)
+{
+    if (prec == LBM_FP16N || prec == LBM_BF16N) return hipErrorNotSupported;
+    by_prec(prec, [&](auto sp, auto ns) {
+        using SP = decltype(sp);
+        if constexpr (!decltype(ns)::value)
+            k_d2q9_block<SP, kBlkT, kBlkK><<<grid, 1024, 0, s>>>(t, parity);
+    });
     return hipGetLastError();
 }
 

@@ -31,9 +31,10 @@ namespace {
 
 // Kernel classes of one step: D2Q9 row-marching kernels by row width, the
 // two-pass kernels (psi then collide) of each stencil and the D3Q19 march;
-// kMass is the per-tick mass check that follows the steps of a tick.
+// kBlk9 the D2Q9 temporal blocks (several steps per launch); kMass is the
+// per-tick mass check that follows the steps of a tick.
 enum Cls { kRow64, kRow128, kRow256, kRow512, kRow1024, kTp9Psi, kTp9Col, kTp19Psi, kTp19Col, kMarch19,
-           kMass, kNumCls };
+           kBlk9, kMass, kNumCls };
 constexpr int kRowWidth[5] = {64, 128, 256, 512, 1024};
 // Nodes per two-pass work item.  Serving stream (200 requests, 77% of the
 // work in ragged D3Q19 jobs on the two passes), GLUPS / p99 s by item size:
@@ -109,7 +110,7 @@ struct lbm_batch {
     bool dirty = true;
     // launch configuration
     int chunk = 32;               // D2Q9 rows per row-kernel item
-    int rows_kernel = 1;          // D2Q9 jobs on the row kernel (1) or two-pass (0)
+    int rows_kernel = 1;          // D2Q9 jobs on the row kernel (1), two-pass (0) or temporal blocks (2)
     int march19 = 1;              // D3Q19 kernel mode: 0 two-pass, 1 by shape, 2 march where it fits
     int tune_left = 0;            // ticks left to try candidate configurations
     struct Config {
@@ -140,9 +141,16 @@ int row_class(int ny)
     return -1;
 }
 
+// D2Q9 temporal blocks: the constant-shift forms (the node-shift forms keep
+// the row kernel).
+bool use_blocks(const lbm_batch* b, const HostJob& j)
+{
+    return j.d.q == 9 && b->rows_kernel == 2 && !node_shift(b->d.precision);
+}
+
 bool use_rows(const lbm_batch* b, const HostJob& j)
 {
-    return j.d.q == 9 && b->rows_kernel && row_class(j.d.ny) >= 0;
+    return j.d.q == 9 && b->rows_kernel && !use_blocks(b, j) && row_class(j.d.ny) >= 0;
 }
 
 // D3Q19 kernel of a job -- a function of its own shape and the batch's mode
@@ -211,9 +219,10 @@ int ensure_mass(lbm_batch* b, int n)
     return LBM_OK;
 }
 
-Job device_view(const HostJob& j, int prec, double rs)
+Job device_view(const HostJob& j, int prec, double rs, bool blk = false)
 {
     Job v{};
+    v.blk = blk ? 1 : 0;
     v.P[0] = j.P[0];
     v.P[1] = j.P[1];
     v.psi = j.psi;
@@ -296,7 +305,7 @@ int upload_tables(lbm_batch* b)
     }
     for (int ji = 0; ji < static_cast<int>(b->order.size()); ji++) {
         const HostJob& j = *b->order[ji];
-        b->h_jobs[ji] = device_view(j, b->d.precision, b->d.rho_shift);
+        b->h_jobs[ji] = device_view(j, b->d.precision, b->d.rho_shift, use_blocks(b, j));
         for (int64_t c0 = 0; c0 < j.n; c0 += kMassItem) {
             Item it{};
             it.job = ji;
@@ -305,7 +314,16 @@ int upload_tables(lbm_batch* b)
             items[kMass].push_back(it);
             b->mass_job.push_back(ji);
         }
-        if (use_rows(b, j)) {
+        if (use_blocks(b, j)) {
+            for (int x0 = 0; x0 < j.d.nx; x0 += lbm::batch::kBlkT)
+                for (int y0 = 0; y0 < j.d.ny; y0 += lbm::batch::kBlkT) {
+                    Item it{};
+                    it.job = ji;
+                    it.x0 = x0;
+                    it.y0 = y0;
+                    items[kBlk9].push_back(it);
+                }
+        } else if (use_rows(b, j)) {
             const int cls = row_class(j.d.ny);
             for (int x0 = 0; x0 < j.d.nx; x0 += chunk[cls]) {
                 Item it{};
@@ -368,6 +386,50 @@ int upload_tables(lbm_batch* b)
     return LBM_OK;
 }
 
+int launch_step(lbm_batch* b, int parity);
+int launch_mass(lbm_batch* b);
+
+// Temporal-block launches of a tick: an even number (the ticket parity of
+// the class returns to 0 for the next tick) of at least steps_per_tick /
+// kBlkK, so block jobs advance at least steps_per_tick steps per tick.
+int block_launches(const lbm_batch* b)
+{
+    const int k = lbm::batch::kBlkK;
+    const int l = (b->d.steps_per_tick + k - 1) / k;
+    return l + (l & 1);
+}
+
+int launch_blocks(lbm_batch* b, int parity)
+{
+    Tables t{b->d_jobs, b->d_items[kBlk9], b->d_nitems + kBlk9, b->d_tickets + 2 * kBlk9};
+    BHIP(lbm::batch::d2q9_block_step(b->d.precision, t, parity, persistent_grid(b, 1024), b->stream));
+    BHIP(lbm::batch::advance_blocks(b->d_jobs, b->cap_jobs, b->d_tickets + 2 * kMass, b->stream));
+    return LBM_OK;
+}
+
+// Classes advanced one step per launch (all but the blocks and the mass check).
+bool has_single_steps(const lbm_batch* b)
+{
+    return (b->class_mask & ~((1u << kBlk9) | (1u << kMass))) != 0;
+}
+
+// The launches of a tick: the block launches (if any job takes them), the
+// steps_per_tick single steps (if any job takes them), the mass check.
+int launch_tick(lbm_batch* b)
+{
+    if (b->class_mask & (1u << kBlk9))
+        for (int m = 0; m < block_launches(b); m++) {
+            int rc = launch_blocks(b, m & 1);
+            if (rc) return rc;
+        }
+    if (has_single_steps(b))
+        for (int s = 0; s < b->d.steps_per_tick; s++) {
+            int rc = launch_step(b, s & 1);
+            if (rc) return rc;
+        }
+    return launch_mass(b);
+}
+
 // One step of every job: row kernels, two-pass kernels, counters.  Only the
 // classes that hold items are launched.
 int launch_step(lbm_batch* b, int parity)
@@ -419,9 +481,7 @@ int prepare_tick(lbm_batch* b)
             b->graph = nullptr;
             hipGraph_t g = nullptr;
             BHIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-            int rc = LBM_OK;
-            for (int s = 0; s < K && rc == LBM_OK; s++) rc = launch_step(b, s & 1);
-            if (rc == LBM_OK) rc = launch_mass(b);
+            int rc = launch_tick(b);
             hipError_t e = hipStreamEndCapture(b->stream, &g);
             if (rc) return rc;
             if (e != hipSuccess) return lbm::set_error(LBM_EHIP, "graph capture: %s", hipGetErrorString(e));
@@ -436,17 +496,11 @@ int prepare_tick(lbm_batch* b)
 
 int run_tick(lbm_batch* b)
 {
-    const int K = b->d.steps_per_tick;
     if (b->d.use_graph) {
         BHIP(hipGraphLaunch(b->graph, b->stream));
-    } else {
-        for (int s = 0; s < K; s++) {
-            int rc = launch_step(b, s & 1);
-            if (rc) return rc;
-        }
-        return launch_mass(b);
+        return LBM_OK;
     }
-    return LBM_OK;
+    return launch_tick(b);
 }
 
 void free_job(lbm_batch* b, HostJob* j)
@@ -508,6 +562,7 @@ int lbm_batch_create(const lbm_batch_desc* desc, lbm_batch** out)
         for (int rk : {1, 0})
             for (int ch : {16, 32, 64})
                 if (rk || ch == 32) b->candidates.push_back({rk, ch});
+        if (!node_shift(b->d.precision)) b->candidates.push_back({2, 32});   // temporal blocks
         b->cand_rate.assign(b->candidates.size(), 0.0);
         b->tune_left = static_cast<int>(b->candidates.size());
         b->rows_kernel = b->candidates[0].rows_kernel;
@@ -612,8 +667,8 @@ int lbm_batch_tick(lbm_batch* b, int* n_finished)
         if (rc) return rc;
     }
     const auto t0 = std::chrono::steady_clock::now();
-    int64_t nodes = 0;
-    for (HostJob* j : b->order) nodes += j->n * std::min<int64_t>(j->remaining, b->d.steps_per_tick);
+    std::vector<int64_t> done_before(b->order.size());
+    for (size_t ji = 0; ji < b->order.size(); ji++) done_before[ji] = b->order[ji]->done;
     if (!b->order.empty()) {
         int rc = run_tick(b);
         if (rc) return rc;
@@ -625,6 +680,9 @@ int lbm_batch_tick(lbm_batch* b, int* n_finished)
     }
     BHIP(hipStreamSynchronize(b->stream));
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    int64_t nodes = 0;   // lattice updates of the tick
+    for (size_t ji = 0; ji < b->order.size(); ji++)
+        nodes += b->order[ji]->n * (b->h_jobs[ji].done - done_before[ji]);
     b->steps_run += b->d.steps_per_tick;
     std::vector<double> mass(b->order.size(), 0.0);
     for (int k = 0; k < b->nitems[kMass]; k++) mass[b->mass_job[k]] += b->h_mass[k];
@@ -849,8 +907,10 @@ int lbm_batch_set_launch(lbm_batch* b, int d2q9_chunk, int d2q9_rows_kernel, int
 {
     if (!b) return lbm::set_error(LBM_EINVAL, "lbm_batch_set_launch: NULL batch");
     if (d2q9_chunk < 1) return lbm::set_error(LBM_EINVAL, "lbm_batch_set_launch: chunk must be >= 1");
+    if (d2q9_rows_kernel < 0 || d2q9_rows_kernel > 2)
+        return lbm::set_error(LBM_EINVAL, "lbm_batch_set_launch: d2q9_rows_kernel must be 0, 1 or 2");
     b->chunk = d2q9_chunk;
-    b->rows_kernel = d2q9_rows_kernel ? 1 : 0;
+    b->rows_kernel = d2q9_rows_kernel;
     if (d3q19_march < 0 || d3q19_march > 2)
         return lbm::set_error(LBM_EINVAL, "lbm_batch_set_launch: d3q19_march must be 0, 1 or 2");
     b->march19 = d3q19_march;

@@ -231,11 +231,14 @@ def make_batched_jobs(n=512, seed=11):
 
 
 def run_batched(jobs, steps=2000, precision=FP16S, steps_per_tick=20, autotune=True,
-                clock=time.perf_counter):
+                clock=time.perf_counter, launch=None):
     """All jobs submitted, then ticked to completion; returns a summary.
     Jobs the engine retires as failed (non-finite mass) count their work up
     to their last finite mass check only."""
-    b = Batch(precision=precision, steps_per_tick=steps_per_tick, use_graph=True, autotune=autotune)
+    b = Batch(precision=precision, steps_per_tick=steps_per_tick, use_graph=True,
+              autotune=autotune and launch is None)
+    if launch is not None:
+        b.set_launch(*launch)   # (d2q9_chunk, d2q9_rows_kernel, d3q19_march); the tuner stays off
     ids = []
     for j in jobs:
         ids.append(b.submit(9, j["shape"], j["tau"], j["G"], steps, noise_rho(j["shape"], j["seed"])))
@@ -288,10 +291,13 @@ def main():
                     help="nodes advanced per step, least remaining work first (0: all)")
     ap.add_argument("--precision", default=None, choices=["fp32", "fp16", "bf16", "fp16n", "bf16n"],
                     help="storage form (default: fp16n for the stream, fp16 for --batched)")
+    ap.add_argument("--d2q9-kernel", default="auto", choices=["auto", "rows", "two_pass", "blocks"],
+                    help="--batched: D2Q9 kernel (auto: the tuner picks)")
     args = ap.parse_args()
+    launch = {"auto": None, "rows": (32, 1, 1), "two_pass": (32, 0, 1), "blocks": (32, 2, 1)}[args.d2q9_kernel]
     prec = PRECISIONS[args.precision or ("fp16" if args.batched else "fp16n")]
     if args.batched:
-        s = run_batched(make_batched_jobs(args.jobs), args.steps, prec, args.steps_per_tick)
+        s = run_batched(make_batched_jobs(args.jobs), args.steps, prec, args.steps_per_tick, launch=launch)
     else:
         s = run_stream(make_requests(args.requests, args.rate, args.seed), prec,
                        int(args.budget_gb * (1 << 30)), args.max_active, args.steps_per_tick,

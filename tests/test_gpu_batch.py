@@ -261,3 +261,64 @@ def test_wait_bound_keeps_a_large_job_from_starving(max_wait):
         while not alone.finished():
             alone.tick()
         assert same_bits(alone.collect(jid)["rho"], got[1]["rho"])
+
+
+BLOCK_JOBS = [dict(q=9, shape=(64, 64), tau=1.0, G=-5.0, steps=13, seed=41),
+              dict(q=9, shape=(33, 70), tau=0.7, G=-5.5, steps=10, seed=42),
+              dict(q=9, shape=(100, 37), tau=1.3, G=-4.8, steps=7, seed=43),
+              dict(q=9, shape=(20, 20), tau=0.9, G=-5.2, steps=29, seed=44),    # smaller than a tile
+              dict(q=9, shape=(3, 5), tau=1.1, G=-5.0, steps=5, seed=45),       # smaller than the halo
+              dict(q=9, shape=(97, 130), tau=0.8, G=-5.0, steps=44, seed=46),
+              dict(q=19, shape=(12, 10, 16), tau=1.0, G=-5.5, steps=9, seed=47)]
+
+
+@pytest.mark.parametrize("prec", [mb.FP32, mb.FP16S, mb.BF16S])
+@pytest.mark.parametrize("spt", [4, 20])
+def test_temporal_blocks_equal_the_row_kernel_bitwise(prec, spt):
+    # D2Q9 jobs in temporal blocks (3 steps per launch, 1 when fewer are
+    # left) against the row kernel: ragged sizes, lattices smaller than a
+    # tile and than the halo, odd and even step counts, the LBfluid2D-order
+    # first step inside a block, a D3Q19 job alongside; tick lengths 4 and 20
+    rows = Batch(precision=prec, steps_per_tick=spt)
+    rows.set_launch(32, 1, 1)
+    blocks = Batch(precision=prec, steps_per_tick=spt)
+    blocks.set_launch(32, 2, 1)
+    assert blocks.tuning["d2q9_rows_kernel"] == 2
+    print(f"blocks vs rows precision {prec} steps/tick {spt}", flush=True)
+    a = _run(rows, BLOCK_JOBS)
+    b = _run(blocks, BLOCK_JOBS)
+    for k in range(len(BLOCK_JOBS)):
+        for f in ("rho", "u", "v", "w"):
+            assert same_bits(a[k][f], b[k][f]), (k, f, BLOCK_JOBS[k])
+        assert a[k]["steps_done"] == b[k]["steps_done"] == BLOCK_JOBS[k]["steps"]
+
+
+def test_temporal_blocks_read_extend_and_node_shift_fallback():
+    # a block job stopped at an odd count, read, extended, equals one run
+    # through; the node-shift forms keep the row kernel under the same setting
+    shape, tau, G = (48, 40), 0.9, -5.5
+    rho0 = mb.noise_rho(shape, 9)
+    ref = Batch(precision=mb.FP16S, steps_per_tick=6)
+    ref.set_launch(32, 1, 1)
+    want = _run(ref, [dict(q=9, shape=shape, tau=tau, G=G, steps=17, seed=9)])[0]
+    b = Batch(precision=mb.FP16S, steps_per_tick=6)
+    b.set_launch(32, 2, 1)
+    jid = b.submit(9, shape, tau, G, 7, rho0)
+    while jid not in b.finished():
+        b.tick()
+    assert b.status(jid)["steps_done"] == 7
+    b.extend(jid, 10)
+    while jid not in b.finished():
+        b.tick()
+    got = b.collect(jid)
+    for f in ("rho", "u", "v"):
+        assert same_bits(got[f], want[f]), f
+    for prec in (mb.FP16N, mb.BF16N):
+        r = Batch(precision=prec, steps_per_tick=6)
+        r.set_launch(32, 1, 1)
+        n = Batch(precision=prec, steps_per_tick=6)
+        n.set_launch(32, 2, 1)
+        x = _run(r, BLOCK_JOBS[:3])
+        y = _run(n, BLOCK_JOBS[:3])
+        for k in range(3):
+            assert same_bits(x[k]["rho"], y[k]["rho"]), (prec, k)

@@ -96,6 +96,27 @@ def test_read_and_extend(prec, q, shape):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("prec", [mb.FP32, mb.FP16S])
+def test_read_after_temporal_blocks(prec):
+    # D2Q9 jobs advanced in temporal blocks (3 steps per launch): a job read
+    # at its stop still sees f(N-1) in its other buffer (the last step before
+    # a stop is a single one), so its force array is that of rho(N-1)
+    shape, tau, G = (40, 36), 0.9, -5.5
+    rho0 = mb.noise_rho(shape, 12)
+    b = Batch(precision=prec, steps_per_tick=6)
+    b.set_launch(32, 2, 1)
+    ja, jb = b.submit(9, shape, tau, G, 11, rho0), b.submit(9, shape, tau, G, 10, rho0)
+    _run_until(b, [ja, jb])
+    got = b.read(ja)
+    prev = b.collect(jb)
+    ref = b.collect(ja)
+    for k in ("rho", "u", "v"):
+        assert same_bits(got[k], ref[k]), k
+    F = _oracle_force(9, shape, G, prev["rho"])
+    assert same_bits(got["fx"], F[0]) and same_bits(got["fy"], F[1])
+
+
+@pytest.mark.gpu
 def test_extend_refuses_failed_jobs():
     shape = (24, 20)
     rho0 = mb.noise_rho(shape, 3)
