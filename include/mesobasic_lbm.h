@@ -429,6 +429,15 @@ int lbm_batch_set_work_limit(lbm_batch* b, int64_t max_nodes);
  * it. */
 int lbm_batch_set_max_wait(lbm_batch* b, int ticks);
 
+/* Concurrent kernel classes (1, default): within a tick, the steps of each
+ * kernel class (a D2Q9 row width, a two-pass stencil, the D3Q19 march, the
+ * temporal blocks) run on a stream of their own, forked from the batch's
+ * stream and joined before the tick's mass check, so a class of a few small
+ * jobs overlaps the large ones.  0: every launch on the batch's stream, one
+ * class after another.  Results do not depend on it (no reference
+ * counterpart: the reference runs one lattice per process). */
+int lbm_batch_set_concurrency(lbm_batch* b, int on);
+
 /* Launch configuration the batch uses (after autotuning): D2Q9 rows per
  * work item (at most; a row-width class whose items would not fill two
  * rounds of resident workgroups gets shorter items, down to 2 rows) and
@@ -443,8 +452,9 @@ int lbm_batch_tuning(const lbm_batch* b, int* d2q9_chunk, int* d2q9_rows_kernel)
  * LDS, single steps for the last few; the constant-shift forms -- the
  * node-shift forms keep the row kernel), and the kernel of D3Q19 jobs: 0 the two-pass kernels, 2 the
  * fused march kernel for every job with nx >= 2, ny >= 8 and nz >= 64, 1
- * (default) the march for such jobs in the node-shift forms and the two
- * passes in the others.  A job's D3Q19 kernel depends only on its shape and
+ * (default) the march for such jobs in the node-shift forms when their y-z
+ * cross-section fills at least 90% of the 8 x 64 march tiles it needs, and
+ * the two passes for the others.  A job's D3Q19 kernel depends only on its shape and
  * this mode.  D2Q9 results
  * do not depend on the configuration; D3Q19 results of the two kernels
  * agree to FP32 rounding, not bit for bit, and the tuner never changes the

@@ -27,6 +27,8 @@ struct Job {
     int blk;             // 1: D2Q9 temporal blocks (k_d2q9_block; advances by kBlkK steps per
                          // launch while more are left, then by 1: always an odd count, so
                          // P[done & 1] stays current, and the last step is a single one)
+    int chain;           // launch chain that steps the job (one per kernel class, see
+                         // advance_counters)
 };
 
 // Unit of work of one launch: a range of x planes (D2Q9 row kernel), of
@@ -68,8 +70,10 @@ hipError_t twopass_collide(int prec, int q, const Tables& t, int parity, int gri
 // passes above to rounding.
 hipError_t d3q19_march_step(int prec, const Tables& t, int parity, int grid, hipStream_t s);
 // After a step: remaining -= 1, done += 1 and first = 0 for every stepped
-// job; also clears *mass_ticket, the counter of the next job_mass launch.
-hipError_t advance_counters(Job* jobs, int njobs, unsigned* mass_ticket, hipStream_t s);
+// job of launch chain `chain` (every single-step job if chain < 0); also
+// clears *mass_ticket, the counter of the next job_mass launch.  Chains are
+// disjoint sets of jobs, so the chains of one tick can run concurrently.
+hipError_t advance_counters(Job* jobs, int njobs, int chain, unsigned* mass_ticket, hipStream_t s);
 // D2Q9 temporal-block jobs (constant-shift forms) in tiles: each launch runs
 // kBlkK steps of every block job with more than kBlkK left, 1 of the others,
 // bit-identical to as many single steps of the row kernel; advance_blocks

@@ -637,11 +637,11 @@ __global__ void __launch_bounds__(kMarchTY * kMarchTZ) __attribute__((amdgpu_wav
     }
 }
 
-__global__ void k_advance(Job* jobs, int njobs, unsigned* mass_ticket)
+__global__ void k_advance(Job* jobs, int njobs, int chain, unsigned* mass_ticket)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) *mass_ticket = 0u;
-    if (i < njobs && !jobs[i].blk && jobs[i].remaining > 0) {
+    if (i < njobs && !jobs[i].blk && (chain < 0 || jobs[i].chain == chain) && jobs[i].remaining > 0) {
         jobs[i].remaining -= 1;
         jobs[i].done += 1;
         jobs[i].first = 0;
@@ -861,9 +861,9 @@ hipError_t d3q19_march_step(int prec, const Tables& t, int parity, int grid, hip
     return hipGetLastError();
 }
 
-hipError_t advance_counters(Job* jobs, int njobs, unsigned* mass_ticket, hipStream_t s)
+hipError_t advance_counters(Job* jobs, int njobs, int chain, unsigned* mass_ticket, hipStream_t s)
 {
-    k_advance<<<(std::max(njobs, 1) + 255) / 256, 256, 0, s>>>(jobs, njobs, mass_ticket);
+    k_advance<<<(std::max(njobs, 1) + 255) / 256, 256, 0, s>>>(jobs, njobs, chain, mass_ticket);
     return hipGetLastError();
 }
 

@@ -122,6 +122,16 @@ struct lbm_batch {
     // tick graph
     hipGraphExec_t graph = nullptr;
     bool graph_valid = false;
+    // launch chains: the steps of each kernel class go to a stream of their
+    // own (default priority), forked from `stream` at the start of a tick and
+    // joined into it before the mass check.  A class's jobs depend only on
+    // its own launches, so the classes of a mix overlap on the GPU: a class of
+    // a few small jobs, whose launches are short serial chains on an almost
+    // idle GPU, runs beside the large classes instead of after them.
+    hipStream_t cstream[kNumCls] = {};
+    hipEvent_t ev_fork = nullptr;
+    hipEvent_t ev_join[kNumCls] = {};
+    bool concurrent = true;
     // pinned staging for per-tick counters
     Job* h_jobs = nullptr;
     // per-tick mass check: one FP64 partial per kMass item, the job (table
@@ -155,18 +165,37 @@ bool use_rows(const lbm_batch* b, const HostJob& j)
 
 // D3Q19 kernel of a job -- a function of its own shape and the batch's mode
 // only, so a job runs the same kernel in any mix.  Mode 1 (default): the
-// march for the node-shift forms (FP16, GLUPS, march vs two passes: 35.7 vs
-// 27.4 on 128^3 jobs, 20.0 vs 22.5 on 64..192-per-side jobs); the two
-// passes for the other forms, where the march wins on tile-aligned jobs
-// (39.4 vs 34.4) but loses on ragged ones (21.9 vs 26.8) and, run for only
-// part of a mix, slowed the serving stream (an extra, under-filled launch
-// per step).  2: the march wherever it fits; 0: the two passes.
+// march for node-shift jobs whose y-z cross-section fills at least
+// kMarchMinFill of the march tiles it needs, the two passes for the others
+// and for the constant-shift forms.  The march's partial tiles cost in
+// proportion to the unfilled part.  Measured on MI355X (FP16 node shift,
+// GLUPS, march vs two passes): 128^3 jobs 35.7 vs 27.4; the 64 D3Q19
+// requests of the serving stream (64..192 per side) 21.9 vs 27.5; constant
+// shift, the same requests 23.8 vs 34.9.  2: the march wherever it fits;
+// 0: the two passes.
+constexpr double kMarchMinFill = 0.9;
+
 bool use_march(const lbm_batch* b, const HostJob& j)
 {
     const bool fits = j.d.q == 19 && j.d.nx >= 2 && j.d.ny >= lbm::batch::kMarchTY &&
                       j.d.nz >= lbm::batch::kMarchTZ;
     if (!fits || b->march19 == 0) return false;
-    return b->march19 == 2 || node_shift(b->d.precision);
+    if (b->march19 == 2) return true;
+    if (!node_shift(b->d.precision)) return false;
+    const int64_t ty = lbm::batch::kMarchTY, tz = lbm::batch::kMarchTZ;
+    const double cover = static_cast<double>((j.d.ny + ty - 1) / ty * ty) *
+                         static_cast<double>((j.d.nz + tz - 1) / tz * tz);
+    return static_cast<double>(j.d.ny) * j.d.nz >= kMarchMinFill * cover;
+}
+
+// Launch chain of a job: the kernel class that steps it (the two-pass
+// classes by their psi pass).
+int chain_of(const lbm_batch* b, const HostJob& j)
+{
+    if (use_blocks(b, j)) return kBlk9;
+    if (use_rows(b, j)) return row_class(j.d.ny);
+    if (use_march(b, j)) return kMarch19;
+    return j.d.q == 9 ? kTp9Psi : kTp19Psi;
 }
 
 int ensure_items(lbm_batch* b, int cls, int n)
@@ -219,10 +248,11 @@ int ensure_mass(lbm_batch* b, int n)
     return LBM_OK;
 }
 
-Job device_view(const HostJob& j, int prec, double rs, bool blk = false)
+Job device_view(const HostJob& j, int prec, double rs, bool blk = false, int chain = -1)
 {
     Job v{};
     v.blk = blk ? 1 : 0;
+    v.chain = chain;
     v.P[0] = j.P[0];
     v.P[1] = j.P[1];
     v.psi = j.psi;
@@ -305,7 +335,7 @@ int upload_tables(lbm_batch* b)
     }
     for (int ji = 0; ji < static_cast<int>(b->order.size()); ji++) {
         const HostJob& j = *b->order[ji];
-        b->h_jobs[ji] = device_view(j, b->d.precision, b->d.rho_shift, use_blocks(b, j));
+        b->h_jobs[ji] = device_view(j, b->d.precision, b->d.rho_shift, use_blocks(b, j), chain_of(b, j));
         for (int64_t c0 = 0; c0 < j.n; c0 += kMassItem) {
             Item it{};
             it.job = ji;
@@ -399,63 +429,103 @@ int block_launches(const lbm_batch* b)
     return l + (l & 1);
 }
 
-int launch_blocks(lbm_batch* b, int parity)
+int launch_blocks(lbm_batch* b, int parity, hipStream_t s)
 {
     Tables t{b->d_jobs, b->d_items[kBlk9], b->d_nitems + kBlk9, b->d_tickets + 2 * kBlk9};
-    BHIP(lbm::batch::d2q9_block_step(b->d.precision, t, parity, persistent_grid(b, 1024), b->stream));
-    BHIP(lbm::batch::advance_blocks(b->d_jobs, b->cap_jobs, b->d_tickets + 2 * kMass, b->stream));
+    BHIP(lbm::batch::d2q9_block_step(b->d.precision, t, parity, persistent_grid(b, 1024), s));
+    BHIP(lbm::batch::advance_blocks(b->d_jobs, b->cap_jobs, b->d_tickets + 2 * kMass, s));
     return LBM_OK;
 }
 
-// Classes advanced one step per launch (all but the blocks and the mass check).
-bool has_single_steps(const lbm_batch* b)
+// The launches of one step of chain c (a row class, a two-pass class by its
+// psi pass, or the D3Q19 march) on stream s, without the counter update.
+int launch_class(lbm_batch* b, int c, int parity, hipStream_t s)
 {
-    return (b->class_mask & ~((1u << kBlk9) | (1u << kMass))) != 0;
+    if (c < 5) {
+        Tables t{b->d_jobs, b->d_items[c], b->d_nitems + c, b->d_tickets + 2 * c};
+        BHIP(lbm::batch::d2q9_rows_step(b->d.precision, t, parity, kRowWidth[c],
+                                        persistent_grid(b, kRowWidth[c]), s));
+    } else if (c == kTp9Psi || c == kTp19Psi) {
+        const int q = c == kTp9Psi ? 9 : 19;
+        Tables tp{b->d_jobs, b->d_items[c], b->d_nitems + c, b->d_tickets + 2 * c};
+        Tables tc{b->d_jobs, b->d_items[c + 1], b->d_nitems + c + 1, b->d_tickets + 2 * (c + 1)};
+        BHIP(lbm::batch::twopass_psi(b->d.precision, q, tp, parity, persistent_grid(b, 256), s));
+        BHIP(lbm::batch::twopass_collide(b->d.precision, q, tc, parity, persistent_grid(b, 256), s));
+    } else if (c == kMarch19) {
+        Tables tm{b->d_jobs, b->d_items[kMarch19], b->d_nitems + kMarch19, b->d_tickets + 2 * kMarch19};
+        BHIP(lbm::batch::d3q19_march_step(b->d.precision, tm, parity,
+                                          persistent_grid(b, lbm::batch::kMarchTY * lbm::batch::kMarchTZ), s));
+    }
+    return LBM_OK;
 }
 
-// The launches of a tick: the block launches (if any job takes them), the
-// steps_per_tick single steps (if any job takes them), the mass check.
+// Chains of the current tables, in launch order: the row classes, the
+// two-pass classes, the D3Q19 march, the temporal blocks.
+std::vector<int> tick_chains(const lbm_batch* b)
+{
+    std::vector<int> c;
+    for (int k : {kRow64, kRow128, kRow256, kRow512, kRow1024, kTp9Psi, kTp19Psi, kMarch19, kBlk9})
+        if (b->class_mask & (1u << k)) c.push_back(k);
+    return c;
+}
+
+// Every launch of chain c in one tick, on stream s.
+int launch_chain(lbm_batch* b, int c, hipStream_t s)
+{
+    if (c == kBlk9) {
+        for (int m = 0; m < block_launches(b); m++) {
+            int rc = launch_blocks(b, m & 1, s);
+            if (rc) return rc;
+        }
+        return LBM_OK;
+    }
+    for (int st = 0; st < b->d.steps_per_tick; st++) {
+        int rc = launch_class(b, c, st & 1, s);
+        if (rc) return rc;
+        BHIP(lbm::batch::advance_counters(b->d_jobs, b->cap_jobs, c, b->d_tickets + 2 * kMass, s));
+    }
+    return LBM_OK;
+}
+
+// The launches of a tick: each chain's steps (on its own stream when there
+// are several and chains run concurrently), then the mass check on the
+// batch's stream.
 int launch_tick(lbm_batch* b)
 {
-    if (b->class_mask & (1u << kBlk9))
-        for (int m = 0; m < block_launches(b); m++) {
-            int rc = launch_blocks(b, m & 1);
+    const std::vector<int> chains = tick_chains(b);
+    if (chains.size() > 1 && b->concurrent) {
+        BHIP(hipEventRecord(b->ev_fork, b->stream));
+        for (int c : chains) {
+            BHIP(hipStreamWaitEvent(b->cstream[c], b->ev_fork, 0));
+            int rc = launch_chain(b, c, b->cstream[c]);
+            if (rc) return rc;
+            BHIP(hipEventRecord(b->ev_join[c], b->cstream[c]));
+            BHIP(hipStreamWaitEvent(b->stream, b->ev_join[c], 0));
+        }
+    } else {
+        if (b->class_mask & (1u << kBlk9)) {
+            int rc = launch_chain(b, kBlk9, b->stream);
             if (rc) return rc;
         }
-    if (has_single_steps(b))
-        for (int s = 0; s < b->d.steps_per_tick; s++) {
-            int rc = launch_step(b, s & 1);
-            if (rc) return rc;
-        }
+        const bool single = std::any_of(chains.begin(), chains.end(), [](int c) { return c != kBlk9; });
+        if (single)
+            for (int st = 0; st < b->d.steps_per_tick; st++) {
+                int rc = launch_step(b, st & 1);
+                if (rc) return rc;
+            }
+    }
     return launch_mass(b);
 }
 
-// One step of every job: row kernels, two-pass kernels, counters.  Only the
-// classes that hold items are launched.
+// One step of every single-step job, all classes on the batch's stream.
 int launch_step(lbm_batch* b, int parity)
 {
-    for (int k = 0; k < 5; k++) {
-        if (!(b->class_mask & (1u << k))) continue;
-        Tables t{b->d_jobs, b->d_items[k], b->d_nitems + k, b->d_tickets + 2 * k};
-        BHIP(lbm::batch::d2q9_rows_step(b->d.precision, t, parity, kRowWidth[k],
-                                        persistent_grid(b, kRowWidth[k]), b->stream));
+    for (int c : tick_chains(b)) {
+        if (c == kBlk9) continue;
+        int rc = launch_class(b, c, parity, b->stream);
+        if (rc) return rc;
     }
-    for (int q : {9, 19}) {
-        const int cp = q == 9 ? kTp9Psi : kTp19Psi;
-        if (!(b->class_mask & (1u << cp))) continue;
-        Tables tp{b->d_jobs, b->d_items[cp], b->d_nitems + cp, b->d_tickets + 2 * cp};
-        Tables tc{b->d_jobs, b->d_items[cp + 1], b->d_nitems + cp + 1, b->d_tickets + 2 * (cp + 1)};
-        BHIP(lbm::batch::twopass_psi(b->d.precision, q, tp, parity, persistent_grid(b, 256), b->stream));
-        BHIP(lbm::batch::twopass_collide(b->d.precision, q, tc, parity, persistent_grid(b, 256),
-                                         b->stream));
-    }
-    if (b->class_mask & (1u << kMarch19)) {
-        Tables tm{b->d_jobs, b->d_items[kMarch19], b->d_nitems + kMarch19, b->d_tickets + 2 * kMarch19};
-        BHIP(lbm::batch::d3q19_march_step(b->d.precision, tm, parity,
-                                          persistent_grid(b, lbm::batch::kMarchTY * lbm::batch::kMarchTZ),
-                                          b->stream));
-    }
-    BHIP(lbm::batch::advance_counters(b->d_jobs, b->cap_jobs, b->d_tickets + 2 * kMass, b->stream));
+    BHIP(lbm::batch::advance_counters(b->d_jobs, b->cap_jobs, -1, b->d_tickets + 2 * kMass, b->stream));
     return LBM_OK;
 }
 
@@ -552,6 +622,11 @@ int lbm_batch_create(const lbm_batch_desc* desc, lbm_batch** out)
         b->d.hbm_budget = static_cast<int64_t>(0.9 * static_cast<double>(free_b));
     }
     BHIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    for (int c = 0; c < kNumCls; c++) {
+        BHIP(hipStreamCreateWithFlags(&b->cstream[c], hipStreamNonBlocking));
+        BHIP(hipEventCreateWithFlags(&b->ev_join[c], hipEventDisableTiming));
+    }
+    BHIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
     BHIP(lbm::dev_malloc(&b->d_nitems, sizeof(int) * kNumCls));
     BHIP(lbm::dev_malloc(&b->d_tickets, sizeof(unsigned) * 2 * kNumCls));
     BHIP(lbm::batch::zero_u32(b->d_tickets, 2 * kNumCls, b->stream));
@@ -586,6 +661,12 @@ void lbm_batch_destroy(lbm_batch* b)
     if (b->d_tickets) (void)hipFree(b->d_tickets);
     if (b->d_mass) (void)hipFree(b->d_mass);
     if (b->h_mass) (void)hipHostFree(b->h_mass);
+    for (int c = 0; c < kNumCls; c++) {
+        if (b->cstream[c]) (void)hipStreamSynchronize(b->cstream[c]);
+        if (b->cstream[c]) (void)hipStreamDestroy(b->cstream[c]);
+        if (b->ev_join[c]) (void)hipEventDestroy(b->ev_join[c]);
+    }
+    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -900,6 +981,14 @@ int lbm_batch_set_max_wait(lbm_batch* b, int ticks)
     if (!b || ticks < 0) return lbm::set_error(LBM_EINVAL, "lbm_batch_set_max_wait: bad argument");
     b->max_wait = ticks;
     b->dirty = true;
+    return LBM_OK;
+}
+
+int lbm_batch_set_concurrency(lbm_batch* b, int on)
+{
+    if (!b || (on != 0 && on != 1)) return lbm::set_error(LBM_EINVAL, "lbm_batch_set_concurrency: bad argument");
+    if (b->concurrent != (on == 1)) b->graph_valid = false;   // the captured launch sequence changes
+    b->concurrent = on == 1;
     return LBM_OK;
 }
 

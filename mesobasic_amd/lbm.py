@@ -118,6 +118,7 @@ def load():
         "lbm_batch_tuning": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
         "lbm_batch_set_work_limit": (ctypes.c_int, [P, ctypes.c_int64]),
         "lbm_batch_set_max_wait": (ctypes.c_int, [P, ctypes.c_int]),
+        "lbm_batch_set_concurrency": (ctypes.c_int, [P, ctypes.c_int]),
         "lbm_batch_set_launch": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
         "lbm_batch_get_launch": (ctypes.c_int, [P] + [ctypes.POINTER(ctypes.c_int)] * 3),
         "lbm_batch_job_status": (ctypes.c_int, [P, ctypes.c_int64, ctypes.POINTER(JobStatus)]),
@@ -390,7 +391,7 @@ class Batch:
     submit(), advanced by tick(), and read back and freed by collect()."""
 
     def __init__(self, precision=FP16S, rho_shift=0.7, hbm_budget=0, steps_per_tick=20,
-                 use_graph=True, autotune=False, work_limit=0, max_wait=0):
+                 use_graph=True, autotune=False, work_limit=0, max_wait=0, concurrent=True):
         lib = load()
         if isinstance(precision, str):
             precision = PRECISIONS[precision]
@@ -404,6 +405,8 @@ class Batch:
             _check(lib.lbm_batch_set_work_limit(h, int(work_limit)))
         if max_wait:
             _check(lib.lbm_batch_set_max_wait(h, int(max_wait)))
+        if not concurrent:
+            _check(lib.lbm_batch_set_concurrency(h, 0))
 
     def close(self):
         if getattr(self, "_h", None):
@@ -509,9 +512,15 @@ class Batch:
         _check(self._lib.lbm_batch_get_launch(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
         return dict(d2q9_chunk=a.value, d2q9_rows_kernel=b.value, d3q19_march=c.value)
 
+    def set_concurrency(self, on):
+        """Kernel classes on streams of their own within a tick (True, the
+        default) or one after another on the batch's stream."""
+        _check(self._lib.lbm_batch_set_concurrency(self._h, int(bool(on))))
+
     def set_launch(self, d2q9_chunk=32, d2q9_rows_kernel=1, d3q19_march=1):
         """Fixes the launch configuration (stops the tuner).  d3q19_march: 0
-        two-pass, 1 the fused march for the node-shift forms (default), 2 the
+        two-pass, 1 the fused march for node-shift jobs that fill 90% of its
+        tiles (default), 2 the
         march wherever it fits; D3Q19 results of the two kernels agree to
         FP32 rounding."""
         _check(self._lib.lbm_batch_set_launch(self._h, int(d2q9_chunk), int(d2q9_rows_kernel),

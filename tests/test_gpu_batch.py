@@ -66,6 +66,31 @@ def test_graph_tuner_and_tick_length_do_not_change_results(prec):
         assert same_bits(a[k]["u"], b[k]["u"]), (k, jobs[k])
 
 
+@pytest.mark.parametrize("prec,launch", [(mb.FP16S, None), (mb.FP16N, None), (mb.FP16S, (16, 2, 1)),
+                                         (mb.FP32, (16, 0, 2))])
+def test_concurrent_classes_equal_one_stream_bitwise(prec, launch):
+    # kernel classes on streams of their own (the default) or one after
+    # another on the batch's stream, replayed as graphs or launched directly:
+    # the same bits.  The mix spans D2Q9 row widths 64..1024, two-pass D2Q9
+    # and D3Q19 jobs, D3Q19 march jobs and (launch (16, 2, 1)) temporal blocks.
+    rng = np.random.default_rng(11)
+    jobs = _jobs(rng, 12)
+    for shape in ((40, 700), (24, 1500), (6, 16, 64), (5, 9, 70)):
+        jobs.append(dict(q=19 if len(shape) == 3 else 9, shape=shape, tau=1.0, G=-5.0,
+                         steps=int(rng.integers(9, 45)), seed=int(rng.integers(1 << 30))))
+    runs = []
+    for conc, graph in ((True, True), (False, True), (True, False)):
+        b = Batch(precision=prec, steps_per_tick=6, use_graph=graph, concurrent=conc)
+        if launch is not None:
+            b.set_launch(*launch)
+        runs.append(_run(b, jobs))
+        b.close()
+    for k in range(len(jobs)):
+        for f in ("rho", "u", "v", "w"):
+            assert same_bits(runs[0][k][f], runs[1][k][f]), (k, f, jobs[k])
+            assert same_bits(runs[0][k][f], runs[2][k][f]), (k, f, jobs[k])
+
+
 @pytest.mark.parametrize("q,shape,steps", [(9, (64, 48), 37), (9, (40, 200), 20), (19, (20, 18, 16), 23)])
 def test_job_matches_fp64_oracle(q, shape, steps):
     tau, G = 0.9, -5.0
@@ -217,10 +242,12 @@ def test_d3q19_march_agrees_with_two_pass_and_oracle(prec):
 
 
 def test_d3q19_march_default_alone_equals_in_the_mix():
-    # by default node-shift batches run D3Q19 jobs that fit on the march, the
-    # others on the two passes; a job's bits do not depend on the other jobs
+    # by default node-shift batches run D3Q19 jobs that fill 90% of the
+    # march tiles on the march ((6, 16, 128)), the others on the two passes
+    # ((10, 12, 100) fills 59%); a job's bits do not depend on the other jobs
     # or on the tuner
-    for prec, shape in ((mb.FP16N, (10, 12, 100)), (mb.BF16N, (10, 12, 100)), (mb.FP16S, (10, 16, 64))):
+    for prec, shape in ((mb.FP16N, (10, 12, 100)), (mb.BF16N, (10, 12, 100)), (mb.FP16N, (6, 16, 128)),
+                        (mb.FP16S, (10, 16, 64))):
         job = dict(q=19, shape=shape, tau=0.9, G=-5.0, steps=25, seed=4)
         b = Batch(precision=prec, steps_per_tick=6)
         assert b.tuning["d3q19_march"] == 1
