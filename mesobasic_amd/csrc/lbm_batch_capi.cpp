@@ -33,9 +33,14 @@ namespace {
 // two-pass kernels (psi then collide) of each stencil and the D3Q19 march;
 // kBlk9 the D2Q9 temporal blocks (several steps per launch); kMass is the
 // per-tick mass check that follows the steps of a tick.
-enum Cls { kRow64, kRow128, kRow256, kRow512, kRow1024, kTp9Psi, kTp9Col, kTp19Psi, kTp19Col, kMarch19,
-           kBlk9, kMass, kNumCls };
-constexpr int kRowWidth[5] = {64, 128, 256, 512, 1024};
+// Row classes by row width: a workgroup has as many threads as its class's
+// width.  Finer classes (every multiple of 64 nodes, which leave fewer idle
+// wavefronts per row) measured slower on MI355X with concurrent classes: 512
+// D2Q9 jobs of 64..512 per side, row kernel, GLUPS 95-98 with these widths,
+// 92-94 adding 384, 86-88 adding 192, 384 and 768.
+constexpr int kNumRow = 5;
+constexpr int kRowWidth[kNumRow] = {64, 128, 256, 512, 1024};
+enum Cls { kRow0, kTp9Psi = kRow0 + kNumRow, kTp9Col, kTp19Psi, kTp19Col, kMarch19, kBlk9, kMass, kNumCls };
 // Nodes per two-pass work item.  Serving stream (200 requests, 77% of the
 // work in ragged D3Q19 jobs on the two passes), GLUPS / p99 s by item size:
 // 1024 30.5 / 16.6, 2048 33.2 / 15.1, 4096 32.7 / 15.4, 8192 31.5 / 16.0,
@@ -146,7 +151,7 @@ namespace {
 
 int row_class(int ny)
 {
-    for (int k = 0; k < 5; k++)
+    for (int k = 0; k < kNumRow; k++)
         if (ny <= kRowWidth[k]) return k;
     return -1;
 }
@@ -284,17 +289,30 @@ int persistent_grid(const lbm_batch* b, int threads)
     return b->cus * std::min(per_cu, 8);
 }
 
-// x rows per item of row class k, which holds `rows` rows in all: the tuned
-// chunk, shortened (to kMinRowChunk at least) until the items fill
-// kRowRounds rounds of the class's resident workgroups.  A class of few
-// narrow jobs otherwise runs a few long items -- each a serial chain of
-// row-load round trips -- on an almost idle GPU, and the classes run one
-// after another.  Results do not depend on the chunk.
-int row_chunk(const lbm_batch* b, int k, int64_t rows)
+// x rows per item of each row class (rows[k]: the rows of class k): the
+// tuned chunk, shortened (to kMinRowChunk at least) until the items fill
+// kRowRounds rounds of resident workgroups.  A few narrow jobs otherwise run
+// as a few long items -- each a serial chain of row-load round trips -- on
+// an almost idle GPU.  Classes that run one after another (no concurrency)
+// each fill their own rounds; concurrent classes share the GPU, so their
+// items together fill rounds of its resident threads (2048 per CU) and a
+// class is not cut into short items, each of which re-reads three rows,
+// only because it is small.  Results do not depend on the chunk.
+void row_chunks(const lbm_batch* b, const int64_t (&rows)[kNumRow], int (&chunk)[kNumRow])
 {
-    const int64_t per_round = static_cast<int64_t>(kRowRounds) * persistent_grid(b, kRowWidth[k]);
-    const int64_t c = rows / std::max<int64_t>(1, per_round);
-    return static_cast<int>(std::clamp<int64_t>(c, std::min(kMinRowChunk, b->chunk), b->chunk));
+    const int lo = std::min(kMinRowChunk, b->chunk);
+    if (b->concurrent) {
+        int64_t threads = 0;   // row threads of all items of one row each
+        for (int k = 0; k < kNumRow; k++) threads += rows[k] * kRowWidth[k];
+        const int64_t c = threads / std::max<int64_t>(1, int64_t(kRowRounds) * b->cus * 2048);
+        for (int k = 0; k < kNumRow; k++) chunk[k] = static_cast<int>(std::clamp<int64_t>(c, lo, b->chunk));
+        return;
+    }
+    for (int k = 0; k < kNumRow; k++) {
+        const int64_t per_round = static_cast<int64_t>(kRowRounds) * persistent_grid(b, kRowWidth[k]);
+        const int64_t c = rows[k] / std::max<int64_t>(1, per_round);
+        chunk[k] = static_cast<int>(std::clamp<int64_t>(c, lo, b->chunk));
+    }
 }
 
 // Rebuilds and uploads the job and item tables (between ticks only).
@@ -326,12 +344,12 @@ int upload_tables(lbm_batch* b)
     if (rc) return rc;
     std::vector<Item> items[kNumCls];
     b->mass_job.clear();
-    int chunk[5];
+    int chunk[kNumRow];
     {
-        int64_t rows[5] = {};
+        int64_t rows[kNumRow] = {};
         for (const HostJob* j : b->order)
             if (use_rows(b, *j)) rows[row_class(j->d.ny)] += j->d.nx;
-        for (int k = 0; k < 5; k++) chunk[k] = row_chunk(b, k, rows[k]);
+        row_chunks(b, rows, chunk);
     }
     for (int ji = 0; ji < static_cast<int>(b->order.size()); ji++) {
         const HostJob& j = *b->order[ji];
@@ -441,7 +459,7 @@ int launch_blocks(lbm_batch* b, int parity, hipStream_t s)
 // psi pass, or the D3Q19 march) on stream s, without the counter update.
 int launch_class(lbm_batch* b, int c, int parity, hipStream_t s)
 {
-    if (c < 5) {
+    if (c < kNumRow) {
         Tables t{b->d_jobs, b->d_items[c], b->d_nitems + c, b->d_tickets + 2 * c};
         BHIP(lbm::batch::d2q9_rows_step(b->d.precision, t, parity, kRowWidth[c],
                                         persistent_grid(b, kRowWidth[c]), s));
@@ -464,7 +482,9 @@ int launch_class(lbm_batch* b, int c, int parity, hipStream_t s)
 std::vector<int> tick_chains(const lbm_batch* b)
 {
     std::vector<int> c;
-    for (int k : {kRow64, kRow128, kRow256, kRow512, kRow1024, kTp9Psi, kTp19Psi, kMarch19, kBlk9})
+    for (int k = kRow0; k < kRow0 + kNumRow; k++)
+        if (b->class_mask & (1u << k)) c.push_back(k);
+    for (int k : {kTp9Psi, kTp19Psi, kMarch19, kBlk9})
         if (b->class_mask & (1u << k)) c.push_back(k);
     return c;
 }
@@ -987,7 +1007,10 @@ int lbm_batch_set_max_wait(lbm_batch* b, int ticks)
 int lbm_batch_set_concurrency(lbm_batch* b, int on)
 {
     if (!b || (on != 0 && on != 1)) return lbm::set_error(LBM_EINVAL, "lbm_batch_set_concurrency: bad argument");
-    if (b->concurrent != (on == 1)) b->graph_valid = false;   // the captured launch sequence changes
+    if (b->concurrent != (on == 1)) {
+        b->graph_valid = false;   // the captured launch sequence changes
+        b->dirty = true;          // and the row items (row_chunks)
+    }
     b->concurrent = on == 1;
     return LBM_OK;
 }
