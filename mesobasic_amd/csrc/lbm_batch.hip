@@ -56,22 +56,89 @@ __device__ __forceinline__ float pulled_rho(const float (&d)[Q], float R)
 {
     float r = 0.f;
     static_for<Q>([&](auto I) { r += d[decltype(I)::value]; });
-    return R + r * (1.0f / kScale<SP, NS>);
+    return fmaf(r, 1.0f / kScale<SP, NS>, R);
 }
 
-// Stored value of population n after the collision: the post-collision f
-// relative to the destination's reference density Rw, scaled.  dn is the
-// pulled stored value (relative to the reader's Rr); deq the equilibrium in
-// the reader's frame.
-template <typename SP, bool NS>
-__device__ __forceinline__ float post_value(float dn, float w, float rho, float poly, float Rr, float Rw,
-                                            float itau)
+// The collision of one node, shared by every batch kernel (D2Q9 row march,
+// temporal blocks, two passes of either stencil), so that a job's bits do
+// not depend on which of them runs it.  The kernels are VALU-bound (the
+// D2Q9 row kernel's vector ALU was busy 62% of its time on MI355X), so the
+// terms are arranged for few instructions: one approximate reciprocal of
+// rho (1 ulp) instead of two IEEE divides, fused multiply-adds throughout,
+// and per population
+//   stored_n = d_n omt + (S w_n rho omega) pm1_n + S w_n (omega (rho - Rr) + Rr - Rw)
+// with omega = 1/tau, omt = 1 - omega and pm1_n = poly_n - 1 = 3 ev +
+// 4.5 ev^2 - 1.5 |u_eq|^2.  That is S (f*_n - w_n Rw) for f*_n = f_n - (f_n -
+// feq_n)/tau, f_n = d_n/S + w_n Rr: d_n is the pulled stored value (relative
+// to the reading node's Rr), Rw the reference density of the destination
+// (Rr = Rw = rs for the constant shift).
+
+// Force sum sum_n w_n e_n psi(x + e_n) over n >= 1 in population order;
+// psi_at(I) gives psi at the neighbour of population I.
+template <int Q, typename PF>
+__device__ __forceinline__ void force_sum(PF psi_at, float& gx, float& gy, float& gz)
 {
-    constexpr float S = kScale<SP, NS>;
-    const float deq = (S * w) * (rho * poly - Rr);
-    const float out = dn - (dn - deq) * itau;
-    if constexpr (NS) return out + (S * w) * (Rr - Rw);
-    else return out;
+    using S = St<Q>;
+    gx = 0.f; gy = 0.f; gz = 0.f;
+    static_for<Q>([&](auto I) {
+        constexpr int n = decltype(I)::value;
+        if constexpr (n > 0) {
+            const float p = psi_at(I);
+            constexpr float w = static_cast<float>(S::w[n]);
+            if constexpr (S::ex[n] != 0) gx = fmaf(S::ex[n] * w, p, gx);
+            if constexpr (S::ey[n] != 0) gy = fmaf(S::ey[n] * w, p, gy);
+            if constexpr (S::ez[n] != 0) gz = fmaf(S::ez[n] * w, p, gz);
+        }
+    });
+}
+
+// Per-node terms: equilibrium velocity, 1.5 |u_eq|^2, S rho omega and the
+// constant part omega (rho - Rr) (+ Rr with the node shift).
+struct BColl {
+    float ue, ve, we, usq15, rom, ct, omt;
+};
+
+// rho: the node's density (rho0 at a first step in the LBfluid2D order,
+// which takes u = 0: at_rest); psi0: psi at the node; g: force_sum.
+template <int Q, typename SP, bool NS>
+__device__ __forceinline__ BColl batch_prep(const float (&d)[Q], float rho, bool at_rest, float psi0,
+                                            float gx, float gy, float gz, float Rr, float tau, float G,
+                                            float om, float omt)
+{
+    using S = St<Q>;
+    float jx = 0.f, jy = 0.f, jz = 0.f;
+    static_for<Q>([&](auto I) {
+        constexpr int n = decltype(I)::value;
+        acc_e<S::ex[n]>(jx, d[n]);
+        acc_e<S::ey[n]>(jy, d[n]);
+        acc_e<S::ez[n]>(jz, d[n]);
+    });
+    const float ir = __builtin_amdgcn_rcpf(rho);
+    const float js = at_rest ? 0.f : ir * (1.0f / kScale<SP, NS>);   // j is summed from scaled values
+    const float kf = (-G * psi0) * (tau * ir);
+    BColl k;
+    k.ue = fmaf(kf, gx, jx * js);
+    k.ve = fmaf(kf, gy, jy * js);
+    k.we = S::D == 3 ? fmaf(kf, gz, jz * js) : 0.f;
+    k.usq15 = 1.5f * fmaf(k.ue, k.ue, fmaf(k.ve, k.ve, k.we * k.we));
+    k.rom = rho * om;
+    k.ct = om * (rho - Rr);
+    k.omt = omt;
+    return k;
+}
+
+// Stored value of population n after the collision (Rw: reference density
+// of the destination node x + e_n; unused for the constant shift).
+template <int Q, typename SP, bool NS, int n>
+__device__ __forceinline__ float batch_out(const BColl& k, float dn, float Rr, float Rw)
+{
+    using S = St<Q>;
+    constexpr float sw = kScale<SP, NS> * static_cast<float>(S::w[n]);
+    const float ev = edot<S::ex[n], S::ey[n], S::ez[n]>(k.ue, k.ve, k.we);
+    const float pm1 = fmaf(fmaf(4.5f, ev, 3.0f), ev, -k.usq15);
+    float c = k.ct;
+    if constexpr (NS) c += Rr - Rw;
+    return fmaf(dn, k.omt, fmaf(sw * k.rom, pm1, sw * c));
 }
 
 __device__ __forceinline__ int wrapi(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
@@ -106,10 +173,14 @@ __device__ __forceinline__ void pull_row9(const typename SP::T* __restrict__ P, 
     const int ym = y == 0 ? ny - 1 : y - 1, yp = y == ny - 1 ? 0 : y + 1;
     const int xs[3] = {xm, x, xp};
     const int ys[3] = {ym, y, yp};
+    // 32-bit node offsets from the plane bases (uniform across the
+    // workgroup: one job per item), so each gather takes one VGPR of address
+    // and no 64-bit arithmetic; a job holds fewer than 2^30 nodes
     static_for<9>([&](auto I) {
         constexpr int n = decltype(I)::value;
-        const int64_t src = static_cast<int64_t>(xs[1 - S::ex[n]]) * ny + ys[1 - S::ey[n]];
-        d[n] = SP::d(P[n * ps + src]);
+        const uint32_t src = static_cast<uint32_t>(xs[1 - S::ex[n]]) * static_cast<uint32_t>(ny) +
+                             static_cast<uint32_t>(ys[1 - S::ey[n]]);
+        d[n] = SP::d(*fused::elem(P + n * ps, src));
     });
 }
 
@@ -132,7 +203,7 @@ __global__ void __launch_bounds__(T) k_d2q9_rows(Tables t, int parity)
         float* Rout = NS ? job.R[p ^ 1] : nullptr;
         const int nx = job.nx, ny = job.ny;
         const int64_t ps = job.ps;
-        const float tau = job.tau, G = job.G, itau = 1.0f / tau;
+        const float tau = job.tau, G = job.G, om = 1.0f / tau, omt = 1.0f - om;
         // constant shift: one register for the whole item (the row values
         // below then all name it; re-reading job.rs per row, which the
         // stores may alias, cost 5 VGPRs and a wave of occupancy)
@@ -149,17 +220,19 @@ __global__ void __launch_bounds__(T) k_d2q9_rows(Tables t, int parity)
         // rows of this item)
         auto put = [&](int xu, const float (&d)[9], float R) {
             const int xr = xw(xu);
-            const int64_t c = static_cast<int64_t>(xr) * ny + y;
+            const uint32_t c = static_cast<uint32_t>(xr) * static_cast<uint32_t>(ny) + static_cast<uint32_t>(y);
             const float rp = pulled_rho<9, SP, NS>(d, R);
-            psi_lds[slot(xu) + y] = psi_of((first && rho0) ? rho0[c] : rp);
+            psi_lds[slot(xu) + y] = psi_of((first && rho0) ? *fused::elem(rho0, c) : rp);
             if constexpr (NS) {
                 rho_lds[slot(xu) + y] = rp;
-                if (xu >= x0 && xu < x1) Rout[c] = rp;
+                if (xu >= x0 && xu < x1) *fused::elem(Rout, c) = rp;
             }
         };
         auto pullR = [&](int xr, float (&d)[9]) {
             pull_row9<SP>(Pin, ps, nx, ny, xr, y, d);
-            if constexpr (NS) return job.R[p][static_cast<int64_t>(xr) * ny + y];
+            if constexpr (NS)
+                return *fused::elem(static_cast<const float*>(job.R[p]),
+                                    static_cast<uint32_t>(xr) * static_cast<uint32_t>(ny) + static_cast<uint32_t>(y));
             else return rs;
         };
         // prologue: the three rows' pulls are all issued before the first
@@ -182,49 +255,29 @@ __global__ void __launch_bounds__(T) k_d2q9_rows(Tables t, int parity)
                 const float* pm = psi_lds + slot(x - 1);
                 const float* p0 = psi_lds + slot(x);
                 const float* pp = psi_lds + slot(x + 1);
-                float jx = 0.f, jy = 0.f;
-                static_for<9>([&](auto I) {
-                    constexpr int n = decltype(I)::value;
-                    acc_e<S::ex[n]>(jx, fcur[n]);
-                    acc_e<S::ey[n]>(jy, fcur[n]);
-                });
-                float rho, ux, uy;
-                if (first && rho0) {
-                    rho = rho0[static_cast<int64_t>(x) * ny + y];
-                    ux = 0.f; uy = 0.f;
-                } else {
-                    rho = pulled_rho<9, SP, NS>(fcur, Rcur);
-                    const float ir = 1.0f / rho;
-                    const float js = 1.0f / kScale<SP, NS>;
-                    ux = (jx * js) * ir; uy = (jy * js) * ir;
-                }
+                const bool at_rest = first && rho0;
+                const float rho = at_rest ? *fused::elem(rho0, static_cast<uint32_t>(x) * static_cast<uint32_t>(ny) +
+                                                                   static_cast<uint32_t>(y))
+                                          : pulled_rho<9, SP, NS>(fcur, Rcur);
                 const int yy[3] = {ym, y, yp};
-                float gx = 0.f, gy = 0.f;
+                float gx, gy, gz;
+                force_sum<9>([&](auto I) {
+                    constexpr int n = decltype(I)::value;
+                    const float* pl = S::ex[n] < 0 ? pm : (S::ex[n] > 0 ? pp : p0);
+                    return pl[yy[1 + S::ey[n]]];
+                }, gx, gy, gz);
+                const BColl k = batch_prep<9, SP, NS>(fcur, rho, at_rest, p0[y], gx, gy, 0.f, Rcur, tau, G,
+                                                      om, omt);
+                const uint32_t c = static_cast<uint32_t>(x) * static_cast<uint32_t>(ny) + static_cast<uint32_t>(y);
                 static_for<9>([&](auto I) {
                     constexpr int n = decltype(I)::value;
-                    if constexpr (n > 0) {
-                        const float* pl = S::ex[n] < 0 ? pm : (S::ex[n] > 0 ? pp : p0);
-                        const float tt = static_cast<float>(S::w[n]) * pl[yy[1 + S::ey[n]]];
-                        acc_e<S::ex[n]>(gx, tt);
-                        acc_e<S::ey[n]>(gy, tt);
-                    }
-                });
-                const float k = (-G * p0[y]) * tau / rho;
-                const float ue = ux + k * gx, ve = uy + k * gy;
-                const float usq15 = 1.5f * (ue * ue + ve * ve);
-                const int64_t c = static_cast<int64_t>(x) * ny + y;
-                static_for<9>([&](auto I) {
-                    constexpr int n = decltype(I)::value;
-                    const float ev = edot<S::ex[n], S::ey[n], 0>(ue, ve, 0.f);
-                    const float poly = 1.0f + 3.0f * ev + 4.5f * ev * ev - usq15;
                     float Rw = 0.f;
                     if constexpr (NS) {
                         const float* rl = rho_lds + slot(x + S::ex[n]);
                         Rw = rl[yy[1 + S::ey[n]]];
                     }
-                    const float v = post_value<SP, NS>(fcur[n], static_cast<float>(S::w[n]), rho, poly,
-                                                       Rcur, Rw, itau);
-                    SP::put_nt(Pout + n * ps + c, SP::of_d(v));
+                    const float v = batch_out<9, SP, NS, n>(k, fcur[n], Rcur, Rw);
+                    SP::put_nt(fused::elem(Pout + n * ps, c), SP::of_d(v));
                 });
                 put(x + 2, fnn, Rnn);
             }
@@ -316,7 +369,7 @@ __global__ void __launch_bounds__(1024) k_d2q9_block(Tables t, int parity)
         auto* Pout = static_cast<ST*>(job.P[(job.done & 1) ^ 1]);
         const int nx = job.nx, ny = job.ny;
         const int64_t ps = job.ps;
-        const float tau = job.tau, G = job.G, itau = 1.0f / tau, rs = job.rs;
+        const float tau = job.tau, G = job.G, om = 1.0f / tau, omt = 1.0f - om, rs = job.rs;
         const bool first = job.first != 0 && job.rho0 != nullptr;
         if (live) {
             for (int q = 0; q < NPT; q++) {
@@ -357,41 +410,17 @@ __global__ void __launch_bounds__(1024) k_d2q9_block(Tables t, int parity)
                     const int l = threadIdx.x + q * 1024;
                     const int i = si[q], j = sj[q];
                     if (l < W2 && i > lo && i < hi - 1 && j > lo && j < hi - 1) {
-                        float jx = 0.f, jy = 0.f;
+                        const float rho = f0 ? r0[l] : rp[q];
+                        float gx, gy, gz;
+                        force_sum<9>([&](auto I) {
+                            constexpr int n = decltype(I)::value;
+                            return pw[l + S::ex[n] * W + S::ey[n]];
+                        }, gx, gy, gz);
+                        const BColl kc = batch_prep<9, SP, false>(d[q], rho, f0, pw[l], gx, gy, 0.f, rs, tau, G,
+                                                                  om, omt);
                         static_for<9>([&](auto I) {
                             constexpr int n = decltype(I)::value;
-                            acc_e<S::ex[n]>(jx, d[q][n]);
-                            acc_e<S::ey[n]>(jy, d[q][n]);
-                        });
-                        float rho, ux, uy;
-                        if (f0) {
-                            rho = r0[l];
-                            ux = 0.f; uy = 0.f;
-                        } else {
-                            rho = rp[q];
-                            const float ir = 1.0f / rho;
-                            const float js = 1.0f / kScale<SP, false>;
-                            ux = (jx * js) * ir; uy = (jy * js) * ir;
-                        }
-                        float gx = 0.f, gy = 0.f;
-                        static_for<9>([&](auto I) {
-                            constexpr int n = decltype(I)::value;
-                            if constexpr (n > 0) {
-                                const float tt = static_cast<float>(S::w[n]) * pw[l + S::ex[n] * W + S::ey[n]];
-                                acc_e<S::ex[n]>(gx, tt);
-                                acc_e<S::ey[n]>(gy, tt);
-                            }
-                        });
-                        const float kk = (-G * pw[l]) * tau / rho;
-                        const float ue = ux + kk * gx, ve = uy + kk * gy;
-                        const float usq15 = 1.5f * (ue * ue + ve * ve);
-                        static_for<9>([&](auto I) {
-                            constexpr int n = decltype(I)::value;
-                            const float ev = edot<S::ex[n], S::ey[n], 0>(ue, ve, 0.f);
-                            const float poly = 1.0f + 3.0f * ev + 4.5f * ev * ev - usq15;
-                            const float v = post_value<SP, false>(d[q][n], static_cast<float>(S::w[n]), rho,
-                                                                  poly, rs, 0.f, itau);
-                            fs[n * W2 + l] = SP::of_d(v);
+                            fs[n * W2 + l] = SP::of_d(batch_out<9, SP, false, n>(kc, d[q][n], rs, rs));
                         });
                     }
                 }
@@ -529,7 +558,7 @@ __global__ void __launch_bounds__(kTP) k_tp_collide(Tables t, int parity)
         const int p = parity ^ job.phase;
         const auto* Pin = static_cast<const typename SP::T*>(job.P[p]);
         auto* Pout = static_cast<typename SP::T*>(job.P[p ^ 1]);
-        const float tau = job.tau, G = job.G, itau = 1.0f / tau;
+        const float tau = job.tau, G = job.G, om = 1.0f / tau, omt = 1.0f - om;
         const float* psi = job.psi;
         const float rs = job.rs;
         const uint32_t c1 = static_cast<uint32_t>(item.c1);
@@ -541,49 +570,22 @@ __global__ void __launch_bounds__(kTP) k_tp_collide(Tables t, int parity)
             pull_node32<Q, SP>(Pin, job, c, o, d);
             float Rr = rs;
             if constexpr (NS) Rr = *fused::elem(job.R[p], c);
-            float jx = 0.f, jy = 0.f, jz = 0.f;
-            static_for<Q>([&](auto I) {
-                constexpr int n = decltype(I)::value;
-                acc_e<S::ex[n]>(jx, d[n]);
-                acc_e<S::ey[n]>(jy, d[n]);
-                acc_e<S::ez[n]>(jz, d[n]);
-            });
-            float rho, ux, uy, uz;
-            if (job.first && job.rho0) {
-                rho = *fused::elem(job.rho0, c);
-                ux = uy = uz = 0.f;
-            } else {
-                rho = pulled_rho<Q, SP, NS>(d, Rr);
-                const float ir = 1.0f / rho;
-                const float js = 1.0f / kScale<SP, NS>;
-                ux = (jx * js) * ir; uy = (jy * js) * ir; uz = (jz * js) * ir;
-            }
+            const bool at_rest = job.first && job.rho0;
+            const float rho = at_rest ? *fused::elem(job.rho0, c) : pulled_rho<Q, SP, NS>(d, Rr);
             // node c + e_n
             auto nb = [&](auto N) {
                 constexpr int n = decltype(N)::value;
                 return c + o.template to<S::ex[n], S::ey[n], S::ez[n]>();
             };
-            float gx = 0.f, gy = 0.f, gz = 0.f;
+            float gx, gy, gz;
+            force_sum<Q>([&](auto I) { return *fused::elem(psi, nb(I)); }, gx, gy, gz);
+            const BColl k = batch_prep<Q, SP, NS>(d, rho, at_rest, *fused::elem(psi, c), gx, gy, gz, Rr, tau, G,
+                                                  om, omt);
             static_for<Q>([&](auto I) {
                 constexpr int n = decltype(I)::value;
-                if constexpr (n > 0) {
-                    const float tt = static_cast<float>(S::w[n]) * *fused::elem(psi, nb(I));
-                    acc_e<S::ex[n]>(gx, tt);
-                    acc_e<S::ey[n]>(gy, tt);
-                    acc_e<S::ez[n]>(gz, tt);
-                }
-            });
-            const float k = (-G * *fused::elem(psi, c)) * tau / rho;
-            const float ue = ux + k * gx, ve = uy + k * gy, we = (S::D == 3) ? uz + k * gz : 0.f;
-            const float usq15 = 1.5f * (ue * ue + ve * ve + we * we);
-            static_for<Q>([&](auto I) {
-                constexpr int n = decltype(I)::value;
-                const float ev = edot<S::ex[n], S::ey[n], S::ez[n]>(ue, ve, we);
-                const float poly = 1.0f + 3.0f * ev + 4.5f * ev * ev - usq15;
                 float Rw = 0.f;
                 if constexpr (NS) Rw = *fused::elem(job.R[p ^ 1], nb(I));
-                const float v = post_value<SP, NS>(d[n], static_cast<float>(S::w[n]), rho, poly, Rr, Rw,
-                                                   itau);
+                const float v = batch_out<Q, SP, NS, n>(k, d[n], Rr, Rw);
                 SP::put_nt(fused::elem(Pout + n * job.ps, c), SP::of_d(v));
             });
         }

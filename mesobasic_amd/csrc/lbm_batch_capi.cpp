@@ -112,6 +112,7 @@ struct lbm_batch {
     unsigned* d_tickets = nullptr; // [kNumCls][2]
     int nitems[kNumCls] = {};
     unsigned class_mask = 0;      // classes with items; only these are launched
+    int64_t class_nodes[kNumCls] = {};   // nodes of the jobs of each launch chain
     bool dirty = true;
     // launch configuration
     int chunk = 32;               // D2Q9 rows per row-kernel item
@@ -430,6 +431,8 @@ int upload_tables(lbm_batch* b)
     BHIP(hipStreamSynchronize(b->stream));
     if (mask != b->class_mask) b->graph_valid = false;   // the launch sequence changes
     b->class_mask = mask;
+    std::fill(std::begin(b->class_nodes), std::end(b->class_nodes), int64_t(0));
+    for (const HostJob* j : b->order) b->class_nodes[chain_of(b, *j)] += j->n;
     b->dirty = false;
     return LBM_OK;
 }
@@ -477,8 +480,7 @@ int launch_class(lbm_batch* b, int c, int parity, hipStream_t s)
     return LBM_OK;
 }
 
-// Chains of the current tables, in launch order: the row classes, the
-// two-pass classes, the D3Q19 march, the temporal blocks.
+// Chains of the current tables, the largest (by nodes) first.
 std::vector<int> tick_chains(const lbm_batch* b)
 {
     std::vector<int> c;
@@ -486,47 +488,62 @@ std::vector<int> tick_chains(const lbm_batch* b)
         if (b->class_mask & (1u << k)) c.push_back(k);
     for (int k : {kTp9Psi, kTp19Psi, kMarch19, kBlk9})
         if (b->class_mask & (1u << k)) c.push_back(k);
+    std::stable_sort(c.begin(), c.end(),
+                     [b](int x, int y) { return b->class_nodes[x] > b->class_nodes[y]; });
     return c;
 }
 
-// Every launch of chain c in one tick, on stream s.
-int launch_chain(lbm_batch* b, int c, hipStream_t s)
+// Launch units of chain c in one tick: its block launches, or one step
+// (the class's kernels and the counter update) per step of the tick.
+int chain_units(const lbm_batch* b, int c)
 {
-    if (c == kBlk9) {
-        for (int m = 0; m < block_launches(b); m++) {
-            int rc = launch_blocks(b, m & 1, s);
-            if (rc) return rc;
-        }
-        return LBM_OK;
-    }
-    for (int st = 0; st < b->d.steps_per_tick; st++) {
-        int rc = launch_class(b, c, st & 1, s);
-        if (rc) return rc;
-        BHIP(lbm::batch::advance_counters(b->d_jobs, b->cap_jobs, c, b->d_tickets + 2 * kMass, s));
-    }
+    return c == kBlk9 ? block_launches(b) : b->d.steps_per_tick;
+}
+
+// Unit m of chain c on stream s.
+int launch_unit(lbm_batch* b, int c, int m, hipStream_t s)
+{
+    if (c == kBlk9) return launch_blocks(b, m & 1, s);
+    int rc = launch_class(b, c, m & 1, s);
+    if (rc) return rc;
+    BHIP(lbm::batch::advance_counters(b->d_jobs, b->cap_jobs, c, b->d_tickets + 2 * kMass, s));
     return LBM_OK;
 }
 
-// The launches of a tick: each chain's steps (on its own stream when there
+// The launches of a tick: each chain's units (on its own stream when there
 // are several and chains run concurrently), then the mass check on the
-// batch's stream.
+// batch's stream.  Concurrent chains are issued unit by unit across the
+// chains, the largest chain first: a hipGraph replay submits its kernel
+// nodes in about the order they were captured, and captured chain after
+// chain the largest class started only after the small ones had run most of
+// their steps alone on an almost idle GPU (MI355X kernel trace, 512 D2Q9
+// jobs: the 64-wide class ran 13 steps alone before the next chain began).
 int launch_tick(lbm_batch* b)
 {
     const std::vector<int> chains = tick_chains(b);
     if (chains.size() > 1 && b->concurrent) {
         BHIP(hipEventRecord(b->ev_fork, b->stream));
+        int units = 0;
         for (int c : chains) {
             BHIP(hipStreamWaitEvent(b->cstream[c], b->ev_fork, 0));
-            int rc = launch_chain(b, c, b->cstream[c]);
-            if (rc) return rc;
+            units = std::max(units, chain_units(b, c));
+        }
+        for (int m = 0; m < units; m++)
+            for (int c : chains)
+                if (m < chain_units(b, c)) {
+                    int rc = launch_unit(b, c, m, b->cstream[c]);
+                    if (rc) return rc;
+                }
+        for (int c : chains) {
             BHIP(hipEventRecord(b->ev_join[c], b->cstream[c]));
             BHIP(hipStreamWaitEvent(b->stream, b->ev_join[c], 0));
         }
     } else {
-        if (b->class_mask & (1u << kBlk9)) {
-            int rc = launch_chain(b, kBlk9, b->stream);
-            if (rc) return rc;
-        }
+        if (b->class_mask & (1u << kBlk9))
+            for (int m = 0; m < chain_units(b, kBlk9); m++) {
+                int rc = launch_unit(b, kBlk9, m, b->stream);
+                if (rc) return rc;
+            }
         const bool single = std::any_of(chains.begin(), chains.end(), [](int c) { return c != kBlk9; });
         if (single)
             for (int st = 0; st < b->d.steps_per_tick; st++) {
