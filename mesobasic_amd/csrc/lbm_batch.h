@@ -27,8 +27,6 @@ struct Job {
     int blk;             // 1: D2Q9 temporal blocks (k_d2q9_block; advances by kBlkK steps per
                          // launch while more are left, then by 1: always an odd count, so
                          // P[done & 1] stays current, and the last step is a single one)
-    int chain;           // launch chain that steps the job (one per kernel class, see
-                         // advance_counters)
 };
 
 // Unit of work of one launch: a range of x planes (D2Q9 row kernel), of
@@ -61,19 +59,23 @@ struct Tables {
 };
 
 // prec: LBM_FP32 / LBM_FP16S / LBM_BF16S / LBM_FP16N / LBM_BF16N (FP32
-// compute in every case).
-hipError_t d2q9_rows_step(int prec, const Tables& t, int parity, int threads, int grid,
+// compute in every case).  `step` is the index of the step within the tick
+// (launch parity step & 1): a launch advances the jobs with remaining >
+// step, whose counters (remaining, done, first) are those of the tick's
+// start until advance_counters updates them after the tick's last step.
+hipError_t d2q9_rows_step(int prec, const Tables& t, int step, int threads, int grid,
                           hipStream_t s);
-hipError_t twopass_psi(int prec, int q, const Tables& t, int parity, int grid, hipStream_t s);
-hipError_t twopass_collide(int prec, int q, const Tables& t, int parity, int grid, hipStream_t s);
+hipError_t twopass_psi(int prec, int q, const Tables& t, int step, int grid, hipStream_t s);
+hipError_t twopass_collide(int prec, int q, const Tables& t, int step, int grid, hipStream_t s);
 // D3Q19 jobs in one fused pass (psi window in LDS); agrees with the two
 // passes above to rounding.
-hipError_t d3q19_march_step(int prec, const Tables& t, int parity, int grid, hipStream_t s);
-// After a step: remaining -= 1, done += 1 and first = 0 for every stepped
-// job of launch chain `chain` (every single-step job if chain < 0); also
-// clears *mass_ticket, the counter of the next job_mass launch.  Chains are
-// disjoint sets of jobs, so the chains of one tick can run concurrently.
-hipError_t advance_counters(Job* jobs, int njobs, int chain, unsigned* mass_ticket, hipStream_t s);
+hipError_t d3q19_march_step(int prec, const Tables& t, int step, int grid, hipStream_t s);
+// After the `steps` single steps of a tick: remaining -= k, done += k with
+// k = min(steps, remaining), and first = 0, for every job that is not on
+// temporal blocks; also clears *mass_ticket, the counter of the next
+// job_mass launch.  One update per tick, so the kernel classes of a tick
+// depend on nothing but their own launches and run concurrently.
+hipError_t advance_counters(Job* jobs, int njobs, int steps, unsigned* mass_ticket, hipStream_t s);
 // D2Q9 temporal-block jobs (constant-shift forms) in tiles: each launch runs
 // kBlkK steps of every block job with more than kBlkK left, 1 of the others,
 // bit-identical to as many single steps of the row kernel; advance_blocks

@@ -185,8 +185,9 @@ __device__ __forceinline__ void pull_row9(const typename SP::T* __restrict__ P, 
 }
 
 template <typename SP, bool NS, int T>
-__global__ void __launch_bounds__(T) k_d2q9_rows(Tables t, int parity)
+__global__ void __launch_bounds__(T) k_d2q9_rows(Tables t, int step)
 {
+    const int parity = step & 1;
     using S = St<9>;
     __shared__ float psi_lds[4 * T];
     __shared__ float rho_lds[NS ? 4 * T : 1];
@@ -196,7 +197,7 @@ __global__ void __launch_bounds__(T) k_d2q9_rows(Tables t, int parity)
     for (int it = next_item(t, parity, &slot_s); it >= 0; it = next_item(t, parity, &slot_s)) {
         const Item item = t.items[it];
         Job& job = t.jobs[item.job];
-        if (job.remaining <= 0) continue;                 // uniform across the workgroup
+        if (job.remaining <= step) continue;              // uniform across the workgroup
         const int p = parity ^ job.phase;
         const auto* Pin = static_cast<const typename SP::T*>(job.P[p]);
         auto* Pout = static_cast<typename SP::T*>(job.P[p ^ 1]);
@@ -208,7 +209,7 @@ __global__ void __launch_bounds__(T) k_d2q9_rows(Tables t, int parity)
         // below then all name it; re-reading job.rs per row, which the
         // stores may alias, cost 5 VGPRs and a wave of occupancy)
         const float rs = job.rs;
-        const bool first = job.first != 0;
+        const bool first = job.first != 0 && step == 0;
         const float* rho0 = job.rho0;
         const bool active = y < ny;
         const int ym = y == 0 ? ny - 1 : y - 1, yp = y == ny - 1 ? 0 : y + 1;
@@ -516,19 +517,20 @@ __device__ __forceinline__ void pull_node32(const typename SP::T* __restrict__ P
 constexpr int kTP = 256;
 
 template <int Q, typename SP, bool NS>
-__global__ void __launch_bounds__(kTP) k_tp_psi(Tables t, int parity)
+__global__ void __launch_bounds__(kTP) k_tp_psi(Tables t, int step)
 {
+    const int parity = step & 1;
     __shared__ int slot_s;
     reset_other(t, parity);
     for (int it = next_item(t, parity, &slot_s); it >= 0; it = next_item(t, parity, &slot_s)) {
         const Item item = t.items[it];
         const Job& job = t.jobs[item.job];
-        if (job.remaining <= 0) continue;
+        if (job.remaining <= step) continue;
         const int p = parity ^ job.phase;
         const auto* Pin = static_cast<const typename SP::T*>(job.P[p]);
         const uint32_t c1 = static_cast<uint32_t>(item.c1);
         for (uint32_t c = static_cast<uint32_t>(item.c0) + threadIdx.x; c < c1; c += kTP) {
-            const bool from_rho0 = job.first && job.rho0;
+            const bool from_rho0 = job.first && step == 0 && job.rho0;
             float rp = 0.f;
             if (!from_rho0 || NS) {
                 int x, y, z;
@@ -546,15 +548,16 @@ __global__ void __launch_bounds__(kTP) k_tp_psi(Tables t, int parity)
 }
 
 template <int Q, typename SP, bool NS>
-__global__ void __launch_bounds__(kTP) k_tp_collide(Tables t, int parity)
+__global__ void __launch_bounds__(kTP) k_tp_collide(Tables t, int step)
 {
+    const int parity = step & 1;
     using S = St<Q>;
     __shared__ int slot_s;
     reset_other(t, parity);
     for (int it = next_item(t, parity, &slot_s); it >= 0; it = next_item(t, parity, &slot_s)) {
         const Item item = t.items[it];
         const Job& job = t.jobs[item.job];
-        if (job.remaining <= 0) continue;
+        if (job.remaining <= step) continue;
         const int p = parity ^ job.phase;
         const auto* Pin = static_cast<const typename SP::T*>(job.P[p]);
         auto* Pout = static_cast<typename SP::T*>(job.P[p ^ 1]);
@@ -570,7 +573,7 @@ __global__ void __launch_bounds__(kTP) k_tp_collide(Tables t, int parity)
             pull_node32<Q, SP>(Pin, job, c, o, d);
             float Rr = rs;
             if constexpr (NS) Rr = *fused::elem(job.R[p], c);
-            const bool at_rest = job.first && job.rho0;
+            const bool at_rest = job.first && step == 0 && job.rho0;
             const float rho = at_rest ? *fused::elem(job.rho0, c) : pulled_rho<Q, SP, NS>(d, Rr);
             // node c + e_n
             auto nb = [&](auto N) {
@@ -613,8 +616,9 @@ template <typename SP, bool NS>
 // Occupancy target 2 wavefronts per SIMD: held to 128 VGPRs the item loop
 // spills 36-108 bytes per thread, and measured slower.
 __global__ void __launch_bounds__(kMarchTY * kMarchTZ) __attribute__((amdgpu_waves_per_eu(2)))
-    k_d3q19_march(Tables t, int parity)
+    k_d3q19_march(Tables t, int step)
 {
+    const int parity = step & 1;
     using TL = fused::Tile<kMarchTY, kMarchTZ, false>;
     constexpr int PL = TL::LY * TL::LZ;
     __shared__ float wpsi[4 * PL], wrho[4 * PL];
@@ -623,7 +627,7 @@ __global__ void __launch_bounds__(kMarchTY * kMarchTZ) __attribute__((amdgpu_wav
     for (int it = next_item(t, parity, &slot_s); it >= 0; it = next_item(t, parity, &slot_s)) {
         const Item item = t.items[it];
         const Job& job = t.jobs[item.job];
-        if (job.remaining <= 0) continue;                 // uniform across the workgroup
+        if (job.remaining <= step) continue;              // uniform across the workgroup
         const int p = parity ^ job.phase;
         Grid g;
         g.nx = job.nx;
@@ -639,13 +643,16 @@ __global__ void __launch_bounds__(kMarchTY * kMarchTZ) __attribute__((amdgpu_wav
     }
 }
 
-__global__ void k_advance(Job* jobs, int njobs, int chain, unsigned* mass_ticket)
+// After the single steps of a tick: each job took min(steps, remaining) of
+// them (a step launch skips jobs with remaining <= its step index).
+__global__ void k_advance(Job* jobs, int njobs, int steps, unsigned* mass_ticket)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) *mass_ticket = 0u;
-    if (i < njobs && !jobs[i].blk && (chain < 0 || jobs[i].chain == chain) && jobs[i].remaining > 0) {
-        jobs[i].remaining -= 1;
-        jobs[i].done += 1;
+    if (i < njobs && !jobs[i].blk && jobs[i].remaining > 0) {
+        const int64_t k = jobs[i].remaining < steps ? jobs[i].remaining : steps;
+        jobs[i].remaining -= k;
+        jobs[i].done += k;
         jobs[i].first = 0;
     }
 }
@@ -815,57 +822,57 @@ inline unsigned blocks(int64_t n) { return static_cast<unsigned>((n + kTP - 1) /
 
 }  // namespace
 
-hipError_t d2q9_rows_step(int prec, const Tables& t, int parity, int threads, int grid,
+hipError_t d2q9_rows_step(int prec, const Tables& t, int step, int threads, int grid,
                           hipStream_t s)
 {
     if (threads < 1 || threads > 1024) return hipErrorInvalidValue;   // one thread per row node
     by_prec(prec, [&](auto sp, auto ns) {
         using SP = decltype(sp);
         constexpr bool NS = decltype(ns)::value;
-        if (threads <= 64) k_d2q9_rows<SP, NS, 64><<<grid, 64, 0, s>>>(t, parity);
-        else if (threads <= 128) k_d2q9_rows<SP, NS, 128><<<grid, 128, 0, s>>>(t, parity);
-        else if (threads <= 256) k_d2q9_rows<SP, NS, 256><<<grid, 256, 0, s>>>(t, parity);
-        else if (threads <= 512) k_d2q9_rows<SP, NS, 512><<<grid, 512, 0, s>>>(t, parity);
-        else k_d2q9_rows<SP, NS, 1024><<<grid, 1024, 0, s>>>(t, parity);
+        if (threads <= 64) k_d2q9_rows<SP, NS, 64><<<grid, 64, 0, s>>>(t, step);
+        else if (threads <= 128) k_d2q9_rows<SP, NS, 128><<<grid, 128, 0, s>>>(t, step);
+        else if (threads <= 256) k_d2q9_rows<SP, NS, 256><<<grid, 256, 0, s>>>(t, step);
+        else if (threads <= 512) k_d2q9_rows<SP, NS, 512><<<grid, 512, 0, s>>>(t, step);
+        else k_d2q9_rows<SP, NS, 1024><<<grid, 1024, 0, s>>>(t, step);
     });
     return hipGetLastError();
 }
 
-hipError_t twopass_psi(int prec, int q, const Tables& t, int parity, int grid, hipStream_t s)
+hipError_t twopass_psi(int prec, int q, const Tables& t, int step, int grid, hipStream_t s)
 {
     by_prec(prec, [&](auto sp, auto ns) {
         using SP = decltype(sp);
         constexpr bool NS = decltype(ns)::value;
-        if (q == 9) k_tp_psi<9, SP, NS><<<grid, kTP, 0, s>>>(t, parity);
-        else k_tp_psi<19, SP, NS><<<grid, kTP, 0, s>>>(t, parity);
+        if (q == 9) k_tp_psi<9, SP, NS><<<grid, kTP, 0, s>>>(t, step);
+        else k_tp_psi<19, SP, NS><<<grid, kTP, 0, s>>>(t, step);
     });
     return hipGetLastError();
 }
 
-hipError_t twopass_collide(int prec, int q, const Tables& t, int parity, int grid, hipStream_t s)
+hipError_t twopass_collide(int prec, int q, const Tables& t, int step, int grid, hipStream_t s)
 {
     by_prec(prec, [&](auto sp, auto ns) {
         using SP = decltype(sp);
         constexpr bool NS = decltype(ns)::value;
-        if (q == 9) k_tp_collide<9, SP, NS><<<grid, kTP, 0, s>>>(t, parity);
-        else k_tp_collide<19, SP, NS><<<grid, kTP, 0, s>>>(t, parity);
+        if (q == 9) k_tp_collide<9, SP, NS><<<grid, kTP, 0, s>>>(t, step);
+        else k_tp_collide<19, SP, NS><<<grid, kTP, 0, s>>>(t, step);
     });
     return hipGetLastError();
 }
 
-hipError_t d3q19_march_step(int prec, const Tables& t, int parity, int grid, hipStream_t s)
+hipError_t d3q19_march_step(int prec, const Tables& t, int step, int grid, hipStream_t s)
 {
     by_prec(prec, [&](auto sp, auto ns) {
         using SP = decltype(sp);
         constexpr bool NS = decltype(ns)::value;
-        k_d3q19_march<SP, NS><<<grid, kMarchTY * kMarchTZ, 0, s>>>(t, parity);
+        k_d3q19_march<SP, NS><<<grid, kMarchTY * kMarchTZ, 0, s>>>(t, step);
     });
     return hipGetLastError();
 }
 
-hipError_t advance_counters(Job* jobs, int njobs, int chain, unsigned* mass_ticket, hipStream_t s)
+hipError_t advance_counters(Job* jobs, int njobs, int steps, unsigned* mass_ticket, hipStream_t s)
 {
-    k_advance<<<(std::max(njobs, 1) + 255) / 256, 256, 0, s>>>(jobs, njobs, chain, mass_ticket);
+    k_advance<<<(std::max(njobs, 1) + 255) / 256, 256, 0, s>>>(jobs, njobs, steps, mass_ticket);
     return hipGetLastError();
 }
 

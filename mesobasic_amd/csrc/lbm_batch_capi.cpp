@@ -254,11 +254,10 @@ int ensure_mass(lbm_batch* b, int n)
     return LBM_OK;
 }
 
-Job device_view(const HostJob& j, int prec, double rs, bool blk = false, int chain = -1)
+Job device_view(const HostJob& j, int prec, double rs, bool blk = false)
 {
     Job v{};
     v.blk = blk ? 1 : 0;
-    v.chain = chain;
     v.P[0] = j.P[0];
     v.P[1] = j.P[1];
     v.psi = j.psi;
@@ -354,7 +353,7 @@ int upload_tables(lbm_batch* b)
     }
     for (int ji = 0; ji < static_cast<int>(b->order.size()); ji++) {
         const HostJob& j = *b->order[ji];
-        b->h_jobs[ji] = device_view(j, b->d.precision, b->d.rho_shift, use_blocks(b, j), chain_of(b, j));
+        b->h_jobs[ji] = device_view(j, b->d.precision, b->d.rho_shift, use_blocks(b, j));
         for (int64_t c0 = 0; c0 < j.n; c0 += kMassItem) {
             Item it{};
             it.job = ji;
@@ -437,7 +436,7 @@ int upload_tables(lbm_batch* b)
     return LBM_OK;
 }
 
-int launch_step(lbm_batch* b, int parity);
+int launch_step(lbm_batch* b, int step);
 int launch_mass(lbm_batch* b);
 
 // Temporal-block launches of a tick: an even number (the ticket parity of
@@ -458,23 +457,23 @@ int launch_blocks(lbm_batch* b, int parity, hipStream_t s)
     return LBM_OK;
 }
 
-// The launches of one step of chain c (a row class, a two-pass class by its
-// psi pass, or the D3Q19 march) on stream s, without the counter update.
-int launch_class(lbm_batch* b, int c, int parity, hipStream_t s)
+// The launches of step `step` of the tick for chain c (a row class, a
+// two-pass class by its psi pass, or the D3Q19 march) on stream s.
+int launch_class(lbm_batch* b, int c, int step, hipStream_t s)
 {
     if (c < kNumRow) {
         Tables t{b->d_jobs, b->d_items[c], b->d_nitems + c, b->d_tickets + 2 * c};
-        BHIP(lbm::batch::d2q9_rows_step(b->d.precision, t, parity, kRowWidth[c],
+        BHIP(lbm::batch::d2q9_rows_step(b->d.precision, t, step, kRowWidth[c],
                                         persistent_grid(b, kRowWidth[c]), s));
     } else if (c == kTp9Psi || c == kTp19Psi) {
         const int q = c == kTp9Psi ? 9 : 19;
         Tables tp{b->d_jobs, b->d_items[c], b->d_nitems + c, b->d_tickets + 2 * c};
         Tables tc{b->d_jobs, b->d_items[c + 1], b->d_nitems + c + 1, b->d_tickets + 2 * (c + 1)};
-        BHIP(lbm::batch::twopass_psi(b->d.precision, q, tp, parity, persistent_grid(b, 256), s));
-        BHIP(lbm::batch::twopass_collide(b->d.precision, q, tc, parity, persistent_grid(b, 256), s));
+        BHIP(lbm::batch::twopass_psi(b->d.precision, q, tp, step, persistent_grid(b, 256), s));
+        BHIP(lbm::batch::twopass_collide(b->d.precision, q, tc, step, persistent_grid(b, 256), s));
     } else if (c == kMarch19) {
         Tables tm{b->d_jobs, b->d_items[kMarch19], b->d_nitems + kMarch19, b->d_tickets + 2 * kMarch19};
-        BHIP(lbm::batch::d3q19_march_step(b->d.precision, tm, parity,
+        BHIP(lbm::batch::d3q19_march_step(b->d.precision, tm, step,
                                           persistent_grid(b, lbm::batch::kMarchTY * lbm::batch::kMarchTZ), s));
     }
     return LBM_OK;
@@ -493,8 +492,8 @@ std::vector<int> tick_chains(const lbm_batch* b)
     return c;
 }
 
-// Launch units of chain c in one tick: its block launches, or one step
-// (the class's kernels and the counter update) per step of the tick.
+// Launch units of chain c in one tick: its block launches (each followed by
+// its counter update), or the class's kernels of each step of the tick.
 int chain_units(const lbm_batch* b, int c)
 {
     return c == kBlk9 ? block_launches(b) : b->d.steps_per_tick;
@@ -504,15 +503,12 @@ int chain_units(const lbm_batch* b, int c)
 int launch_unit(lbm_batch* b, int c, int m, hipStream_t s)
 {
     if (c == kBlk9) return launch_blocks(b, m & 1, s);
-    int rc = launch_class(b, c, m & 1, s);
-    if (rc) return rc;
-    BHIP(lbm::batch::advance_counters(b->d_jobs, b->cap_jobs, c, b->d_tickets + 2 * kMass, s));
-    return LBM_OK;
+    return launch_class(b, c, m, s);
 }
 
 // The launches of a tick: each chain's units (on its own stream when there
-// are several and chains run concurrently), then the mass check on the
-// batch's stream.  Concurrent chains are issued unit by unit across the
+// are several and chains run concurrently), the counter update of the
+// single-step jobs, then the mass check, on the batch's stream.  Concurrent chains are issued unit by unit across the
 // chains, the largest chain first: a hipGraph replay submits its kernel
 // nodes in about the order they were captured, and captured chain after
 // chain the largest class started only after the small ones had run most of
@@ -544,25 +540,27 @@ int launch_tick(lbm_batch* b)
                 int rc = launch_unit(b, kBlk9, m, b->stream);
                 if (rc) return rc;
             }
-        const bool single = std::any_of(chains.begin(), chains.end(), [](int c) { return c != kBlk9; });
-        if (single)
+        if (std::any_of(chains.begin(), chains.end(), [](int c) { return c != kBlk9; }))
             for (int st = 0; st < b->d.steps_per_tick; st++) {
-                int rc = launch_step(b, st & 1);
+                int rc = launch_step(b, st);
                 if (rc) return rc;
             }
     }
+    if (std::any_of(chains.begin(), chains.end(), [](int c) { return c != kBlk9; }))
+        BHIP(lbm::batch::advance_counters(b->d_jobs, b->cap_jobs, b->d.steps_per_tick,
+                                          b->d_tickets + 2 * kMass, b->stream));
     return launch_mass(b);
 }
 
-// One step of every single-step job, all classes on the batch's stream.
-int launch_step(lbm_batch* b, int parity)
+// Step `step` of the tick for every single-step job, all classes on the
+// batch's stream.
+int launch_step(lbm_batch* b, int step)
 {
     for (int c : tick_chains(b)) {
         if (c == kBlk9) continue;
-        int rc = launch_class(b, c, parity, b->stream);
+        int rc = launch_class(b, c, step, b->stream);
         if (rc) return rc;
     }
-    BHIP(lbm::batch::advance_counters(b->d_jobs, b->cap_jobs, -1, b->d_tickets + 2 * kMass, b->stream));
     return LBM_OK;
 }
 
