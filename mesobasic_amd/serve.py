@@ -20,9 +20,13 @@ are not starved; then one tick (steps_per_tick steps of every active job,
 one graph replay) runs, and finished jobs are read back and freed, which
 makes room for the next admissions.  Latency = completion - arrival, on
 the host's monotonic clock; aggregate MLUPS = sum(nodes * steps) / wall
-time from the first arrival to the last completion.  Storage is FP16 with
-the node shift by default: droplet requests (densities 2.0 / 0.15) stay
-within 1e-3 of the FP64 path in it, where the constant shift does not.
+time from the first arrival to the last completion.  The command line
+serves D2Q9 requests in FP32 storage and D3Q19 requests in FP16 with the
+node shift, on two engines that share the budget: droplet requests
+(densities 2.0 / 0.15) then stay within 1e-3 of the FP64 path.  The D3Q19
+node-shift form is closer to FP64 than FP32 storage on droplets (1e-5
+against 5e-5 at 128^3), the D2Q9 one is not (up to 3e-3 after 3,000 steps
+at G = -5.7, against 2e-4), and the constant shift is 0.2-0.7 off.
 
 Failure handling: the engine retires a job whose FP64 mass total goes
 non-finite at the end of the tick that sees it (outcome "failed", its work
@@ -78,21 +82,40 @@ def run_stream(reqs, precision=FP16N, budget_bytes=24 << 30, max_active=0, steps
     keep_results, the per-request fields of every request, else of the ids in
     keep_ids, as a second value).
 
-    work_limit > 0: each tick advances the admitted jobs with the least
-    remaining work, up to that many nodes; a job left waiting for max_wait
-    ticks goes first (no starvation of large requests).  A request may carry a
-    "deadline" (seconds after its arrival): a request still queued or
-    running past it is cancelled and its memory freed.  Outcomes: "ok";
-    "failed" (the engine retired the job because its FP64 mass total went
-    non-finite; its work counts up to the last finite check); "cancelled";
-    "rejected" (larger than the whole budget, or its allocation kept failing
-    with nothing else resident).  `engine` replaces the GPU batch (an object
-    with the Batch methods; tests pass a host-only stand-in)."""
-    b = engine if engine is not None else Batch(
-        precision=precision, hbm_budget=budget_bytes, steps_per_tick=steps_per_tick, use_graph=True,
-        autotune=False, work_limit=work_limit, max_wait=max_wait if work_limit else 0)
+    precision: the storage form of every request, or a dict {q: form} that
+    chooses it by stencil; each form runs on a batch engine of its own, the
+    engines share the HBM budget and each loop turn ticks every engine that
+    holds jobs.  work_limit > 0: each tick advances the admitted jobs with
+    the least remaining work, up to that many nodes (split evenly between
+    the engines); a job left
+    waiting for max_wait ticks goes first (no starvation of large requests).
+    A request may carry a "deadline" (seconds after its arrival): a request
+    still queued or running past it is cancelled and its memory freed.
+    Outcomes: "ok"; "failed" (the engine retired the job because its FP64
+    mass total went non-finite; its work counts up to the last finite
+    check); "cancelled"; "rejected" (larger than the whole budget, or its
+    allocation kept failing with nothing else resident).  `engine` replaces
+    the GPU batch for every request (an object with the Batch methods; tests
+    pass a host-only stand-in)."""
+    forms = dict(precision) if isinstance(precision, dict) else None
+    engines = {}                    # form -> engine
+    # the node limit per tick is shared by the engines (each turn ticks them all)
+    n_forms = len(set(forms.values())) if forms is not None else 1
+    limit = -(-work_limit // n_forms) if work_limit else 0
+
+    def engine_of(r):
+        key = None if engine is not None else (forms[r["q"]] if forms is not None else precision)
+        if key not in engines:
+            engines[key] = engine if engine is not None else Batch(
+                precision=key, hbm_budget=budget_bytes, steps_per_tick=steps_per_tick, use_graph=True,
+                autotune=False, work_limit=limit, max_wait=max_wait if limit else 0)
+        return engines[key]
+
+    def in_use():
+        return sum(e.bytes_in_use for e in engines.values())
+
     pending = list(reqs)            # not yet admitted, arrival order
-    live = {}                       # job id -> request
+    live = {}                       # (engine, job id) -> request
     outcome, latency, fail_info = {}, {}, {}
     results = {}
     keep_ids = set(keep_ids)
@@ -110,10 +133,10 @@ def run_stream(reqs, precision=FP16N, budget_bytes=24 << 30, max_active=0, steps
     while pending or live:
         now = clock() - t0
         # deadlines: queued requests past theirs are dropped, running ones cancelled
-        for jid, r in list(live.items()):
+        for (b, jid), r in list(live.items()):
             if r.get("deadline") is not None and now - r["arrival"] > r["deadline"]:
                 b.cancel(jid)
-                del live[jid]
+                del live[(b, jid)]
                 finish(r, "cancelled", now)
         while pending and pending[0].get("deadline") is not None and \
                 now - pending[0]["arrival"] > pending[0]["deadline"]:
@@ -121,10 +144,14 @@ def run_stream(reqs, precision=FP16N, budget_bytes=24 << 30, max_active=0, steps
         # admissions, in arrival order, no overtaking
         while pending and pending[0]["arrival"] <= now and (max_active <= 0 or len(live) < max_active):
             r = pending[0]
-            if b.job_bytes(r["q"], r["shape"], r["tau"], r["G"], r["steps"]) > budget_bytes:
+            b = engine_of(r)
+            need = b.job_bytes(r["q"], r["shape"], r["tau"], r["G"], r["steps"])
+            if need > budget_bytes:
                 finish(pending.pop(0), "rejected", now)
                 continue
-            jid = b.submit(r["q"], r["shape"], r["tau"], r["G"], r["steps"], initial_rho(r))
+            # the engines share the budget
+            jid = None if (len(engines) > 1 and in_use() + need > budget_bytes) else \
+                b.submit(r["q"], r["shape"], r["tau"], r["G"], r["steps"], initial_rho(r))
             if jid is None:
                 if live:
                     break           # waits for memory to free up
@@ -136,30 +163,33 @@ def run_stream(reqs, precision=FP16N, budget_bytes=24 << 30, max_active=0, steps
                     retries = 0
                 continue
             retries = 0
-            live[jid] = r
+            live[(b, jid)] = r
             pending.pop(0)
         max_live = max(max_live, len(live))
-        peak_bytes = max(peak_bytes, b.bytes_in_use)
+        peak_bytes = max(peak_bytes, in_use())
         if not live:
             if pending:
                 time.sleep(max(0.0, pending[0]["arrival"] - (clock() - t0)))
             continue
-        b.tick()
+        busy = {b for (b, _) in live}
+        for b in busy:
+            b.tick()
         ticks += 1
-        for jid in b.finished():
-            r = live.pop(jid)
-            out = b.collect(jid)
-            now = clock() - t0
-            nodes = int(np.prod(r["shape"]))
-            if out["failed"]:
-                finish(r, "failed", now)
-                fail_info[r["id"]] = dict(steps_done=out["steps_done"], steps_finite=out["steps_finite"])
-                work += nodes * out["steps_finite"]
-            else:
-                finish(r, "ok", now)
-                work += nodes * r["steps"]
-            if keep_results or r["id"] in keep_ids:
-                results[r["id"]] = out
+        for b in busy:
+            for jid in b.finished():
+                r = live.pop((b, jid))
+                out = b.collect(jid)
+                now = clock() - t0
+                nodes = int(np.prod(r["shape"]))
+                if out["failed"]:
+                    finish(r, "failed", now)
+                    fail_info[r["id"]] = dict(steps_done=out["steps_done"], steps_finite=out["steps_finite"])
+                    work += nodes * out["steps_finite"]
+                else:
+                    finish(r, "ok", now)
+                    work += nodes * r["steps"]
+                if keep_results or r["id"] in keep_ids:
+                    results[r["id"]] = out
     end = clock() - t0
     wall = end - reqs[0]["arrival"]
     ok = [r["id"] for r in reqs if outcome[r["id"]] == "ok"]
@@ -171,7 +201,8 @@ def run_stream(reqs, precision=FP16N, budget_bytes=24 << 30, max_active=0, steps
                    max_active=max_live, peak_hbm_bytes=int(peak_bytes), outcomes=counts,
                    failed=fail_info)
     if engine is None:
-        b.close()
+        for b in engines.values():
+            b.close()
     return (summary, results) if (keep_results or keep_ids) else summary
 
 
@@ -289,13 +320,20 @@ def main():
     ap.add_argument("--steps-per-tick", type=int, default=20)
     ap.add_argument("--work-limit", type=int, default=0,
                     help="nodes advanced per step, least remaining work first (0: all)")
-    ap.add_argument("--precision", default=None, choices=["fp32", "fp16", "bf16", "fp16n", "bf16n"],
-                    help="storage form (default: fp16n for the stream, fp16 for --batched)")
+    ap.add_argument("--precision", default=None,
+                    help="storage form, fp32 / fp16 / bf16 / fp16n / bf16n, or FORM2D,FORM3D for the "
+                         "stream's D2Q9 and D3Q19 requests (default: fp32,fp16n for the stream, fp16 for "
+                         "--batched)")
     ap.add_argument("--d2q9-kernel", default="auto", choices=["auto", "rows", "two_pass", "blocks"],
                     help="--batched: D2Q9 kernel (auto: the tuner picks)")
     args = ap.parse_args()
     launch = {"auto": None, "rows": (32, 1, 1), "two_pass": (32, 0, 1), "blocks": (32, 2, 1)}[args.d2q9_kernel]
-    prec = PRECISIONS[args.precision or ("fp16" if args.batched else "fp16n")]
+    spec = args.precision or ("fp16" if args.batched else "fp32,fp16n")
+    if "," in spec and not args.batched:
+        p2, p3 = spec.split(",")
+        prec = {9: PRECISIONS[p2], 19: PRECISIONS[p3]}
+    else:
+        prec = PRECISIONS[spec]
     if args.batched:
         s = run_batched(make_batched_jobs(args.jobs), args.steps, prec, args.steps_per_tick, launch=launch)
     else:

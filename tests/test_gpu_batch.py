@@ -159,8 +159,10 @@ def test_large_rows_take_the_two_pass_path():
     assert rel_l2(res["rho"], o.moments()["rho"]) < 2e-6
 
 
-@pytest.mark.parametrize("work_limit", [0, 3000])
-def test_serving_stream_admits_as_memory_frees_and_matches_alone(work_limit):
+@pytest.mark.parametrize("work_limit,forms", [(0, None), (3000, None), (3000, {9: mb.FP32, 19: mb.FP16N})])
+def test_serving_stream_admits_as_memory_frees_and_matches_alone(work_limit, forms):
+    # forms: one storage form for every request (FP16 constant shift), or one
+    # per stencil, each on an engine of its own sharing the budget
     from mesobasic_amd import serve
     rng = np.random.default_rng(9)
     reqs = []
@@ -171,8 +173,9 @@ def test_serving_stream_admits_as_memory_frees_and_matches_alone(work_limit):
         reqs.append(dict(id=k, arrival=0.004 * max(0, k - 3), q=q, shape=shape, steps=int(rng.integers(20, 90)),
                          tau=1.0, G=float(rng.uniform(-6.0, -4.5)),
                          init="droplet" if k % 2 else "noise", seed=100 + k))
-    need = max(Batch(precision=mb.FP16S).job_bytes(r["q"], r["shape"]) for r in reqs)
-    summary, res = serve.run_stream(reqs, precision=mb.FP16S, budget_bytes=3 * need, steps_per_tick=8,
+    form = (lambda r: forms[r["q"]]) if forms else (lambda r: mb.FP16S)
+    need = max(Batch(precision=form(r)).job_bytes(r["q"], r["shape"]) for r in reqs)
+    summary, res = serve.run_stream(reqs, precision=forms or mb.FP16S, budget_bytes=3 * need, steps_per_tick=8,
                                     work_limit=work_limit, keep_results=True)
     print(summary, flush=True)
     # the first four arrive together: at least three fit the budget at once
@@ -180,7 +183,7 @@ def test_serving_stream_admits_as_memory_frees_and_matches_alone(work_limit):
     assert summary["peak_hbm_bytes"] <= 3 * need
     assert summary["p50_s"] <= summary["p99_s"]
     for r in reqs:
-        b = Batch(precision=mb.FP16S, steps_per_tick=4)
+        b = Batch(precision=form(r), steps_per_tick=4)
         jid = b.submit(r["q"], r["shape"], r["tau"], r["G"], r["steps"], serve.initial_rho(r))
         while not b.finished():
             b.tick()
