@@ -524,16 +524,17 @@ int launch_tick(lbm_batch* b)
             BHIP(hipStreamWaitEvent(b->cstream[c], b->ev_fork, 0));
             units = std::max(units, chain_units(b, c));
         }
-        for (int m = 0; m < units; m++)
+        int rc = LBM_OK;
+        for (int m = 0; m < units && rc == LBM_OK; m++)
             for (int c : chains)
-                if (m < chain_units(b, c)) {
-                    int rc = launch_unit(b, c, m, b->cstream[c]);
-                    if (rc) return rc;
-                }
+                if (rc == LBM_OK && m < chain_units(b, c)) rc = launch_unit(b, c, m, b->cstream[c]);
+        // every forked stream is joined back, also after a failed launch (a
+        // capture cannot end with a stream left out)
         for (int c : chains) {
             BHIP(hipEventRecord(b->ev_join[c], b->cstream[c]));
             BHIP(hipStreamWaitEvent(b->stream, b->ev_join[c], 0));
         }
+        if (rc) return rc;
     } else {
         if (b->class_mask & (1u << kBlk9))
             for (int m = 0; m < chain_units(b, kBlk9); m++) {
