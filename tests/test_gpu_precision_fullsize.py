@@ -13,7 +13,8 @@ storage ends 5e-3 off FP64 at 5,000 steps and no reduced form can hold the
 pointwise L2 to 1e-3 there.  The L2 is checked at 500 steps (FP32 and FP16
 node shift below 1e-3; FP32 stays below it through 1,000 steps, FP16 with
 the node shift crosses it between 500 and 1,000), and at 5,000 steps the
-size-independent results: coexistence densities and total-mass drift."""
+size-independent results: bulk coexistence densities (phase medians) and
+total-mass drift."""
 import numpy as np
 import pytest
 
