@@ -208,9 +208,19 @@ def copy_calibration(nbytes=4 << 30):
     nbytes, read + write bytes per second."""
     ms_k = mb.lbm.probe_copy(nbytes, reps=5, mode=4)
     ms_m = mb.lbm.probe_memcpy(nbytes, reps=5)
+    # the fused march's own access pattern without its arithmetic (19 pulls
+    # and 19 stores per node, 512^3, the population-plane layout), per
+    # storage width: the attainable rate of the step kernels' traffic
+    skel = {}
+    for name, eb in (("fp64", 8), ("fp32", 4), ("fp16", 2)):
+        ms = mb.lbm.probe_march(512, eb, 0, reps=10)
+        gbs = 2 * 19 * eb * 512 ** 3 / (ms * 1e-3) / 1e9
+        skel[name] = {"GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+                      "mlups": round(512 ** 3 / (ms * 1e-3) / 1e6, 1)}
     return {"bytes": nbytes,
             "copy_kernel_GBs": round(2 * nbytes / (ms_k * 1e-3) / 1e9, 1),
-            "hipMemcpyDtoD_GBs": round(2 * nbytes / (ms_m * 1e-3) / 1e9, 1)}
+            "hipMemcpyDtoD_GBs": round(2 * nbytes / (ms_m * 1e-3) / 1e9, 1),
+            "march_skeleton_512": skel}
 
 
 def main():
@@ -233,7 +243,20 @@ def main():
                     help="requests in the serving stream (Poisson, 200/s)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the streaming copy kernel over MIB MiB (for counter scaling)")
+    ap.add_argument("--march-skeleton", action="store_true",
+                    help="only run the fused march's memory skeleton (256^3 and 512^3, 8/4/2-byte "
+                         "values, both layouts), one JSON line each")
     args = ap.parse_args()
+    if args.march_skeleton:
+        for n in (256, 512):
+            for eb in (8, 4, 2):
+                for layout in (0, 1):
+                    ms = mb.lbm.probe_march(n, eb, layout, reps=10)
+                    gbs = 2 * 19 * eb * n ** 3 / (ms * 1e-3) / 1e9
+                    print(json.dumps({"n": n, "elem_bytes": eb, "layout": layout, "ms": round(ms, 4),
+                                      "GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+                                      "mlups": round(n ** 3 / (ms * 1e-3) / 1e6, 1)}), flush=True)
+        return
     if args.copy_calibration:
         nbytes = args.copy_calibration * 2**20
         res = {"copy_bytes": nbytes}

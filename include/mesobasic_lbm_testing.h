@@ -35,6 +35,17 @@ int lbm_probe_copy(int64_t bytes, int reps, int mode, double* ms);
  * depend on this library's kernels. */
 int lbm_probe_memcpy(int64_t bytes, int reps, double* ms);
 
+/* Memory skeleton of the fused D3Q19 march on an n^3 periodic lattice (n a
+ * multiple of 64, at most 1024): the kernel's loads and stores (8x64 tiles,
+ * its block order and x chunks, each node's 19 upstream values pulled and
+ * stored unchanged) without its arithmetic, LDS window, ring or barriers,
+ * on two buffers it allocates, `reps` times after two untimed runs;
+ * *ms = average time per run.  elem_bytes 2, 4 or 8; layout 0: one plane per
+ * population (the lattice handles' layout), 1: the 19 populations of each
+ * 64-node z segment stored together.  The attainable rate of that access
+ * pattern, against which the step kernels are measured. */
+int lbm_probe_march(int n, int elem_bytes, int layout, int reps, double* ms);
+
 /* Device allocations the library has made so far: every hipMalloc of the
  * lattice handles and the batch engine goes through one wrapper that counts
  * it (the probes above allocate on their own and are not counted). */

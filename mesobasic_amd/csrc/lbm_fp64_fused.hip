@@ -466,3 +466,117 @@ extern "C" int lbm_probe_tile_map(int nty, int ntz, int ncx, int order, int* ty,
         lbm::fused::tile_of_block(b, nty, ntz, ncx, order, ty[b], tz[b], cx[b]);
     return 0;
 }
+
+// Memory skeleton of the fused D3Q19 march: the access pattern of k_fused
+// (8x64 tiles of 512 threads, one node each, the same block -> tile band
+// order, x chunks of 128 planes for 8-byte values and 64 for narrower ones,
+// the pulls of the next plane issued before the stores of this one) with
+// the arithmetic, the LDS window, the ring and the barriers taken out: every
+// node loads its 19 upstream values and stores them unchanged.  Its rate is
+// the ceiling of that pattern on the memory system, for the layout of this
+// library (layout 0: one plane per population) or a blocked one (layout 1:
+// the 19 populations of each 64-node z segment stored together).
+namespace {
+
+template <typename T, int LAYOUT>
+__global__ void __launch_bounds__(512) k_probe_march(const T* __restrict__ A, T* __restrict__ B, int n,
+                                                   int64_t ps, int chunk)
+{
+    using S = lbm::St<19>;
+    constexpr int TY = 8, TZ = 64;
+    const int nt = n / TY, nzt = n / TZ, ncx = (n + chunk - 1) / chunk;
+    int ty, tz, cx;
+    lbm::fused::tile_of_block(blockIdx.x, nt, nzt, ncx, 3, ty, tz, cx);
+    const int y = ty * TY + static_cast<int>(threadIdx.x) / TZ;
+    const int z = tz * TZ + static_cast<int>(threadIdx.x) % TZ;
+    const int x0 = cx * chunk, x1 = min(n, x0 + chunk);
+    auto at = [&](int q, int xx, int yy, int zz) -> size_t {
+        if constexpr (LAYOUT == 0)
+            return static_cast<size_t>(q) * ps + (static_cast<size_t>(xx) * n + yy) * n + zz;
+        else
+            return (((static_cast<size_t>(xx) * n + yy) * nzt + (zz >> 6)) * 19 + q) * 64 + (zz & 63);
+    };
+    auto wrap = [&](int v) { return v < 0 ? v + n : (v >= n ? v - n : v); };
+    auto load = [&](int x, T (&v)[19]) {
+        lbm::static_for<19>([&](auto I) {
+            constexpr int q = decltype(I)::value;
+            v[q] = A[at(q, wrap(x - S::ex[q]), wrap(y - S::ey[q]), wrap(z - S::ez[q]))];
+        });
+    };
+    auto store = [&](int x, const T (&v)[19]) {
+        lbm::static_for<19>([&](auto I) {
+            constexpr int q = decltype(I)::value;
+            __builtin_nontemporal_store(v[q], B + at(q, x, y, z));
+        });
+    };
+    T a[19], b[19];
+    load(x0, a);
+    int x = x0;
+    for (; x + 1 < x1; x += 2) {
+        load(x + 1, b);
+        store(x, a);
+        if (x + 2 < x1) load(x + 2, a);
+        store(x + 1, b);
+    }
+    if (x < x1) store(x, a);
+}
+
+template <typename T, int LAYOUT>
+void launch_probe_march(const void* A, void* B, int n, int64_t ps)
+{
+    const int chunk = sizeof(T) == 8 ? 128 : 64;
+    const unsigned nb = static_cast<unsigned>((n / 8) * (n / 64) * ((n + chunk - 1) / chunk));
+    k_probe_march<T, LAYOUT><<<nb, 512>>>(static_cast<const T*>(A), static_cast<T*>(B), n, ps, chunk);
+}
+
+}  // namespace
+
+extern "C" int lbm_probe_march(int n, int elem_bytes, int layout, int reps, double* ms)
+{
+    if (n < 64 || n % 64 != 0 || n > 1024 || reps < 1 || (layout != 0 && layout != 1) ||
+        (elem_bytes != 2 && elem_bytes != 4 && elem_bytes != 8))
+        return -1;
+    const int64_t nodes = static_cast<int64_t>(n) * n * n;
+    // planes padded as the lattice handles pad them (256 B, skewed by 512 B)
+    const int64_t per = 256 / elem_bytes;
+    int64_t ps = (nodes + per - 1) / per * per;
+    if ((ps * elem_bytes) % 4096 == 0) ps += 512 / elem_bytes;
+    const size_t bytes = static_cast<size_t>(19 * ps * elem_bytes);
+    void* a = nullptr;
+    void* b = nullptr;
+    if (hipMalloc(&a, bytes) != hipSuccess) return -2;
+    if (hipMalloc(&b, bytes) != hipSuccess) { (void)hipFree(a); return -2; }
+    int rc = 0;
+    if (hipMemset(a, 0, bytes) != hipSuccess || hipMemset(b, 0, bytes) != hipSuccess) rc = -3;
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    if (!rc) {
+        // two untimed runs first; the buffers swap every run, as the lattice's do
+        for (int r = -2; r < reps; r++) {
+            if (r == 0) (void)hipEventRecord(e0, nullptr);
+            const void* src = (r & 1) ? b : a;
+            void* dst = (r & 1) ? a : b;
+            if (elem_bytes == 8) {
+                if (layout == 0) launch_probe_march<double, 0>(src, dst, n, ps);
+                else launch_probe_march<double, 1>(src, dst, n, ps);
+            } else if (elem_bytes == 4) {
+                if (layout == 0) launch_probe_march<float, 0>(src, dst, n, ps);
+                else launch_probe_march<float, 1>(src, dst, n, ps);
+            } else {
+                if (layout == 0) launch_probe_march<uint16_t, 0>(src, dst, n, ps);
+                else launch_probe_march<uint16_t, 1>(src, dst, n, ps);
+            }
+        }
+        (void)hipEventRecord(e1, nullptr);
+        if (hipEventSynchronize(e1) != hipSuccess || hipGetLastError() != hipSuccess) rc = -3;
+        float t = 0.f;
+        (void)hipEventElapsedTime(&t, e0, e1);
+        if (ms) *ms = t / reps;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return rc;
+}

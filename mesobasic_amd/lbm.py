@@ -143,6 +143,7 @@ def load():
         "lbm_snapshot_begin": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),
         "lbm_snapshot_wait": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int64)] + [_D] * 7),
         "lbm_probe_tile_map": (ctypes.c_int, [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_int)] * 3),
+        "lbm_probe_march": (ctypes.c_int, [ctypes.c_int] * 4 + [_D]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -216,6 +217,17 @@ def probe_memcpy(nbytes, reps=5):
     lib = load()
     ms = ctypes.c_double()
     _check(lib.lbm_probe_memcpy(int(nbytes), int(reps), ctypes.byref(ms)))
+    return ms.value
+
+
+def probe_march(n, elem_bytes=8, layout=0, reps=10):
+    """Memory skeleton of the fused D3Q19 march on an n^3 lattice (its loads
+    and stores, no arithmetic): returns ms per run.  layout 0: one plane per
+    population (this library's layout); 1: the 19 populations of each 64-node
+    z segment together."""
+    lib = load()
+    ms = ctypes.c_double()
+    _check(lib.lbm_probe_march(int(n), int(elem_bytes), int(layout), int(reps), ctypes.byref(ms)))
     return ms.value
 
 
