@@ -202,6 +202,34 @@ def roofline_block(q, prec, n, ms_per_step, kname, workload):
             "kernel": f"step ({kname}); HIP-event time per step over the timed region"}
 
 
+# Two-component Shan-Chen (mcmp3D / mcmp2D), FP64: name -> q, shape, tau_a,
+# tau_b, G.  Both components start as seeded noise around 0.5.
+MC_WORKLOADS = {
+    "mcmp3d_fp64_256_noise": (19, (256, 256, 256), 1.0, 0.8, 3.0),
+    "mcmp2d_fp64_4096_noise": (9, (4096, 4096), 1.0, 0.8, 3.0),
+}
+
+
+def time_mcmp(name, steps=10, warmup=2):
+    """MLUPS of a two-component workload; algorithmic bytes per update are
+    both components' populations read and written once, 2 x 2 x Q x 8 B."""
+    q, shape, ta, tb, G = MC_WORKLOADS[name]
+    lat = mb.Lattice(q, shape, ta, G, precision=mb.FP64, components=2, tau_b=tb)
+    lat.set_rho(mb.noise_rho(shape, 31, mean=0.5, amp=0.1), comp=0)
+    lat.set_rho(mb.noise_rho(shape, 32, mean=0.5, amp=0.1), comp=1)
+    lat.init()
+    lat.step_timed(warmup)
+    ms = lat.step_timed(steps) / steps
+    n = int(np.prod(shape))
+    bpl = 2 * 2 * q * 8
+    out = {"mlups": round(n / (ms * 1e-3) / 1e6, 1), "ms_per_step": round(ms, 4),
+           "algorithmic_bytes_per_lu": bpl,
+           "hbm_frac": round(bpl * n / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "kernel": KERNEL_NAMES.get(lat.kernel)}
+    lat.close()
+    return out
+
+
 def copy_calibration(nbytes=4 << 30):
     """Attainable HBM bandwidth on this box: the library's fastest streaming
     copy kernel and the HIP runtime's device-to-device copy, both over
@@ -243,10 +271,15 @@ def main():
                     help="requests in the serving stream (Poisson, 200/s)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the streaming copy kernel over MIB MiB (for counter scaling)")
+    ap.add_argument("--mcmp", action="store_true",
+                    help="only time the two-component workloads, one JSON line")
     ap.add_argument("--march-skeleton", action="store_true",
                     help="only run the fused march's memory skeleton (256^3 and 512^3, 8/4/2-byte "
                          "values, both layouts), one JSON line each")
     args = ap.parse_args()
+    if args.mcmp:
+        print(json.dumps({k: time_mcmp(k, steps=args.steps) for k in MC_WORKLOADS}), flush=True)
+        return
     if args.march_skeleton:
         for n in (256, 512):
             for eb in (8, 4, 2):
@@ -342,6 +375,8 @@ def main():
         # and a 4096^2 one (HBM-bound), automatic kernel choice
         out["d2q9_fp64"] = {"256x256": time_workload("d2q9_fp64_256_noise", steps=2000, warmup=10),
                             "4096x4096": time_workload("d2q9_fp64_4096_noise", steps=30)}
+        # the two-component model (two-pass FP64 kernels)
+        out["mcmp_fp64"] = {k: time_mcmp(k, steps=10) for k in MC_WORKLOADS}
     if args.workload == "d3q19_fp64_256_droplet":
         from mesobasic_amd import serve
         if not args.no_batched:
