@@ -230,6 +230,82 @@ def time_mcmp(name, steps=10, warmup=2):
     return out
 
 
+def run_driver(text, timeout=900):
+    """Runs the C++ drop-in driver (mesobasic_amd/host/meso_amd) on an
+    input.dat of the given text in a scratch directory; returns (seconds,
+    MLUPS including output) from its timing line."""
+    import re
+    import subprocess
+    import tempfile
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "mesobasic_amd", "host", "meso_amd")
+    if not os.path.exists(exe):
+        raise RuntimeError(f"{exe} is not built (make -C mesobasic_amd/host)")
+    with tempfile.TemporaryDirectory() as d:
+        with open(os.path.join(d, "input.dat"), "w") as f:
+            f.write(text)
+        r = subprocess.run([exe], cwd=d, capture_output=True, text=True, timeout=timeout)
+        if r.returncode != 0:
+            raise RuntimeError(f"meso_amd failed: {r.stderr[-2000:]}")
+        m = re.search(r"(\d+) steps in ([0-9.]+) s \(([0-9.]+) MLUPS", r.stderr)
+        return float(m.group(2)), float(m.group(3))
+
+
+def driver_measurements():
+    """The drop-in driver end to end: scmp3D 192^3 for 3,000 steps with no
+    output, with 6 VTK outputs written synchronously (Output/async = 0, the
+    reference's behaviour) and with asynchronous snapshots (the default); and
+    capsule2D (256 x 128, 64 IB nodes) against scmp2D on the same lattice,
+    10,000 steps without output."""
+    scmp = """[Domain]
+  nx = 192
+  ny = 192
+  nz = 192
+[Time]
+  nstep = 3000
+[Output]
+  numOutputs = {nout}
+  async = {asy}
+[LBApp]
+  app = true
+  type = scmp3D
+  G = -5.5
+  rhoAi = 0.7
+  rhoAi_noise = 0.02
+  tauA = 1.0
+  seed = 9
+"""
+    out = {"scmp3d_192_3000_steps": {}}
+    for label, nout, asy in (("no_output", 0, 1), ("sync_6_outputs", 6, 0), ("async_6_outputs", 6, 1)):
+        secs, mlups = run_driver(scmp.format(nout=nout, asy=asy))
+        out["scmp3d_192_3000_steps"][label] = {"s": round(secs, 3), "mlups_incl_output": round(mlups, 1)}
+    lat2d = """[Domain]
+  nx = 256
+  ny = 128
+[Time]
+  nstep = 10000
+[Output]
+  numOutputs = 0
+[LBApp]
+  app = true
+  type = {typ}
+  tauA = 0.9
+  G = -5.0
+  rhoAi = 0.7
+  rhoAi_noise = 0.01
+  nnodes = 64
+  kstiff = 0.8
+  capsule_x = 64.0
+  capsule_y = 63.5
+  capsule_r = 16.0
+  seed = 5
+"""
+    for typ in ("capsule2D", "scmp2D"):
+        secs, mlups = run_driver(lat2d.format(typ=typ))
+        out[f"{typ}_256x128_10000_steps"] = {"s": round(secs, 3), "us_per_step": round(secs / 1e4 * 1e6, 2),
+                                            "mlups": round(mlups, 1)}
+    return out
+
+
 def copy_calibration(nbytes=4 << 30):
     """Attainable HBM bandwidth on this box: the library's fastest streaming
     copy kernel and the HIP runtime's device-to-device copy, both over
@@ -271,12 +347,19 @@ def main():
                     help="requests in the serving stream (Poisson, 200/s)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the streaming copy kernel over MIB MiB (for counter scaling)")
+    ap.add_argument("--no-driver", action="store_true",
+                    help="skip the C++ drop-in driver side measurements (output path, capsule)")
+    ap.add_argument("--driver", action="store_true",
+                    help="only run the C++ drop-in driver measurements (output path, capsule), one JSON line")
     ap.add_argument("--mcmp", action="store_true",
                     help="only time the two-component workloads, one JSON line")
     ap.add_argument("--march-skeleton", action="store_true",
                     help="only run the fused march's memory skeleton (256^3 and 512^3, 8/4/2-byte "
                          "values, both layouts), one JSON line each")
     args = ap.parse_args()
+    if args.driver:
+        print(json.dumps(driver_measurements()), flush=True)
+        return
     if args.mcmp:
         print(json.dumps({k: time_mcmp(k, steps=args.steps) for k in MC_WORKLOADS}), flush=True)
         return
@@ -416,6 +499,12 @@ def main():
             out["serving"] = r
             print(f"# serving: p50 {r['p50_s']:.2f} s p99 {r['p99_s']:.2f} s "
                   f"{r['aggregate_mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
+        if not args.no_driver:
+            try:
+                out["driver"] = driver_measurements()
+            except Exception as e:   # the line is still printed, with the reason
+                print(f"# driver measurements: {e}", file=sys.stderr, flush=True)
+                out["driver"] = {"error": str(e)[:300]}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(q, shape, tau, G, init)
         if args.workload == "d3q19_fp64_256_droplet":

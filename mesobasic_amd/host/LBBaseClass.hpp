@@ -20,8 +20,9 @@ public:
     virtual void updateLatticeBoltzmann() = 0;
     virtual void outputLatticeBoltzmann() = 0;
     virtual void setTimeStep(int) = 0;
-    // Completes output still in flight at the end of the run (the reference
-    // writes each output before returning and needs none).
+    // Completes output still in flight at the end of the run and waits for
+    // the device to finish the steps queued after it (the reference writes
+    // each output before returning and steps synchronously).
     virtual void finishOutput() {}
 
     virtual ~LBBaseClass() {}

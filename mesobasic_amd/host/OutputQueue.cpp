@@ -100,6 +100,8 @@ void OutputQueue::finish(lbm_handle* h)
 {
     if (pending_) collect_and_write(h);
     join_writer();
+    // the steps after the last output are still queued on the device
+    check(lbm_synchronize(h), "lbm_synchronize");
 }
 
 }  // namespace mesobasic

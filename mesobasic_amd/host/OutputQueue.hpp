@@ -38,7 +38,8 @@ public:
 
     // The output of step `step` of component comp of handle h.
     void output(lbm_handle* h, int comp, int step);
-    // Writes what is still pending; rethrows a writer's error.
+    // Writes what is still pending, then waits for the handle's queued
+    // device work; rethrows a writer's error.
     void finish(lbm_handle* h);
 
 private:

@@ -1,13 +1,63 @@
 #include "VtkWriter.hpp"
 
+#include <algorithm>
+#include <charconv>
 #include <fstream>
 #include <iomanip>
 #include <sstream>
 #include <stdexcept>
+#include <thread>
 
 namespace mesobasic {
 
+// v as printf("%.3f") prints it in the C locale (std::to_chars in fixed form
+// with a precision is specified to give exactly that, "nan", "-nan", "inf"
+// and "-0.000" included).  Needs kMaxFixed3 bytes at p.
+char* put_fixed3(char* p, double v)
+{
+    return std::to_chars(p, p + kMaxFixed3, v, std::chars_format::fixed, 3).ptr;
+}
+
 namespace {
+
+// Formats lines [0, nlines) into text, line l by emit(l, p) -> end pointer,
+// each line at most max_line bytes; large files are formatted by several
+// threads (each its own block of lines, joined in order).  The reference
+// ends every line with std::endl, which flushes the stream per value; the
+// bytes are the same.
+template <typename F>
+std::string format_lines(long long nlines, size_t max_line, F emit)
+{
+    const long long kMinPerThread = 1 << 16;
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const int nt = static_cast<int>(std::max(1LL, std::min<long long>(hw, nlines / kMinPerThread)));
+    std::vector<std::string> parts(nt);
+    auto work = [&](int t) {
+        const long long l0 = nlines * t / nt, l1 = nlines * (t + 1) / nt;
+        std::string& s = parts[t];
+        s.resize(static_cast<size_t>(l1 - l0) * 8 + max_line);
+        size_t at = 0;
+        for (long long l = l0; l < l1; l++) {
+            if (s.size() - at < max_line) s.resize(std::max(s.size() * 2, at + max_line));
+            at = static_cast<size_t>(emit(l, &s[at]) - s.data());
+        }
+        s.resize(at);
+    };
+    if (nt == 1) {
+        work(0);
+        return std::move(parts[0]);
+    }
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; t++) th.emplace_back(work, t);
+    work(0);
+    for (auto& t : th) t.join();
+    size_t total = 0;
+    for (auto& s : parts) total += s.size();
+    std::string out;
+    out.reserve(total);
+    for (auto& s : parts) out += s;
+    return out;
+}
 
 std::ofstream open_vtk(const std::string& dir, const std::string& tag, int step)
 {
@@ -37,15 +87,21 @@ void write_vtk_3d(const std::string& dir, const std::string& tag, int step, int 
     out << "POINT_DATA " << (NX / iskip) * (NY / jskip) * (NZ / kskip) << std::endl;
     out << "SCALARS " << tag << " float" << std::endl;
     out << "LOOKUP_TABLE default" << std::endl;
-    // loop order and selection of LBfluid3D.cpp:399-414 (one rank, xOffset 0)
-    for (int k = 1; k < NZ + 1; k += kskip)
-        for (int j = 1; j < NY + 1; j += jskip)
-            for (int i = 1; i < NX + 1; i++) {
-                if (i == 0 || i % iskip == 0) {
-                    const long long c = (static_cast<long long>(i - 1) * NY + (j - 1)) * NZ + (k - 1);
-                    out << std::fixed << std::setprecision(3) << rho[c] << std::endl;
-                }
-            }
+    // loop order and selection of LBfluid3D.cpp:399-414 (one rank, xOffset
+    // 0): k, then j, then the i that iskip divides, x fastest
+    const int nk = (NZ + kskip - 1) / kskip, nj = (NY + jskip - 1) / jskip, ni = NX / iskip;
+    const std::string text = format_lines(
+        static_cast<long long>(nk) * nj * ni, kMaxFixed3 + 1, [&](long long l, char* p) {
+            const int i = static_cast<int>(l % ni + 1) * iskip;
+            const long long kj = l / ni;
+            const int j = static_cast<int>(kj % nj) * jskip + 1, k = static_cast<int>(kj / nj) * kskip + 1;
+            const long long c = (static_cast<long long>(i - 1) * NY + (j - 1)) * NZ + (k - 1);
+            p = put_fixed3(p, rho[c]);
+            *p++ = '\n';
+            return p;
+        });
+    out.write(text.data(), static_cast<std::streamsize>(text.size()));
+    out.flush();
 }
 
 void write_vtk_2d(const std::string& dir, const std::string& tag, int step, int NX, int NY,
@@ -66,22 +122,34 @@ void write_vtk_2d(const std::string& dir, const std::string& tag, int step, int 
     out << "POINT_DATA " << (NX / iskip) * (NY / jskip) << std::endl;
     out << "SCALARS " << tag << " float" << std::endl;
     out << "LOOKUP_TABLE default" << std::endl;
-    // LBfluid2D.cpp:440-453
-    for (int j = 1; j < NY + 1; j += jskip)
-        for (int i = 1; i < NX + 1; i++)
-            if (i == 0 || i % iskip == 0)
-                out << std::fixed << std::setprecision(3)
-                    << rho[static_cast<long long>(i - 1) * NY + (j - 1)] << std::endl;
+    // LBfluid2D.cpp:440-453: j, then the i that iskip divides, x fastest
+    const int nj = (NY + jskip - 1) / jskip, ni = NX / iskip;
+    const long long nl = static_cast<long long>(nj) * ni;
+    auto node = [&](long long l) {
+        const int i = static_cast<int>(l % ni + 1) * iskip, j = static_cast<int>(l / ni) * jskip + 1;
+        return static_cast<long long>(i - 1) * NY + (j - 1);
+    };
+    std::string text = format_lines(nl, kMaxFixed3 + 1, [&](long long l, char* p) {
+        p = put_fixed3(p, rho[node(l)]);
+        *p++ = '\n';
+        return p;
+    });
+    out.write(text.data(), static_cast<std::streamsize>(text.size()));
     // LBfluid2D.cpp:461-479
     out << "   " << std::endl;
     out << "VECTORS Velocity float" << std::endl;
-    for (int j = 1; j < NY + 1; j += jskip)
-        for (int i = 1; i < NX + 1; i++)
-            if (i == 0 || i % iskip == 0) {
-                const long long c = static_cast<long long>(i - 1) * NY + (j - 1);
-                out << std::fixed << std::setprecision(3) << ustar[c] << " " << vstar[c] << " "
-                    << 0.0 << std::endl;
-            }
+    text = format_lines(nl, 3 * kMaxFixed3 + 3, [&](long long l, char* p) {
+        const long long c = node(l);
+        p = put_fixed3(p, ustar[c]);
+        *p++ = ' ';
+        p = put_fixed3(p, vstar[c]);
+        *p++ = ' ';
+        p = put_fixed3(p, 0.0);
+        *p++ = '\n';
+        return p;
+    });
+    out.write(text.data(), static_cast<std::streamsize>(text.size()));
+    out.flush();
 }
 
 void write_model_vtk(const std::string& dir, const CommonParams& p, int q, int step,

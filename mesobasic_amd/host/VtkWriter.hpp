@@ -11,6 +11,14 @@
 
 namespace mesobasic {
 
+// Longest text printf("%.3f") gives for a double: sign, 309 integer
+// digits, the point and 3 decimals.
+constexpr int kMaxFixed3 = 320;
+
+// Writes v at p as printf("%.3f") would in the C locale and returns the end
+// (no terminating zero); p must have kMaxFixed3 bytes of room.
+char* put_fixed3(char* p, double v);
+
 // rho: interior array (x slowest, z fastest) of NX*NY*NZ values.
 void write_vtk_3d(const std::string& dir, const std::string& tag, int step, int NX, int NY,
                   int NZ, int iskip, int jskip, int kskip, const std::vector<double>& rho);

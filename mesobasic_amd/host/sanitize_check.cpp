@@ -6,9 +6,14 @@
 #include <sys/stat.h>
 #include <unistd.h>
 
+#include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <fstream>
+#include <iomanip>
+#include <limits>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -119,6 +124,101 @@ void vtk_checks()
     rmdir(dir);
 }
 
+// put_fixed3 against printf("%.3f"): special values, ties (exactly
+// representable values halfway between two 3-decimal results), values next
+// to the rounding boundaries, every exponent, and random ones.
+void format_checks()
+{
+    auto same = [](double v) {
+        char a[400], b[kMaxFixed3 + 1];
+        std::snprintf(a, sizeof a, "%.3f", v);
+        char* e = put_fixed3(b, v);
+        *e = 0;
+        return std::string(a) == std::string(b);
+    };
+    int bad = 0;
+    std::vector<double> vals = {0.0, -0.0, 1.0, -1.0, 0.0005, -0.0005, 0.0015, 0.0025, 0.0625, 0.1875,
+                                1.0005, 2.0625, 0.9995, 999.9995, 1e-300, 5e-324, 1e308, -1e308,
+                                1.7976931348623157e308, 123456789.0125, 0.7, 2.0, 0.15};
+    vals.push_back(std::numeric_limits<double>::quiet_NaN());
+    vals.push_back(-std::numeric_limits<double>::quiet_NaN());
+    vals.push_back(std::numeric_limits<double>::infinity());
+    vals.push_back(-std::numeric_limits<double>::infinity());
+    for (int k = -20000; k <= 20000; k++) {
+        const double t = (k + 0.5) / 1000.0;   // near the boundaries
+        vals.push_back(t);
+        vals.push_back(std::nextafter(t, 1e9));
+        vals.push_back(std::nextafter(t, -1e9));
+        vals.push_back(k / 16.0);               // exact binary ties at 3 decimals
+    }
+    uint64_t x = 0x9E3779B97F4A7C15ull;
+    for (int i = 0; i < 200000; i++) {
+        x ^= x << 13; x ^= x >> 7; x ^= x << 17;
+        double v;
+        std::memcpy(&v, &x, sizeof v);         // every exponent
+        vals.push_back(v);
+        vals.push_back(static_cast<double>(x >> 11) / 9007199254740992.0 * 20.0 - 10.0);
+    }
+    for (double v : vals) bad += !same(v);
+    check(bad == 0, "put_fixed3 equals printf %.3f (" + std::to_string(bad) + " differ)");
+}
+
+// The writers' files against the same loops printed through an ostream
+// (fixed, 3 decimals, one value per line), on a lattice large enough for
+// several formatting threads, with and without skips.
+void vtk_bytes_checks()
+{
+    char tmpl[] = "/tmp/san_vtkb_XXXXXX";
+    const char* dir = mkdtemp(tmpl);
+    check(dir != nullptr, "mkdtemp");
+    if (!dir) return;
+    const int NX = 70, NY = 64, NZ = 40;
+    std::vector<double> rho(static_cast<size_t>(NX) * NY * NZ);
+    uint64_t x = 12345;
+    for (auto& r : rho) {
+        x = x * 6364136223846793005ull + 1442695040888963407ull;
+        r = 0.1 + 2.0 * static_cast<double>(x >> 11) / 9007199254740992.0;
+    }
+    rho[7] = -0.0;
+    rho[99] = std::numeric_limits<double>::quiet_NaN();
+    for (int sk : {1, 3}) {
+        const int step = 100 + sk;
+        write_vtk_3d(dir, "rhoA", step, NX, NY, NZ, sk, sk, sk, rho);
+        const std::string got = slurp(std::string(dir) + "/rhoA_" + std::to_string(step) + ".vtk");
+        std::ostringstream want;
+        for (int k = 1; k < NZ + 1; k += sk)
+            for (int j = 1; j < NY + 1; j += sk)
+                for (int i = 1; i < NX + 1; i++)
+                    if (i % sk == 0)
+                        want << std::fixed << std::setprecision(3)
+                             << rho[(static_cast<size_t>(i - 1) * NY + (j - 1)) * NZ + (k - 1)] << "\n";
+        const size_t body = got.find("LOOKUP_TABLE default\n");
+        check(body != std::string::npos && got.substr(body + 21) == want.str(),
+              "3-D VTK body equals the ostream loop, skip " + std::to_string(sk));
+        std::remove((std::string(dir) + "/rhoA_" + std::to_string(step) + ".vtk").c_str());
+    }
+    const int MX = 700, MY = 400;
+    std::vector<double> r2(static_cast<size_t>(MX) * MY), us(r2.size()), vs(r2.size());
+    for (size_t c = 0; c < r2.size(); c++) r2[c] = rho[c % rho.size()], us[c] = 0.01 * (c % 97) - 0.4, vs[c] = -1e-4 * c;
+    write_vtk_2d(dir, "rhoA", 5, MX, MY, 2, 1, r2, us, vs);
+    const std::string got = slurp(std::string(dir) + "/rhoA_5.vtk");
+    std::ostringstream want;
+    for (int j = 1; j < MY + 1; j += 1)
+        for (int i = 1; i < MX + 1; i++)
+            if (i % 2 == 0) want << std::fixed << std::setprecision(3) << r2[static_cast<size_t>(i - 1) * MY + (j - 1)] << "\n";
+    want << "   " << "\n" << "VECTORS Velocity float" << "\n";
+    for (int j = 1; j < MY + 1; j += 1)
+        for (int i = 1; i < MX + 1; i++)
+            if (i % 2 == 0) {
+                const size_t c = static_cast<size_t>(i - 1) * MY + (j - 1);
+                want << std::fixed << std::setprecision(3) << us[c] << " " << vs[c] << " " << 0.0 << "\n";
+            }
+    const size_t body = got.find("LOOKUP_TABLE default\n");
+    check(body != std::string::npos && got.substr(body + 21) == want.str(), "2-D VTK body equals the ostream loop");
+    std::remove((std::string(dir) + "/rhoA_5.vtk").c_str());
+    rmdir(dir);
+}
+
 void factory_checks()
 {
     CommonParams cp{};
@@ -150,6 +250,8 @@ int main()
 {
     parser_checks();
     vtk_checks();
+    format_checks();
+    vtk_bytes_checks();
     factory_checks();
     if (failures) {
         std::fprintf(stderr, "%d check(s) failed\n", failures);
