@@ -125,6 +125,84 @@ def cpu_baseline_reference(q, shape, tau, G, init, ranks=None, seconds=10.0):
                       f"steps ({dt:.1f} s)"}
 
 
+def _time_harness(make_cmd, n, k1, k2, timeout=180):
+    """Seconds of k2 - k1 steps of a reference harness: two runs (k1 and
+    k2 steps, the same start-up and final dump) differenced; None when a run
+    fails.  make_cmd(k, outdir) gives the command line of a k-step run."""
+    import subprocess
+    import tempfile
+    times = []
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        for k in (k1, k2):
+            t0 = time.perf_counter()
+            try:
+                r = subprocess.run(make_cmd(k, d), capture_output=True, timeout=timeout)
+            except (OSError, subprocess.TimeoutExpired):
+                return None
+            times.append(time.perf_counter() - t0)
+            if r.returncode != 0:
+                return None
+    dt = times[1] - times[0]
+    return dt if dt > 0 else None
+
+
+def cpu_baseline_mcmp(name, ranks=None, seconds=10.0):
+    """The reference's own two-component CPU path (its mcmp2D / mcmp3D
+    classes, compiled in place into oracle/_ref/ref_harness_mcmp, x slabs over
+    MPI ranks) on the same workload and initial densities as time_mcmp."""
+    import shutil
+    import tempfile
+    harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness_mcmp")
+    mpiexec = shutil.which("mpiexec") or "/opt/conda/bin/mpiexec"
+    if not (os.path.exists(harness) and os.path.exists(mpiexec)):
+        return None
+    q, shape, ta, tb, G = MC_WORKLOADS[name]
+    ranks = ranks or min(len(os.sched_getaffinity(0)), 16)
+    n = int(np.prod(shape))
+    full = tuple(shape) + ((1,) if len(shape) == 2 else ())
+    k1 = 2
+    k2 = k1 + max(10, min(20000, int(seconds * 2e6 * ranks / n)))
+    with tempfile.TemporaryDirectory(dir="/tmp") as d0:
+        ra, rb = os.path.join(d0, "rA.bin"), os.path.join(d0, "rB.bin")
+        mb.noise_rho(shape, 31, mean=0.5, amp=0.1).astype(np.float64).ravel().tofile(ra)
+        mb.noise_rho(shape, 32, mean=0.5, amp=0.1).astype(np.float64).ravel().tofile(rb)
+        dt = _time_harness(lambda k, d: [mpiexec, "-np", str(ranks), harness, str(q), *map(str, full),
+                                         repr(ta), repr(tb), repr(G), ra, rb, os.path.join(d, "out"),
+                                         f"+{k}"], n, k1, k2)
+    if dt is None:
+        return None
+    return {"value": round(n * (k2 - k1) / dt / 1e6, 3), "unit": "MLUPS", "cores": ranks,
+            "kind": "reference",
+            "sample": f"reference two-component CPU path (its LB sources compiled in place, -O3, MPI x "
+                      f"slabs), mpiexec -np {ranks}, {'x'.join(map(str, shape))}, {k2} minus {k1} steps "
+                      f"({dt:.1f} s)"}
+
+
+def cpu_baseline_capsule(nx=256, ny=128, nnodes=64, seconds=5.0):
+    """The reference's own capsule2D CPU path (capsule2D / IBcapsules2D
+    compiled in place into oracle/_ref/ref_harness_capsule, one rank) on the
+    lattice of the driver's capsule measurement (tau 0.9, k_stiff 0.8, circle
+    of radius 16 at (64, 63.5)); microseconds per step."""
+    harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness_capsule")
+    if not os.path.exists(harness):
+        return None
+    import tempfile
+    n = nx * ny
+    k1 = 2
+    k2 = k1 + max(100, int(seconds * 1.7e7 / n))   # about 17 M updates per second on one core
+    with tempfile.TemporaryDirectory(dir="/tmp") as d0:
+        r0 = os.path.join(d0, "rho0.bin")
+        np.full(n, 1.0, dtype=np.float64).tofile(r0)
+        dt = _time_harness(lambda k, d: [harness, str(nx), str(ny), "0.9", str(nnodes), "0.8", "64.0",
+                                         "63.5", "16.0", r0, os.path.join(d, "out"), f"+{k}"], n, k1, k2)
+    if dt is None:
+        return None
+    return {"us_per_step": round(dt / (k2 - k1) * 1e6, 2), "mlups": round(n * (k2 - k1) / dt / 1e6, 3),
+            "cores": 1, "kind": "reference",
+            "sample": f"reference capsule2D CPU path (compiled in place, -O3, one rank), {nx}x{ny}, "
+                      f"{nnodes} IB nodes, uniform density 1.0, {k2} minus {k1} steps ({dt:.1f} s)"}
+
+
 def cpu_baseline(q, shape, tau, G, init, budget_s=15.0):
     """The reference's CPU path where its harness is built
     (cpu_baseline_reference), else the oracle (CPU restatement of the
@@ -358,10 +436,16 @@ def main():
                          "values, both layouts), one JSON line each")
     args = ap.parse_args()
     if args.driver:
-        print(json.dumps(driver_measurements()), flush=True)
+        res = driver_measurements()
+        if not args.no_cpu_baseline:
+            res["cpu_baseline_capsule2D_256x128"] = cpu_baseline_capsule()
+        print(json.dumps(res), flush=True)
         return
     if args.mcmp:
-        print(json.dumps({k: time_mcmp(k, steps=args.steps) for k in MC_WORKLOADS}), flush=True)
+        res = {k: time_mcmp(k, steps=args.steps) for k in MC_WORKLOADS}
+        if not args.no_cpu_baseline:
+            res["cpu_baseline_mcmp3d_fp64_256_noise"] = cpu_baseline_mcmp("mcmp3d_fp64_256_noise")
+        print(json.dumps(res), flush=True)
         return
     if args.march_skeleton:
         for n in (256, 512):
@@ -458,8 +542,11 @@ def main():
         # and a 4096^2 one (HBM-bound), automatic kernel choice
         out["d2q9_fp64"] = {"256x256": time_workload("d2q9_fp64_256_noise", steps=2000, warmup=10),
                             "4096x4096": time_workload("d2q9_fp64_4096_noise", steps=30)}
-        # the two-component model (two-pass FP64 kernels)
+        # the two-component model (two-pass FP64 kernels), with the
+        # reference's own CPU path on the 3-D workload
         out["mcmp_fp64"] = {k: time_mcmp(k, steps=10) for k in MC_WORKLOADS}
+        if not args.no_cpu_baseline:
+            out["mcmp_fp64"]["cpu_baseline_mcmp3d_fp64_256_noise"] = cpu_baseline_mcmp("mcmp3d_fp64_256_noise")
     if args.workload == "d3q19_fp64_256_droplet":
         from mesobasic_amd import serve
         if not args.no_batched:
@@ -502,6 +589,8 @@ def main():
         if not args.no_driver:
             try:
                 out["driver"] = driver_measurements()
+                if not args.no_cpu_baseline:
+                    out["driver"]["cpu_baseline_capsule2D_256x128"] = cpu_baseline_capsule()
             except Exception as e:   # the line is still printed, with the reason
                 print(f"# driver measurements: {e}", file=sys.stderr, flush=True)
                 out["driver"] = {"error": str(e)[:300]}

@@ -78,7 +78,11 @@ int main(int argc, char** argv)
         std::string tok;
         while (std::getline(ss, tok, ',')) dumps.push_back(std::atoi(tok.c_str()));
     }
-    const int last = dumps.empty() ? 0 : *std::max_element(dumps.begin(), dumps.end());
+    int last = dumps.empty() ? 0 : *std::max_element(dumps.begin(), dumps.end());
+    if (argv[11][0] == '+') {   // "+K": K steps, no dump (timing the reference's step)
+        last = std::atoi(argv[11] + 1);
+        dumps.clear();
+    }
 
     CommonParams p;
     std::memset(&p, 0, sizeof(p));

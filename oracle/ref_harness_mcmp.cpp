@@ -17,7 +17,8 @@
 //   ref_harness_mcmp q nx ny nz tauA tauB G rhoA0.bin rhoB0.bin out_prefix step[,step...]
 //                    [rhoA_sol rhoB_sol]
 // With the two wall densities the film classes run instead (their keys
-// LBApp/rhoA_sol, rhoB_sol).
+// LBApp/rhoA_sol, rhoB_sol).  A step list "+K" runs K steps and dumps nothing
+// (for timing the reference's step).
 // Writes out_prefix_<c>_<step>.bin for c = A, B: rho | u | v | w | fx | fy | fz
 // | f (float64, interior nodes, x slowest and the last axis fastest; f is
 // [node][q]; w and fz are zeros in 2-D).  Run under mpirun -np N to split x
@@ -257,6 +258,10 @@ int main(int argc, char** argv)
         while (std::getline(ss, tok, ',')) r.dumps.push_back(std::atoi(tok.c_str()));
     }
     r.last = r.dumps.empty() ? 0 : *std::max_element(r.dumps.begin(), r.dumps.end());
+    if (argv[11][0] == '+') {
+        r.last = std::atoi(argv[11] + 1);
+        r.dumps.clear();
+    }
     double sol[2] = {0.0, 0.0};
     const bool film = argc == 14;
     if (film) {
