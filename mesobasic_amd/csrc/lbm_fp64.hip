@@ -673,67 +673,128 @@ __global__ void __launch_bounds__(kBlock)
 // u* = u + F/(2 rho) of the new fluid state (IBcapsules2D.cpp:218-287,
 // LBfluid2D.cpp:164-172; the ghost rows read r = u = 0) and the move
 // x += v dt (IBcapsules2D.cpp:463-469).
+//
+// The nodes are taken kIbStage at a time: their positions, footprint
+// origins and membrane forces are staged in LDS for the ghost-row sums, and
+// each of their footprint points is one thread's work item (pull the 9
+// values, moments, the weighted u*), so the four points of a node are
+// evaluated at once instead of one after another.  A node's velocity then
+// adds its points' terms in the order of the reference's loops (px outer,
+// py inner), so the sums round as they do there.
+constexpr int kIbStage = 256;
+
 template <bool WALLS>
 __global__ void __launch_bounds__(1024) k_ib_finish(const double* __restrict__ P, int kind, Grid g, Capsule cap)
 {
+    static_assert(4 * kIbStage <= 1024, "one thread per footprint point of a staged node");
+    __shared__ double s_x[kIbStage], s_y[kIbStage], s_fx[kIbStage], s_fy[kIbStage];
+    __shared__ int s_ox[kIbStage], s_oy[kIbStage];
+    __shared__ bool s_ok[kIbStage];
+    __shared__ double s_u[4 * kIbStage], s_v[4 * kIbStage];
+    __shared__ bool s_use[4 * kIbStage];
     const int nx = g.nx, ny = g.ny;
-    for (int t = threadIdx.x; t < 2 * nx; t += blockDim.x) {
-        const int side = t / nx, px = t - side * nx;
-        const int py = side ? ny : -1;
-        double gx = cap.gfx[t], gy = cap.gfy[t];
-        for (int n = 0; n < cap.nnodes; n++) {
-            int ox, oy;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    auto stage = [&](int base) {
+        const int cnt = min(kIbStage, cap.nnodes - base);
+        for (int k = tid; k < cnt; k += nt) {
+            const int n = base + k;
             const double xn = cap.x[n], yn = cap.y[n];
-            if (!ib_origin(xn, yn, nx, ny, ox, oy)) continue;
-            const int ax = px - ox, ay = py - oy;
-            if ((ax == 0 || ax == 1) && (ay == 0 || ay == 1)) {
-                const double del = ib_weight(xn, yn, px, py);
-                gx = gx + del * (-cap.kstiff * (xn - cap.x0[n]));
-                gy = gy + del * (-cap.kstiff * (yn - cap.y0[n]));
+            int ox = 0, oy = 0;
+            s_ok[k] = ib_origin(xn, yn, nx, ny, ox, oy);
+            s_x[k] = xn;
+            s_y[k] = yn;
+            s_ox[k] = ox;
+            s_oy[k] = oy;
+            // membrane force, IBcapsules2D.cpp:480-483
+            s_fx[k] = -cap.kstiff * (xn - cap.x0[n]);
+            s_fy[k] = -cap.kstiff * (yn - cap.y0[n]);
+        }
+        return cnt;
+    };
+    // ghost rows: each position adds the spread of every node, in node order
+    for (int t0 = 0; t0 < 2 * nx; t0 += nt) {
+        const int t = t0 + tid;
+        const bool mine = t < 2 * nx;
+        const int side = mine ? t / nx : 0, px = mine ? t - side * nx : 0;
+        const int py = side ? ny : -1;
+        double gx = mine ? cap.gfx[t] : 0.0, gy = mine ? cap.gfy[t] : 0.0;
+        for (int base = 0; base < cap.nnodes; base += kIbStage) {
+            __syncthreads();
+            const int cnt = stage(base);
+            __syncthreads();
+            if (!mine) continue;
+            for (int k = 0; k < cnt; k++) {
+                if (!s_ok[k]) continue;
+                const int ax = px - s_ox[k], ay = py - s_oy[k];
+                if ((ax == 0 || ax == 1) && (ay == 0 || ay == 1)) {
+                    const double del = ib_weight(s_x[k], s_y[k], px, py);
+                    gx = gx + del * s_fx[k];
+                    gy = gy + del * s_fy[k];
+                }
             }
         }
-        cap.gfx[t] = gx;
-        cap.gfy[t] = gy;
+        if (mine) {
+            cap.gfx[t] = gx;
+            cap.gfy[t] = gy;
+        }
     }
-    for (int n = threadIdx.x; n < cap.nnodes; n += blockDim.x) {
+    for (int n = tid; n < cap.nnodes; n += nt) {
         cap.nfx[n] = -cap.kstiff * (cap.x[n] - cap.x0[n]);
         cap.nfy[n] = -cap.kstiff * (cap.y[n] - cap.y0[n]);
     }
     __syncthreads();
-    for (int n = threadIdx.x; n < cap.nnodes; n += blockDim.x) {
-        const double xn = cap.x[n], yn = cap.y[n];
-        double vx = 0.0, vy = 0.0;
-        int ox, oy;
-        if (ib_origin(xn, yn, nx, ny, ox, oy)) {
-            for (int px = ox; px <= ox + 1; px++) {
-                if (px < 0 || px > nx - 1) continue;
-                for (int py = oy; py <= oy + 1; py++) {
-                    const double del = ib_weight(xn, yn, px, py);
-                    double us, vs;
-                    if (py >= 0 && py < ny) {
-                        const int64_t c = static_cast<int64_t>(px) * ny + py;
-                        const Coord q{px, py, 0};
-                        const Nbr nb = neighbours(q, g);
-                        double f[9];
-                        load_f<9, WALLS>(P, g, kind, q, nb, c, f);
-                        double r, sx, sy, sz;
-                        moments<9>(f, r, sx, sy, sz);
-                        us = sx / r + (0.5 * cap.fx[c]) / r;
-                        vs = sy / r + (0.5 * cap.fy[c]) / r;
-                    } else {
-                        const int t = (py < 0 ? 0 : nx) + px;
-                        us = 0.0 + (0.5 * cap.gfx[t]) / 0.0;
-                        vs = 0.0 + (0.5 * cap.gfy[t]) / 0.0;
-                    }
-                    vx = vx + del * us;
-                    vy = vy + del * vs;
+    for (int base = 0; base < cap.nnodes; base += kIbStage) {
+        const int cnt = min(kIbStage, cap.nnodes - base);
+        // item i: point (i & 3) of node base + i / 4, point = 2 (px - ox) + (py - oy)
+        if (tid < 4 * cnt) {
+            const int k = tid >> 2, pt = tid & 3;
+            const double xn = cap.x[base + k], yn = cap.y[base + k];
+            int ox = 0, oy = 0;
+            const bool ok = ib_origin(xn, yn, nx, ny, ox, oy);
+            const int px = ox + (pt >> 1), py = oy + (pt & 1);
+            const bool use = ok && px >= 0 && px <= nx - 1;
+            double cu = 0.0, cv = 0.0;
+            if (use) {
+                const double del = ib_weight(xn, yn, px, py);
+                double us, vs;
+                if (py >= 0 && py < ny) {
+                    const int64_t c = static_cast<int64_t>(px) * ny + py;
+                    const Coord q{px, py, 0};
+                    const Nbr nb = neighbours(q, g);
+                    double f[9];
+                    load_f<9, WALLS>(P, g, kind, q, nb, c, f);
+                    double r, sx, sy, sz;
+                    moments<9>(f, r, sx, sy, sz);
+                    us = sx / r + (0.5 * cap.fx[c]) / r;
+                    vs = sy / r + (0.5 * cap.fy[c]) / r;
+                } else {
+                    const int t = (py < 0 ? 0 : nx) + px;
+                    us = 0.0 + (0.5 * cap.gfx[t]) / 0.0;
+                    vs = 0.0 + (0.5 * cap.gfy[t]) / 0.0;
                 }
+                cu = del * us;
+                cv = del * vs;
             }
+            s_u[tid] = cu;
+            s_v[tid] = cv;
+            s_use[tid] = use;
         }
-        cap.vx[n] = vx;
-        cap.vy[n] = vy;
-        cap.x[n] = xn + vx * cap.dt;
-        cap.y[n] = yn + vy * cap.dt;
+        __syncthreads();
+        if (tid < cnt) {
+            const int n = base + tid;
+            double vx = 0.0, vy = 0.0;
+            for (int pt = 0; pt < 4; pt++) {
+                if (!s_use[4 * tid + pt]) continue;
+                vx = vx + s_u[4 * tid + pt];
+                vy = vy + s_v[4 * tid + pt];
+            }
+            const double xn = cap.x[n], yn = cap.y[n];
+            cap.vx[n] = vx;
+            cap.vy[n] = vy;
+            cap.x[n] = xn + vx * cap.dt;
+            cap.y[n] = yn + vy * cap.dt;
+        }
+        __syncthreads();
     }
 }
 
