@@ -146,17 +146,19 @@ def _time_harness(make_cmd, n, k1, k2, timeout=180):
     return dt if dt > 0 else None
 
 
-def cpu_baseline_mcmp(name, ranks=None, seconds=10.0):
+def cpu_baseline_mcmp(name, ranks=None, seconds=10.0, shape=None):
     """The reference's own two-component CPU path (its mcmp2D / mcmp3D
     classes, compiled in place into oracle/_ref/ref_harness_mcmp, x slabs over
-    MPI ranks) on the same workload and initial densities as time_mcmp."""
+    MPI ranks) on the same workload and initial densities as time_mcmp
+    (shape: another lattice of the same stencil)."""
     import shutil
     import tempfile
     harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness_mcmp")
     mpiexec = shutil.which("mpiexec") or "/opt/conda/bin/mpiexec"
     if not (os.path.exists(harness) and os.path.exists(mpiexec)):
         return None
-    q, shape, ta, tb, G = MC_WORKLOADS[name]
+    q, wshape, ta, tb, G = MC_WORKLOADS[name]
+    shape = tuple(shape) if shape is not None else wshape
     ranks = ranks or min(len(os.sched_getaffinity(0)), 16)
     n = int(np.prod(shape))
     full = tuple(shape) + ((1,) if len(shape) == 2 else ())
