@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <charconv>
+#include <exception>
 #include <fstream>
 #include <iomanip>
 #include <sstream>
@@ -47,10 +48,21 @@ std::string format_lines(long long nlines, size_t max_line, F emit)
         work(0);
         return std::move(parts[0]);
     }
+    // an exception in a worker (an allocation failing) is rethrown here
+    std::vector<std::exception_ptr> err(nt);
+    auto guarded = [&](int t) {
+        try {
+            work(t);
+        } catch (...) {
+            err[t] = std::current_exception();
+        }
+    };
     std::vector<std::thread> th;
-    for (int t = 1; t < nt; t++) th.emplace_back(work, t);
-    work(0);
+    for (int t = 1; t < nt; t++) th.emplace_back(guarded, t);
+    guarded(0);
     for (auto& t : th) t.join();
+    for (auto& e : err)
+        if (e) std::rethrow_exception(e);
     size_t total = 0;
     for (auto& s : parts) total += s.size();
     std::string out;
