@@ -2,14 +2,18 @@
 //
 // Same grammar and lookup semantics as the GetPot calls the reference makes
 // (src/utils/GetPot, used by MesoExecute.cpp:58-116, LBApp.cpp:18-28,
-// scmp3D.cpp:21-31): the file is a stream of whitespace-separated tokens,
-// '#' starts a comment to the end of the line, "[Name]" opens a section
+// scmp3D.cpp:21-31), restated from GetPot's reading and parsing code: the
+// file is a stream of tokens split at spaces, tabs and newlines, '#' at the
+// start of a token comments to the end of the line, "[Name]" opens a section
 // ("[./sub]" relative to the current one, "[../sub]" relative to the parent,
-// "[]" back to the root), "key = value" (with or without spaces round '=')
-// assigns Section/key, and a later assignment of the same key wins.  Numbers
-// convert with "%lf" semantics; integers go through double, as GetPot's int
-// conversion does (GetPot:802-818).  A missing or unconvertible key yields
-// the caller's default.
+// "[]" back to the root), "key = value" or "key=value" assigns Section/key
+// (other spacing round '=' does not: "key =v" assigns the section itself),
+// and a later assignment of the same key wins.  Numbers convert with "%lf"
+// semantics; integers go through double, as GetPot's int conversion does
+// (GetPot:802-818).  A missing or unconvertible key yields the caller's
+// default.  GetPot's "${...}" expressions are not evaluated: a file with one
+// is refused.  tests/test_getpot_parity.py compares this reader with GetPot
+// itself on a corpus of inputs.
 #pragma once
 
 #include <string>

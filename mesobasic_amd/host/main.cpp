@@ -18,6 +18,7 @@
 
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <dirent.h>
 #include <exception>
@@ -55,6 +56,27 @@ int print_params(const char* path)
     for (const auto& s : in.section_names()) std::printf("section %s\n", s.c_str());
     for (const auto& kv : in.variables()) std::printf("%s = %s\n", kv.first.c_str(), kv.second.c_str());
     std::printf("nstep %d numOutputs %d\n", in("Time/nstep", 0), in("Output/numOutputs", 1));
+    return 0;
+}
+
+// The parse of `path` as GetPot reports it (oracle/getpot_check prints the
+// same lines from the reference's own GetPot, and a test compares them):
+// sections, every variable with its string value, and the typed lookups
+// i:KEY:DEFAULT, d:KEY:DEFAULT, s:KEY:DEFAULT.
+int query(int argc, char** argv)
+{
+    InputParams in(argv[2]);
+    for (const auto& s : in.section_names()) std::printf("section %s\n", s.c_str());
+    for (const auto& kv : in.variables()) std::printf("var %s = [%s]\n", kv.first.c_str(), kv.second.c_str());
+    for (int i = 3; i < argc; i++) {
+        const std::string q = argv[i];
+        const size_t a = q.find(':'), b = q.find(':', a + 1);
+        if (a != 1 || b == std::string::npos) throw std::runtime_error("query must be t:key:default");
+        const std::string key = q.substr(2, b - 2), def = q.substr(b + 1);
+        if (q[0] == 'i') std::printf("i %s %d\n", key.c_str(), in(key.c_str(), std::atoi(def.c_str())));
+        else if (q[0] == 'd') std::printf("d %s %.17g\n", key.c_str(), in(key.c_str(), std::strtod(def.c_str(), nullptr)));
+        else std::printf("s %s [%s]\n", key.c_str(), in(key.c_str(), def.c_str()).c_str());
+    }
     return 0;
 }
 
@@ -104,6 +126,7 @@ int main(int argc, char** argv)
 {
     try {
         if (argc == 3 && std::strcmp(argv[1], "--print-params") == 0) return print_params(argv[2]);
+        if (argc >= 3 && std::strcmp(argv[1], "--query") == 0) return query(argc, argv);
         if (argc >= 2 && std::strcmp(argv[1], "--serve") == 0) return serve(argc, argv);
         const char* path = argc > 1 ? argv[1] : "input.dat";
         InputParams in(path);
