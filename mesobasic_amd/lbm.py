@@ -457,6 +457,11 @@ class Batch:
         return dict(state=st.state, steps_done=st.steps_done, steps_finite=st.steps_finite,
                     mass=st.mass)
 
+    def set_work_limit(self, max_nodes):
+        """Nodes advanced per tick at most (0: every job); the jobs with the
+        least remaining work go first (lbm_batch_set_work_limit)."""
+        _check(self._lib.lbm_batch_set_work_limit(self._h, int(max_nodes)))
+
     def cancel(self, job_id):
         """Drops an admitted job (running or finished) and frees its memory."""
         _check(self._lib.lbm_batch_cancel(self._h, int(job_id)))

@@ -77,7 +77,7 @@ def initial_rho(req):
 
 def run_stream(reqs, precision=FP16N, budget_bytes=24 << 30, max_active=0, steps_per_tick=20,
                work_limit=0, keep_results=False, clock=time.perf_counter, engine=None,
-               alloc_retries=3, keep_ids=(), max_wait=50):
+               alloc_retries=3, keep_ids=(), max_wait=50, split="global"):
     """Serves `reqs` in real time; returns a summary dict (and, when
     keep_results, the per-request fields of every request, else of the ids in
     keep_ids, as a second value).
@@ -89,6 +89,12 @@ def run_stream(reqs, precision=FP16N, budget_bytes=24 << 30, max_active=0, steps
     the least remaining work, up to that many nodes (split evenly between
     the engines); a job left
     waiting for max_wait ticks goes first (no starvation of large requests).
+    split: how the node limit is shared between the engines of several
+    forms -- "global" (each turn, the live requests of all engines are
+    ranked by remaining work, nodes x steps left, and each engine gets the
+    nodes of its requests in the prefix that fits the limit; an engine with
+    none waits that turn) or "even" (each engine gets limit / number of
+    forms).
     A request may carry a "deadline" (seconds after its arrival): a request
     still queued or running past it is cancelled and its memory freed.
     Outcomes: "ok"; "failed" (the engine retired the job because its FP64
@@ -116,6 +122,7 @@ def run_stream(reqs, precision=FP16N, budget_bytes=24 << 30, max_active=0, steps
 
     pending = list(reqs)            # not yet admitted, arrival order
     live = {}                       # (engine, job id) -> request
+    skipped = {}                    # engine -> turns in a row without a tick (split="global")
     outcome, latency, fail_info = {}, {}, {}
     results = {}
     keep_ids = set(keep_ids)
@@ -172,10 +179,33 @@ def run_stream(reqs, precision=FP16N, budget_bytes=24 << 30, max_active=0, steps
                 time.sleep(max(0.0, pending[0]["arrival"] - (clock() - t0)))
             continue
         busy = {b for (b, _) in live}
+        if limit and len(engines) > 1 and split == "global":
+            # the global shortest-remaining-work prefix, shared out by engine
+            rank = []
+            for (b, jid), r in live.items():
+                left = r["steps"] - b.status(jid)["steps_done"]
+                rank.append((int(np.prod(r["shape"])) * left, int(np.prod(r["shape"])), id(b), b))
+            rank.sort(key=lambda t: (t[0], t[2]))
+            share, total = {}, 0
+            for _, n, _, b in rank:
+                if total and total + n > work_limit:
+                    break
+                total += n
+                share[b] = share.get(b, 0) + n
+            # an engine left out max_wait turns in a row runs its smallest
+            # job at least (no starvation across engines)
+            for b in {b for (b, _) in live}:
+                skipped[b] = 0 if b in share else skipped.get(b, 0) + 1
+                if max_wait and skipped[b] >= max_wait:
+                    share[b] = max(share.get(b, 0), min(n for _, n, _, e in rank if e is b))
+                    skipped[b] = 0
+            busy = set(share)
+            for b in busy:
+                b.set_work_limit(share[b])
         for b in busy:
             b.tick()
         ticks += 1
-        for b in busy:
+        for b in {b for (b, _) in live}:
             for jid in b.finished():
                 r = live.pop((b, jid))
                 out = b.collect(jid)

@@ -123,3 +123,66 @@ def test_sample_ids_takes_both_init_types():
     ids = serve.sample_ids(reqs, 6, max_work=1e12)
     inits = {reqs[i]["init"] for i in ids}
     assert len(ids) == 6 and inits == {"droplet", "noise"}
+
+
+class FakeLimitedBatch(FakeBatch):
+    """Stand-in engine with the work limit of the real one: a tick advances
+    the unfinished jobs in order of remaining work (nodes x steps left), the
+    prefix that fits the limit (at least one job)."""
+
+    engines = []
+
+    def __init__(self, precision=None, hbm_budget=0, steps_per_tick=10, work_limit=0, max_wait=0, **_):
+        super().__init__(hbm_budget, steps_per_tick=steps_per_tick)
+        self.form, self.limit, self.ticks = precision, work_limit, []
+        FakeLimitedBatch.engines.append(self)
+
+    def set_work_limit(self, n):
+        self.limit = n
+
+    def status(self, jid):
+        return dict(steps_done=self.jobs[jid]["done"])
+
+    def tick(self):
+        live = [(int(j["rho"].size) * (j["steps"] - j["done"]), i) for i, j in self.jobs.items()
+                if j["done"] < j["steps"]]
+        live.sort()
+        picked, nodes = [], 0
+        for _, i in live:
+            n = int(self.jobs[i]["rho"].size)
+            if picked and self.limit and nodes + n > self.limit:
+                break
+            picked.append(i)
+            nodes += n
+        for i in picked:
+            j = self.jobs[i]
+            j["done"] = min(j["steps"], j["done"] + self.k)
+            j["finite"] = j["done"]
+        self.ticks.append((tuple(picked), nodes))
+
+    def close(self):
+        pass
+
+
+def test_global_split_shares_the_limit_and_starves_no_engine(monkeypatch):
+    # one large D3Q19 request among a stream of small D2Q9 ones: under the
+    # global split the small ones take the node limit, yet the large one
+    # still advances at least once every max_wait + 1 turns
+    FakeLimitedBatch.engines = []
+    monkeypatch.setattr(serve, "Batch", FakeLimitedBatch)
+    reqs = [dict(id=0, arrival=0.0, q=19, shape=(8, 8, 8), steps=200, tau=1.0, G=-5.0, init="noise", seed=1)]
+    reqs += [dict(id=k, arrival=0.0, q=9, shape=(8, 8), steps=40, tau=1.0, G=-5.0, init="noise", seed=k)
+             for k in range(1, 40)]
+    summary, res = serve.run_stream(reqs, precision={9: "A", 19: "B"}, budget_bytes=10 ** 9,
+                                    steps_per_tick=10, work_limit=4 * 64, max_wait=3, keep_results=True)
+    assert summary["outcomes"]["ok"] == len(reqs)
+    by_form = {e.form: e for e in FakeLimitedBatch.engines}
+    a, b = by_form["A"], by_form["B"]
+    # the small jobs went first: A ran every one of its ticks at the limit
+    # while B was held back, and B was never left out more than 3 turns
+    assert all(n <= 4 * 64 for _, n in a.ticks)
+    assert summary["ticks"] > len(b.ticks)
+    assert len(b.ticks) >= summary["ticks"] // 4
+    # results are the requests' own (the stand-in's result depends on nothing else)
+    for r in reqs:
+        np.testing.assert_array_equal(res[r["id"]]["rho"], serve.initial_rho(r) * 2.0)
