@@ -356,6 +356,17 @@ int lbm_batch_submit(lbm_batch* b, const lbm_job_desc* job, const double* rho0, 
  * leaves the other jobs' results unchanged. */
 int lbm_batch_tick(lbm_batch* b, int* n_finished);
 
+/* lbm_batch_tick in two halves, so that the ticks of several batches run on
+ * the device together: begin uploads the tables and launches the tick (with
+ * the read-back of its counters and mass totals) on the batch's stream and
+ * returns; end waits for it and does the rest of lbm_batch_tick (retiring
+ * failed jobs, reporting finished ones).  Between the two, calls that change
+ * the batch (submit, collect, read, extend, cancel, the set_* calls and
+ * another begin) return LBM_ESTATE.  No counterpart in the reference (the
+ * batched form is this library's own). */
+int lbm_batch_tick_begin(lbm_batch* b);
+int lbm_batch_tick_end(lbm_batch* b, int* n_finished);
+
 /* Ids of finished, uncollected jobs (up to max), failed ones included. */
 int lbm_batch_finished(lbm_batch* b, int64_t* ids, int max, int* n);
 

@@ -125,6 +125,10 @@ struct lbm_batch {
     std::vector<Config> candidates;
     std::vector<double> cand_rate;
     int cand_idx = 0;
+    // a tick launched by lbm_batch_tick_begin and not yet ended
+    bool tick_pending = false;
+    std::chrono::steady_clock::time_point tick_t0;
+    std::vector<int64_t> tick_done_before;
     // tick graph
     hipGraphExec_t graph = nullptr;
     bool graph_valid = false;
@@ -313,6 +317,12 @@ void row_chunks(const lbm_batch* b, const int64_t (&rows)[kNumRow], int (&chunk)
         const int64_t c = rows[k] / std::max<int64_t>(1, per_round);
         chunk[k] = static_cast<int>(std::clamp<int64_t>(c, lo, b->chunk));
     }
+}
+
+// Error of a call that would change a batch while its tick is in flight.
+int in_flight(const char* what)
+{
+    return lbm::set_error(LBM_ESTATE, "%s: a tick is in flight (lbm_batch_tick_end first)", what);
 }
 
 // Rebuilds and uploads the job and item tables (between ticks only).
@@ -715,6 +725,7 @@ int64_t lbm_batch_job_bytes(const lbm_batch* b, const lbm_job_desc* job)
 int lbm_batch_submit(lbm_batch* b, const lbm_job_desc* job, const double* rho0, int64_t* job_id)
 {
     if (!b || !job || !rho0) return lbm::set_error(LBM_EINVAL, "lbm_batch_submit: NULL argument");
+    if (b->tick_pending) return in_flight("lbm_batch_submit");
     if (job->q != 9 && job->q != 19) return lbm::set_error(LBM_EINVAL, "job q must be 9 or 19");
     const int nz = job->q == 9 ? 1 : job->nz;
     if (job->nx < 3 || job->ny < 3 || nz < 1 || (job->q == 19 && nz < 3))
@@ -772,9 +783,10 @@ int lbm_batch_submit(lbm_batch* b, const lbm_job_desc* job, const double* rho0, 
     return LBM_OK;
 }
 
-int lbm_batch_tick(lbm_batch* b, int* n_finished)
+int lbm_batch_tick_begin(lbm_batch* b)
 {
-    if (!b) return lbm::set_error(LBM_EINVAL, "lbm_batch_tick: NULL batch");
+    if (!b) return lbm::set_error(LBM_EINVAL, "lbm_batch_tick_begin: NULL batch");
+    if (b->tick_pending) return lbm::set_error(LBM_ESTATE, "lbm_batch_tick_begin: a tick is already in flight");
     if (b->dirty) {
         int rc = upload_tables(b);
         if (rc) return rc;
@@ -783,9 +795,9 @@ int lbm_batch_tick(lbm_batch* b, int* n_finished)
         int rc = prepare_tick(b);
         if (rc) return rc;
     }
-    const auto t0 = std::chrono::steady_clock::now();
-    std::vector<int64_t> done_before(b->order.size());
-    for (size_t ji = 0; ji < b->order.size(); ji++) done_before[ji] = b->order[ji]->done;
+    b->tick_t0 = std::chrono::steady_clock::now();
+    b->tick_done_before.resize(b->order.size());
+    for (size_t ji = 0; ji < b->order.size(); ji++) b->tick_done_before[ji] = b->order[ji]->done;
     if (!b->order.empty()) {
         int rc = run_tick(b);
         if (rc) return rc;
@@ -795,8 +807,27 @@ int lbm_batch_tick(lbm_batch* b, int* n_finished)
             BHIP(hipMemcpyAsync(b->h_mass, b->d_mass, sizeof(double) * b->nitems[kMass],
                                 hipMemcpyDeviceToHost, b->stream));
     }
+    b->tick_pending = true;
+    return LBM_OK;
+}
+
+int lbm_batch_tick(lbm_batch* b, int* n_finished)
+{
+    if (!b) return lbm::set_error(LBM_EINVAL, "lbm_batch_tick: NULL batch");
+    int rc = lbm_batch_tick_begin(b);
+    if (rc) return rc;
+    return lbm_batch_tick_end(b, n_finished);
+}
+
+int lbm_batch_tick_end(lbm_batch* b, int* n_finished)
+{
+    if (!b) return lbm::set_error(LBM_EINVAL, "lbm_batch_tick_end: NULL batch");
+    if (!b->tick_pending) return lbm::set_error(LBM_ESTATE, "lbm_batch_tick_end: no tick in flight");
+    b->tick_pending = false;
     BHIP(hipStreamSynchronize(b->stream));
-    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    const double secs =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - b->tick_t0).count();
+    const std::vector<int64_t>& done_before = b->tick_done_before;
     int64_t nodes = 0;   // lattice updates of the tick
     for (size_t ji = 0; ji < b->order.size(); ji++)
         nodes += b->order[ji]->n * (b->h_jobs[ji].done - done_before[ji]);
@@ -864,6 +895,7 @@ int lbm_batch_finished(lbm_batch* b, int64_t* ids, int max, int* n)
 int lbm_batch_collect(lbm_batch* b, int64_t job_id, double* rho, double* u, double* v, double* w)
 {
     if (!b) return lbm::set_error(LBM_EINVAL, "lbm_batch_collect: NULL batch");
+    if (b->tick_pending) return in_flight("lbm_batch_collect");
     auto it = b->jobs.find(job_id);
     if (it == b->jobs.end()) return lbm::set_error(LBM_EINVAL, "unknown job %lld", (long long)job_id);
     HostJob* j = it->second.get();
@@ -897,6 +929,7 @@ int lbm_batch_read(lbm_batch* b, int64_t job_id, double* rho, double* u, double*
                    double* fy, double* fz)
 {
     if (!b) return lbm::set_error(LBM_EINVAL, "lbm_batch_read: NULL batch");
+    if (b->tick_pending) return in_flight("lbm_batch_read");
     auto it = b->jobs.find(job_id);
     if (it == b->jobs.end()) return lbm::set_error(LBM_EINVAL, "unknown job %lld", (long long)job_id);
     const HostJob& j = *it->second;
@@ -953,6 +986,7 @@ int lbm_batch_read(lbm_batch* b, int64_t job_id, double* rho, double* u, double*
 int lbm_batch_extend(lbm_batch* b, int64_t job_id, int64_t nsteps)
 {
     if (!b) return lbm::set_error(LBM_EINVAL, "lbm_batch_extend: NULL batch");
+    if (b->tick_pending) return in_flight("lbm_batch_extend");
     if (nsteps < 1) return lbm::set_error(LBM_EINVAL, "lbm_batch_extend: nsteps %lld < 1", (long long)nsteps);
     auto it = b->jobs.find(job_id);
     if (it == b->jobs.end()) return lbm::set_error(LBM_EINVAL, "unknown job %lld", (long long)job_id);
@@ -988,6 +1022,7 @@ int lbm_batch_job_status(const lbm_batch* b, int64_t job_id, lbm_job_status* st)
 int lbm_batch_cancel(lbm_batch* b, int64_t job_id)
 {
     if (!b) return lbm::set_error(LBM_EINVAL, "lbm_batch_cancel: NULL batch");
+    if (b->tick_pending) return in_flight("lbm_batch_cancel");
     auto it = b->jobs.find(job_id);
     if (it == b->jobs.end()) return lbm::set_error(LBM_EINVAL, "unknown job %lld", (long long)job_id);
     // ticks are synchronous, so no launch of this batch is in flight here
@@ -1007,6 +1042,7 @@ int64_t lbm_batch_steps_run(const lbm_batch* b) { return b ? b->steps_run : -1; 
 int lbm_batch_set_work_limit(lbm_batch* b, int64_t max_nodes)
 {
     if (!b || max_nodes < 0) return lbm::set_error(LBM_EINVAL, "lbm_batch_set_work_limit: bad argument");
+    if (b->tick_pending) return in_flight("lbm_batch_set_work_limit");
     b->work_limit = max_nodes;
     b->dirty = true;
     return LBM_OK;
@@ -1015,6 +1051,7 @@ int lbm_batch_set_work_limit(lbm_batch* b, int64_t max_nodes)
 int lbm_batch_set_max_wait(lbm_batch* b, int ticks)
 {
     if (!b || ticks < 0) return lbm::set_error(LBM_EINVAL, "lbm_batch_set_max_wait: bad argument");
+    if (b->tick_pending) return in_flight("lbm_batch_set_max_wait");
     b->max_wait = ticks;
     b->dirty = true;
     return LBM_OK;
@@ -1023,6 +1060,7 @@ int lbm_batch_set_max_wait(lbm_batch* b, int ticks)
 int lbm_batch_set_concurrency(lbm_batch* b, int on)
 {
     if (!b || (on != 0 && on != 1)) return lbm::set_error(LBM_EINVAL, "lbm_batch_set_concurrency: bad argument");
+    if (b->tick_pending) return in_flight("lbm_batch_set_concurrency");
     if (b->concurrent != (on == 1)) {
         b->graph_valid = false;   // the captured launch sequence changes
         b->dirty = true;          // and the row items (row_chunks)
@@ -1034,6 +1072,7 @@ int lbm_batch_set_concurrency(lbm_batch* b, int on)
 int lbm_batch_set_launch(lbm_batch* b, int d2q9_chunk, int d2q9_rows_kernel, int d3q19_march)
 {
     if (!b) return lbm::set_error(LBM_EINVAL, "lbm_batch_set_launch: NULL batch");
+    if (b->tick_pending) return in_flight("lbm_batch_set_launch");
     if (d2q9_chunk < 1) return lbm::set_error(LBM_EINVAL, "lbm_batch_set_launch: chunk must be >= 1");
     if (d2q9_rows_kernel < 0 || d2q9_rows_kernel > 2)
         return lbm::set_error(LBM_EINVAL, "lbm_batch_set_launch: d2q9_rows_kernel must be 0, 1 or 2");

@@ -109,6 +109,8 @@ def load():
         "lbm_batch_job_bytes": (ctypes.c_int64, [P, ctypes.POINTER(JobDesc)]),
         "lbm_batch_submit": (ctypes.c_int, [P, ctypes.POINTER(JobDesc), _D, ctypes.POINTER(ctypes.c_int64)]),
         "lbm_batch_tick": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int)]),
+        "lbm_batch_tick_begin": (ctypes.c_int, [P]),
+        "lbm_batch_tick_end": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int)]),
         "lbm_batch_finished": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int64), ctypes.c_int,
                                               ctypes.POINTER(ctypes.c_int)]),
         "lbm_batch_collect": (ctypes.c_int, [P, ctypes.c_int64, _D, _D, _D, _D]),
@@ -484,6 +486,17 @@ class Batch:
     def tick(self):
         n = ctypes.c_int()
         _check(self._lib.lbm_batch_tick(self._h, ctypes.byref(n)))
+        return n.value
+
+    def tick_begin(self):
+        """Launches a tick and returns (lbm_batch_tick_begin); tick_end()
+        completes it.  Ticks of several batches then run on the device
+        together."""
+        _check(self._lib.lbm_batch_tick_begin(self._h))
+
+    def tick_end(self):
+        n = ctypes.c_int()
+        _check(self._lib.lbm_batch_tick_end(self._h, ctypes.byref(n)))
         return n.value
 
     def finished(self, max_ids=1 << 16):

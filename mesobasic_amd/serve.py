@@ -69,6 +69,30 @@ def make_requests(n, rate=200.0, seed=7):
     return reqs
 
 
+def tick_all(engines):
+    """One tick of every engine: all are launched first (tick_begin), then
+    completed (tick_end), so that their kernels share the device; an engine
+    without the split interface ticks on its own.  A tick that was launched
+    is always completed, even when a later launch fails."""
+    begun = []
+    try:
+        for b in engines:
+            if hasattr(b, "tick_begin"):
+                b.tick_begin()
+                begun.append(b)
+            else:
+                b.tick()
+    finally:
+        first = None
+        for b in begun:
+            try:
+                b.tick_end()
+            except Exception as e:   # completes the others, then reports the first
+                first = first or e
+        if first is not None:
+            raise first
+
+
 def initial_rho(req):
     if req["init"] == "droplet":
         return droplet_rho(req["shape"], 3.0 * min(req["shape"]) / 16.0)
@@ -202,8 +226,7 @@ def run_stream(reqs, precision=FP16N, budget_bytes=24 << 30, max_active=0, steps
             busy = set(share)
             for b in busy:
                 b.set_work_limit(share[b])
-        for b in busy:
-            b.tick()
+        tick_all(busy)
         ticks += 1
         for b in {b for (b, _) in live}:
             for jid in b.finished():

@@ -91,6 +91,47 @@ def test_concurrent_classes_equal_one_stream_bitwise(prec, launch):
             assert same_bits(runs[0][k][f], runs[2][k][f]), (k, f, jobs[k])
 
 
+def test_overlapped_ticks_of_two_batches_equal_their_own_ticks_bitwise():
+    # two batches of different forms ticked together (both launched with
+    # tick_begin, then both completed with tick_end) give the bits each gives
+    # ticked alone; between begin and end a batch refuses changes
+    rng = np.random.default_rng(21)
+    jobs_a, jobs_b = _jobs(rng, 8), _jobs(rng, 8, q9_share=0.3)
+    alone_a = _run(Batch(precision=mb.FP32, steps_per_tick=6), jobs_a)
+    alone_b = _run(Batch(precision=mb.FP16N, steps_per_tick=6), jobs_b)
+    a, b = Batch(precision=mb.FP32, steps_per_tick=6), Batch(precision=mb.FP16N, steps_per_tick=6)
+    ids = {}
+    for batch, jobs, tag in ((a, jobs_a, "a"), (b, jobs_b, "b")):
+        for k, j in enumerate(jobs):
+            jid = batch.submit(j["q"], j["shape"], j["tau"], j["G"], j["steps"], mb.noise_rho(j["shape"], j["seed"]))
+            ids[(tag, jid)] = k
+    out = {"a": {}, "b": {}}
+    checked = False
+    while len(out["a"]) < len(jobs_a) or len(out["b"]) < len(jobs_b):
+        a.tick_begin()
+        b.tick_begin()
+        if not checked:
+            j = jobs_a[0]
+            for call in (lambda: a.submit(j["q"], j["shape"], j["tau"], j["G"], 4, mb.noise_rho(j["shape"], 1)),
+                         lambda: a.set_work_limit(100), lambda: a.tick_begin()):
+                with pytest.raises(mb.lbm.LbmError):
+                    call()
+            checked = True
+        a.tick_end()
+        b.tick_end()
+        for batch, tag in ((a, "a"), (b, "b")):
+            for jid in batch.finished():
+                out[tag][ids[(tag, jid)]] = batch.collect(jid)
+    with pytest.raises(mb.lbm.LbmError):
+        a.tick_end()            # nothing in flight
+    a.close()
+    b.close()
+    for got, ref, jobs in ((out["a"], alone_a, jobs_a), (out["b"], alone_b, jobs_b)):
+        for k in range(len(jobs)):
+            for f in ("rho", "u", "v", "w"):
+                assert same_bits(got[k][f], ref[k][f]), (k, f, jobs[k])
+
+
 @pytest.mark.parametrize("q,shape,steps", [(9, (64, 48), 37), (9, (40, 200), 20), (19, (20, 18, 16), 23)])
 def test_job_matches_fp64_oracle(q, shape, steps):
     tau, G = 0.9, -5.0
