@@ -33,11 +33,16 @@ import mesobasic_amd as mb  # noqa: E402
 
 METRIC = "D3Q19 Shan-Chen MLUPS & % HBM roofline; batched-serve p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
-# nodes advanced per step by the serving loop (least remaining work first);
-# measured on MI355X (200-request stream, FP16 node shift, p50 / p99 s,
-# GLUPS): all jobs 13.2 / 17.8 / 27.6, 48 M 4.8 / 17.6 / 27.3, 32 M 3.5 /
-# 17.7 / 27.0, 24 M 3.0 / 18.0 / 26.7
-SERVE_WORK_LIMIT = 32_000_000
+# nodes advanced per step by the serving loop (least remaining work first,
+# ranked over both engines) and steps per tick; measured on MI355X
+# (200-request stream, the forms below, engines' ticks overlapped; p50 / p99
+# s, GLUPS; one box): 32 M x 20 steps 3.7 / 17.9 / 26.1, 32 M x 40 3.6 /
+# 17.4 / 26.8, 32 M x 80 3.5 / 17.1 / 27.3, 24 M x 60 2.9 / 17.9 / 26.9,
+# 20 M x 80 2.45 / 17.4 / 26.8, 16 M x 120 2.1 / 17.1 / 26.5, 12 M x 80 1.9 /
+# 17.8 / 25.5.  Longer ticks cost fewer synchronisations and mass checks;
+# a smaller limit lets short requests through sooner.
+SERVE_WORK_LIMIT = 16_000_000
+SERVE_STEPS_PER_TICK = 120
 # storage form by stencil: FP16 with the node shift is closer to FP64 than
 # FP32 storage on D3Q19 droplets, but not on D2Q9 ones (up to 3e-3 against
 # 2e-4), so D2Q9 requests are served in FP32 storage; measured on MI355X
@@ -563,13 +568,14 @@ def main():
             reqs = serve.make_requests(args.serve_requests, 200.0, 7)
             sample = serve.sample_ids(reqs, 6)
             r, kept = serve.run_stream(reqs, precision=SERVE_FORMS, work_limit=SERVE_WORK_LIMIT,
-                                       keep_ids=sample)
+                                       steps_per_tick=SERVE_STEPS_PER_TICK, keep_ids=sample)
             r["config"] = (f"{args.serve_requests} requests, Poisson 200/s (seed 7), 70% D2Q9 "
                            "128..1024 per side, 30% D3Q19 64..192 per side, steps U[500, 5000], "
                            "G U[-6, -4.5], droplet or noisy init, D2Q9 requests in FP32 storage and "
                            "D3Q19 requests in FP16 node-shifted storage (two engines sharing the HBM "
                            "budget of 24 GiB), shortest remaining work first within "
-                           f"{SERVE_WORK_LIMIT} nodes per step in all")
+                           f"{SERVE_WORK_LIMIT} nodes per step in all, ticks of "
+                           f"{SERVE_STEPS_PER_TICK} steps")
             # accuracy of a seeded sample of served requests against the FP64
             # drop-in path (bit-identical to the reference), after the stream
             ok = [i for i in sample if i in kept and not kept[i]["failed"]]
