@@ -215,10 +215,78 @@ struct PairHeld {
     }
 };
 
+// Collision of a pair from the window (win[i] = {psi, rho} of window node
+// i): the arithmetic of lp_prep / lp_out (tau = 1) in the same order, so the
+// results are the same bits.  The pair's neighbours e_n are adjacent window
+// entries, read in one LDS instruction for psi and rho of both nodes; the
+// neighbour densities (the node shift's reference densities) stay in
+// registers from here to pair_out.
+template <typename SP, bool NS, int LZ>
+__device__ __forceinline__ void pair_prep(const PairHeld& d, const int (&sl)[3], int li,
+                                          const float2* win, float tau, float G, LpColl (&k)[2],
+                                          float (&rn)[2][19])
+{
+    using S = St<19>;
+    const float4 own = *reinterpret_cast<const float4*>(win + sl[1] + li);
+    float psi[2][19];
+    static_for<19>([&](auto I) {
+        constexpr int n = decltype(I)::value;
+        if constexpr (n > 0) {
+            const int b = sl[1 + S::ex[n]] + li + S::ey[n] * LZ;
+            if constexpr (S::ez[n] == 0) {
+                const float4 q = *reinterpret_cast<const float4*>(win + b);
+                psi[0][n] = q.x; rn[0][n] = q.y; psi[1][n] = q.z; rn[1][n] = q.w;
+            } else {
+                const float2 qa = win[b + S::ez[n]], qb = win[b + S::ez[n] + 1];
+                psi[0][n] = qa.x; rn[0][n] = qa.y; psi[1][n] = qb.x; rn[1][n] = qb.y;
+            }
+        }
+    });
+    const float orho[2] = {own.y, own.w}, opsi[2] = {own.x, own.z};
+    static_for<2>([&](auto K) {
+        constexpr int i = decltype(K)::value;
+        LpColl& c = k[i];
+        c.rho = orho[i];
+        rn[i][0] = orho[i];
+        const float jx = d.j[i].jx, jy = d.j[i].jy, jz = d.j[i].jz;
+        float gx = 0.f, gy = 0.f, gz = 0.f;
+        static_for<19>([&](auto I) {
+            constexpr int n = decltype(I)::value;
+            if constexpr (n > 0) {
+                const float p = static_cast<float>(S::w[n]) * psi[i][n];
+                acc_e<S::ex[n]>(gx, p);
+                acc_e<S::ey[n]>(gy, p);
+                acc_e<S::ez[n]>(gz, p);
+            }
+        });
+        const float ir = __builtin_amdgcn_rcpf(c.rho);
+        const float jr = ir * (1.0f / kStoreScale<SP, NS>);
+        const float kf = (tau * ir) * ((-G) * opsi[i]);
+        c.ue = fmaf(kf, gx, jx * jr);
+        c.ve = fmaf(kf, gy, jy * jr);
+        c.we = fmaf(kf, gz, jz * jr);
+        c.usq15 = 1.5f * fmaf(c.ue, c.ue, fmaf(c.ve, c.ve, c.we * c.we));
+        c.t = 0.f;
+    });
+}
+
+template <typename SP, bool NS, int n>
+__device__ __forceinline__ typename SP::T pair_out(const LpColl& k, const float (&rn)[19], float rs)
+{
+    using S = St<19>;
+    constexpr float wn = static_cast<float>(S::w[n]);
+    constexpr float sw = kStoreScale<SP, NS> * wn;
+    const float ev = edot<S::ex[n], S::ey[n], S::ez[n]>(k.ue, k.ve, k.we);
+    const float pm1 = fmaf(fmaf(4.5f, ev, 3.0f), ev, -k.usq15);
+    const float rnb = NS ? rn[n] : rs;
+    return SP::of_d(fmaf(sw * k.rho, pm1, fmaf(-sw, rnb, sw * k.rho)));
+}
+
 // The pair step kernel: one TY x TZ tile per workgroup (z0 a multiple of
 // TZ = 128 nodes).
-// Occupancy target 2 wavefronts per SIMD: the kernel then takes 135 VGPRs
-// with no spills (3 wavefronts fit by registers).  Held to 128 VGPRs (4 per
+// Occupancy target 2 wavefronts per SIMD: the kernel then takes 130 VGPRs
+// (171 with the node shift, whose neighbour densities stay in registers
+// through the collision) with no spills.  Held to 128 VGPRs (4 per
 // SIMD) the compiler spills 80-100 bytes per thread, and the kernel runs at
 // 35 GLUPS instead of 57 (D3Q19 512^3 FP16, MI355X).
 template <typename SP, bool NS, typename TL, bool NTS>
@@ -230,7 +298,9 @@ __global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(2))
     constexpr int TY = TL::TY, TZ = TL::TZ, PZ = TL::PZ, LZ = TL::LZ, PL = TL::LY * TL::LZ;
     constexpr int NT = TL::NT;
     constexpr int RW = (TL::NRING + 63) / 64 * 64;   // threads of the ring wavefronts
-    __shared__ float wpsi[4 * PL], wrho[4 * PL];
+    // window: {psi, rho} of each node (16-byte aligned: pairs start at even entries)
+    __shared__ float4 win4[2 * PL];
+    float2* const win = reinterpret_cast<float2*>(win4);
     // seam patch slots: own pair (5 x NT) and ring pair (5 x RW)
     __shared__ uint32_t pown[5 * NT], pring[5 * RW];
 
@@ -293,10 +363,7 @@ __global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(2))
     auto put_win = [&](int at, const PairVals& d, const float (&R)[2], float (&rho)[2]) {
         rho[0] = fmaf(sum19(PairNode<SP, 0>{d.v}), isc, R[0]);
         rho[1] = fmaf(sum19(PairNode<SP, 1>{d.v}), isc, R[1]);
-        wpsi[at] = psi_lp(rho[0]);
-        wpsi[at + 1] = psi_lp(rho[1]);
-        wrho[at] = rho[0];
-        wrho[at + 1] = rho[1];
+        *reinterpret_cast<float4*>(win + at) = make_float4(psi_lp(rho[0]), rho[0], psi_lp(rho[1]), rho[1]);
     };
     // after wait_vmem(): merge the patches, fill the window; the density
     // field is written for this thread's pairs of the planes of this chunk
@@ -342,18 +409,16 @@ __global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(2))
     prologue(x0, Ra, &fa);
     prologue(x0 + 1, Rb, &fb);
 
-    const float om = 1.0f / tau, omt = 1.0f - om;
-
     auto collide = [&](int x, const PairHeld& d, const float (&R)[2]) {
         const int sl[3] = {slot(x - 1), slot(x), slot(x + 1)};
-        const LpNode n0{sl, li, -1, 1}, n1{sl, li + 1, -1, 1};
-        const LpColl k0 = lp_prep<SP, NS, true, LZ>(d.j[0], n0, wpsi, wrho, R[0], tau, G, om, omt);
-        const LpColl k1 = lp_prep<SP, NS, true, LZ>(d.j[1], n1, wpsi, wrho, R[1], tau, G, om, omt);
+        LpColl k[2];
+        float rn[2][19];
+        pair_prep<SP, NS, LZ>(d, sl, li, win, tau, G, k, rn);
         const uint32_t c = static_cast<uint32_t>(x) * plane + cyz;
         static_for<19>([&](auto I) {
             constexpr int n = decltype(I)::value;
-            const typename SP::T v0 = lp_out<SP, NS, true, LZ, n>(k0, d.j[0], n0, wrho, rs, om, omt);
-            const typename SP::T v1 = lp_out<SP, NS, true, LZ, n>(k1, d.j[1], n1, wrho, rs, om, omt);
+            const typename SP::T v0 = pair_out<SP, NS, n>(k[0], rn[0], rs);
+            const typename SP::T v1 = pair_out<SP, NS, n>(k[1], rn[1], rs);
             const uint32_t pv = static_cast<uint32_t>(__builtin_bit_cast(unsigned short, v0)) |
                                 (static_cast<uint32_t>(__builtin_bit_cast(unsigned short, v1)) << 16);
             uint32_t* base = reinterpret_cast<uint32_t*>(elem(Pout + n * g.ps, c));
