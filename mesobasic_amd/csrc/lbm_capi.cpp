@@ -320,12 +320,13 @@ int lbm_create(const lbm_desc* desc, lbm_handle** out)
     h->tune.shape = desc->precision == LBM_FP64
                         ? lbm::fp64_default_shape(h->g, desc->q)
                         : lbm::lowp_default_shape(desc->precision, h->a, h->g);
-    // x planes per workgroup: at most 128 for the FP64 and the pair tiles
-    // (fewer prologue planes; measured 1.5-2.3% faster than 64 at 256^3 and
-    // 512^3 on MI355X), 64 for the others (flat between 32 and 256), fewer
-    // where the lattice would not fill the CUs (fused::default_chunk)
+    // x planes per workgroup: at most 256 for the FP64 and the pair tiles
+    // (fewer prologue planes; 128 measured 1.5-2.3% faster than 64 at 256^3
+    // and 512^3 on MI355X, and 256 another 0.3-0.9% at 512^3), 64 for the
+    // others (flat between 32 and 256), fewer where the lattice would not
+    // fill the CUs (fused::default_chunk)
     const bool wide = desc->precision == LBM_FP64 || lbm::fused::is_pair_shape(h->tune.shape);
-    const int max_chunk = wide ? 128 : 64;
+    const int max_chunk = wide ? 256 : 64;
     // resident waves per CU: 2 per SIMD for FP64, the pair tiles and the
     // one-node 16-bit tiles with walls, 4 for the other FP32-compute tiles
     const int waves_per_cu = (wide || desc->walls) ? 8 : 16;
